@@ -1,0 +1,115 @@
+/*
+ * mini_raytrace_ext.h -- additive extensions of the MiniRaytrace C ABI served
+ * by this project's libmrt.so.  Nothing in mini_raytrace.h changes; callers
+ * that only know that header keep working.
+ *
+ * Extensions:
+ *   1. Ray queries over a standalone triangle set (closest hit / any hit) on
+ *      device-resident ray batches.  These expose, as a C entry point, what
+ *      the reference runs internally in core/shaders/intersect.glsl:48-178
+ *      and restates on the CPU in tests/bvh_tests.cpp:49-73.
+ *   2. Per-engine path options: RNG seed, path guiding on/off and its
+ *      parameters (the reference has no counterpart; see each declaration).
+ *   3. Batched rendering of several engines in one persistent launch.
+ *   4. Per-frame GPU timing and counters for profiling.
+ *
+ * All entry points follow the conventions of mini_raytrace.h: structSize
+ * first, mrtResult codes, no exceptions across the boundary, borrowed
+ * pointers, one thread per engine / query scene.
+ */
+#ifndef MINI_RAYTRACE_EXT_H
+#define MINI_RAYTRACE_EXT_H
+
+#include "mini_raytrace.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- 1. ray queries ------------------------------------------------------- */
+
+/* Host-side BVH build (no device needed): the reference BvhBuilder::build
+ * (core/src/accel/Bvh.cpp:137-156) as a C entry point.  bounds: 6 floats per
+ * primitive (lo xyz, hi xyz).  Writes up to nodeCap 32-byte nodes and `count`
+ * primitive indices; *outNodeCount receives the node count (0 for count 0).
+ * Returns MRT_ERROR_INVALID_ARGUMENT when nodeCap is too small. */
+MRT_API mrtResult mrtxBuildBvh(const float* bounds, uint32_t count, uint32_t maxLeaf,
+                               void* nodes, uint64_t nodeCap, uint32_t* primOrder,
+                               uint64_t* outNodeCount);
+
+typedef struct mrtxQueryScene mrtxQueryScene;
+
+/* Ray record on the device: 8 floats, 32 bytes:
+ *   origin.x origin.y origin.z tmin  dir.x dir.y dir.z tmax
+ * A hit must satisfy tmin < t < tmax. */
+typedef struct mrtxRay {
+    float org[3];
+    float tmin;
+    float dir[3];
+    float tmax;
+} mrtxRay;
+
+/* Closest-hit record on the device: 8 bytes.  triId is the index of the
+ * triangle in the order it was passed to mrtxCreateQueryScene, or
+ * MRTX_NO_HIT; t is the hit distance (the ray's tmax on a miss).  Exact ties
+ * in t resolve to the lowest triId. */
+typedef struct mrtxHit {
+    uint32_t triId;
+    float    t;
+} mrtxHit;
+#define MRTX_NO_HIT 0xFFFFFFFFu
+
+typedef struct mrtxQuerySceneInfo {
+    size_t   structSize;
+    uint32_t triCount;
+    uint32_t nodeCount;     /* 32-byte BVH nodes (reference Bvh.hpp:15-26 layout) */
+    uint32_t leafCount;
+    uint32_t maxDepth;      /* root = depth 0 */
+    float    buildMs;       /* host wall time of the BVH build */
+    uint64_t deviceBytes;   /* device memory held by the scene */
+} mrtxQuerySceneInfo;
+
+/* Traversal launch shape.  0 in a field = library default (or the value the
+ * autotuner picked for this scene). */
+typedef struct mrtxTraversalConfig {
+    size_t   structSize;
+    uint32_t blockSize;     /* threads per workgroup: 64, 128, 256 */
+    uint32_t stackDepth;    /* LDS stack entries per lane: 8, 16, 32 */
+} mrtxTraversalConfig;
+
+/* Builds a binned-SAH BVH (reference core/src/accel/Bvh.cpp:137-156
+ * semantics, max leaf 4) over triCount triangles and uploads it to HIP device
+ * gpuIndex1-1 (0 = current device).  positions: xyz per vertex; indices: 3 per
+ * triangle.  Host pointers, borrowed. */
+MRT_API mrtResult mrtxCreateQueryScene(const float* positions, uint32_t vertexCount,
+                                       const uint32_t* indices, uint32_t triCount,
+                                       uint32_t gpuIndex1, mrtxQueryScene** outScene);
+MRT_API void      mrtxDestroyQueryScene(mrtxQueryScene* scene);
+MRT_API mrtResult mrtxGetQuerySceneInfo(const mrtxQueryScene* scene, mrtxQuerySceneInfo* out);
+/* Copies the BVH nodes (32 bytes each, nodeCount of them) and the primitive
+ * order (triCount uint32) to host memory.  Either pointer may be NULL. */
+MRT_API mrtResult mrtxExportQueryBvh(const mrtxQueryScene* scene, void* nodes, size_t nodesBytes,
+                                     uint32_t* primOrder, size_t primCount);
+MRT_API mrtResult mrtxSetTraversalConfig(mrtxQueryScene* scene, const mrtxTraversalConfig* cfg);
+MRT_API mrtResult mrtxGetTraversalConfig(const mrtxQueryScene* scene, mrtxTraversalConfig* out);
+
+/* Closest hit for n device-resident rays; results to device memory.  Work is
+ * enqueued on `hipStream` (a hipStream_t; NULL = the null stream) and the
+ * call returns without synchronising. */
+MRT_API mrtResult mrtxTraceClosest(mrtxQueryScene* scene, const mrtxRay* dRays, uint32_t n,
+                                   mrtxHit* dHits, void* hipStream);
+/* Any hit: dOccluded[i] = 1 if some triangle has tmin < t < tmax, else 0. */
+MRT_API mrtResult mrtxTraceAny(mrtxQueryScene* scene, const mrtxRay* dRays, uint32_t n,
+                               uint32_t* dOccluded, void* hipStream);
+/* Synchronises the device and returns sticky status bits of all traces so
+ * far: bit 0 = a traversal stack overflowed (results of that call invalid). */
+MRT_API mrtResult mrtxGetQueryStatus(mrtxQueryScene* scene, uint32_t* flags);
+/* Times every supported launch shape (block size x LDS stack depth) on the
+ * given rays and keeps the fastest for this scene.  Overwrites dHits. */
+MRT_API mrtResult mrtxAutotuneTraversal(mrtxQueryScene* scene, const mrtxRay* dRays, uint32_t n,
+                                        mrtxHit* dHits, mrtxTraversalConfig* chosen);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+#endif /* MINI_RAYTRACE_EXT_H */

@@ -1,0 +1,26 @@
+// Host-callable launchers for the HIP kernels (implemented in *.hip files).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mrt {
+namespace dev {
+
+struct Node;
+
+struct QueryLaunch {
+    const Node* nodes;     // 32-byte BVH nodes, root at 0
+    const float4* tris;    // 3 float4 per leaf-ordered triangle, v0.w = triangle id bits
+    const float4* rays;    // 2 float4 per ray
+    uint32_t n;
+    void* out;             // uint2 hits (closest) or uint32 flags (any)
+    uint32_t* status;      // bit 0 set on traversal stack overflow
+};
+
+hipError_t launchTraceClosest(const QueryLaunch& q, uint32_t block, uint32_t stack, hipStream_t s);
+hipError_t launchTraceAny(const QueryLaunch& q, uint32_t block, uint32_t stack, hipStream_t s);
+bool traversalShapeSupported(uint32_t block, uint32_t stack);
+
+} // namespace dev
+} // namespace mrt

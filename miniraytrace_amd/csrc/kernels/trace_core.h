@@ -1,0 +1,174 @@
+// Device-side BVH traversal building blocks for gfx950 (wave64).
+//
+// Node layout: the reference's 32-byte BvhNode (core/src/accel/Bvh.hpp:15-26)
+// read as two float4: a = (lo.xyz, leftOrPrim bits), b = (hi.xyz, packed bits),
+// packed = (count << 2) | axis, count == 0 -> internal node whose children sit
+// at leftOrPrim and leftOrPrim + 1.
+//
+// Triangles are read from a de-indexed, leaf-ordered array of 3 float4 per
+// triangle (v0.xyz + id bits, v1.xyz, v2.xyz): one 48-byte contiguous read per
+// test instead of the reference's index fetch followed by three dependent
+// position fetches (core/shaders/intersect.glsl:21-25).
+//
+// Intersection is the watertight test of Woop, Benthin and Wald (JCGT 2013).
+// The box test uses IEEE minNum/maxNum (a NaN from 0*inf drops out) and widens
+// far distances by 2^-20 and the hit limit by 2^-16, so rounding never culls a
+// box that holds a hit at t <= limit; together with a (t, id) tie-break this
+// makes traversal results equal brute force exactly.  All floating-point
+// expressions in this file are evaluated without FMA contraction so that the
+// CPU oracle (g++ -ffp-contract=off) reproduces them bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mrt {
+namespace dev {
+
+constexpr float kFarWiden = 1.0f + 1.0f / 1048576.0f;
+constexpr float kLimitWiden = 1.0f + 1.0f / 65536.0f;
+constexpr uint32_t kNoHit = 0xFFFFFFFFu;
+
+struct Node {
+    float4 a;  // lo.xyz, leftOrPrim
+    float4 b;  // hi.xyz, packed
+};
+
+__device__ __forceinline__ Node loadNode(const Node* __restrict__ nodes, uint32_t i) {
+    return nodes[i];
+}
+__device__ __forceinline__ uint32_t nodeLeft(const Node& n) { return __float_as_uint(n.a.w); }
+__device__ __forceinline__ uint32_t nodeCount(const Node& n) { return __float_as_uint(n.b.w) >> 2; }
+
+// Per-ray constants: origin, direction, reciprocal direction and the shear
+// of the watertight test.
+struct RayK {
+    float ox, oy, oz;
+    float dx, dy, dz;
+    float ix, iy, iz;
+    float sx, sy, sz;
+    int kx, ky, kz;
+    float tmin;
+};
+
+__device__ __forceinline__ float pick3(float x, float y, float z, int k) {
+    return k == 0 ? x : (k == 1 ? y : z);
+}
+
+__device__ __forceinline__ RayK setupRay(float ox, float oy, float oz, float dx, float dy,
+                                         float dz, float tmin) {
+#pragma clang fp contract(off)
+    RayK r;
+    r.ox = ox; r.oy = oy; r.oz = oz;
+    r.dx = dx; r.dy = dy; r.dz = dz;
+    r.ix = 1.0f / dx; r.iy = 1.0f / dy; r.iz = 1.0f / dz;
+    const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+    int kz = 0;
+    if (ay > ax) kz = 1;
+    if (az > (kz == 0 ? ax : ay)) kz = 2;
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    const float dkz = pick3(dx, dy, dz, kz);
+    if (dkz < 0.0f) { const int t = kx; kx = ky; ky = t; }
+    r.kx = kx; r.ky = ky; r.kz = kz;
+    r.sx = pick3(dx, dy, dz, kx) / dkz;
+    r.sy = pick3(dx, dy, dz, ky) / dkz;
+    r.sz = 1.0f / dkz;
+    r.tmin = tmin;
+    return r;
+}
+
+// Conservative slab test.  Returns true when the box may hold a hit with
+// t <= limit; tEnter receives the (clamped at 0) entry distance.
+__device__ __forceinline__ bool slab(const Node& n, const RayK& r, float limit, float& tEnter) {
+#pragma clang fp contract(off)
+    const float t0x = (n.a.x - r.ox) * r.ix, t1x = (n.b.x - r.ox) * r.ix;
+    const float t0y = (n.a.y - r.oy) * r.iy, t1y = (n.b.y - r.oy) * r.iy;
+    const float t0z = (n.a.z - r.oz) * r.iz, t1z = (n.b.z - r.oz) * r.iz;
+    const float enter = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.0f));
+    const float far = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * kFarWiden;
+    tEnter = enter;
+    return enter <= fminf(far, limit);
+}
+
+// Watertight ray/triangle test.  Hit iff tmin < t <= tLimit; writes t and the
+// barycentric weights of v1 and v2.
+__device__ __forceinline__ bool isectTri(const RayK& r, const float4& v0, const float4& v1,
+                                         const float4& v2, float tLimit, float& tOut,
+                                         float& b1, float& b2) {
+#pragma clang fp contract(off)
+    const float Ax0 = v0.x - r.ox, Ay0 = v0.y - r.oy, Az0 = v0.z - r.oz;
+    const float Bx0 = v1.x - r.ox, By0 = v1.y - r.oy, Bz0 = v1.z - r.oz;
+    const float Cx0 = v2.x - r.ox, Cy0 = v2.y - r.oy, Cz0 = v2.z - r.oz;
+    const float Akz = pick3(Ax0, Ay0, Az0, r.kz);
+    const float Bkz = pick3(Bx0, By0, Bz0, r.kz);
+    const float Ckz = pick3(Cx0, Cy0, Cz0, r.kz);
+    const float Ax = pick3(Ax0, Ay0, Az0, r.kx) - r.sx * Akz;
+    const float Ay = pick3(Ax0, Ay0, Az0, r.ky) - r.sy * Akz;
+    const float Bx = pick3(Bx0, By0, Bz0, r.kx) - r.sx * Bkz;
+    const float By = pick3(Bx0, By0, Bz0, r.ky) - r.sy * Bkz;
+    const float Cx = pick3(Cx0, Cy0, Cz0, r.kx) - r.sx * Ckz;
+    const float Cy = pick3(Cx0, Cy0, Cz0, r.ky) - r.sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
+        V = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
+        W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = (U + V) + W;
+    if (det == 0.0f) return false;
+    const float Az = r.sz * Akz, Bz = r.sz * Bkz, Cz = r.sz * Ckz;
+    const float T = (U * Az + V * Bz) + W * Cz;
+    const float rcp = 1.0f / det;
+    const float t = T * rcp;
+    if (!(t > r.tmin && t <= tLimit)) return false;
+    tOut = t;
+    b1 = V * rcp;
+    b2 = W * rcp;
+    return true;
+}
+
+// Short traversal stack: the top S entries of each lane live in LDS (entry s
+// of lane `tid` at lds[s * BLOCK + tid], so a wave's accesses to one slot are
+// conflict free); deeper entries spill to a private scratch array.  An entry
+// is (node index, entry distance bits).
+constexpr int kSpill = 64;
+
+template <int S, int BLOCK>
+struct ShortStack {
+    static_assert((S & (S - 1)) == 0, "LDS stack depth must be a power of two");
+    uint2* lds;
+    uint2 spill[kSpill];
+    int sp = 0;
+    int base = 0;
+    bool overflow = false;
+
+    __device__ __forceinline__ explicit ShortStack(uint2* ldsLane) : lds(ldsLane) {}
+    __device__ __forceinline__ bool empty() const { return sp == 0; }
+    __device__ __forceinline__ void push(uint2 e) {
+        if (sp - base == S) {
+            if (base >= kSpill) {  // deeper than the spill area: drop (reported)
+                overflow = true;
+                return;
+            }
+            spill[base] = lds[(base & (S - 1)) * BLOCK];
+            ++base;
+        }
+        lds[(sp & (S - 1)) * BLOCK] = e;
+        ++sp;
+    }
+    __device__ __forceinline__ uint2 pop() {
+        --sp;
+        if (sp < base) {
+            base = sp;
+            return spill[sp];
+        }
+        return lds[(sp & (S - 1)) * BLOCK];
+    }
+};
+
+} // namespace dev
+} // namespace mrt
