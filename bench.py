@@ -115,6 +115,103 @@ def traversal_workload(args, torch) -> dict:
     return out
 
 
+def pt_workload(args, torch) -> dict:
+    """Plain progressive path tracing at 1920x1080, 1 spp per frame, max depth 8:
+    seeded terrain of 1,000,000 triangles + 200 instanced icospheres (1,280
+    triangles each), 3 GGX materials, procedural 2048x1024 sky with a 0.5 deg
+    sun.  One step = one mrtRenderFrame (1 sample for each of the 2,073,600
+    pixels), through the C ABI."""
+    from miniraytrace_amd import scenes
+    from miniraytrace_amd.engine import Engine
+
+    W, H = 1920, 1080
+    e = Engine(width=W, height=H)
+    total = args.warmup + args.steps + 8
+    e.set_render_settings(width=W, height=H, max_bounces=8, spp_limit=total, firefly_clamp=0.0, denoise=False)
+    t0 = time.perf_counter()
+    sky = scenes.sky()
+    info = scenes.load_terrain_scene(e, scenes.CONFIG2, seed=1, sky_img=sky)
+    st = e.commit()
+    _log(f"pt: scene {info['triangles']} triangles, {info['instances']} instances, commit "
+         f"{st['commitMs']:.0f} ms (setup {time.perf_counter() - t0:.1f} s)")
+    for _ in range(args.warmup):
+        e.render_frame()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        e.render_frame()
+    wall = time.perf_counter() - t0
+    msamples = W * H * args.steps / wall / 1e6
+
+    # Profiling pass: per-stage GPU time (HIP events on the engine's stream)
+    # and queue sizes per bounce.
+    e.set_profiling(True)
+    prof = []
+    for _ in range(3):
+        e.render_frame()
+        prof.append(e.frame_stats())
+    e.set_profiling(False)
+    stage = {k: float(np.mean([p["gpuMs"][k] for p in prof])) for k in prof[0]["gpuMs"]}
+    rays = [int(x) for x in prof[-1]["rays"]]
+    shadows = [int(x) for x in prof[-1]["shadowRays"]]
+    out = {
+        "workload": "plain progressive path tracing 1920x1080, 1 spp/frame, max depth 8, "
+                    "terrain 1M tris + 200 icospheres (1.256M tris), 2048x1024 sky",
+        "frames": args.steps,
+        "ms_per_frame": wall * 1e3 / args.steps,
+        "msamples_s": msamples,
+        "gpu_stage_ms": stage,
+        "rays_per_bounce": rays,
+        "shadow_rays_per_bounce": shadows,
+        "status": prof[-1]["status"],
+        "triangles": info["triangles"],
+    }
+
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+        threads = orc.hardware_threads()
+        exp = e.export_scene()
+        cam = orc.camera_lookat(scenes.CONFIG2["camera"]["position"], scenes.CONFIG2["camera"]["target"],
+                                (0, 1, 0), scenes.CONFIG2["camera"]["fov_y_deg"], W, H)
+        # Bounded CPU sample: a 96x54 crop of the frame centre, 1 spp,
+        # FP64 restatement with reference-order fetch counting.
+        cw, ch = 96, 54
+        crop = ((W - cw) // 2, (H - ch) // 2, cw, ch)
+        t1 = time.perf_counter()
+        _, counts = orc.pt_render(exp, sky, cam, W, H, crop=crop, frame0=0, spp=1, max_bounces=8,
+                                  threads=threads, count_fetches=True)
+        t_count = time.perf_counter() - t1
+        q, nn, nt = int(counts[0]), int(counts[1]), int(counts[2])
+        # extend-stage algorithmic bytes per frame: per closest-hit query a
+        # 32-byte ray read, a 20-byte hit record write, 32 B per node popped
+        # and 64 B per triangle tested (reference order), scaled from the
+        # crop's queries per path to the frame's queue sizes.
+        per_query = 32 + 20 + 32 * nn / max(q, 1) + 64 * nt / max(q, 1)
+        frame_queries = sum(rays)
+        bytes_alg = per_query * frame_queries
+        achieved = bytes_alg / (stage["extend"] * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                           "kernel": "kExtend (closest hit, all bounces of one frame)",
+                           "bytes_per_frame": bytes_alg, "queries_per_frame": frame_queries,
+                           "nodes_per_query": nn / max(q, 1), "tris_per_query": nt / max(q, 1),
+                           "crop_queries_per_path": q / (cw * ch)}
+        # CPU baseline: the FP64 path tracer timed on the same crop without counting.
+        reps, t_cpu = 0, 0.0
+        while t_cpu < 10.0 and reps < 64:
+            t1 = time.perf_counter()
+            orc.pt_render(exp, sky, cam, W, H, crop=crop, frame0=reps, spp=1, max_bounces=8, threads=threads)
+            t_cpu += time.perf_counter() - t1
+            reps += 1
+        out["cpu_baseline"] = {"value": cw * ch * reps / t_cpu / 1e6, "unit": "Msamples/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"{reps} frames of a {cw}x{ch} centre crop, FP64 restatement of "
+                                         "pathtrace.comp (no CPU path tracer exists in the reference)",
+                               "count_pass_s": t_count}
+    e.close()
+    return out
+
+
 def copy_workload(args, torch) -> dict:
     """Plain device copy of a known size (1 GiB), used to calibrate the HBM
     byte counters (FETCH_SIZE / WRITE_SIZE) against known traffic."""
@@ -138,9 +235,12 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="traversal", choices=["traversal", "copy"])
-    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--workload", default="pt", choices=["pt", "traversal", "copy"])
+    ap.add_argument("--autotune", action="store_true", default=True)
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--with-traversal", action="store_true", default=True)
+    ap.add_argument("--no-traversal", dest="with_traversal", action="store_false")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -157,26 +257,43 @@ def main() -> int:
         res = copy_workload(args, torch)
         print(json.dumps({"workload": "copy", **res}))
         return 0
-    res = traversal_workload(args, torch)
+    if args.workload == "traversal":
+        res = traversal_workload(args, torch)
+        line = {"metric": "closest-hit Mrays/s, 10k-triangle soup, 1M rays", "value": res["value_mrays_s"],
+                "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+                "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+                "config": {"workload": res["workload"]}, "traversal": res}
+        for k in ("roofline", "cpu_baseline"):
+            if k in res:
+                line[k] = res[k]
+        print(json.dumps(line))
+        return 0
+
+    res = pt_workload(args, torch)
     line = {
         "metric": METRIC,
-        "value": None,
+        "value": res["msamples_s"],
         "unit": "Msamples/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": res["ms_per_step"],
+        "ms_per_step": res["ms_per_frame"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": res["workload"]},
-        "traversal": res,
+        "config": {"workload": res["workload"], "width": 1920, "height": 1080, "spp_per_frame": 1,
+                   "max_depth": 8},
+        "pt": res,
     }
     for k in ("roofline", "cpu_baseline"):
         if k in res:
-            line[k] = res[k]
+            line[k] = res.pop(k)
+    if args.with_traversal:
+        trav = traversal_workload(args, torch)
+        line["traversal"] = trav
     print(json.dumps(line))
     return 0
 

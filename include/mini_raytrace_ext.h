@@ -109,6 +109,64 @@ MRT_API mrtResult mrtxGetQueryStatus(mrtxQueryScene* scene, uint32_t* flags);
 MRT_API mrtResult mrtxAutotuneTraversal(mrtxQueryScene* scene, const mrtxRay* dRays, uint32_t n,
                                         mrtxHit* dHits, mrtxTraversalConfig* chosen);
 
+/* ---- 2. per-engine path options ------------------------------------------- */
+
+/* No counterpart in the reference: the reference keys its RNG by (pixel,
+ * frame) only and always runs one launch shape. */
+typedef struct mrtxPathOptions {
+    size_t   structSize;
+    uint32_t seed;          /* 0 = the reference's RNG keys; else mixed into the frame key */
+    uint32_t useGraph;      /* 1 (default) = replay the frame as a captured HIP graph */
+    uint32_t extendBlock;   /* closest-hit stage launch shape (0 = default / tuned) */
+    uint32_t extendStack;
+    uint32_t shadowBlock;   /* shadow stage launch shape */
+    uint32_t shadowStack;
+} mrtxPathOptions;
+
+MRT_API mrtResult mrtxSetPathOptions(mrtEngine* engine, const mrtxPathOptions* opts);
+MRT_API mrtResult mrtxGetPathOptions(mrtEngine* engine, mrtxPathOptions* out);
+
+/* ---- 4. profiling and inspection ------------------------------------------- */
+
+#define MRTX_MAX_BOUNCE_LEVELS 18
+
+/* Stage order of gpuMs: generate, extend, shade, shadow, accumulate, resolve. */
+typedef struct mrtxFrameStats {
+    size_t   structSize;
+    float    frameMs;                                /* host wall time of the last frame */
+    float    gpuMs[6];                               /* per stage, profiling mode only */
+    uint32_t rays[MRTX_MAX_BOUNCE_LEVELS];           /* rays traced per bounce level */
+    uint32_t shadowRays[MRTX_MAX_BOUNCE_LEVELS];     /* shadow rays per bounce level */
+    uint32_t status;                                 /* bit 0: traversal stack overflow */
+} mrtxFrameStats;
+
+/* Profiling mode: each frame runs stage by stage with HIP events between the
+ * stages (no graph replay) and records ray counts. */
+MRT_API mrtResult mrtxSetProfiling(mrtEngine* engine, int32_t enable);
+MRT_API mrtResult mrtxGetFrameStats(mrtEngine* engine, mrtxFrameStats* out);
+
+/* Copy of the packed device scene (commits pending edits first).  Counts are
+ * always filled; each array pointer may be NULL; arrays must hold the counts
+ * reported by a previous call. */
+typedef struct mrtxSceneExport {
+    size_t   structSize;
+    uint32_t nodeCount, triCount, vertexCount, instanceCount;
+    uint32_t materialCount, lightCount, tlasRoot, blasNodeCount;
+    uint32_t envWidth, envHeight;
+    float    envIntegral;
+    void*    nodes;        /* nodeCount x 32 B: all BLAS nodes, then the TLAS */
+    void*    triangles;    /* triCount x 16 B: global vertex indices, leaf order */
+    float*   positions;    /* vertexCount x 4 floats */
+    float*   normals;      /* vertexCount x 4 floats */
+    float*   uvs;          /* vertexCount x 2 floats */
+    void*    instances;    /* instanceCount x 144 B (TLAS leaf order) */
+    void*    materials;    /* materialCount x 80 B (index 0 = default) */
+    void*    lights;       /* lightCount x 80 B (environment first when present) */
+    void*    envAlias;     /* envWidth*envHeight x (float prob, uint32 alias) */
+} mrtxSceneExport;
+
+MRT_API mrtResult mrtxExportScene(mrtEngine* engine, mrtxSceneExport* inout);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -122,6 +122,32 @@ class mrtxTraversalConfig(_Sized):
                 ("stackDepth", C.c_uint32)]
 
 
+class mrtxPathOptions(_Sized):
+    _fields_ = [("structSize", C.c_size_t), ("seed", C.c_uint32), ("useGraph", C.c_uint32),
+                ("extendBlock", C.c_uint32), ("extendStack", C.c_uint32),
+                ("shadowBlock", C.c_uint32), ("shadowStack", C.c_uint32)]
+
+
+MRTX_MAX_BOUNCE_LEVELS = 18
+
+
+class mrtxFrameStats(_Sized):
+    _fields_ = [("structSize", C.c_size_t), ("frameMs", C.c_float), ("gpuMs", C.c_float * 6),
+                ("rays", C.c_uint32 * MRTX_MAX_BOUNCE_LEVELS),
+                ("shadowRays", C.c_uint32 * MRTX_MAX_BOUNCE_LEVELS), ("status", C.c_uint32)]
+
+
+class mrtxSceneExport(_Sized):
+    _fields_ = [("structSize", C.c_size_t), ("nodeCount", C.c_uint32), ("triCount", C.c_uint32),
+                ("vertexCount", C.c_uint32), ("instanceCount", C.c_uint32),
+                ("materialCount", C.c_uint32), ("lightCount", C.c_uint32), ("tlasRoot", C.c_uint32),
+                ("blasNodeCount", C.c_uint32), ("envWidth", C.c_uint32), ("envHeight", C.c_uint32),
+                ("envIntegral", C.c_float), ("nodes", C.c_void_p), ("triangles", C.c_void_p),
+                ("positions", C.c_void_p), ("normals", C.c_void_p), ("uvs", C.c_void_p),
+                ("instances", C.c_void_p), ("materials", C.c_void_p), ("lights", C.c_void_p),
+                ("envAlias", C.c_void_p)]
+
+
 LOG_FN = C.CFUNCTYPE(None, C.c_int, C.c_char_p, C.c_void_p)
 
 _P = C.c_void_p
@@ -169,6 +195,12 @@ _SIGS = {
     "mrtxTraceAny": (_R, [_P, _P, _U32, _P, _P]),
     "mrtxGetQueryStatus": (_R, [_P, C.POINTER(_U32)]),
     "mrtxAutotuneTraversal": (_R, [_P, _P, _U32, _P, C.POINTER(mrtxTraversalConfig)]),
+    # ext: engine
+    "mrtxSetPathOptions": (_R, [_P, C.POINTER(mrtxPathOptions)]),
+    "mrtxGetPathOptions": (_R, [_P, C.POINTER(mrtxPathOptions)]),
+    "mrtxSetProfiling": (_R, [_P, C.c_int32]),
+    "mrtxGetFrameStats": (_R, [_P, C.POINTER(mrtxFrameStats)]),
+    "mrtxExportScene": (_R, [_P, C.POINTER(mrtxSceneExport)]),
 }
 
 # The 27 entry points of the reference ABI (capi/include/mini_raytrace.h:184-237).

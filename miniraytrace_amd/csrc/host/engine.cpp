@@ -1,0 +1,837 @@
+// Engine implementation: scene packing, device buffers, frame loop, readback.
+#include "engine.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <thread>
+
+namespace mrt {
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+float msSince(Clock::time_point t0) {
+    return std::chrono::duration<float, std::milli>(Clock::now() - t0).count();
+}
+
+// Deepest stack a traversal can need: TLAS depth + instances pushed from one
+// TLAS leaf + BLAS depth.  The kernels hold 64 scratch entries plus the LDS
+// part (at least 8).
+constexpr uint32_t kMaxTraversalStack = 64 + 8;
+
+// A node whose inverted box fails every slab test (reference Renderer.cpp:51-58).
+BvhNode sentinelNode() {
+    BvhNode n{};
+    for (int a = 0; a < 3; ++a) {
+        n.lo[a] = std::numeric_limits<float>::max();
+        n.hi[a] = -std::numeric_limits<float>::max();
+    }
+    n.leftOrPrim = 0;
+    n.packed = 1u << 2;
+    return n;
+}
+
+// Column-major m (4x4) times (x, y, z, 1), in glm's evaluation order:
+// (c0*x + c1*y) + (c2*z + c3).
+void xformPoint(const float m[16], const float p[3], float out[3]) {
+    for (int i = 0; i < 3; ++i)
+        out[i] = (m[0 + i] * p[0] + m[4 + i] * p[1]) + (m[8 + i] * p[2] + m[12 + i] * 1.0f);
+}
+
+// Conservative world box of an object box: all 8 corners transformed
+// (reference MathTypes.hpp:38-48).
+Box3 transformBox(const Box3& b, const float m[16]) {
+    Box3 out;
+    for (int i = 0; i < 8; ++i) {
+        const float c[3] = { (i & 1) ? b.hi[0] : b.lo[0], (i & 2) ? b.hi[1] : b.lo[1], (i & 4) ? b.hi[2] : b.lo[2] };
+        float w[3];
+        xformPoint(m, c, w);
+        out.grow(w);
+    }
+    return out;
+}
+
+// General 4x4 inverse in double (Gauss-Jordan with partial pivoting),
+// rounded to float.  Column-major in and out.
+void invert4(const float m[16], float out[16]) {
+    double a[4][8];
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) {
+            a[r][c] = m[c * 4 + r];
+            a[r][c + 4] = (r == c) ? 1.0 : 0.0;
+        }
+    for (int col = 0; col < 4; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < 4; ++r)
+            if (std::fabs(a[r][col]) > std::fabs(a[piv][col])) piv = r;
+        if (piv != col)
+            for (int c = 0; c < 8; ++c) std::swap(a[col][c], a[piv][c]);
+        const double p = a[col][col];
+        const double inv = p != 0.0 ? 1.0 / p : 0.0;  // singular: zero inverse, rays miss
+        for (int c = 0; c < 8; ++c) a[col][c] *= inv;
+        for (int r = 0; r < 4; ++r) {
+            if (r == col) continue;
+            const double f = a[r][col];
+            if (f != 0.0)
+                for (int c = 0; c < 8; ++c) a[r][c] -= f * a[col][c];
+        }
+    }
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) out[c * 4 + r] = float(a[r][c + 4]);
+}
+
+// Vose's alias method over non-negative weights (double).
+void buildAlias(const std::vector<double>& w, std::vector<AliasEntry>& out) {
+    const size_t n = w.size();
+    out.assign(n, AliasEntry{ 1.0f, 0u });
+    double total = 0.0;
+    for (double x : w) total += x;
+    if (n == 0) return;
+    if (!(total > 0.0)) {
+        for (size_t i = 0; i < n; ++i) out[i] = AliasEntry{ 1.0f, uint32_t(i) };
+        return;
+    }
+    std::vector<double> q(n);
+    std::vector<uint32_t> small, large;
+    small.reserve(n);
+    large.reserve(n);
+    for (size_t i = 0; i < n; ++i) {
+        q[i] = w[i] * double(n) / total;
+        (q[i] < 1.0 ? small : large).push_back(uint32_t(i));
+    }
+    while (!small.empty() && !large.empty()) {
+        const uint32_t s = small.back();
+        small.pop_back();
+        const uint32_t l = large.back();
+        out[s] = AliasEntry{ float(q[s]), l };
+        q[l] = (q[l] + q[s]) - 1.0;
+        if (q[l] < 1.0) {
+            large.pop_back();
+            small.push_back(l);
+        }
+    }
+    for (uint32_t i : large) out[i] = AliasEntry{ 1.0f, i };
+    for (uint32_t i : small) out[i] = AliasEntry{ 1.0f, i };
+}
+
+void uploadVec(DeviceBuffer& b, const void* data, size_t bytes) {
+    b.ensure(std::max<size_t>(bytes, 16));
+    if (bytes) b.upload(data, bytes);
+}
+
+} // namespace
+
+// ------------------------------------------------------------------ lifecycle --
+Engine::Engine(const mrtEngineDesc& desc) {
+    device_ = selectDevice(desc.gpuIndex1);
+    DeviceScope scope(device_);
+    MRT_HIP(hipDeviceGetAttribute(&cuCount_, hipDeviceAttributeMultiprocessorCount, device_));
+    MRT_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    for (auto& e : ev_) MRT_HIP(hipEventCreate(&e));
+    if (desc.width > 0 && desc.height > 0) {
+        settings_.width = desc.width;
+        settings_.height = desc.height;
+    }
+    // Minimal scene: default material, no instances, empty texture table.
+    scene_.markDirty(kDirtyMaterials | kDirtyTextures | kDirtyLights | kDirtyGeometry | kDirtyInstances);
+    allocFrame();
+    chooseShapes();
+    sync(nullptr);
+    frameIndex_ = 0;
+    logMessage(LogLevel::Info, "engine on HIP device " + std::to_string(device_) + " (" +
+                                   std::to_string(cuCount_) + " CUs)");
+}
+
+Engine::~Engine() {
+    DeviceScope scope(device_);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    invalidateGraph();
+    for (auto& e : ev_)
+        if (e) (void)hipEventDestroy(e);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Engine::invalidateGraph() {
+    if (graphExec_) (void)hipGraphExecDestroy(graphExec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    graphExec_ = nullptr;
+    graph_ = nullptr;
+}
+
+void Engine::chooseShapes() {
+    int perCU = 0;
+    MRT_HIP(dev::persistentGrid(extendShape_.block, extendShape_.stack, false, &perCU));
+    extendShape_.grid = uint32_t(std::max(1, perCU) * cuCount_);
+    MRT_HIP(dev::persistentGrid(shadowShape_.block, shadowShape_.stack, true, &perCU));
+    shadowShape_.grid = uint32_t(std::max(1, perCU) * cuCount_);
+    MRT_HIP(dev::shadeBlocksPerCU(&perCU));
+    shadeGrid_ = uint32_t(std::max(1, perCU) * cuCount_);
+    invalidateGraph();
+}
+
+// ------------------------------------------------------------------- settings --
+void Engine::setSettings(const Settings& s) {
+    if (s.width == 0 || s.height == 0) throw Error(MRT_ERROR_INVALID_ARGUMENT, "width and height must be > 0");
+    const bool resize = s.width != settings_.width || s.height != settings_.height;
+    const bool reset = resize || s.maxBounces != settings_.maxBounces || s.fireflyClamp != settings_.fireflyClamp;
+    if (std::min(s.maxBounces, 16u) != std::min(settings_.maxBounces, 16u)) invalidateGraph();
+    settings_ = s;
+    if (resize) {
+        DeviceScope scope(device_);
+        MRT_HIP(hipStreamSynchronize(stream_));
+        allocFrame();
+    }
+    if (reset) resetAccumulation();
+}
+
+void Engine::setPathOptions(const mrtxPathOptions& o) {
+    seed_ = o.seed;
+    useGraph_ = o.useGraph != 0;
+    const uint32_t eb = o.extendBlock ? o.extendBlock : 128, es = o.extendStack ? o.extendStack : 16;
+    const uint32_t sb = o.shadowBlock ? o.shadowBlock : 128, ss = o.shadowStack ? o.shadowStack : 16;
+    if (!dev::ptShapeSupported(eb, es) || !dev::ptShapeSupported(sb, ss))
+        throw Error(MRT_ERROR_INVALID_ARGUMENT, "unsupported traversal launch shape");
+    extendShape_.block = eb;
+    extendShape_.stack = es;
+    shadowShape_.block = sb;
+    shadowShape_.stack = ss;
+    DeviceScope scope(device_);
+    chooseShapes();
+}
+
+void Engine::getPathOptions(mrtxPathOptions& o) const {
+    o.seed = seed_;
+    o.useGraph = useGraph_ ? 1 : 0;
+    o.extendBlock = extendShape_.block;
+    o.extendStack = extendShape_.stack;
+    o.shadowBlock = shadowShape_.block;
+    o.shadowStack = shadowShape_.stack;
+}
+
+// --------------------------------------------------------------------- commit --
+void Engine::commit(mrtCommitStats* out) {
+    const uint32_t light = kDirtyCamera | kDirtyEnvParams;
+    const uint32_t d = scene_.dirty();
+    if (d != kDirtyNone && (d & ~light) == 0) {
+        scene_.clearDirty();
+        resetAccumulation();
+        if (out) {
+            out->blasRebuilt = 0;
+            out->tlasRebuilt = 0;
+            out->commitMs = 0.0f;
+        }
+        return;
+    }
+    sync(out);
+}
+
+void Engine::sync(mrtCommitStats* out) {
+    const auto t0 = Clock::now();
+    const uint32_t d = scene_.dirty();
+    if (d == kDirtyNone) {
+        if (out) {
+            out->blasRebuilt = 0;
+            out->tlasRebuilt = 0;
+            out->commitMs = 0.0f;
+        }
+        return;
+    }
+    DeviceScope scope(device_);
+    MRT_HIP(hipStreamSynchronize(stream_));
+    uint32_t rebuilt = 0;
+    if (d & kDirtyTextures) uploadTextures();
+    if (d & kDirtyMaterials) packMaterials();
+    if (d & kDirtyGeometry) rebuilt = packGeometry();
+    const bool tlas = (d & (kDirtyGeometry | kDirtyInstances | kDirtyMaterials)) != 0;
+    if (tlas) packInstances();
+    if (d & (kDirtyLights | kDirtyEnv)) packLights();
+    if (d & kDirtyEnv) packEnvironment();
+    scene_.clearDirty();
+    resetAccumulation();
+    if (out) {
+        out->blasRebuilt = rebuilt;
+        out->tlasRebuilt = tlas ? 1 : 0;
+        out->commitMs = msSince(t0);
+    }
+}
+
+// -------------------------------------------------------------------- packing --
+// Reference Renderer.cpp:333-446.
+uint32_t Engine::packGeometry() {
+    auto& meshes = scene_.meshes();
+    for (auto it = blas_.begin(); it != blas_.end();)
+        it = meshes.count(it->first) ? std::next(it) : blas_.erase(it);
+
+    std::vector<std::pair<uint32_t, MeshData*>> jobs;
+    for (auto& [id, m] : meshes)
+        if (m.blasDirty || !blas_.count(id)) jobs.emplace_back(id, &m);
+    std::vector<Blas> built(jobs.size());
+    std::atomic<size_t> next{ 0 };
+    auto worker = [&] {
+        for (size_t j; (j = next.fetch_add(1)) < jobs.size();) {
+            const MeshData& m = *jobs[j].second;
+            const uint32_t nt = m.triCount();
+            std::vector<Box3> bounds(nt);
+            for (uint32_t t = 0; t < nt; ++t)
+                for (int k = 0; k < 3; ++k) bounds[t].grow(&m.positions[size_t(m.indices[t * 3 + k]) * 4]);
+            // Large meshes parallelise inside the builder; small ones across meshes.
+            BvhBuild b = buildBvh(bounds, 4, nt >= 65536 ? 0 : 1);
+            built[j].nodes = std::move(b.nodes);
+            built[j].triOrder = std::move(b.primOrder);
+            built[j].bounds = b.rootBounds;
+            built[j].depth = b.maxDepth;
+        }
+    };
+    const unsigned nthreads = std::max(1u, std::min<unsigned>(std::thread::hardware_concurrency(), unsigned(jobs.size())));
+    std::vector<std::thread> pool;
+    for (unsigned i = 1; i < nthreads; ++i) pool.emplace_back(worker);
+    worker();
+    for (auto& t : pool) t.join();
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        blas_[jobs[j].first] = std::move(built[j]);
+        jobs[j].second->blasDirty = false;
+    }
+
+    size_t nv = 0, nt = 0, nn = 0;
+    for (auto& [id, m] : meshes) {
+        nv += m.vertexCount();
+        nt += m.triCount();
+        nn += blas_[id].nodes.size();
+    }
+    std::vector<float> pos, nrm, uv, tan, triVerts;
+    std::vector<GpuTriangle> tris;
+    pos.reserve(nv * 4);
+    nrm.reserve(nv * 4);
+    uv.reserve(nv * 2);
+    tan.reserve(nv * 4);
+    tris.reserve(nt);
+    triVerts.reserve(nt * 12);
+    blasNodes_.clear();
+    blasNodes_.reserve(nn);
+    maxBlasDepth_ = 0;
+    for (auto& [id, m] : meshes) {
+        Blas& b = blas_[id];
+        const uint32_t vb = uint32_t(pos.size() / 4), tb = uint32_t(tris.size()), nb = uint32_t(blasNodes_.size());
+        b.rootGlobal = nb;
+        maxBlasDepth_ = std::max(maxBlasDepth_, b.depth);
+        pos.insert(pos.end(), m.positions.begin(), m.positions.end());
+        nrm.insert(nrm.end(), m.normals.begin(), m.normals.end());
+        if (m.uvs.empty()) uv.insert(uv.end(), size_t(m.vertexCount()) * 2, 0.0f);
+        else uv.insert(uv.end(), m.uvs.begin(), m.uvs.end());
+        if (m.tangents.empty()) {
+            for (uint32_t i = 0; i < m.vertexCount(); ++i) tan.insert(tan.end(), { 1.0f, 0.0f, 0.0f, 1.0f });
+        } else {
+            tan.insert(tan.end(), m.tangents.begin(), m.tangents.end());
+        }
+        for (uint32_t local : b.triOrder) {
+            GpuTriangle t{ m.indices[local * 3] + vb, m.indices[local * 3 + 1] + vb, m.indices[local * 3 + 2] + vb, 0u };
+            const uint32_t slot = uint32_t(tris.size());
+            tris.push_back(t);
+            for (int k = 0; k < 3; ++k) {
+                const float* p = &m.positions[size_t(m.indices[local * 3 + k]) * 4];
+                triVerts.insert(triVerts.end(), { p[0], p[1], p[2], 0.0f });
+            }
+            std::memcpy(&triVerts[size_t(slot) * 12 + 3], &slot, 4);
+        }
+        for (BvhNode n : b.nodes) {
+            n.leftOrPrim += (n.packed >> 2) == 0 ? nb : tb;
+            blasNodes_.push_back(n);
+        }
+    }
+    tlasRoot_ = uint32_t(blasNodes_.size());
+    totalTris_ = tris.size();
+    totalVerts_ = pos.size() / 4;
+    uploadVec(dPos_, pos.data(), pos.size() * 4);
+    uploadVec(dNrm_, nrm.data(), nrm.size() * 4);
+    uploadVec(dUv_, uv.data(), uv.size() * 4);
+    uploadVec(dTan_, tan.data(), tan.size() * 4);
+    uploadVec(dTris_, tris.data(), tris.size() * sizeof(GpuTriangle));
+    uploadVec(dTriVerts_, triVerts.data(), triVerts.size() * 4);
+    invalidateGraph();
+    return uint32_t(jobs.size());
+}
+
+// Reference Renderer.cpp:448-508.
+void Engine::packInstances() {
+    std::vector<const InstanceData*> flat;
+    std::vector<Box3> bounds;
+    for (auto& [id, inst] : scene_.instances()) {
+        auto it = blas_.find(inst.mesh);
+        if (it == blas_.end() || !it->second.bounds.valid()) continue;
+        flat.push_back(&inst);
+        bounds.push_back(transformBox(it->second.bounds, inst.objectToWorld));
+    }
+    BvhBuild tlas = buildBvh(bounds, 1);
+    if (tlas.nodes.empty()) tlas.nodes.push_back(sentinelNode());
+    std::vector<GpuInstance> gi;
+    std::vector<TraceInstance> ti;
+    gi.reserve(flat.size());
+    ti.reserve(flat.size());
+    uint32_t maxLeaf = 1;
+    for (uint32_t idx : tlas.primOrder) {
+        const InstanceData* inst = flat[idx];
+        GpuInstance g{};
+        float inv[16];
+        invert4(inst->objectToWorld, inv);
+        for (int c = 0; c < 4; ++c) {
+            g.objectToWorld[c] = make_float4(inst->objectToWorld[c * 4], inst->objectToWorld[c * 4 + 1],
+                                             inst->objectToWorld[c * 4 + 2], inst->objectToWorld[c * 4 + 3]);
+            g.worldToObject[c] = make_float4(inv[c * 4], inv[c * 4 + 1], inv[c * 4 + 2], inv[c * 4 + 3]);
+        }
+        g.blasRoot = blas_[inst->mesh].rootGlobal;
+        auto mi = materialIndex_.find(inst->material);
+        g.materialIndex = mi != materialIndex_.end() ? mi->second : 0u;
+        gi.push_back(g);
+        TraceInstance t{};
+        t.row0 = make_float4(inv[0], inv[4], inv[8], inv[12]);
+        t.row1 = make_float4(inv[1], inv[5], inv[9], inv[13]);
+        t.row2 = make_float4(inv[2], inv[6], inv[10], inv[14]);
+        t.blasRoot = g.blasRoot;
+        t.materialIndex = g.materialIndex;
+        ti.push_back(t);
+    }
+    for (BvhNode& n : tlas.nodes) {
+        const uint32_t cnt = n.packed >> 2;
+        if (cnt == 0) n.leftOrPrim += tlasRoot_;
+        else maxLeaf = std::max(maxLeaf, cnt);
+    }
+    instanceCount_ = uint32_t(gi.size());
+    tlasNodeCount_ = uint32_t(tlas.nodes.size());
+    tlasDepth_ = tlas.maxDepth;
+    if (tlasDepth_ + maxLeaf + maxBlasDepth_ + 2 > kMaxTraversalStack)
+        throw Error(MRT_ERROR_UNKNOWN, "scene BVH too deep for the traversal stack (TLAS depth " +
+                                           std::to_string(tlasDepth_) + ", BLAS depth " +
+                                           std::to_string(maxBlasDepth_) + ")");
+    std::vector<BvhNode> all;
+    all.reserve(blasNodes_.size() + tlas.nodes.size());
+    all.insert(all.end(), blasNodes_.begin(), blasNodes_.end());
+    all.insert(all.end(), tlas.nodes.begin(), tlas.nodes.end());
+    uploadVec(dNodes_, all.data(), all.size() * sizeof(BvhNode));
+    uploadVec(dInst_, gi.data(), gi.size() * sizeof(GpuInstance));
+    uploadVec(dTraceInst_, ti.data(), ti.size() * sizeof(TraceInstance));
+    invalidateGraph();
+}
+
+// Reference Renderer.cpp:510-550.
+void Engine::packMaterials() {
+    auto slotOf = [&](uint32_t id) -> int32_t {
+        const uint32_t s = scene_.textureSlot(id);
+        return (s == ~0u || s >= kMaxTextureSlots) ? -1 : int32_t(s);
+    };
+    std::vector<GpuMaterial> mats;
+    mats.reserve(scene_.materials().size() + 1);
+    materialIndex_.clear();
+    GpuMaterial def{};
+    def.baseColorOpacity = make_float4(0.8f, 0.8f, 0.8f, 1.0f);
+    def.emissionRoughness = make_float4(0.0f, 0.0f, 0.0f, 0.5f);
+    def.params = make_float4(0.0f, 0.0f, 1.45f, 0.0f);
+    def.baseColorTex = def.roughnessTex = def.metallicTex = def.normalTex = def.emissionTex = -1;
+    mats.push_back(def);
+    for (auto& [id, m] : scene_.materials()) {
+        GpuMaterial g{};
+        g.baseColorOpacity = make_float4(m.baseColor[0], m.baseColor[1], m.baseColor[2], m.opacity);
+        g.emissionRoughness = make_float4(m.emission[0], m.emission[1], m.emission[2], std::clamp(m.roughness, 0.0f, 1.0f));
+        g.params = make_float4(std::clamp(m.metallic, 0.0f, 1.0f), std::clamp(m.transmission, 0.0f, 1.0f),
+                               std::max(m.ior, 1.0f), 0.0f);
+        g.baseColorTex = slotOf(m.baseColorTex);
+        g.roughnessTex = slotOf(m.roughnessTex);
+        g.metallicTex = slotOf(m.metallicTex);
+        g.normalTex = slotOf(m.normalTex);
+        g.emissionTex = slotOf(m.emissionTex);
+        materialIndex_[id] = uint32_t(mats.size());
+        mats.push_back(g);
+    }
+    uploadVec(dMats_, mats.data(), mats.size() * sizeof(GpuMaterial));
+    invalidateGraph();
+}
+
+// Reference Renderer.cpp:552-603 (environment at slot 0 when present).
+void Engine::packLights() {
+    std::vector<GpuLight> lights;
+    if (scene_.hasEnvironment()) {
+        GpuLight g{};
+        g.type = kLightEnv;
+        lights.push_back(g);
+    }
+    for (auto& [id, l] : scene_.lights()) {
+        GpuLight g{};
+        if (l.type == MRT_LIGHT_SUN) {
+            g.type = kLightSun;
+            const float d[3] = { l.direction[0], l.direction[1], l.direction[2] };
+            const float len = std::sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+            const float inv = 1.0f / len;
+            g.p0 = make_float4(d[0] * inv, d[1] * inv, d[2] * inv, std::cos(std::max(l.angularRadius, 1e-4f)));
+            g.p1 = make_float4(l.radiance[0], l.radiance[1], l.radiance[2], 0.0f);
+        } else if (l.type == MRT_LIGHT_POINT) {
+            g.type = kLightPoint;
+            g.p0 = make_float4(l.position[0], l.position[1], l.position[2], 0.0f);
+            g.p1 = make_float4(l.radiance[0], l.radiance[1], l.radiance[2], 0.0f);
+        } else if (l.type == MRT_LIGHT_RECT) {
+            g.type = kLightRect;
+            const float* e0 = l.edge0;
+            const float* e1 = l.edge1;
+            const float cx = e0[1] * e1[2] - e1[1] * e0[2], cy = e0[2] * e1[0] - e1[2] * e0[0],
+                        cz = e0[0] * e1[1] - e1[0] * e0[1];
+            const float area = std::sqrt((cx * cx + cy * cy) + cz * cz);
+            g.p0 = make_float4(l.corner[0], l.corner[1], l.corner[2], area);
+            g.p1 = make_float4(e0[0], e0[1], e0[2], 0.0f);
+            g.p2 = make_float4(e1[0], e1[1], e1[2], 0.0f);
+            g.p3 = make_float4(l.radiance[0], l.radiance[1], l.radiance[2], l.twoSided ? 1.0f : 0.0f);
+        } else {
+            continue;  // unknown light type: ignored
+        }
+        lights.push_back(g);
+    }
+    lightCount_ = uint32_t(lights.size());
+    uploadVec(dLights_, lights.data(), lights.size() * sizeof(GpuLight));
+    invalidateGraph();
+}
+
+// Reference Renderer.cpp:605-662: luminance x sin(theta) texel weights and
+// the solid-angle integral (FP64 sums); an alias table replaces the CDFs.
+void Engine::packEnvironment() {
+    hasEnv_ = scene_.hasEnvironment();
+    if (!hasEnv_) {
+        envW_ = envH_ = 0;
+        invalidateGraph();
+        return;
+    }
+    const TextureData& env = scene_.environment();
+    const uint32_t w = env.width, h = env.height;
+    const float* px = reinterpret_cast<const float*>(env.data.data());
+    const float kPi = 3.14159265358979323846f;
+    const float dTheta = kPi / float(h);
+    const float dPhi = 2.0f * kPi / float(w);
+    std::vector<double> weights(size_t(w) * h);
+    double integral = 0.0;
+    for (uint32_t y = 0; y < h; ++y) {
+        const float sinT = std::sin((float(y) + 0.5f) * dTheta);
+        double row = 0.0;
+        for (uint32_t x = 0; x < w; ++x) {
+            const float* p = px + (size_t(y) * w + x) * 4;
+            const float lum = (0.2126f * p[0] + 0.7152f * p[1]) + 0.0722f * p[2];
+            const double wgt = double(lum) * double(sinT);
+            weights[size_t(y) * w + x] = wgt;
+            row += wgt;
+        }
+        integral += row * double(dTheta) * double(dPhi);
+    }
+    std::vector<AliasEntry> alias;
+    buildAlias(weights, alias);
+    envW_ = w;
+    envH_ = h;
+    envIntegral_ = std::max(float(integral), 1e-8f);
+    uploadVec(dEnv_, px, size_t(w) * h * 16);
+    uploadVec(dAlias_, alias.data(), alias.size() * sizeof(AliasEntry));
+    invalidateGraph();
+}
+
+// Texture pools: RGBA8 texels as uint32 (bytes R,G,B,A), RGBA32F as float4.
+void Engine::uploadTextures() {
+    const auto& tex = scene_.textures();
+    std::vector<GpuTexture> table(kMaxTextureSlots, GpuTexture{ 0, 0, 0, 0 });
+    std::vector<uint32_t> pool8;
+    std::vector<float> pool32;
+    for (size_t slot = 0; slot < tex.size() && slot < kMaxTextureSlots; ++slot) {
+        const TextureData& t = tex[slot];
+        if (t.data.empty()) continue;
+        GpuTexture& g = table[slot];
+        g.width = t.width;
+        g.height = t.height;
+        g.format = t.format;
+        const size_t n = size_t(t.width) * t.height;
+        if (t.format == MRT_TEX_RGBA32F) {
+            g.offset = uint32_t(pool32.size() / 4);
+            const float* f = reinterpret_cast<const float*>(t.data.data());
+            pool32.insert(pool32.end(), f, f + n * 4);
+        } else {
+            g.offset = uint32_t(pool8.size());
+            const size_t base = pool8.size();
+            pool8.resize(base + n);
+            std::memcpy(&pool8[base], t.data.data(), n * 4);
+        }
+    }
+    if (tex.size() > kMaxTextureSlots)
+        logMessage(LogLevel::Warn, "texture count exceeds 64 slots; extra textures ignored");
+    uploadVec(dTexTable_, table.data(), table.size() * sizeof(GpuTexture));
+    uploadVec(dTex8_, pool8.data(), pool8.size() * 4);
+    uploadVec(dTex32_, pool32.data(), pool32.size() * 4);
+    invalidateGraph();
+}
+
+// ------------------------------------------------------------- frame buffers --
+void Engine::allocFrame() {
+    const size_t n = size_t(settings_.width) * settings_.height;
+    frameW_ = settings_.width;
+    frameH_ = settings_.height;
+    dConsts_.ensure(sizeof(FrameConsts));
+    dBeta_.allocate(n * 16);
+    dRad_.allocate(n * 16);
+    dRng_.allocate(n * 4);
+    for (int i = 0; i < 2; ++i) {
+        dRayO_[i].allocate(n * 16);
+        dRayD_[i].allocate(n * 16);
+    }
+    dHit_.allocate(n * 16);
+    dHitInst_.allocate(n * 4);
+    dShO_.allocate(n * 16);
+    dShD_.allocate(n * 16);
+    dShC_.allocate(n * 16);
+    dCounters_.ensure(dev::kCounterWords * 4);
+    dAccum_.allocate(n * 16);
+    dOut8_.allocate(n * 4);
+    dOut32_.allocate(n * 16);
+    dAccum_.zero(stream_);
+    dOut8_.zero(stream_);
+    dCounters_.zero(stream_);
+    MRT_HIP(hipStreamSynchronize(stream_));
+    hConsts_.ensure(sizeof(FrameConsts));
+    hCounters_.ensure(dev::kCounterWords * 4);
+    invalidateGraph();
+}
+
+dev::SceneView Engine::sceneView() const {
+    dev::SceneView s;
+    s.nodes = dNodes_.as<const dev::Node>();
+    s.triVerts = dTriVerts_.as<const float4>();
+    s.tris = dTris_.as<const GpuTriangle>();
+    s.positions = dPos_.as<const float4>();
+    s.normals = dNrm_.as<const float4>();
+    s.uvs = dUv_.as<const float2>();
+    s.tangents = dTan_.as<const float4>();
+    s.instances = dInst_.as<const GpuInstance>();
+    s.traceInst = dTraceInst_.as<const TraceInstance>();
+    s.materials = dMats_.as<const GpuMaterial>();
+    s.lights = dLights_.as<const GpuLight>();
+    s.envMap = dEnv_.as<const float4>();
+    s.envAlias = dAlias_.as<const AliasEntry>();
+    s.texTable = dTexTable_.as<const GpuTexture>();
+    s.tex8 = dTex8_.as<const uint32_t>();
+    s.tex32 = dTex32_.as<const float4>();
+    return s;
+}
+
+dev::PathView Engine::pathView() const {
+    dev::PathView p;
+    p.capacity = frameW_ * frameH_;
+    p.beta = dBeta_.as<float4>();
+    p.radiance = dRad_.as<float4>();
+    p.rngCounter = dRng_.as<uint32_t>();
+    for (int i = 0; i < 2; ++i) {
+        p.rayO[i] = dRayO_[i].as<float4>();
+        p.rayD[i] = dRayD_[i].as<float4>();
+    }
+    p.hit = dHit_.as<float4>();
+    p.hitInst = dHitInst_.as<uint32_t>();
+    p.shadowO = dShO_.as<float4>();
+    p.shadowD = dShD_.as<float4>();
+    p.shadowC = dShC_.as<float4>();
+    p.counters = dCounters_.as<uint32_t>();
+    p.accum = dAccum_.as<float4>();
+    p.outRgba8 = dOut8_.as<uint32_t>();
+    p.outRgba32 = dOut32_.as<float4>();
+    return p;
+}
+
+// Reference Renderer.cpp:711-764.
+void Engine::fillConsts(FrameConsts& g) const {
+    std::memset(&g, 0, sizeof(g));
+    float pos[3], fwd[3], up[3];
+    float left, right, bottom, top;
+    bool ortho = false;
+    auto normalize3 = [](float v[3]) {
+        const float inv = 1.0f / std::sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+        v[0] *= inv; v[1] *= inv; v[2] *= inv;
+    };
+    if (scene_.usesCameraEx()) {
+        const CameraExData& c = scene_.cameraEx();
+        for (int i = 0; i < 3; ++i) { pos[i] = c.position[i]; fwd[i] = c.forward[i]; up[i] = c.up[i]; }
+        normalize3(fwd);
+        left = c.left; right = c.right; bottom = c.bottom; top = c.top;
+        ortho = c.projection == MRT_CAMERA_ORTHOGRAPHIC;
+    } else {
+        const CameraData& c = scene_.camera();
+        for (int i = 0; i < 3; ++i) { pos[i] = c.position[i]; fwd[i] = c.target[i] - c.position[i]; up[i] = c.up[i]; }
+        normalize3(fwd);
+        const float aspect = float(settings_.width) / float(settings_.height);
+        const float tanHalf = std::tan(c.fovYDeg * 0.0174532925199432957692f * 0.5f);
+        right = tanHalf * aspect;
+        left = -right;
+        top = tanHalf;
+        bottom = -top;
+    }
+    float rv[3] = { fwd[1] * up[2] - up[1] * fwd[2], fwd[2] * up[0] - up[2] * fwd[0], fwd[0] * up[1] - up[0] * fwd[1] };
+    const float rlen = std::sqrt((rv[0] * rv[0] + rv[1] * rv[1]) + rv[2] * rv[2]);
+    if (rlen < 1e-6f) { rv[0] = 1.0f; rv[1] = 0.0f; rv[2] = 0.0f; }
+    else normalize3(rv);
+    const float uv[3] = { rv[1] * fwd[2] - fwd[1] * rv[2], rv[2] * fwd[0] - fwd[2] * rv[0], rv[0] * fwd[1] - fwd[0] * rv[1] };
+    g.camPos = make_float4(pos[0], pos[1], pos[2], 0.0f);
+    g.camRight = make_float4(rv[0], rv[1], rv[2], 0.0f);
+    g.camUp = make_float4(uv[0], uv[1], uv[2], 0.0f);
+    g.camForward = make_float4(fwd[0], fwd[1], fwd[2], 0.0f);
+    g.camFrustum = make_float4(left, right, bottom, top);
+    g.width = settings_.width;
+    g.height = settings_.height;
+    g.frameIndex = frameIndex_;
+    g.maxBounces = std::min(settings_.maxBounces, 16u);
+    g.lightCount = lightCount_;
+    g.envWidth = hasEnv_ ? envW_ : 0;
+    g.envHeight = hasEnv_ ? envH_ : 0;
+    g.flags = (hasEnv_ ? kFlagHasEnv : 0u) | (ortho ? kFlagOrthographic : 0u);
+    g.envRotation = scene_.envRotation();
+    g.envIntensity = scene_.envIntensity();
+    g.envIntegral = envIntegral_;
+    g.exposure = std::exp2(settings_.exposureEV);
+    g.fireflyClamp = settings_.fireflyClamp;
+    g.tonemap = settings_.tonemap;
+    g.tlasRoot = tlasRoot_;
+    g.instanceCount = instanceCount_;
+    g.seedMix = seed_ ? (seed_ * 0x9E3779B9u) ^ 0x85EBCA6Bu : 0u;
+}
+
+void Engine::enqueueFrame(hipStream_t s, bool timed) {
+    const dev::SceneView sc = sceneView();
+    const dev::PathView pv = pathView();
+    const FrameConsts* g = dConsts_.as<const FrameConsts>();
+    const uint32_t pixels = frameW_ * frameH_;
+    const uint32_t maxB = std::min(settings_.maxBounces, 16u);
+    if (timed) MRT_HIP(hipEventRecord(ev_[0], s));
+    MRT_HIP(dev::launchClearWords(pv.counters, dev::kCounterWords, s));
+    MRT_HIP(dev::launchGenerate(pv, g, pixels, s));
+    if (timed) MRT_HIP(hipEventRecord(ev_[1], s));
+    for (uint32_t b = 0; b <= maxB; ++b) {
+        if (timed) MRT_HIP(hipEventRecord(ev_[2], s));
+        MRT_HIP(dev::launchExtend(sc, pv, g, b, extendShape_, s));
+        if (timed) MRT_HIP(hipEventRecord(ev_[3], s));
+        MRT_HIP(dev::launchShade(sc, pv, g, b, shadeGrid_, s));
+        if (timed) MRT_HIP(hipEventRecord(ev_[4], s));
+        MRT_HIP(dev::launchShadow(sc, pv, g, b, shadowShape_, s));
+        if (timed) {
+            MRT_HIP(hipEventRecord(ev_[5], s));
+            MRT_HIP(hipEventSynchronize(ev_[5]));
+            float a = 0, bsh = 0, c = 0;
+            MRT_HIP(hipEventElapsedTime(&a, ev_[2], ev_[3]));
+            MRT_HIP(hipEventElapsedTime(&bsh, ev_[3], ev_[4]));
+            MRT_HIP(hipEventElapsedTime(&c, ev_[4], ev_[5]));
+            stats_.stageMs[1] += a;
+            stats_.stageMs[2] += bsh;
+            stats_.stageMs[3] += c;
+        }
+    }
+    if (timed) MRT_HIP(hipEventRecord(ev_[6], s));
+    MRT_HIP(dev::launchAccumulate(pv, g, pixels, s));
+    if (timed) MRT_HIP(hipEventRecord(ev_[7], s));
+    MRT_HIP(dev::launchResolve8(pv, g, pixels, s));
+    if (timed) {
+        MRT_HIP(hipEventRecord(ev_[8], s));
+        MRT_HIP(hipEventSynchronize(ev_[8]));
+        float a = 0, b = 0, c = 0;
+        MRT_HIP(hipEventElapsedTime(&a, ev_[0], ev_[1]));
+        MRT_HIP(hipEventElapsedTime(&b, ev_[6], ev_[7]));
+        MRT_HIP(hipEventElapsedTime(&c, ev_[7], ev_[8]));
+        stats_.stageMs[0] = a;
+        stats_.stageMs[4] = b;
+        stats_.stageMs[5] = c;
+    }
+}
+
+void Engine::renderFrame(mrtFrameInfo* out) {
+    const auto t0 = Clock::now();
+    commit(nullptr);
+    DeviceScope scope(device_);
+    if (frameIndex_ < settings_.sppLimit) {
+        FrameConsts* g = static_cast<FrameConsts*>(hConsts_.get());
+        fillConsts(*g);
+        MRT_HIP(hipMemcpyAsync(dConsts_.get(), g, sizeof(FrameConsts), hipMemcpyHostToDevice, stream_));
+        if (profiling_) {
+            std::memset(stats_.stageMs, 0, sizeof(stats_.stageMs));
+            enqueueFrame(stream_, true);
+            MRT_HIP(hipMemcpyAsync(hCounters_.get(), dCounters_.get(), dev::kCounterWords * 4,
+                                   hipMemcpyDeviceToHost, stream_));
+            MRT_HIP(hipStreamSynchronize(stream_));
+            const uint32_t* c = static_cast<const uint32_t*>(hCounters_.get());
+            for (uint32_t b = 0; b < dev::kMaxBounceLevels; ++b) {
+                stats_.rays[b] = c[b * dev::kCntPerBounce + dev::kCntRays];
+                stats_.shadowRays[b] = c[b * dev::kCntPerBounce + dev::kCntShadows];
+            }
+            stats_.status |= c[dev::kStatusWord];
+        } else if (useGraph_) {
+            if (!graphExec_) {
+                MRT_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+                try {
+                    enqueueFrame(stream_, false);
+                } catch (...) {
+                    hipGraph_t g2 = nullptr;
+                    (void)hipStreamEndCapture(stream_, &g2);
+                    if (g2) (void)hipGraphDestroy(g2);
+                    throw;
+                }
+                MRT_HIP(hipStreamEndCapture(stream_, &graph_));
+                MRT_HIP(hipGraphInstantiate(&graphExec_, graph_, nullptr, nullptr, 0));
+            }
+            MRT_HIP(hipGraphLaunch(graphExec_, stream_));
+            MRT_HIP(hipStreamSynchronize(stream_));
+        } else {
+            enqueueFrame(stream_, false);
+            MRT_HIP(hipStreamSynchronize(stream_));
+        }
+        ++frameIndex_;
+    }
+    stats_.frameMs = msSince(t0);
+    if (out) {
+        out->spp = frameIndex_;
+        out->converged = frameIndex_ >= settings_.sppLimit ? 1 : 0;
+        out->denoised = 0;
+        out->frameMs = stats_.frameMs;
+    }
+}
+
+// Reference Renderer.cpp:892-909.  The RGBA32F division runs on the GPU.
+void Engine::readFramebuffer(mrtPixelFormat fmt, void* dst, size_t dstSize) {
+    const size_t pixels = size_t(frameW_) * frameH_;
+    DeviceScope scope(device_);
+    if (fmt == MRT_PIXEL_RGBA8_SRGB) {
+        if (dstSize < pixels * 4) throw Error(MRT_ERROR_INVALID_ARGUMENT, "destination too small for RGBA8");
+        MRT_HIP(hipMemcpyAsync(dst, dOut8_.get(), pixels * 4, hipMemcpyDeviceToHost, stream_));
+        MRT_HIP(hipStreamSynchronize(stream_));
+        return;
+    }
+    if (fmt != MRT_PIXEL_RGBA32F_LINEAR) throw Error(MRT_ERROR_INVALID_ARGUMENT, "unknown pixel format");
+    if (dstSize < pixels * 16) throw Error(MRT_ERROR_INVALID_ARGUMENT, "destination too small for RGBA32F");
+    MRT_HIP(dev::launchResolve32(pathView(), uint32_t(pixels), frameIndex_, stream_));
+    MRT_HIP(hipMemcpyAsync(dst, dOut32_.get(), pixels * 16, hipMemcpyDeviceToHost, stream_));
+    MRT_HIP(hipStreamSynchronize(stream_));
+}
+
+void Engine::exportScene(mrtxSceneExport& o) {
+    commit(nullptr);
+    DeviceScope scope(device_);
+    MRT_HIP(hipStreamSynchronize(stream_));
+    o.blasNodeCount = uint32_t(blasNodes_.size());
+    o.tlasRoot = tlasRoot_;
+    o.triCount = uint32_t(totalTris_);
+    o.vertexCount = uint32_t(totalVerts_);
+    o.instanceCount = instanceCount_;
+    o.materialCount = uint32_t(scene_.materials().size() + 1);
+    o.lightCount = lightCount_;
+    o.envWidth = hasEnv_ ? envW_ : 0;
+    o.envHeight = hasEnv_ ? envH_ : 0;
+    o.envIntegral = envIntegral_;
+    o.nodeCount = uint32_t(blasNodes_.size()) + tlasNodeCount_;
+    if (o.nodes) dNodes_.download(o.nodes, size_t(o.nodeCount) * 32);
+    if (o.triangles) dTris_.download(o.triangles, totalTris_ * 16);
+    if (o.positions) dPos_.download(o.positions, totalVerts_ * 16);
+    if (o.normals) dNrm_.download(o.normals, totalVerts_ * 16);
+    if (o.uvs) dUv_.download(o.uvs, totalVerts_ * 8);
+    if (o.instances) dInst_.download(o.instances, size_t(instanceCount_) * sizeof(GpuInstance));
+    if (o.materials) dMats_.download(o.materials, size_t(o.materialCount) * sizeof(GpuMaterial));
+    if (o.lights) dLights_.download(o.lights, size_t(lightCount_) * sizeof(GpuLight));
+    if (o.envAlias && hasEnv_) dAlias_.download(o.envAlias, size_t(envW_) * envH_ * sizeof(AliasEntry));
+}
+
+} // namespace mrt

@@ -1,0 +1,132 @@
+// Engine: scene + device resources + the wavefront frame loop behind the C
+// ABI.  Responsibilities match the reference's Engine/Renderer pair
+// (core/src/Engine.cpp:44-60, core/src/render/Renderer.cpp:290-909); the GPU
+// side is the HIP pipeline in kernels/pathtrace.hip.
+#pragma once
+
+#include "bvh_build.h"
+#include "common.h"
+#include "common/gpu_types.h"
+#include "kernels/pt_launch.h"
+#include "mini_raytrace_ext.h"
+#include "scene.h"
+
+#include <map>
+#include <memory>
+#include <vector>
+
+namespace mrt {
+
+struct Settings {
+    uint32_t width = 1280, height = 720;
+    uint32_t maxBounces = 8;
+    uint32_t sppLimit = 1024;
+    float exposureEV = 0.0f;
+    float fireflyClamp = 50.0f;
+    uint32_t tonemap = MRT_TONEMAP_ACES;
+    bool denoise = true;
+};
+
+struct FrameStats {
+    float frameMs = 0.0f;
+    float stageMs[6] = {};  // generate, extend, shade, shadow, accumulate, resolve
+    uint32_t rays[dev::kMaxBounceLevels] = {};
+    uint32_t shadowRays[dev::kMaxBounceLevels] = {};
+    uint32_t status = 0;
+};
+
+class Engine {
+public:
+    explicit Engine(const mrtEngineDesc& desc);
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    Scene& scene() { return scene_; }
+    void setSettings(const Settings& s);
+    const Settings& settings() const { return settings_; }
+    void commit(mrtCommitStats* out);
+    void renderFrame(mrtFrameInfo* out);
+    void readFramebuffer(mrtPixelFormat fmt, void* dst, size_t dstSize);
+    void resetAccumulation() { frameIndex_ = 0; }
+    uint32_t frameIndex() const { return frameIndex_; }
+
+    // extensions
+    void setPathOptions(const mrtxPathOptions& o);
+    void getPathOptions(mrtxPathOptions& o) const;
+    void setProfiling(bool on) { profiling_ = on; }
+    const FrameStats& stats() const { return stats_; }
+    void exportScene(mrtxSceneExport& out);
+    int device() const { return device_; }
+    hipStream_t stream() const { return stream_; }
+
+private:
+    struct Blas {
+        std::vector<BvhNode> nodes;
+        std::vector<uint32_t> triOrder;
+        Box3 bounds;
+        uint32_t depth = 0;
+        uint32_t rootGlobal = 0;
+    };
+
+    void sync(mrtCommitStats* out);
+    uint32_t packGeometry();
+    void packInstances();
+    void packMaterials();
+    void packLights();
+    void packEnvironment();
+    void uploadTextures();
+    void allocFrame();
+    void fillConsts(FrameConsts& g) const;
+    void enqueueFrame(hipStream_t s, bool timed);
+    dev::SceneView sceneView() const;
+    dev::PathView pathView() const;
+    void invalidateGraph();
+    void chooseShapes();
+
+    int device_ = 0;
+    int cuCount_ = 0;
+    hipStream_t stream_ = nullptr;
+    Scene scene_;
+    Settings settings_;
+    uint32_t frameIndex_ = 0;
+    uint32_t seed_ = 0;
+    bool useGraph_ = true;
+    bool profiling_ = false;
+    FrameStats stats_;
+
+    // packing state
+    std::map<uint32_t, Blas> blas_;
+    std::map<uint32_t, uint32_t> materialIndex_;
+    std::vector<BvhNode> blasNodes_;      // concatenated BLAS nodes (global indices)
+    uint32_t tlasRoot_ = 0;
+    uint32_t instanceCount_ = 0;
+    uint32_t tlasNodeCount_ = 0;
+    uint32_t maxBlasDepth_ = 0;
+    uint32_t tlasDepth_ = 0;
+    uint32_t lightCount_ = 0;
+    bool hasEnv_ = false;
+    uint32_t envW_ = 0, envH_ = 0;
+    float envIntegral_ = 1.0f;
+    size_t totalTris_ = 0, totalVerts_ = 0;
+
+    // scene buffers
+    DeviceBuffer dNodes_, dTriVerts_, dTris_, dPos_, dNrm_, dUv_, dTan_;
+    DeviceBuffer dInst_, dTraceInst_, dMats_, dLights_, dEnv_, dAlias_;
+    DeviceBuffer dTexTable_, dTex8_, dTex32_;
+    // frame buffers
+    DeviceBuffer dConsts_, dBeta_, dRad_, dRng_, dRayO_[2], dRayD_[2], dHit_, dHitInst_;
+    DeviceBuffer dShO_, dShD_, dShC_, dCounters_, dAccum_, dOut8_, dOut32_;
+    PinnedBuffer hConsts_, hReadback_, hCounters_;
+    uint32_t frameW_ = 0, frameH_ = 0;
+
+    dev::TraceShape extendShape_{ 128, 16, 0 };
+    dev::TraceShape shadowShape_{ 128, 16, 0 };
+    uint32_t shadeGrid_ = 0;
+
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graphExec_ = nullptr;
+    hipEvent_t ev_[16] = {};
+};
+
+} // namespace mrt

@@ -1,0 +1,457 @@
+// Wavefront progressive path tracer for gfx950.
+//
+// The reference runs one megakernel thread per pixel that loops over bounces
+// (core/shaders/pathtrace.comp:100-234).  Here each bounce is split into
+// stages that communicate through compacted queues in HBM:
+//
+//   generate   camera rays for every pixel -> ray queue 0
+//   extend(b)  closest hit for ray queue b -> hit records (per queue slot)
+//   shade(b)   surface, emission, NEE light sample, BSDF sample, Russian
+//              roulette -> shadow queue b and ray queue b+1
+//   shadow(b)  any hit for shadow queue b; unoccluded -> add contribution
+//   accumulate per-pixel firefly clamp, NaN guard, accum += (L, 1)
+//   resolve    accum -> RGBA8 (exposure, ACES, sRGB) / RGBA32F (accum/spp)
+//
+// Queues are appended with one atomic per wave (ballot + mbcnt prefix) and
+// consumed by persistent waves that fetch 64 slots per atomic.  Per-path
+// random numbers follow the reference's draw order exactly (pcg4d keyed by
+// pixel, frame and a per-path counter), so a path consumes the same random
+// numbers as the reference megakernel would.
+#include "pt_device.h"
+#include "pt_launch.h"
+#include "trace_scene.h"
+
+namespace mrt {
+namespace dev {
+
+struct Surface {
+    V3 pos, nGeom, nShade;
+    float u, v;
+    bool front;
+    GpuMaterial mat;
+};
+
+__device__ __forceinline__ V3 colDot3(const float4* cols, V3 n) {
+    // transpose(mat3(M)) * n: component i = dot(column i, n)
+    return v3(cols[0].x * n.x + cols[0].y * n.y + cols[0].z * n.z,
+              cols[1].x * n.x + cols[1].y * n.y + cols[1].z * n.z,
+              cols[2].x * n.x + cols[2].y * n.y + cols[2].z * n.z);
+}
+__device__ __forceinline__ V3 mulCols3(const float4* cols, V3 t) {
+    // mat3(M) * t
+    return v3(cols[0]) * t.x + v3(cols[1]) * t.y + v3(cols[2]) * t.z;
+}
+
+// pathtrace.comp:39-80
+__device__ Surface makeSurface(const SceneView& sc, V3 dir, float b1, float b2, uint32_t triIdx,
+                               uint32_t instIdx) {
+    Surface s;
+    const GpuInstance& inst = sc.instances[instIdx];
+    const GpuTriangle tri = sc.tris[triIdx];
+    const float b0 = 1.0f - b1 - b2;
+    const V3 p0 = v3(sc.positions[tri.i0]), p1 = v3(sc.positions[tri.i1]), p2 = v3(sc.positions[tri.i2]);
+    const V3 po = p0 * b0 + p1 * b1 + p2 * b2;
+    s.pos = mulCols3(inst.objectToWorld, po) + v3(inst.objectToWorld[3]);
+    V3 nGeom = normalize(colDot3(inst.worldToObject, cross(p1 - p0, p2 - p0)));
+    const V3 no = v3(sc.normals[tri.i0]) * b0 + v3(sc.normals[tri.i1]) * b1 + v3(sc.normals[tri.i2]) * b2;
+    V3 nShade = normalize(colDot3(inst.worldToObject, no));
+    const float2 uv0 = sc.uvs[tri.i0], uv1 = sc.uvs[tri.i1], uv2 = sc.uvs[tri.i2];
+    s.u = uv0.x * b0 + uv1.x * b1 + uv2.x * b2;
+    s.v = uv0.y * b0 + uv1.y * b1 + uv2.y * b2;
+    s.front = dot(dir, nGeom) < 0.0f;
+    if (!s.front) {
+        nGeom = -nGeom;
+        nShade = -nShade;
+    }
+    s.mat = sc.materials[inst.materialIndex];
+    if (s.mat.normalTex >= 0) {
+        const float4 t0 = sc.tangents[tri.i0], t1 = sc.tangents[tri.i1], t2 = sc.tangents[tri.i2];
+        const float4 tanH = make_float4(t0.x * b0 + t1.x * b1 + t2.x * b2, t0.y * b0 + t1.y * b1 + t2.y * b2,
+                                        t0.z * b0 + t1.z * b1 + t2.z * b2, t0.w * b0 + t1.w * b1 + t2.w * b2);
+        const V3 tangent = normalize(mulCols3(inst.objectToWorld, v3(tanH)));
+        const float4 tn4 = sampleTexture(sc, s.mat.normalTex, s.u, s.v);
+        const V3 tn = v3(tn4.x * 2.0f - 1.0f, tn4.y * 2.0f - 1.0f, tn4.z * 2.0f - 1.0f);
+        const V3 bitan = cross(nShade, tangent) * tanH.w;
+        nShade = normalize(tangent * tn.x + bitan * tn.y + nShade * tn.z);
+        if (dot(nShade, nGeom) < 0.0f) nShade = nGeom;
+    }
+    s.nGeom = nGeom;
+    s.nShade = nShade;
+    return s;
+}
+
+// pathtrace.comp:82-94
+__device__ __forceinline__ Bsdf makeBsdf(const SceneView& sc, const Surface& s) {
+    Bsdf b;
+    b.base = v3(s.mat.baseColorOpacity);
+    if (s.mat.baseColorTex >= 0) b.base = b.base * v3(sampleTexture(sc, s.mat.baseColorTex, s.u, s.v));
+    b.roughness = s.mat.emissionRoughness.w;
+    if (s.mat.roughnessTex >= 0) b.roughness *= sampleTexture(sc, s.mat.roughnessTex, s.u, s.v).x;
+    b.metallic = s.mat.params.x;
+    if (s.mat.metallicTex >= 0) b.metallic *= sampleTexture(sc, s.mat.metallicTex, s.u, s.v).x;
+    b.transmission = s.mat.params.y;
+    b.ior = s.mat.params.z;
+    b.entering = s.front;
+    return b;
+}
+
+// ---------------------------------------------------------------- generate --
+__global__ __launch_bounds__(256) void kGenerate(PathView pv, const FrameConsts* __restrict__ gp) {
+    const FrameConsts g = *gp;
+    const uint32_t n = g.width * g.height;
+    if (blockIdx.x == 0 && threadIdx.x == 0) pv.counters[kCntRays] = n;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const uint32_t px = p % g.width, py = p / g.width;
+        Rng rng{ px, py, g.frameIndex ^ g.seedMix, 0u };
+        float jx, jy;
+        rng.next2(jx, jy);
+        const float ux = (float(px) + jx) / float(g.width);
+        const float uy = (float(py) + jy) / float(g.height);
+        const float sx = g.camFrustum.x * (1.0f - ux) + g.camFrustum.y * ux;
+        const float sy = g.camFrustum.w * (1.0f - uy) + g.camFrustum.z * uy;
+        V3 o, d;
+        if (g.flags & kFlagOrthographic) {
+            o = v3(g.camPos) + v3(g.camRight) * sx + v3(g.camUp) * sy;
+            d = v3(g.camForward);
+        } else {
+            o = v3(g.camPos);
+            d = normalize(v3(g.camForward) + v3(g.camRight) * sx + v3(g.camUp) * sy);
+        }
+        pv.rayO[0][p] = f4(o, __uint_as_float(p));
+        pv.rayD[0][p] = f4(d, 0.0f);
+        pv.beta[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        pv.radiance[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        pv.rngCounter[p] = rng.counter;
+    }
+}
+
+// ------------------------------------------------------------------ extend --
+template <int BLOCK, int S>
+__global__ __launch_bounds__(BLOCK) void kExtend(SceneView sc, PathView pv,
+                                                 const FrameConsts* __restrict__ gp, uint32_t bounce) {
+    __shared__ uint2 ldsStack[S * BLOCK];
+    const uint32_t tlasRoot = gp->tlasRoot, instCount = gp->instanceCount;
+    uint32_t* cnt = pv.counters + bounce * kCntPerBounce;
+    const uint32_t count = cnt[kCntRays];
+    const float4* rayO = pv.rayO[bounce & 1];
+    const float4* rayD = pv.rayD[bounce & 1];
+    bool overflow = false;
+    while (true) {
+        const uint32_t base = waveFetch(cnt + kCntExtendFetch);
+        if (base >= count) break;
+        const uint32_t k = base + laneId();
+        if (k < count) {
+            const float4 o = rayO[k], d = rayD[k];
+            SceneHit h;
+            const bool hit = traceScene<false, S, BLOCK>(sc, tlasRoot, instCount, make_float3(o.x, o.y, o.z),
+                                                         make_float3(d.x, d.y, d.z), kEpsRay, 1e30f,
+                                                         ldsStack + threadIdx.x, h, overflow);
+            pv.hit[k] = hit ? make_float4(h.t, h.b1, h.b2, __uint_as_float(h.tri))
+                            : make_float4(1e30f, 0.0f, 0.0f, __uint_as_float(kNoHit));
+            pv.hitInst[k] = hit ? h.inst : kNoHit;
+        }
+    }
+    if (overflow) atomicOr(pv.counters + kStatusWord, 1u);
+}
+
+// ------------------------------------------------------------------- shade --
+__global__ __launch_bounds__(256) void kShade(SceneView sc, PathView pv, const FrameConsts* __restrict__ gp,
+                                              uint32_t bounce) {
+    const FrameConsts g = *gp;
+    uint32_t* cnt = pv.counters + bounce * kCntPerBounce;
+    uint32_t* next = pv.counters + (bounce + 1) * kCntPerBounce;
+    const uint32_t count = cnt[kCntRays];
+    const float4* rayO = pv.rayO[bounce & 1];
+    const float4* rayD = pv.rayD[bounce & 1];
+    float4* nrayO = pv.rayO[(bounce + 1) & 1];
+    float4* nrayD = pv.rayD[(bounce + 1) & 1];
+    const bool last = bounce >= g.maxBounces;
+    while (true) {
+        const uint32_t base = waveFetch(cnt + kCntShadeFetch);
+        if (base >= count) break;
+        const uint32_t k = base + laneId();
+        bool emitRay = false, emitShadow = false;
+        V3 newO, newD, shO, shD, shC;
+        float shMax = 0.0f;
+        uint32_t p = 0;
+        if (k < count) {
+            const float4 ro = rayO[k], rd = rayD[k];
+            p = __float_as_uint(ro.w);
+            const V3 dir = v3(rd);
+            const float4 h = pv.hit[k];
+            const uint32_t tri = __float_as_uint(h.w);
+            float4 beta4 = pv.beta[p];
+            V3 beta = v3(beta4);
+            float prevPdf = beta4.w;
+            float4 L4 = pv.radiance[p];
+            V3 L = v3(L4);
+            Rng rng{ p % g.width, p / g.width, g.frameIndex ^ g.seedMix, pv.rngCounter[p] };
+            if (tri == kNoHit) {
+                // pathtrace.comp:135-147: environment, MIS against NEE after a BSDF bounce.
+                const V3 env = envRadiance(sc, g, dir);
+                if (maxComp(env) > 0.0f) {
+                    float w = 1.0f;
+                    if (bounce > 0) w = balanceHeuristic(prevPdf, envPdf(sc, g, dir) * envPickPdf(g));
+                    L = L + beta * env * w;
+                }
+            } else {
+                const Surface s = makeSurface(sc, dir, h.y, h.z, tri, pv.hitInst[k]);
+                const float opacity = s.mat.baseColorOpacity.w;
+                if (opacity < 1.0f && rng.next1() >= opacity) {
+                    // Stochastic alpha: pass through, consuming one bounce.
+                    newO = s.pos + dir * kEpsRay;
+                    newD = dir;
+                    emitRay = !last;
+                } else {
+                    const V3 emission = s.mat.emissionTex >= 0
+                                            ? v3(s.mat.emissionRoughness) * v3(sampleTexture(sc, s.mat.emissionTex, s.u, s.v))
+                                            : v3(s.mat.emissionRoughness);
+                    if (maxComp(emission) > 0.0f && s.front) L = L + beta * emission;
+                    const Bsdf bsdf = makeBsdf(sc, s);
+                    V3 T, B;
+                    onb(s.nShade, T, B);
+                    V3 wo = v3(dot(-dir, T), dot(-dir, B), dot(-dir, s.nShade));
+                    if (wo.z < 1e-6f) wo.z = 1e-6f;
+                    wo = normalize(wo);
+                    // NEE (pathtrace.comp:181-197)
+                    const LightSample ls = sampleOneLight(sc, g, s.pos, rng);
+                    if (ls.valid && ls.pdf > 1e-9f) {
+                        const V3 wi = v3(dot(ls.wi, T), dot(ls.wi, B), dot(ls.wi, s.nShade));
+                        V3 f;
+                        float bsdfPdf;
+                        bsdfEval(bsdf, wo, wi, f, bsdfPdf);
+                        if (maxComp(f) > 0.0f) {
+                            const float w = ls.delta ? 1.0f : balanceHeuristic(ls.pdf, bsdfPdf);
+                            shO = s.pos + s.nGeom * (wi.z > 0.0f ? kEpsRay : -kEpsRay);
+                            shD = ls.wi;
+                            shMax = ls.dist - 2.0f * kEpsRay;
+                            shC = beta * f * ls.radiance * (fabsf(wi.z) * w / ls.pdf);
+                            emitShadow = true;
+                        }
+                    }
+                    // BSDF bounce and Russian roulette (pathtrace.comp:200-216)
+                    if (!last) {
+                        const BsdfSample bs = bsdfSample(bsdf, wo, rng);
+                        if (bs.valid) {
+                            const V3 wiW = normalize(T * bs.wi.x + B * bs.wi.y + s.nShade * bs.wi.z);
+                            beta = beta * bs.f * (fabsf(bs.wi.z) / bs.pdf);
+                            prevPdf = bs.pdf;
+                            bool alive = true;
+                            if (bounce >= 3) {
+                                const float q = clampf(maxComp(beta), 0.05f, 0.95f);
+                                if (rng.next1() >= q) alive = false;
+                                else beta = beta * (1.0f / q);
+                            }
+                            if (alive) {
+                                newO = s.pos + s.nGeom * (bs.wi.z > 0.0f ? kEpsRay : -kEpsRay);
+                                newD = wiW;
+                                emitRay = true;
+                            }
+                        }
+                    }
+                }
+            }
+            pv.beta[p] = f4(beta, prevPdf);
+            pv.radiance[p] = f4(L, L4.w);
+            pv.rngCounter[p] = rng.counter;
+        }
+        const uint32_t rs = waveAppend(next + kCntRays, emitRay);
+        if (emitRay) {
+            nrayO[rs] = f4(newO, __uint_as_float(p));
+            nrayD[rs] = f4(newD, 0.0f);
+        }
+        const uint32_t ss = waveAppend(cnt + kCntShadows, emitShadow);
+        if (emitShadow) {
+            pv.shadowO[ss] = f4(shO, shMax);
+            pv.shadowD[ss] = f4(shD, __uint_as_float(p));
+            pv.shadowC[ss] = f4(shC, 0.0f);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ shadow --
+template <int BLOCK, int S>
+__global__ __launch_bounds__(BLOCK) void kShadow(SceneView sc, PathView pv,
+                                                 const FrameConsts* __restrict__ gp, uint32_t bounce) {
+    __shared__ uint2 ldsStack[S * BLOCK];
+    const uint32_t tlasRoot = gp->tlasRoot, instCount = gp->instanceCount;
+    uint32_t* cnt = pv.counters + bounce * kCntPerBounce;
+    const uint32_t count = cnt[kCntShadows];
+    bool overflow = false;
+    while (true) {
+        const uint32_t base = waveFetch(cnt + kCntShadowFetch);
+        if (base >= count) break;
+        const uint32_t k = base + laneId();
+        if (k < count) {
+            const float4 o = pv.shadowO[k], d = pv.shadowD[k];
+            SceneHit h;
+            const bool occluded = traceScene<true, S, BLOCK>(sc, tlasRoot, instCount, make_float3(o.x, o.y, o.z),
+                                                             make_float3(d.x, d.y, d.z), kEpsRay, o.w,
+                                                             ldsStack + threadIdx.x, h, overflow);
+            if (!occluded) {
+                const uint32_t p = __float_as_uint(d.w);
+                const float4 c = pv.shadowC[k];
+                float4 L = pv.radiance[p];
+                L.x += c.x;
+                L.y += c.y;
+                L.z += c.z;
+                pv.radiance[p] = L;
+            }
+        }
+    }
+    if (overflow) atomicOr(pv.counters + kStatusWord, 1u);
+}
+
+// -------------------------------------------------------------- accumulate --
+// pathtrace.comp:219-228
+__global__ __launch_bounds__(256) void kAccumulate(PathView pv, const FrameConsts* __restrict__ gp) {
+    const FrameConsts g = *gp;
+    const uint32_t n = g.width * g.height;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        V3 L = v3(pv.radiance[p]);
+        if (g.fireflyClamp > 0.0f) {
+            const float m = maxComp(L);
+            if (m > g.fireflyClamp) L = L * (g.fireflyClamp / m);
+        }
+        if (!isfinite(L.x) || !isfinite(L.y) || !isfinite(L.z)) L = v3(0.0f, 0.0f, 0.0f);
+        const float4 prev = g.frameIndex == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : pv.accum[p];
+        pv.accum[p] = make_float4(prev.x + L.x, prev.y + L.y, prev.z + L.z, prev.w + 1.0f);
+    }
+}
+
+// ----------------------------------------------------------------- resolve --
+// resolve.comp:13-55: exposure, optional ACES (Narkowicz 2015), clamp, sRGB.
+__device__ __forceinline__ float aces(float x) {
+    const float a = 2.51f, b = 0.03f, c = 2.43f, d = 0.59f, e = 0.14f;
+    return clampf((x * (a * x + b)) / (x * (c * x + d) + e), 0.0f, 1.0f);
+}
+__device__ __forceinline__ float linearToSrgb(float c) {
+    return c <= 0.0031308f ? c * 12.92f : 1.055f * powf(c, 1.0f / 2.4f) - 0.055f;
+}
+__device__ __forceinline__ uint32_t unorm8(float c) {
+    return uint32_t(rintf(clampf(c, 0.0f, 1.0f) * 255.0f));
+}
+
+__global__ __launch_bounds__(256) void kResolve8(PathView pv, const FrameConsts* __restrict__ gp) {
+    const FrameConsts g = *gp;
+    const uint32_t n = g.width * g.height;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const float4 acc = pv.accum[p];
+        float c[3] = { 0.0f, 0.0f, 0.0f };
+        if (acc.w > 0.0f) { c[0] = acc.x / acc.w; c[1] = acc.y / acc.w; c[2] = acc.z / acc.w; }
+        uint32_t px = 0xFF000000u;
+        for (int i = 0; i < 3; ++i) {
+            float x = c[i] * g.exposure;
+            if (g.tonemap == 1) x = aces(x);
+            x = linearToSrgb(clampf(x, 0.0f, 1e4f));
+            px |= unorm8(x) << (8 * i);
+        }
+        pv.outRgba8[p] = px;
+    }
+}
+
+// RGBA32F readback value: accum.rgb * (1 / spp), alpha 1 (Renderer.cpp:900-908).
+__global__ __launch_bounds__(256) void kResolve32(PathView pv, uint32_t n, uint32_t spp) {
+    const float inv = spp > 0 ? 1.0f / float(spp) : 0.0f;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const float4 acc = pv.accum[p];
+        pv.outRgba32[p] = make_float4(acc.x * inv, acc.y * inv, acc.z * inv, 1.0f);
+    }
+}
+
+__global__ void kClearWords(uint32_t* words, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) words[i] = 0u;
+}
+
+// ---------------------------------------------------------------- launchers --
+static uint32_t gridFor(uint32_t n, uint32_t block, uint32_t cap) {
+    uint32_t g = (n + block - 1) / block;
+    if (g > cap) g = cap;
+    return g ? g : 1;
+}
+
+hipError_t launchClearWords(uint32_t* words, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(kClearWords, dim3(gridFor(n, 256, 1024)), dim3(256), 0, s, words, n);
+    return hipGetLastError();
+}
+
+hipError_t launchGenerate(const PathView& pv, const FrameConsts* g, uint32_t pixels, hipStream_t s) {
+    hipLaunchKernelGGL(kGenerate, dim3(gridFor(pixels, 256, 8192)), dim3(256), 0, s, pv, g);
+    return hipGetLastError();
+}
+
+template <int B, int S>
+static hipError_t extendT(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t b,
+                          uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL((kExtend<B, S>), dim3(grid), dim3(B), 0, s, sc, pv, g, b);
+    return hipGetLastError();
+}
+template <int B, int S>
+static hipError_t shadowT(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t b,
+                          uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL((kShadow<B, S>), dim3(grid), dim3(B), 0, s, sc, pv, g, b);
+    return hipGetLastError();
+}
+
+#define MRT_PT_SHAPES(X) X(64, 16) X(128, 8) X(128, 16) X(128, 32) X(256, 8) X(256, 16)
+
+hipError_t launchExtend(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t bounce,
+                        const TraceShape& shape, hipStream_t s) {
+#define X(B, S) if (shape.block == B && shape.stack == S) return extendT<B, S>(sc, pv, g, bounce, shape.grid, s);
+    MRT_PT_SHAPES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launchShadow(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t bounce,
+                        const TraceShape& shape, hipStream_t s) {
+#define X(B, S) if (shape.block == B && shape.stack == S) return shadowT<B, S>(sc, pv, g, bounce, shape.grid, s);
+    MRT_PT_SHAPES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launchShade(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t bounce,
+                       uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(kShade, dim3(grid), dim3(256), 0, s, sc, pv, g, bounce);
+    return hipGetLastError();
+}
+
+hipError_t launchAccumulate(const PathView& pv, const FrameConsts* g, uint32_t pixels, hipStream_t s) {
+    hipLaunchKernelGGL(kAccumulate, dim3(gridFor(pixels, 256, 8192)), dim3(256), 0, s, pv, g);
+    return hipGetLastError();
+}
+
+hipError_t launchResolve8(const PathView& pv, const FrameConsts* g, uint32_t pixels, hipStream_t s) {
+    hipLaunchKernelGGL(kResolve8, dim3(gridFor(pixels, 256, 8192)), dim3(256), 0, s, pv, g);
+    return hipGetLastError();
+}
+
+hipError_t launchResolve32(const PathView& pv, uint32_t pixels, uint32_t spp, hipStream_t s) {
+    hipLaunchKernelGGL(kResolve32, dim3(gridFor(pixels, 256, 8192)), dim3(256), 0, s, pv, pixels, spp);
+    return hipGetLastError();
+}
+
+bool ptShapeSupported(uint32_t block, uint32_t stack) {
+#define X(B, S) if (block == B && stack == S) return true;
+    MRT_PT_SHAPES(X)
+#undef X
+    return false;
+}
+
+hipError_t persistentGrid(uint32_t block, uint32_t stack, bool shadow, int* blocksPerCU) {
+#define X(B, S)                                                                                           \
+    if (block == B && stack == S)                                                                        \
+        return shadow ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kShadow<B, S>, B, 0)    \
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kExtend<B, S>, B, 0);
+    MRT_PT_SHAPES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t shadeBlocksPerCU(int* blocksPerCU) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kShade, 256, 0);
+}
+
+} // namespace dev
+} // namespace mrt

@@ -1,0 +1,64 @@
+// Parameter blocks of the wavefront path-tracing kernels (plain device
+// pointers; passed by value to every launch).
+#pragma once
+
+#include "common/gpu_types.h"
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mrt {
+namespace dev {
+
+struct Node;
+
+struct SceneView {
+    const Node* nodes;               // all BLAS nodes, then the TLAS at FrameConsts::tlasRoot
+    const float4* triVerts;          // 3 float4 per leaf-ordered triangle (object space)
+    const GpuTriangle* tris;         // leaf-ordered vertex indices (global)
+    const float4* positions;
+    const float4* normals;
+    const float2* uvs;
+    const float4* tangents;
+    const GpuInstance* instances;    // TLAS leaf order
+    const TraceInstance* traceInst;  // same order, compact traversal copy
+    const GpuMaterial* materials;
+    const GpuLight* lights;
+    const float4* envMap;            // envWidth x envHeight RGBA32F, row 0 = top (theta = 0)
+    const AliasEntry* envAlias;      // envWidth * envHeight entries
+    const GpuTexture* texTable;      // 64 slots
+    const uint32_t* tex8;            // RGBA8 texel pool
+    const float4* tex32;             // RGBA32F texel pool
+};
+
+// Per-bounce counters, 8 words per bounce level.
+enum CounterSlot : uint32_t {
+    kCntRays = 0,        // rays queued for this bounce
+    kCntShadows = 1,     // shadow rays queued at this bounce
+    kCntExtendFetch = 2, // work-fetch cursor of the extend kernel
+    kCntShadeFetch = 3,
+    kCntShadowFetch = 4,
+    kCntGuideRecords = 5,
+    kCntPerBounce = 8,
+};
+
+struct PathView {
+    uint32_t capacity;               // paths (= pixels) per frame
+    float4* beta;                    // rgb throughput + previous BSDF pdf
+    float4* radiance;                // rgb radiance + flag bits
+    uint32_t* rngCounter;            // draws consumed so far
+    float4* rayO[2];                 // queued ray origin xyz + path index bits
+    float4* rayD[2];                 // queued ray direction xyz
+    float4* hit;                     // t, b1, b2, triangle index bits (per queue slot)
+    uint32_t* hitInst;               // instance index (per queue slot)
+    float4* shadowO;                 // origin xyz + tmax
+    float4* shadowD;                 // direction xyz + path index bits
+    float4* shadowC;                 // unoccluded contribution rgb
+    uint32_t* counters;              // kCntPerBounce words per bounce
+    float4* accum;                   // rgb sum + sample count, per pixel
+    uint32_t* outRgba8;              // resolved image
+    float4* outRgba32;               // accum / spp, alpha 1
+};
+
+} // namespace dev
+} // namespace mrt

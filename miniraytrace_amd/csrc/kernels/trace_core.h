@@ -80,13 +80,20 @@ __device__ __forceinline__ RayK setupRay(float ox, float oy, float oz, float dx,
 
 // Conservative slab test.  Returns true when the box may hold a hit with
 // t <= limit; tEnter receives the (clamped at 0) entry distance.
+//
+// The near and far plane of each axis are chosen from the sign of the
+// reciprocal direction, so the only NaN (0 * inf, a ray parallel to and lying
+// on a box face) lands in the max over near distances or the min over far
+// distances, where IEEE maxNum/minNum drop it: a vacuous constraint, which is
+// the right answer for a ray inside that slab.
 __device__ __forceinline__ bool slab(const Node& n, const RayK& r, float limit, float& tEnter) {
 #pragma clang fp contract(off)
-    const float t0x = (n.a.x - r.ox) * r.ix, t1x = (n.b.x - r.ox) * r.ix;
-    const float t0y = (n.a.y - r.oy) * r.iy, t1y = (n.b.y - r.oy) * r.iy;
-    const float t0z = (n.a.z - r.oz) * r.iz, t1z = (n.b.z - r.oz) * r.iz;
-    const float enter = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.0f));
-    const float far = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * kFarWiden;
+    const bool px = r.ix >= 0.0f, py = r.iy >= 0.0f, pz = r.iz >= 0.0f;
+    const float nx = ((px ? n.a.x : n.b.x) - r.ox) * r.ix, fx = ((px ? n.b.x : n.a.x) - r.ox) * r.ix;
+    const float ny = ((py ? n.a.y : n.b.y) - r.oy) * r.iy, fy = ((py ? n.b.y : n.a.y) - r.oy) * r.iy;
+    const float nz = ((pz ? n.a.z : n.b.z) - r.oz) * r.iz, fz = ((pz ? n.b.z : n.a.z) - r.oz) * r.iz;
+    const float enter = fmaxf(fmaxf(nx, ny), fmaxf(nz, 0.0f));
+    const float far = fminf(fminf(fx, fy), fz) * kFarWiden;
     tEnter = enter;
     return enter <= fminf(far, limit);
 }

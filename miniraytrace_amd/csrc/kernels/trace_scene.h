@@ -1,0 +1,159 @@
+// Two-level traversal (TLAS over instances, BLAS over triangles) for the
+// path tracer's extend (closest hit) and shadow (any hit) stages.
+//
+// Semantics follow core/shaders/intersect.glsl:88-178: the world ray is
+// transformed into each instance's object space with an unnormalised
+// direction, so hit distances are shared between spaces and one limit prunes
+// both levels.  Differences from the reference, all in the direction of
+// exactness: watertight triangle test, conservative box test, TLAS leaves
+// holding several instances (coincident centroids) are all visited, ties in
+// t resolve to the lowest (instance, triangle) pair, and the stack spills
+// instead of overflowing.
+#pragma once
+
+#include "common/gpu_types.h"
+#include "pt_params.h"
+#include "trace_core.h"
+
+namespace mrt {
+namespace dev {
+
+struct SceneHit {
+    float t, b1, b2;
+    uint32_t tri, inst;
+};
+
+constexpr uint32_t kInstEntry = 0x80000000u;  // stack entry tag: "enter instance"
+
+__device__ __forceinline__ void toObject(const TraceInstance& ti, float ox, float oy, float oz,
+                                         float dx, float dy, float dz, float3& o, float3& d) {
+    o.x = ((ti.row0.x * ox + ti.row0.y * oy) + ti.row0.z * oz) + ti.row0.w;
+    o.y = ((ti.row1.x * ox + ti.row1.y * oy) + ti.row1.z * oz) + ti.row1.w;
+    o.z = ((ti.row2.x * ox + ti.row2.y * oy) + ti.row2.z * oz) + ti.row2.w;
+    d.x = (ti.row0.x * dx + ti.row0.y * dy) + ti.row0.z * dz;
+    d.y = (ti.row1.x * dx + ti.row1.y * dy) + ti.row1.z * dz;
+    d.z = (ti.row2.x * dx + ti.row2.y * dy) + ti.row2.z * dz;
+}
+
+// ANY = false: closest hit in (tmin, tmax) -> returns true and fills `out`.
+// ANY = true: returns true as soon as some triangle has tmin < t < tmax.
+template <bool ANY, int S, int BLOCK>
+__device__ bool traceScene(const SceneView& sc, uint32_t tlasRoot, uint32_t instanceCount,
+                           float3 o, float3 d, float tmin, float tmax, uint2* ldsLane,
+                           SceneHit& out, bool& overflow) {
+    if (instanceCount == 0) return false;
+    const RayK world = setupRay(o.x, o.y, o.z, d.x, d.y, d.z, tmin);
+    RayK r = world;
+    ShortStack<S, BLOCK> stack(ldsLane);
+    bool inBlas = false;
+    int blasBase = 0;
+    uint32_t curInst = 0;
+    float best = tmax;
+    bool found = false;
+    uint32_t bestTri = kNoHit, bestInst = kNoHit;
+    float bestB1 = 0.0f, bestB2 = 0.0f;
+
+    Node cur = loadNode(sc.nodes, tlasRoot);
+    float te;
+    bool have = slab(cur, r, best * kLimitWiden, te);
+    bool pendingInst = false;
+    uint32_t pendingIdx = 0;
+
+    while (true) {
+        if (pendingInst) {
+            // Enter an instance: object-space ray, BLAS root.
+            pendingInst = false;
+            const TraceInstance ti = sc.traceInst[pendingIdx];
+            float3 oo, dd;
+            toObject(ti, world.ox, world.oy, world.oz, world.dx, world.dy, world.dz, oo, dd);
+            r = setupRay(oo.x, oo.y, oo.z, dd.x, dd.y, dd.z, tmin);
+            inBlas = true;
+            blasBase = stack.sp;
+            curInst = pendingIdx;
+            cur = loadNode(sc.nodes, ti.blasRoot);
+            have = slab(cur, r, best * kLimitWiden, te);
+        }
+        if (have) {
+            const uint32_t lp = nodeLeft(cur);
+            const uint32_t cnt = nodeCount(cur);
+            if (cnt == 0) {
+                const Node c0 = loadNode(sc.nodes, lp);
+                const Node c1 = loadNode(sc.nodes, lp + 1);
+                const float lim = best * kLimitWiden;
+                float t0, t1;
+                const bool h0 = slab(c0, r, lim, t0);
+                const bool h1 = slab(c1, r, lim, t1);
+                if (h0 && h1) {
+                    const bool firstNear = t0 <= t1;
+                    stack.push(make_uint2(firstNear ? lp + 1 : lp, __float_as_uint(firstNear ? t1 : t0)));
+                    cur = firstNear ? c0 : c1;
+                    continue;
+                }
+                if (h0 || h1) {
+                    cur = h0 ? c0 : c1;
+                    continue;
+                }
+            } else if (inBlas) {
+                for (uint32_t k = lp; k < lp + cnt; ++k) {
+                    const float4 v0 = sc.triVerts[3 * size_t(k)];
+                    const float4 v1 = sc.triVerts[3 * size_t(k) + 1];
+                    const float4 v2 = sc.triVerts[3 * size_t(k) + 2];
+                    float t, b1, b2;
+                    if (!isectTri(r, v0, v1, v2, best, t, b1, b2)) continue;
+                    if (ANY) {
+                        if (t < tmax) return true;
+                        continue;
+                    }
+                    const bool better = t < best ||
+                                        (found && t == best &&
+                                         (curInst < bestInst || (curInst == bestInst && k < bestTri)));
+                    if (better) {
+                        best = t;
+                        found = true;
+                        bestTri = k;
+                        bestInst = curInst;
+                        bestB1 = b1;
+                        bestB2 = b2;
+                    }
+                }
+            } else {
+                // TLAS leaf: one or more instances [lp, lp + cnt).
+                for (uint32_t j = cnt - 1; j >= 1; --j) stack.push(make_uint2(kInstEntry | (lp + j), 0u));
+                pendingInst = true;
+                pendingIdx = lp;
+                continue;
+            }
+        }
+        have = false;
+        while (true) {
+            if (inBlas && stack.sp == blasBase) {
+                inBlas = false;
+                r = world;
+            }
+            if (stack.empty()) break;
+            const uint2 e = stack.pop();
+            if (e.x & kInstEntry) {
+                pendingInst = true;
+                pendingIdx = e.x & ~kInstEntry;
+                break;
+            }
+            if (__uint_as_float(e.y) <= best * kLimitWiden) {
+                cur = loadNode(sc.nodes, e.x);
+                have = true;
+                break;
+            }
+        }
+        if (!have && !pendingInst) break;
+    }
+    if (stack.overflow) overflow = true;
+    if (ANY || !found) return false;
+    out.t = best;
+    out.b1 = bestB1;
+    out.b2 = bestB2;
+    out.tri = bestTri;
+    out.inst = bestInst;
+    return true;
+}
+
+} // namespace dev
+} // namespace mrt

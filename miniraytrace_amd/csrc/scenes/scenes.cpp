@@ -5,9 +5,11 @@
 // Random numbers: std::mt19937_64 (its output sequence is fixed by the C++
 // standard), converted to floats as u = (x >> 40) * 2^-24, which is exact in
 // binary32 and lies in [0, 1).
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <random>
 #include <vector>
 
@@ -61,6 +63,197 @@ void mrts_soup_rays(uint64_t n, uint64_t seed, float* rays8) {
         float* o = rays8 + i * 8;
         o[0] = float(ox); o[1] = float(oy); o[2] = float(oz); o[3] = 1e-4f;
         o[4] = float(dx); o[5] = float(dy); o[6] = float(dz); o[7] = INFINITY;
+    }
+}
+
+// ---- terrain ----------------------------------------------------------------
+// Improved Perlin gradient noise (Perlin 2002) in 2D with a permutation
+// shuffled by the seeded generator; fBm of 4 octaves, normalised to [-1, 1].
+struct Perlin {
+    int perm[512];
+    explicit Perlin(uint64_t seed) {
+        Rng r(seed);
+        int p[256];
+        for (int i = 0; i < 256; ++i) p[i] = i;
+        for (int i = 255; i > 0; --i) {
+            const int j = int(r.g() % uint64_t(i + 1));
+            const int t = p[i]; p[i] = p[j]; p[j] = t;
+        }
+        for (int i = 0; i < 512; ++i) perm[i] = p[i & 255];
+    }
+    static double fade(double t) { return t * t * t * (t * (t * 6 - 15) + 10); }
+    static double grad(int h, double x, double y) {
+        switch (h & 7) {
+            case 0: return x + y;  case 1: return -x + y; case 2: return x - y;  case 3: return -x - y;
+            case 4: return x;      case 5: return -x;     case 6: return y;      default: return -y;
+        }
+    }
+    double noise(double x, double y) const {
+        const int xi = int(std::floor(x)) & 255, yi = int(std::floor(y)) & 255;
+        const double xf = x - std::floor(x), yf = y - std::floor(y);
+        const double u = fade(xf), v = fade(yf);
+        const int aa = perm[perm[xi] + yi], ab = perm[perm[xi] + yi + 1];
+        const int ba = perm[perm[xi + 1] + yi], bb = perm[perm[xi + 1] + yi + 1];
+        const double x1 = grad(aa, xf, yf) + u * (grad(ba, xf - 1, yf) - grad(aa, xf, yf));
+        const double x2 = grad(ab, xf, yf - 1) + u * (grad(bb, xf - 1, yf - 1) - grad(ab, xf, yf - 1));
+        return x1 + v * (x2 - x1);
+    }
+    double fbm(double x, double y) const {
+        double s = 0, a = 1, f = 1, norm = 0;
+        for (int o = 0; o < 4; ++o) {
+            s += a * noise(x * f, y * f);
+            norm += a;
+            a *= 0.5;
+            f *= 2.0;
+        }
+        return s / norm;
+    }
+};
+
+// Height of the terrain at (x, z): amplitude * fbm(x * 0.35, z * 0.35).
+static double terrainHeight(const Perlin& p, double x, double z, double amplitude) {
+    return amplitude * p.fbm(x * 0.35 + 17.0, z * 0.35 + 31.0);
+}
+
+// Heightfield of nx x nz quads over [-sx/2, sx/2] x [-sz/2, sz/2]
+// ((nx+1) x (nz+1) vertices, 2 triangles per quad, normals facing +y).
+// positions: (nx+1)(nz+1) x 3; indices: nx*nz*6.
+void mrts_terrain(uint32_t nx, uint32_t nz, float sx, float sz, float amplitude, uint64_t seed,
+                  float* positions, uint32_t* indices) {
+    const Perlin p(seed);
+    for (uint32_t j = 0; j <= nz; ++j)
+        for (uint32_t i = 0; i <= nx; ++i) {
+            const double x = -0.5 * sx + double(sx) * i / nx, z = -0.5 * sz + double(sz) * j / nz;
+            float* v = positions + (size_t(j) * (nx + 1) + i) * 3;
+            v[0] = float(x);
+            v[1] = float(terrainHeight(p, x, z, amplitude));
+            v[2] = float(z);
+        }
+    for (uint32_t j = 0; j < nz; ++j)
+        for (uint32_t i = 0; i < nx; ++i) {
+            const uint32_t v00 = j * (nx + 1) + i, v10 = v00 + 1, v01 = v00 + (nx + 1), v11 = v01 + 1;
+            uint32_t* t = indices + (size_t(j) * nx + i) * 6;
+            t[0] = v00; t[1] = v01; t[2] = v11;
+            t[3] = v00; t[4] = v11; t[5] = v10;
+        }
+}
+
+float mrts_terrain_height(float x, float z, float amplitude, uint64_t seed) {
+    const Perlin p(seed);
+    return float(terrainHeight(p, x, z, amplitude));
+}
+
+// ---- icosphere -----------------------------------------------------------------
+// Unit icosphere by recursive 4:1 subdivision of an icosahedron.  Level n:
+// 10*4^n + 2 vertices, 20*4^n triangles.  Returns the vertex count written.
+uint32_t mrts_icosphere(uint32_t level, float* positions, uint32_t* indices) {
+    const double t = (1.0 + std::sqrt(5.0)) / 2.0;
+    std::vector<double> v = { -1, t, 0, 1, t, 0, -1, -t, 0, 1, -t, 0, 0, -1, t, 0, 1, t,
+                              0, -1, -t, 0, 1, -t, t, 0, -1, t, 0, 1, -t, 0, -1, -t, 0, 1 };
+    for (size_t i = 0; i < v.size(); i += 3) {
+        const double l = std::sqrt(v[i] * v[i] + v[i + 1] * v[i + 1] + v[i + 2] * v[i + 2]);
+        v[i] /= l; v[i + 1] /= l; v[i + 2] /= l;
+    }
+    std::vector<uint32_t> f = { 0, 11, 5, 0, 5, 1, 0, 1, 7, 0, 7, 10, 0, 10, 11, 1, 5, 9, 5, 11, 4,
+                                11, 10, 2, 10, 7, 6, 7, 1, 8, 3, 9, 4, 3, 4, 2, 3, 2, 6, 3, 6, 8,
+                                3, 8, 9, 4, 9, 5, 2, 4, 11, 6, 2, 10, 8, 6, 7, 9, 8, 1 };
+    for (uint32_t lv = 0; lv < level; ++lv) {
+        std::vector<uint32_t> nf;
+        std::map<uint64_t, uint32_t> cache;
+        auto mid = [&](uint32_t a, uint32_t b) {
+            const uint64_t key = (uint64_t(std::min(a, b)) << 32) | std::max(a, b);
+            auto it = cache.find(key);
+            if (it != cache.end()) return it->second;
+            double x = v[a * 3] + v[b * 3], y = v[a * 3 + 1] + v[b * 3 + 1], z = v[a * 3 + 2] + v[b * 3 + 2];
+            const double l = std::sqrt(x * x + y * y + z * z);
+            v.push_back(x / l); v.push_back(y / l); v.push_back(z / l);
+            const uint32_t id = uint32_t(v.size() / 3 - 1);
+            cache.emplace(key, id);
+            return id;
+        };
+        for (size_t i = 0; i < f.size(); i += 3) {
+            const uint32_t a = f[i], b = f[i + 1], c = f[i + 2];
+            const uint32_t ab = mid(a, b), bc = mid(b, c), ca = mid(c, a);
+            const uint32_t tri[12] = { a, ab, ca, b, bc, ab, c, ca, bc, ab, bc, ca };
+            nf.insert(nf.end(), tri, tri + 12);
+        }
+        f.swap(nf);
+    }
+    for (size_t i = 0; i < v.size(); ++i) positions[i] = float(v[i]);
+    std::memcpy(indices, f.data(), f.size() * 4);
+    return uint32_t(v.size() / 3);
+}
+
+// ---- sky ---------------------------------------------------------------------
+// Procedural equirectangular sky (row 0 = zenith, u = atan2(z, x) / 2pi):
+// horizon-to-zenith gradient and dark ground after the reference's
+// tools/sky_gen.py, plus a sun disc of angular radius sunRadius whose
+// luminance is sunScale x the zenith luminance, towards (0.4, 0.6, 0.3).
+// rgba: h x w x 4 floats.
+void mrts_sky(uint32_t w, uint32_t h, float sunRadius, float sunScale, float* rgba) {
+    const double kPi = 3.14159265358979323846;
+    const double sl = std::sqrt(0.4 * 0.4 + 0.6 * 0.6 + 0.3 * 0.3);
+    const double sx = 0.4 / sl, sy = 0.6 / sl, sz = 0.3 / sl;
+    const double zr = 0.22, zg = 0.42, zb = 0.85;
+    const double zenithLum = 0.2126 * zr + 0.7152 * zg + 0.0722 * zb;
+    const double sunTint[3] = { 1.0, 0.93, 0.83 };
+    const double tintLum = 0.2126 * sunTint[0] + 0.7152 * sunTint[1] + 0.0722 * sunTint[2];
+    const double sunK = sunScale * zenithLum / tintLum;
+    const double cosSun = std::cos(double(sunRadius));
+    for (uint32_t y = 0; y < h; ++y)
+        for (uint32_t x = 0; x < w; ++x) {
+            const double theta = (y + 0.5) / h * kPi, phi = (x + 0.5) / w * 2.0 * kPi;
+            const double dx = std::sin(theta) * std::cos(phi), dy = std::cos(theta), dz = std::sin(theta) * std::sin(phi);
+            double r, g, b;
+            if (dy < 0.0) {
+                const double f = 1.0 + dy;
+                r = 0.28 * f + 0.05; g = 0.26 * f + 0.05; b = 0.24 * f + 0.05;
+            } else {
+                const double t = std::pow(dy, 0.55);
+                r = 0.95 * (1 - t) + zr * t; g = 0.97 * (1 - t) + zg * t; b = 1.00 * (1 - t) + zb * t;
+                const double c = dx * sx + dy * sy + dz * sz;
+                if (c >= cosSun) {
+                    r = sunK * sunTint[0]; g = sunK * sunTint[1]; b = sunK * sunTint[2];
+                } else if (c > 0.0) {
+                    const double glow = std::exp(-std::acos(std::min(1.0, c)) * 14.0) * 3.0;
+                    r += glow; g += glow * 0.9; b += glow * 0.7;
+                }
+            }
+            float* o = rgba + (size_t(y) * w + x) * 4;
+            o[0] = float(r); o[1] = float(g); o[2] = float(b); o[3] = 1.0f;
+        }
+}
+
+// ---- scattered instances --------------------------------------------------------
+// count uniformly scaled instances on the terrain: radius U[rmin, rmax],
+// x in [-sx/2, sx/2] * 0.9, z likewise, resting on the surface; material slot
+// uniform in [0, nmat).  xforms: count x 12 (row-major 3x4); mats: count.
+void mrts_scatter_on_terrain(uint32_t count, float sx, float sz, float amplitude, uint64_t terrainSeed,
+                             float rmin, float rmax, uint32_t nmat, uint64_t seed, float* xforms,
+                             uint32_t* mats) {
+    Rng r(seed);
+    const Perlin p(terrainSeed);
+    for (uint32_t i = 0; i < count; ++i) {
+        const double rad = rmin + (rmax - rmin) * r.ud();
+        const double x = (r.ud() - 0.5) * sx * 0.9, z = (r.ud() - 0.5) * sz * 0.9;
+        const double y = terrainHeight(p, x, z, amplitude) + rad;
+        float* m = xforms + size_t(i) * 12;
+        for (int k = 0; k < 12; ++k) m[k] = 0.0f;
+        m[0] = m[5] = m[10] = float(rad);
+        m[3] = float(x); m[7] = float(y); m[11] = float(z);
+        mats[i] = uint32_t(r.g() % nmat);
+    }
+}
+
+// GGX material parameters: roughness U[0.05, 0.8], metallic 0 or 1,
+// albedo U[0.2, 0.9]^3.  out: count x 5 (r, g, b, roughness, metallic).
+void mrts_materials(uint32_t count, uint64_t seed, float* out) {
+    Rng r(seed);
+    for (uint32_t i = 0; i < count; ++i) {
+        float* o = out + size_t(i) * 5;
+        o[3] = float(0.05 + 0.75 * r.ud());
+        o[4] = r.ud() < 0.5 ? 0.0f : 1.0f;
+        for (int c = 0; c < 3; ++c) o[c] = float(0.2 + 0.7 * r.ud());
     }
 }
 

@@ -21,8 +21,91 @@ def lib() -> C.CDLL:
         h = C.CDLL(str(_PATH))
         h.mrts_soup.argtypes = [C.c_uint32, C.c_uint64, C.c_void_p]
         h.mrts_soup_rays.argtypes = [C.c_uint64, C.c_uint64, C.c_void_p]
+        _bind(h)
         _lib = h
     return _lib
+
+
+def _bind(h):
+    P, U32, U64, F = C.c_void_p, C.c_uint32, C.c_uint64, C.c_float
+    h.mrts_terrain.argtypes = [U32, U32, F, F, F, U64, P, P]
+    h.mrts_terrain_height.argtypes = [F, F, F, U64]
+    h.mrts_terrain_height.restype = F
+    h.mrts_icosphere.argtypes = [U32, P, P]
+    h.mrts_icosphere.restype = U32
+    h.mrts_sky.argtypes = [U32, U32, F, F, P]
+    h.mrts_scatter_on_terrain.argtypes = [U32, F, F, F, U64, F, F, U32, U64, P, P]
+    h.mrts_materials.argtypes = [U32, U64, P]
+
+
+def terrain(nx: int = 1000, nz: int = 500, sx: float = 20.0, sz: float = 10.0, amplitude: float = 0.2,
+            seed: int = 1) -> tuple[np.ndarray, np.ndarray]:
+    """Perlin heightfield: ((nx+1)(nz+1), 3) positions, (2 nx nz, 3) indices."""
+    pos = np.empty(((nx + 1) * (nz + 1), 3), np.float32)
+    idx = np.empty((nx * nz * 2, 3), np.uint32)
+    lib().mrts_terrain(nx, nz, sx, sz, amplitude, seed, pos.ctypes.data, idx.ctypes.data)
+    return pos, idx
+
+
+def icosphere(level: int = 3) -> tuple[np.ndarray, np.ndarray]:
+    nv, nf = 10 * 4 ** level + 2, 20 * 4 ** level
+    pos = np.empty((nv, 3), np.float32)
+    idx = np.empty((nf, 3), np.uint32)
+    got = lib().mrts_icosphere(level, pos.ctypes.data, idx.ctypes.data)
+    assert got == nv
+    return pos, idx
+
+
+def sky(width: int = 2048, height: int = 1024, sun_radius: float = np.radians(0.25),
+        sun_scale: float = 5e4) -> np.ndarray:
+    """Equirect RGBA32F sky (height, width, 4), sun disc of 0.5 deg diameter."""
+    out = np.empty((height, width, 4), np.float32)
+    lib().mrts_sky(width, height, float(sun_radius), float(sun_scale), out.ctypes.data)
+    return out
+
+
+def scatter_on_terrain(count, sx, sz, amplitude, terrain_seed, rmin, rmax, nmat, seed):
+    xf = np.empty((count, 12), np.float32)
+    mats = np.empty(count, np.uint32)
+    lib().mrts_scatter_on_terrain(count, sx, sz, amplitude, terrain_seed, rmin, rmax, nmat, seed,
+                                  xf.ctypes.data, mats.ctypes.data)
+    return xf, mats
+
+
+def materials(count: int = 3, seed: int = 1) -> np.ndarray:
+    out = np.empty((count, 5), np.float32)
+    lib().mrts_materials(count, seed, out.ctypes.data)
+    return out
+
+
+CONFIG2 = dict(nx=1000, nz=500, sx=20.0, sz=10.0, amplitude=0.2, spheres=200, sphere_level=3,
+               rmin=0.15, rmax=0.45, sky=(2048, 1024), camera=dict(position=(0.0, 2.2, 7.5),
+                                                                  target=(0.0, 0.0, 0.0), fov_y_deg=45.0))
+
+
+def load_terrain_scene(engine, cfg: dict = CONFIG2, seed: int = 1, sky_img: np.ndarray | None = None) -> dict:
+    """Terrain + instanced icospheres + GGX materials + sky into an Engine.
+
+    Geometry: one terrain mesh (nx x nz quads) and one icosphere mesh
+    instanced `spheres` times; 3 materials; the terrain uses material 0."""
+    tpos, tidx = terrain(cfg["nx"], cfg["nz"], cfg["sx"], cfg["sz"], cfg["amplitude"], seed)
+    spos, sidx = icosphere(cfg["sphere_level"])
+    mats = materials(3, seed + 100)
+    mat_ids = [engine.add_material(base_color=tuple(m[:3]), roughness=float(m[3]), metallic=float(m[4]))
+               for m in mats]
+    tm = engine.add_mesh(tpos, tidx)
+    sm = engine.add_mesh(spos, sidx, normals=spos)
+    engine.add_instance(tm, None, mat_ids[0])
+    xf, which = scatter_on_terrain(cfg["spheres"], cfg["sx"], cfg["sz"], cfg["amplitude"], seed,
+                                   cfg["rmin"], cfg["rmax"], 3, seed + 200)
+    for i in range(cfg["spheres"]):
+        engine.add_instance(sm, xf[i], mat_ids[int(which[i])])
+    if sky_img is None:
+        sky_img = sky(*cfg["sky"])
+    engine.set_environment(sky_img, 0.0, 1.0)
+    engine.set_camera(**cfg["camera"])
+    return {"triangles": tidx.shape[0] + cfg["spheres"] * sidx.shape[0], "instances": cfg["spheres"] + 1,
+            "materials": mats}
 
 
 def soup(count: int = 10_000, seed: int = 1) -> np.ndarray:

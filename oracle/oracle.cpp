@@ -302,19 +302,21 @@ float slabGlsl(const OrNode& n, V3 o, V3 inv, float tMax) {
     const float exit = std::fmin(std::fmin(mxx, mxy), std::fmin(mxz, tMax));
     return enter <= exit ? enter : tMax;
 }
-// Conservative slab test used with the (t, id) tie-break: IEEE minNum/maxNum
-// (a NaN from 0*inf drops out), far distances widened by 2^-20 and the limit
-// by 2^-16 so rounding never culls a box holding a hit at t <= limit.
+// Conservative slab test used with the (t, id) tie-break.  Near/far planes
+// are picked from the sign of the reciprocal direction, so the only NaN
+// (0 * inf: a ray parallel to and lying on a box face) enters the max over
+// near distances or the min over far distances, where IEEE maxNum/minNum drop
+// it (a vacuous constraint).  Far distances are widened by 2^-20 and the
+// limit by 2^-16 so rounding never culls a box holding a hit at t <= limit.
 constexpr float kFarWiden = 1.0f + 1.0f / 1048576.0f;
 constexpr float kLimitWiden = 1.0f + 1.0f / 65536.0f;
 bool slabSafe(const OrNode& n, V3 o, V3 inv, float limit, float* tEnter) {
-    const float t0x = (n.lo[0] - o.x) * inv.x, t1x = (n.hi[0] - o.x) * inv.x;
-    const float t0y = (n.lo[1] - o.y) * inv.y, t1y = (n.hi[1] - o.y) * inv.y;
-    const float t0z = (n.lo[2] - o.z) * inv.z, t1z = (n.hi[2] - o.z) * inv.z;
-    const float enter = std::fmax(std::fmax(std::fmin(t0x, t1x), std::fmin(t0y, t1y)),
-                                  std::fmax(std::fmin(t0z, t1z), 0.0f));
-    const float far = std::fmin(std::fmin(std::fmax(t0x, t1x), std::fmax(t0y, t1y)),
-                                std::fmax(t0z, t1z)) * kFarWiden;
+    const bool px = inv.x >= 0.0f, py = inv.y >= 0.0f, pz = inv.z >= 0.0f;
+    const float nx = ((px ? n.lo[0] : n.hi[0]) - o.x) * inv.x, fx = ((px ? n.hi[0] : n.lo[0]) - o.x) * inv.x;
+    const float ny = ((py ? n.lo[1] : n.hi[1]) - o.y) * inv.y, fy = ((py ? n.hi[1] : n.lo[1]) - o.y) * inv.y;
+    const float nz = ((pz ? n.lo[2] : n.hi[2]) - o.z) * inv.z, fz = ((pz ? n.hi[2] : n.lo[2]) - o.z) * inv.z;
+    const float enter = std::fmax(std::fmax(nx, ny), std::fmax(nz, 0.0f));
+    const float far = std::fmin(std::fmin(fx, fy), fz) * kFarWiden;
     *tEnter = enter;
     return enter <= std::fmin(far, limit);
 }

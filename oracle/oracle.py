@@ -172,3 +172,95 @@ def brute_any(verts9, rays, threads: int = 0) -> np.ndarray:
 
 def hardware_threads() -> int:
     return int(lib().or_hardware_threads())
+
+
+# ---- path tracer (core/shaders/pathtrace.comp restated in FP64) -------------
+class _PtScene(C.Structure):
+    _fields_ = [("nodes", C.c_void_p), ("tlasRoot", C.c_uint32), ("instanceCount", C.c_uint32),
+                ("triangles", C.c_void_p), ("positions", C.c_void_p), ("normals", C.c_void_p),
+                ("instances", C.c_void_p), ("materials", C.c_void_p), ("lights", C.c_void_p),
+                ("lightCount", C.c_uint32), ("env", C.c_void_p), ("envW", C.c_uint32), ("envH", C.c_uint32),
+                ("envRotation", C.c_float), ("envIntensity", C.c_float), ("envAlias", C.c_void_p)]
+
+
+class _PtCamera(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("right", C.c_float * 3), ("up", C.c_float * 3),
+                ("forward", C.c_float * 3), ("frustum", C.c_float * 4), ("ortho", C.c_int32)]
+
+
+class _PtParams(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("x0", C.c_uint32), ("y0", C.c_uint32),
+                ("cw", C.c_uint32), ("ch", C.c_uint32), ("frame0", C.c_uint32), ("spp", C.c_uint32),
+                ("maxBounces", C.c_uint32), ("fireflyClamp", C.c_float), ("seedMix", C.c_uint32),
+                ("threads", C.c_int32), ("countFetches", C.c_int32)]
+
+
+def _bind_pt(h):
+    h.or_pt_render.argtypes = [C.POINTER(_PtScene), C.POINTER(_PtCamera), C.POINTER(_PtParams),
+                               C.c_void_p, C.c_void_p]
+    h.or_pt_render.restype = None
+    h.or_camera_lookat.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_uint32,
+                                   C.c_uint32, C.POINTER(_PtCamera)]
+    h.or_env_cdf_probs.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
+    h.or_env_cdf_probs.restype = C.c_double
+
+
+def camera_lookat(position, target, up, fov_y_deg, width, height) -> _PtCamera:
+    _bind_pt(lib())
+    cam = _PtCamera()
+    a = [_f32(x) for x in (position, target, up)]
+    lib().or_camera_lookat(a[0].ctypes.data, a[1].ctypes.data, a[2].ctypes.data, fov_y_deg, width,
+                           height, C.byref(cam))
+    return cam
+
+
+def env_cdf_probs(env_rgba: np.ndarray) -> tuple[np.ndarray, float]:
+    """Per-texel probability of the reference CDF sampler and the solid-angle
+    luminance integral (Renderer.cpp:620-652)."""
+    _bind_pt(lib())
+    e = _f32(env_rgba)
+    h, w = e.shape[:2]
+    prob = np.zeros(h * w, np.float64)
+    integral = lib().or_env_cdf_probs(e.ctypes.data, w, h, prob.ctypes.data)
+    return prob.reshape(h, w), float(integral)
+
+
+def pt_render(exported: dict, env: np.ndarray | None, camera: _PtCamera, width: int, height: int,
+              crop=(0, 0, None, None), frame0: int = 0, spp: int = 1, max_bounces: int = 8,
+              firefly_clamp: float = 0.0, env_rotation: float = 0.0, env_intensity: float = 1.0,
+              use_alias: bool = False, seed_mix: int = 0, threads: int = 0, count_fetches: bool = False):
+    """Renders a crop with the FP64 restatement.  `exported` is the packed
+    scene from Engine.export_scene().  Returns (sum of radiance over the spp
+    frames, shape (ch, cw, 3), counts[6] or None)."""
+    _bind_pt(lib())
+    x0, y0, cw, ch = crop
+    cw = width - x0 if cw is None else cw
+    ch = height - y0 if ch is None else ch
+    keep = {k: np.ascontiguousarray(v) for k, v in exported.items() if isinstance(v, np.ndarray)}
+    sc = _PtScene()
+    sc.nodes = keep["nodes"].ctypes.data
+    sc.tlasRoot = exported["tlasRoot"]
+    sc.instanceCount = keep["instances"].shape[0]
+    sc.triangles = keep["triangles"].ctypes.data
+    sc.positions = keep["positions"].ctypes.data
+    sc.normals = keep["normals"].ctypes.data
+    sc.instances = keep["instances"].ctypes.data if keep["instances"].size else None
+    sc.materials = keep["materials"].ctypes.data
+    sc.lights = keep["lights"].ctypes.data if keep["lights"].size else None
+    sc.lightCount = keep["lights"].shape[0]
+    env_arr = None
+    if env is not None:
+        env_arr = _f32(env)
+        sc.env = env_arr.ctypes.data
+        sc.envH, sc.envW = env_arr.shape[:2]
+        sc.envRotation = env_rotation
+        sc.envIntensity = env_intensity
+        if use_alias:
+            sc.envAlias = keep["envAlias"].ctypes.data
+    p = _PtParams(width=width, height=height, x0=x0, y0=y0, cw=cw, ch=ch, frame0=frame0, spp=spp,
+                  maxBounces=max_bounces, fireflyClamp=firefly_clamp, seedMix=seed_mix, threads=threads,
+                  countFetches=int(count_fetches))
+    out = np.zeros((ch, cw, 3), np.float64)
+    counts = np.zeros(6, np.uint64)
+    lib().or_pt_render(C.byref(sc), C.byref(camera), C.byref(p), out.ctypes.data, counts.ctypes.data)
+    return out, (counts if count_fetches else None)

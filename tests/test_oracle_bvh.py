@@ -197,3 +197,24 @@ def test_product_builder_matches_oracle(oracle):
     tris = random_triangles(257, 4)
     b = oracle.tri_bounds(tris)
     assert oracle.bvh_build(b, 1)[0].tobytes() == build_bvh(b, 1)[0].tobytes()
+
+
+def test_axis_parallel_rays_on_box_faces(oracle):
+    """Rays with zero direction components lying exactly on a node's face
+    (0 * inf in the slab test) must not be culled."""
+    tri = np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0]], np.float32)
+    quad = np.array([[0, 0, 0, 1, 0, 0, 1, 1, 0], [0, 0, 0, 1, 1, 0, 0, 1, 0]], np.float32)
+    rays = []
+    for x in (0.0, 0.25, 1.0):
+        for y in (0.0, 0.5, 1.0):
+            rays.append([x, y, 5, 1e-4, 0, 0, -1, np.inf])
+            rays.append([x, y, -5, 1e-4, 0, 0, 1, np.inf])
+            rays.append([x, y, 5, 1e-4, -0.0, -0.0, -1, np.inf])
+    rays.append([-1, 0.0, 0.0, 1e-4, 1, 0, 0, np.inf])  # in the triangle's plane
+    rays = np.array(rays, np.float32)
+    for soup in (tri, quad, np.concatenate([tri, quad + 2.0])):
+        nodes, order = oracle.bvh_build(oracle.tri_bounds(soup), 4)
+        assert np.array_equal(oracle.bvh_closest(nodes, order, soup, rays, 1),
+                              oracle.brute_closest(soup, rays, 1))
+    hits = oracle.brute_closest(tri, rays, 1)
+    assert hits[0, 0] == 0  # the vertex at the origin is hit
