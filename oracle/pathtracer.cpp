@@ -218,11 +218,9 @@ bool trace(const Scene& sc, D3 o, D3 d, double tmax, bool any, Hit& hit) {
     const D3 inv = { 1.0 / d.x, 1.0 / d.y, 1.0 / d.z };
     double best = tmax;
     bool found = false;
-    std::vector<uint32_t> tstack;
-    tstack.reserve(64);
+    thread_local std::vector<uint32_t> tstack, bstack;
+    tstack.clear();
     tstack.push_back(s.tlasRoot);
-    std::vector<uint32_t> bstack;
-    bstack.reserve(128);
     while (!tstack.empty()) {
         const OrNode& n = s.nodes[tstack.back()];
         tstack.pop_back();

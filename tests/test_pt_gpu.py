@@ -1,0 +1,131 @@
+"""Path tracer on the GPU vs the FP64 oracle (oracle/pathtracer.cpp).
+
+* Environment alias table: its implied texel distribution equals the
+  reference CDF sampler's (Renderer.cpp:620-652) to float32 CDF precision.
+* Per-sample parity: with the same RNG stream and the engine's alias table,
+  GPU and oracle agree pixel by pixel except where FP32/FP64 rounding sends
+  a path another way; the fraction of such pixels is bounded and the mean
+  image agrees.
+* Convergence: on a crop, the GPU image's RMSE against an FP64 reference
+  (reference CDF sampling, many more samples) falls as 1/sqrt(spp).
+* White furnace against the quadrature albedo of the reference BSDF.
+"""
+import numpy as np
+import pytest
+
+from tests.conftest import cpu_threads
+from tests.test_oracle_pt import _albedo_quadrature
+
+pytestmark = pytest.mark.gpu
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+def small_terrain_engine(w, h, spp_limit, seed=1, sky=None):
+    from miniraytrace_amd import scenes
+    from miniraytrace_amd.engine import Engine
+    cfg = dict(scenes.CONFIG2)
+    cfg.update(nx=120, nz=60, spheres=24, sphere_level=2, sky=(256, 128))
+    e = Engine(width=w, height=h)
+    e.set_render_settings(width=w, height=h, max_bounces=8, spp_limit=spp_limit, firefly_clamp=0.0,
+                          denoise=False)
+    sky = scenes.sky(256, 128) if sky is None else sky
+    scenes.load_terrain_scene(e, cfg, seed=seed, sky_img=sky)
+    e.commit()
+    return e, sky, cfg
+
+
+@pytest.fixture(scope="module")
+def gpu(torch_gpu):
+    return torch_gpu
+
+
+def test_alias_table_distribution(gpu, oracle):
+    _print("case: alias table vs reference CDF")
+    e, sky, _ = small_terrain_engine(32, 32, 1)
+    ex = e.export_scene()
+    prob_ref, integral = oracle.env_cdf_probs(sky)
+    a = ex["envAlias"]
+    n = a.shape[0]
+    keep = a[:, 0].view(np.float32).astype(np.float64)
+    implied = keep / n
+    np.add.at(implied, a[:, 1], (1.0 - keep) / n)
+    diff = np.abs(implied - prob_ref.reshape(-1))
+    _print(f"max |p_alias - p_cdf| = {diff.max():.3e}; env integral {ex['envIntegral']:.6f} vs {integral:.6f}")
+    assert diff.max() < 5e-7
+    assert ex["envIntegral"] == pytest.approx(integral, rel=1e-6)
+
+
+def test_per_sample_parity(gpu, oracle):
+    _print("case: per-sample parity GPU vs FP64 oracle")
+    from miniraytrace_amd import scenes
+    W, H, F = 96, 64, 4
+    e, sky, cfg = small_terrain_engine(W, H, F)
+    for _ in range(F):
+        e.render_frame()
+    g = e.read_rgba32f(W, H)[..., :3].astype(np.float64)
+    ex = e.export_scene()
+    c = cfg["camera"]
+    cam = oracle.camera_lookat(c["position"], c["target"], (0, 1, 0), c["fov_y_deg"], W, H)
+    ref, _ = oracle.pt_render(ex, sky, cam, W, H, spp=F, max_bounces=8, use_alias=True, threads=cpu_threads())
+    ref /= F
+    rel = np.abs(g - ref).max(-1) / np.maximum(np.abs(ref).max(-1), 1e-3)
+    agree = (rel < 1e-3).mean()
+    _print(f"pixels agreeing to 1e-3: {agree:.4f}; mean GPU {g.mean():.5f} oracle {ref.mean():.5f}")
+    assert agree > 0.9
+    assert abs(g.mean() - ref.mean()) < 0.05 * ref.mean()
+
+
+def test_convergence_rmse(gpu, oracle):
+    _print("case: convergence vs FP64 reference")
+    W, H = 64, 48
+    spps = [4, 16, 64, 256]
+    e, sky, cfg = small_terrain_engine(W, H, max(spps))
+    imgs = {}
+    done = 0
+    for s in spps:
+        while done < s:
+            e.render_frame()
+            done += 1
+        imgs[s] = e.read_rgba32f(W, H)[..., :3].astype(np.float64)
+    ex = e.export_scene()
+    c = cfg["camera"]
+    cam = oracle.camera_lookat(c["position"], c["target"], (0, 1, 0), c["fov_y_deg"], W, H)
+    # reference: reference CDF env sampling, different frames (independent samples)
+    ref_spp = 2048
+    ref, _ = oracle.pt_render(ex, sky, cam, W, H, frame0=100000, spp=ref_spp, max_bounces=8,
+                              threads=cpu_threads())
+    ref /= ref_spp
+    rmse = {s: float(np.sqrt(((imgs[s] - ref) ** 2).mean())) for s in spps}
+    _print("RMSE vs FP64 reference:", {s: f"{v:.4e}" for s, v in rmse.items()})
+    for a, b in zip(spps, spps[1:]):
+        ratio = rmse[b] / rmse[a]
+        # 1/sqrt(4) = 0.5 per 4x samples, while well above the reference's own noise
+        assert 0.3 < ratio < 0.75, (a, b, ratio)
+
+
+def test_white_furnace_quadrature(gpu):
+    _print("case: GPU white furnace vs quadrature")
+    from miniraytrace_amd import scenes
+    from miniraytrace_amd.engine import Engine
+    v, f = scenes.icosphere(5)
+    for rough, metal, base in [(1.0, 0.0, 0.5), (0.6, 1.0, 0.6)]:
+        W = H = 64
+        e = Engine(width=W, height=H)
+        e.set_render_settings(width=W, height=H, max_bounces=8, spp_limit=1024, firefly_clamp=0.0, denoise=False)
+        m = e.add_mesh(v, f, normals=v)
+        mat = e.add_material(base_color=(base,) * 3, roughness=rough, metallic=metal)
+        e.add_instance(m, None, mat)
+        e.set_environment(np.ones((64, 128, 4), np.float32))
+        e.set_camera(position=(0, 0, 4), target=(0, 0, 0), fov_y_deg=30.0)
+        for _ in range(1024):
+            e.render_frame()
+        img = e.read_rgba32f(W, H)
+        est = img[30:34, 30:34, 0]
+        expect = _albedo_quadrature(base, rough, metal)
+        _print(f"furnace rough={rough} metal={metal}: {est.mean():.5f} vs {expect:.5f}")
+        assert abs(est.mean() - expect) < max(4 * est.std() / 4, 2e-3)
+        assert np.allclose(img[:2, :2, :3], 1.0)
+        e.close()
