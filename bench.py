@@ -48,10 +48,10 @@ def traversal_workload(args, torch) -> dict:
     hits = torch.empty((n_rays, 2), dtype=torch.int32, device="cuda")
     stream = torch.cuda.current_stream()
     if args.autotune:
-        blk, stk = qs.autotune(d_rays, hits)
+        blk, stk, loop = qs.autotune(d_rays, hits)
     else:
-        blk, stk = qs.config()
-    _log(f"traversal: {info['nodeCount']} nodes, depth {info['maxDepth']}, shape {blk}x{stk}")
+        blk, stk, loop = qs.config()
+    _log(f"traversal: {info['nodeCount']} nodes, depth {info['maxDepth']}, shape {blk}x{stk} loop {loop}")
 
     for _ in range(args.warmup):
         qs.trace_closest(d_rays, hits, stream)
@@ -74,7 +74,7 @@ def traversal_workload(args, torch) -> dict:
         "ms_per_step": wall * 1e3 / args.steps,
         "kernel_ms": kernel_ms,
         "value_mrays_s": n_rays / (kernel_ms * 1e-3) / 1e6,
-        "shape": [blk, stk],
+        "shape": {"block": blk, "lds_stack": stk, "loop": ["step", "while-while"][loop]},
         "bvh": {k: info[k] for k in ("nodeCount", "maxDepth", "leafCount", "buildMs")},
     }
 

@@ -119,7 +119,7 @@ class mrtxQuerySceneInfo(_Sized):
 
 class mrtxTraversalConfig(_Sized):
     _fields_ = [("structSize", C.c_size_t), ("blockSize", C.c_uint32),
-                ("stackDepth", C.c_uint32)]
+                ("stackDepth", C.c_uint32), ("loopMode", C.c_uint32)]
 
 
 class mrtxPathOptions(_Sized):

@@ -23,6 +23,7 @@ struct mrtxQueryScene {
     float buildMs = 0.0f;
     uint32_t block = 128;
     uint32_t stack = 16;
+    uint32_t loop = 0;
 };
 
 namespace {
@@ -191,12 +192,13 @@ MRT_API mrtResult mrtxSetTraversalConfig(mrtxQueryScene* s, const mrtxTraversalC
     if (!cfg || cfg->structSize < sizeof(mrtxTraversalConfig)) return MRT_ERROR_INVALID_ARGUMENT;
     const uint32_t block = cfg->blockSize ? cfg->blockSize : 128;
     const uint32_t stack = cfg->stackDepth ? cfg->stackDepth : 16;
-    if (!mrt::dev::traversalShapeSupported(block, stack)) {
+    if (!mrt::dev::traversalShapeSupported(block, stack, cfg->loopMode)) {
         mrt::setLastErrorMessage("unsupported traversal shape");
         return MRT_ERROR_INVALID_ARGUMENT;
     }
     s->block = block;
     s->stack = stack;
+    s->loop = cfg->loopMode;
     return MRT_SUCCESS;
 }
 
@@ -205,6 +207,7 @@ MRT_API mrtResult mrtxGetTraversalConfig(const mrtxQueryScene* s, mrtxTraversalC
     if (!out || out->structSize < sizeof(mrtxTraversalConfig)) return MRT_ERROR_INVALID_ARGUMENT;
     out->blockSize = s->block;
     out->stackDepth = s->stack;
+    out->loopMode = s->loop;
     return MRT_SUCCESS;
 }
 
@@ -215,7 +218,7 @@ MRT_API mrtResult mrtxTraceClosest(mrtxQueryScene* s, const mrtxRay* dRays, uint
     return mrt::guarded([&] {
         mrt::DeviceScope scope(s->device);
         MRT_HIP(mrt::dev::launchTraceClosest(makeLaunch(s, dRays, n, dHits), s->block, s->stack,
-                                             static_cast<hipStream_t>(hipStream)));
+                                             s->loop, static_cast<hipStream_t>(hipStream)));
         return MRT_SUCCESS;
     });
 }
@@ -227,7 +230,7 @@ MRT_API mrtResult mrtxTraceAny(mrtxQueryScene* s, const mrtxRay* dRays, uint32_t
     return mrt::guarded([&] {
         mrt::DeviceScope scope(s->device);
         MRT_HIP(mrt::dev::launchTraceAny(makeLaunch(s, dRays, n, dOccluded), s->block, s->stack,
-                                         static_cast<hipStream_t>(hipStream)));
+                                         s->loop, static_cast<hipStream_t>(hipStream)));
         return MRT_SUCCESS;
     });
 }
@@ -255,15 +258,16 @@ MRT_API mrtResult mrtxAutotuneTraversal(mrtxQueryScene* s, const mrtxRay* dRays,
         MRT_HIP(hipEventCreate(&e0));
         MRT_HIP(hipEventCreate(&e1));
         float bestMs = 1e30f;
-        uint32_t bestBlock = s->block, bestStack = s->stack;
+        uint32_t bestBlock = s->block, bestStack = s->stack, bestLoop = s->loop;
         const uint32_t blocks[] = { 64, 128, 256 };
         const uint32_t stacks[] = { 8, 16, 32 };
+        for (uint32_t loop = 0; loop < 2; ++loop)
         for (uint32_t blk : blocks)
             for (uint32_t stk : stacks) {
                 const auto q = makeLaunch(s, dRays, n, dHits);
-                MRT_HIP(mrt::dev::launchTraceClosest(q, blk, stk, st));  // warm-up
+                MRT_HIP(mrt::dev::launchTraceClosest(q, blk, stk, loop, st));  // warm-up
                 MRT_HIP(hipEventRecord(e0, st));
-                for (int rep = 0; rep < 3; ++rep) MRT_HIP(mrt::dev::launchTraceClosest(q, blk, stk, st));
+                for (int rep = 0; rep < 3; ++rep) MRT_HIP(mrt::dev::launchTraceClosest(q, blk, stk, loop, st));
                 MRT_HIP(hipEventRecord(e1, st));
                 MRT_HIP(hipEventSynchronize(e1));
                 float ms = 0.0f;
@@ -272,6 +276,7 @@ MRT_API mrtResult mrtxAutotuneTraversal(mrtxQueryScene* s, const mrtxRay* dRays,
                     bestMs = ms;
                     bestBlock = blk;
                     bestStack = stk;
+                    bestLoop = loop;
                 }
             }
         (void)hipEventDestroy(e0);
@@ -279,9 +284,11 @@ MRT_API mrtResult mrtxAutotuneTraversal(mrtxQueryScene* s, const mrtxRay* dRays,
         (void)hipStreamDestroy(st);
         s->block = bestBlock;
         s->stack = bestStack;
+        s->loop = bestLoop;
         if (chosen && chosen->structSize >= sizeof(mrtxTraversalConfig)) {
             chosen->blockSize = bestBlock;
             chosen->stackDepth = bestStack;
+            chosen->loopMode = bestLoop;
         }
         return MRT_SUCCESS;
     });

@@ -18,9 +18,12 @@ struct QueryLaunch {
     uint32_t* status;      // bit 0 set on traversal stack overflow
 };
 
-hipError_t launchTraceClosest(const QueryLaunch& q, uint32_t block, uint32_t stack, hipStream_t s);
-hipError_t launchTraceAny(const QueryLaunch& q, uint32_t block, uint32_t stack, hipStream_t s);
-bool traversalShapeSupported(uint32_t block, uint32_t stack);
+// loop: 0 = one step per iteration, 1 = while-while.
+hipError_t launchTraceClosest(const QueryLaunch& q, uint32_t block, uint32_t stack, uint32_t loop,
+                              hipStream_t s);
+hipError_t launchTraceAny(const QueryLaunch& q, uint32_t block, uint32_t stack, uint32_t loop,
+                          hipStream_t s);
+bool traversalShapeSupported(uint32_t block, uint32_t stack, uint32_t loop);
 
 } // namespace dev
 } // namespace mrt

@@ -6,13 +6,80 @@
 // with its entry distance; a popped entry is skipped without a fetch when its
 // entry distance is beyond the current limit.  Closest-hit ties in t resolve
 // to the lowest triangle id.
+//
+// Two loop structures (chosen per scene by the autotuner):
+//   LOOP 0  one step per iteration: an inner-node step or a leaf step;
+//   LOOP 1  "while-while" (Aila & Laine 2009): descend until the lane holds
+//           a leaf, so the lanes of a wave test their leaf triangles together.
 #include "launch.h"
 #include "trace_core.h"
 
 namespace mrt {
 namespace dev {
 
-template <int BLOCK, int S>
+struct ClosestState {
+    float best;
+    uint32_t bestId;
+    bool hit;
+};
+
+__device__ __forceinline__ void leafClosest(const float4* __restrict__ tris, const RayK& r, uint32_t lp,
+                                            uint32_t cnt, ClosestState& st) {
+    for (uint32_t k = lp; k < lp + cnt; ++k) {
+        const float4 v0 = tris[3 * size_t(k)];
+        const float4 v1 = tris[3 * size_t(k) + 1];
+        const float4 v2 = tris[3 * size_t(k) + 2];
+        float t, b1, b2;
+        if (!isectTri(r, v0, v1, v2, st.best, t, b1, b2)) continue;
+        const uint32_t id = __float_as_uint(v0.w);
+        if (t < st.best || (st.hit && t == st.best && id < st.bestId)) {
+            st.best = t;
+            st.bestId = id;
+            st.hit = true;
+        }
+    }
+}
+
+// Pops until an entry within the limit is found; returns false when empty.
+template <int S, int BLOCK>
+__device__ __forceinline__ bool popNext(ShortStack<S, BLOCK>& stack, const Node* __restrict__ nodes,
+                                        float limit, Node& cur) {
+    while (!stack.empty()) {
+        const uint2 e = stack.pop();
+        if (__uint_as_float(e.y) <= limit) {
+            cur = loadNode(nodes, e.x);
+            return true;
+        }
+    }
+    return false;
+}
+
+// Inner-node step: returns true and sets cur to the next node to visit
+// (pushing the farther child when both are hit); false when neither child
+// is hit.
+template <int S, int BLOCK>
+__device__ __forceinline__ bool innerStep(const Node* __restrict__ nodes, const RayK& r, float lim,
+                                          ShortStack<S, BLOCK>& stack, Node& cur) {
+    const uint32_t lp = nodeLeft(cur);
+    const Node c0 = loadNode(nodes, lp);
+    const Node c1 = loadNode(nodes, lp + 1);
+    float t0, t1;
+    const bool h0 = slab(c0, r, lim, t0);
+    const bool h1 = slab(c1, r, lim, t1);
+    if (h0 && h1) {
+        const bool firstNear = t0 <= t1;
+        stack.push(make_uint2(firstNear ? lp + 1 : lp, __float_as_uint(firstNear ? t1 : t0)));
+        cur = firstNear ? c0 : c1;
+        return true;
+    }
+    if (h0 || h1) {
+        cur = h0 ? c0 : c1;
+        return true;
+    }
+    return false;
+}
+
+template <int BLOCK, int S, int LOOP>
 __global__ __launch_bounds__(BLOCK) void kTraceClosest(const Node* __restrict__ nodes,
                                                        const float4* __restrict__ tris,
                                                        const float4* __restrict__ rays,
@@ -25,67 +92,39 @@ __global__ __launch_bounds__(BLOCK) void kTraceClosest(const Node* __restrict__ 
     const float4 r1 = rays[2 * size_t(i) + 1];
     const RayK r = setupRay(r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r0.w);
 
-    float best = r1.w;
-    uint32_t bestId = kNoHit;
-    bool hit = false;
-
-    ShortStack<S, BLOCK> stack(ldsStack + threadIdx.x);
+    ClosestState st{ r1.w, kNoHit, false };
+    uint2 spill[kSpill];
+    ShortStack<S, BLOCK> stack(ldsStack + threadIdx.x, spill);
     Node cur = loadNode(nodes, 0);
     float te;
-    bool have = slab(cur, r, best * kLimitWiden, te);
-    while (true) {
-        if (have) {
-            const uint32_t lp = nodeLeft(cur);
+    bool have = slab(cur, r, st.best * kLimitWiden, te);
+    if (LOOP == 0) {
+        while (have) {
             const uint32_t cnt = nodeCount(cur);
             if (cnt == 0) {
-                const Node c0 = loadNode(nodes, lp);
-                const Node c1 = loadNode(nodes, lp + 1);
-                const float lim = best * kLimitWiden;
-                float t0, t1;
-                const bool h0 = slab(c0, r, lim, t0);
-                const bool h1 = slab(c1, r, lim, t1);
-                if (h0 && h1) {
-                    const bool firstNear = t0 <= t1;
-                    stack.push(make_uint2(firstNear ? lp + 1 : lp, __float_as_uint(firstNear ? t1 : t0)));
-                    cur = firstNear ? c0 : c1;
-                    continue;
-                }
-                if (h0 || h1) {
-                    cur = h0 ? c0 : c1;
-                    continue;
-                }
+                if (innerStep(nodes, r, st.best * kLimitWiden, stack, cur)) continue;
             } else {
-                for (uint32_t k = lp; k < lp + cnt; ++k) {
-                    const float4 v0 = tris[3 * size_t(k)];
-                    const float4 v1 = tris[3 * size_t(k) + 1];
-                    const float4 v2 = tris[3 * size_t(k) + 2];
-                    float t, b1, b2;
-                    if (!isectTri(r, v0, v1, v2, best, t, b1, b2)) continue;
-                    const uint32_t id = __float_as_uint(v0.w);
-                    if (t < best || (hit && t == best && id < bestId)) {
-                        best = t;
-                        bestId = id;
-                        hit = true;
-                    }
-                }
+                leafClosest(tris, r, nodeLeft(cur), cnt, st);
             }
+            have = popNext(stack, nodes, st.best * kLimitWiden, cur);
         }
-        have = false;
-        while (!stack.empty()) {
-            const uint2 e = stack.pop();
-            if (__uint_as_float(e.y) <= best * kLimitWiden) {
-                cur = loadNode(nodes, e.x);
-                have = true;
-                break;
+    } else {
+        while (have) {
+            // descend to a leaf
+            while (have && nodeCount(cur) == 0) {
+                if (!innerStep(nodes, r, st.best * kLimitWiden, stack, cur))
+                    have = popNext(stack, nodes, st.best * kLimitWiden, cur);
             }
+            if (!have) break;
+            leafClosest(tris, r, nodeLeft(cur), nodeCount(cur), st);
+            have = popNext(stack, nodes, st.best * kLimitWiden, cur);
         }
-        if (!have) break;
     }
-    hits[i] = make_uint2(bestId, __float_as_uint(best));
+    hits[i] = make_uint2(st.bestId, __float_as_uint(st.best));
     if (stack.overflow) atomicOr(status, 1u);
 }
 
-template <int BLOCK, int S>
+template <int BLOCK, int S, int LOOP>
 __global__ __launch_bounds__(BLOCK) void kTraceAny(const Node* __restrict__ nodes,
                                                    const float4* __restrict__ tris,
                                                    const float4* __restrict__ rays, uint32_t n,
@@ -100,86 +139,81 @@ __global__ __launch_bounds__(BLOCK) void kTraceAny(const Node* __restrict__ node
     const float tmax = r1.w;
     const float lim = tmax * kLimitWiden;
 
-    ShortStack<S, BLOCK> stack(ldsStack + threadIdx.x);
+    uint2 spill[kSpill];
+    ShortStack<S, BLOCK> stack(ldsStack + threadIdx.x, spill);
     uint32_t occ = 0;
     Node cur = loadNode(nodes, 0);
     float te;
     bool have = slab(cur, r, lim, te);
-    while (!occ) {
-        if (have) {
-            const uint32_t lp = nodeLeft(cur);
-            const uint32_t cnt = nodeCount(cur);
-            if (cnt == 0) {
-                const Node c0 = loadNode(nodes, lp);
-                const Node c1 = loadNode(nodes, lp + 1);
-                float t0, t1;
-                const bool h0 = slab(c0, r, lim, t0);
-                const bool h1 = slab(c1, r, lim, t1);
-                if (h0 && h1) {
-                    stack.push(make_uint2(lp + 1, __float_as_uint(t1)));
-                    cur = c0;
-                    continue;
-                }
-                if (h0 || h1) {
-                    cur = h0 ? c0 : c1;
-                    continue;
-                }
-            } else {
-                for (uint32_t k = lp; k < lp + cnt; ++k) {
-                    float t, b1, b2;
-                    if (isectTri(r, tris[3 * size_t(k)], tris[3 * size_t(k) + 1],
-                                 tris[3 * size_t(k) + 2], tmax, t, b1, b2) && t < tmax) {
-                        occ = 1;
-                        break;
-                    }
-                }
-                if (occ) break;
+    while (have) {
+        if (LOOP != 0) {
+            while (have && nodeCount(cur) == 0) {
+                if (!innerStep(nodes, r, lim, stack, cur)) have = popNext(stack, nodes, lim, cur);
             }
+            if (!have) break;
         }
-        if (stack.empty()) break;
-        cur = loadNode(nodes, stack.pop().x);
-        have = true;
+        const uint32_t cnt = nodeCount(cur);
+        if (cnt == 0) {
+            if (innerStep(nodes, r, lim, stack, cur)) continue;
+        } else {
+            const uint32_t lp = nodeLeft(cur);
+            for (uint32_t k = lp; k < lp + cnt; ++k) {
+                float t, b1, b2;
+                if (isectTri(r, tris[3 * size_t(k)], tris[3 * size_t(k) + 1], tris[3 * size_t(k) + 2], tmax, t,
+                             b1, b2) &&
+                    t < tmax) {
+                    occ = 1;
+                    break;
+                }
+            }
+            if (occ) break;
+        }
+        have = popNext(stack, nodes, lim, cur);
     }
     occluded[i] = occ;
     if (stack.overflow) atomicOr(status, 1u);
 }
 
-template <int BLOCK, int S>
+template <int BLOCK, int S, int LOOP>
 static hipError_t launchClosestT(const QueryLaunch& q, hipStream_t s) {
     const uint32_t grid = (q.n + BLOCK - 1) / BLOCK;
-    hipLaunchKernelGGL((kTraceClosest<BLOCK, S>), dim3(grid), dim3(BLOCK), 0, s, q.nodes, q.tris,
+    hipLaunchKernelGGL((kTraceClosest<BLOCK, S, LOOP>), dim3(grid), dim3(BLOCK), 0, s, q.nodes, q.tris,
                        q.rays, q.n, reinterpret_cast<uint2*>(q.out), q.status);
     return hipGetLastError();
 }
-template <int BLOCK, int S>
+template <int BLOCK, int S, int LOOP>
 static hipError_t launchAnyT(const QueryLaunch& q, hipStream_t s) {
     const uint32_t grid = (q.n + BLOCK - 1) / BLOCK;
-    hipLaunchKernelGGL((kTraceAny<BLOCK, S>), dim3(grid), dim3(BLOCK), 0, s, q.nodes, q.tris,
+    hipLaunchKernelGGL((kTraceAny<BLOCK, S, LOOP>), dim3(grid), dim3(BLOCK), 0, s, q.nodes, q.tris,
                        q.rays, q.n, reinterpret_cast<uint32_t*>(q.out), q.status);
     return hipGetLastError();
 }
 
-#define MRT_QUERY_SHAPES(X) \
-    X(64, 8) X(64, 16) X(64, 32) X(128, 8) X(128, 16) X(128, 32) X(256, 8) X(256, 16) X(256, 32)
+#define MRT_QUERY_SHAPES(X)                                                                          \
+    X(64, 8, 0) X(64, 16, 0) X(64, 32, 0) X(128, 8, 0) X(128, 16, 0) X(128, 32, 0) X(256, 8, 0)       \
+    X(256, 16, 0) X(256, 32, 0) X(64, 8, 1) X(64, 16, 1) X(64, 32, 1) X(128, 8, 1) X(128, 16, 1)     \
+    X(128, 32, 1) X(256, 8, 1) X(256, 16, 1) X(256, 32, 1)
 
-hipError_t launchTraceClosest(const QueryLaunch& q, uint32_t block, uint32_t stack, hipStream_t s) {
+hipError_t launchTraceClosest(const QueryLaunch& q, uint32_t block, uint32_t stack, uint32_t loop,
+                              hipStream_t s) {
     if (q.n == 0) return hipSuccess;
-#define X(B, S) if (block == B && stack == S) return launchClosestT<B, S>(q, s);
+#define X(B, S, L) if (block == B && stack == S && loop == L) return launchClosestT<B, S, L>(q, s);
     MRT_QUERY_SHAPES(X)
 #undef X
     return hipErrorInvalidValue;
 }
 
-hipError_t launchTraceAny(const QueryLaunch& q, uint32_t block, uint32_t stack, hipStream_t s) {
+hipError_t launchTraceAny(const QueryLaunch& q, uint32_t block, uint32_t stack, uint32_t loop,
+                          hipStream_t s) {
     if (q.n == 0) return hipSuccess;
-#define X(B, S) if (block == B && stack == S) return launchAnyT<B, S>(q, s);
+#define X(B, S, L) if (block == B && stack == S && loop == L) return launchAnyT<B, S, L>(q, s);
     MRT_QUERY_SHAPES(X)
 #undef X
     return hipErrorInvalidValue;
 }
 
-bool traversalShapeSupported(uint32_t block, uint32_t stack) {
-#define X(B, S) if (block == B && stack == S) return true;
+bool traversalShapeSupported(uint32_t block, uint32_t stack, uint32_t loop) {
+#define X(B, S, L) if (block == B && stack == S && loop == L) return true;
     MRT_QUERY_SHAPES(X)
 #undef X
     return false;

@@ -144,16 +144,20 @@ __device__ __forceinline__ bool isectTri(const RayK& r, const float4& v0, const 
 // is (node index, entry distance bits).
 constexpr int kSpill = 64;
 
+//
+// The spill array is a separate private object from the stack's scalar state
+// so that the scalars stay in registers (an aggregate holding a dynamically
+// indexed array lives in scratch as a whole).
 template <int S, int BLOCK>
 struct ShortStack {
     static_assert((S & (S - 1)) == 0, "LDS stack depth must be a power of two");
     uint2* lds;
-    uint2 spill[kSpill];
+    uint2* spill;  // kSpill entries of private (scratch) memory
     int sp = 0;
     int base = 0;
     bool overflow = false;
 
-    __device__ __forceinline__ explicit ShortStack(uint2* ldsLane) : lds(ldsLane) {}
+    __device__ __forceinline__ ShortStack(uint2* ldsLane, uint2* spillArray) : lds(ldsLane), spill(spillArray) {}
     __device__ __forceinline__ bool empty() const { return sp == 0; }
     __device__ __forceinline__ void push(uint2 e) {
         if (sp - base == S) {
@@ -167,12 +171,14 @@ struct ShortStack {
         lds[(sp & (S - 1)) * BLOCK] = e;
         ++sp;
     }
+    // An empty LDS part is first refilled with one entry from the spill
+    // area, so the pop itself is always one 64-bit LDS read.
     __device__ __forceinline__ uint2 pop() {
-        --sp;
-        if (sp < base) {
-            base = sp;
-            return spill[sp];
+        if (sp == base) {
+            --base;
+            lds[(base & (S - 1)) * BLOCK] = spill[base];
         }
+        --sp;
         return lds[(sp & (S - 1)) * BLOCK];
     }
 };

@@ -44,7 +44,8 @@ __device__ bool traceScene(const SceneView& sc, uint32_t tlasRoot, uint32_t inst
     if (instanceCount == 0) return false;
     const RayK world = setupRay(o.x, o.y, o.z, d.x, d.y, d.z, tmin);
     RayK r = world;
-    ShortStack<S, BLOCK> stack(ldsLane);
+    uint2 spill[kSpill];
+    ShortStack<S, BLOCK> stack(ldsLane, spill);
     bool inBlas = false;
     int blasBase = 0;
     uint32_t curInst = 0;

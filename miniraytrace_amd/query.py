@@ -68,14 +68,14 @@ class QueryScene:
                 "mrtxExportQueryBvh")
         return nodes, order
 
-    def set_config(self, block: int, stack: int) -> None:
-        cfg = L.mrtxTraversalConfig(blockSize=block, stackDepth=stack)
+    def set_config(self, block: int, stack: int, loop: int = 0) -> None:
+        cfg = L.mrtxTraversalConfig(blockSize=block, stackDepth=stack, loopMode=loop)
         L.check(L.lib().mrtxSetTraversalConfig(self._h, C.byref(cfg)), "mrtxSetTraversalConfig")
 
-    def config(self) -> tuple[int, int]:
+    def config(self) -> tuple[int, int, int]:
         cfg = L.mrtxTraversalConfig()
         L.check(L.lib().mrtxGetTraversalConfig(self._h, C.byref(cfg)), "mrtxGetTraversalConfig")
-        return cfg.blockSize, cfg.stackDepth
+        return cfg.blockSize, cfg.stackDepth, cfg.loopMode
 
     def trace_closest(self, rays, hits, stream=None) -> None:
         """rays: device tensor (n, 8) float32; hits: device tensor (n, 2) int32."""
@@ -88,12 +88,12 @@ class QueryScene:
         L.check(L.lib().mrtxTraceAny(self._h, rays.data_ptr(), n, occluded.data_ptr(),
                                      _stream_handle(stream)), "mrtxTraceAny")
 
-    def autotune(self, rays, hits) -> tuple[int, int]:
+    def autotune(self, rays, hits) -> tuple[int, int, int]:
         cfg = L.mrtxTraversalConfig()
         L.check(L.lib().mrtxAutotuneTraversal(self._h, rays.data_ptr(), rays.shape[0],
                                               hits.data_ptr(), C.byref(cfg)),
                 "mrtxAutotuneTraversal")
-        return cfg.blockSize, cfg.stackDepth
+        return cfg.blockSize, cfg.stackDepth, cfg.loopMode
 
     def status(self) -> int:
         flags = C.c_uint32()

@@ -86,16 +86,16 @@ def test_closest_vs_reference_traversal(c1, oracle):
     assert (rel > 1e-6).mean() < 0.01
 
 
-@pytest.mark.parametrize("block,stack", [(64, 8), (64, 32), (128, 16), (256, 8), (256, 32)])
-def test_all_launch_shapes_identical(c1, oracle, block, stack):
-    _print(f"case: c1 shape block={block} stack={stack}")
-    from miniraytrace_amd.query import QueryScene
+@pytest.mark.parametrize("block,stack,loop", [(64, 8, 0), (64, 32, 0), (128, 16, 0), (256, 8, 0), (256, 32, 0),
+                                              (64, 8, 1), (128, 16, 1), (256, 32, 1)])
+def test_all_launch_shapes_identical(c1, oracle, block, stack, loop):
+    _print(f"case: c1 shape block={block} stack={stack} loop={loop}")
     base = _trace(c1)
-    c1["qs"].set_config(block, stack)
+    c1["qs"].set_config(block, stack, loop)
     try:
         other = _trace(c1)
     finally:
-        c1["qs"].set_config(128, 16)
+        c1["qs"].set_config(128, 16, 0)
     assert np.array_equal(base, other)
 
 
