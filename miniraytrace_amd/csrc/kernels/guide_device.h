@@ -1,0 +1,106 @@
+// Path-guiding distribution on the device: feature encoding of a shading
+// point, decoding of the network's 40 outputs into a mixture of 8 von
+// Mises-Fisher lobes, and the mixture's pdf and sampling routine.
+//
+// No counterpart in the reference (one of the three additions on top of the
+// reference path tracer).  Numerics follow the FP64 oracle (oracle/guide.py):
+//   vMF(w; mu, k) = k / (2 pi (1 - e^{-2k})) * exp(k (mu.w - 1))
+//   log k clamped to [-5, 7]; lobe weights = softmax of the logits.
+#pragma once
+
+#include "pt_device.h"
+
+namespace mrt {
+namespace dev {
+
+constexpr int kGuideIn = 78;       // encoded features
+constexpr int kGuideInPad = 80;    // padded to a multiple of the MFMA k step
+constexpr int kGuideHidden = 64;
+constexpr int kGuideOut = 40;      // 8 lobes x (mu 3, log kappa, logit)
+constexpr int kGuideLobes = 8;
+constexpr int kGuideBands = 12;
+constexpr int kGuideParams = 78 * 64 + 64 + 3 * (64 * 64 + 64) + 64 * 40 + 40;  // 20,136
+constexpr float kLogKappaLo = -5.0f, kLogKappaHi = 7.0f;
+
+// Offsets of each layer's weights and biases in the flat parameter vector
+// (weights output-major: row n holds the weights of output n).
+__host__ __device__ constexpr int guideLayerIn(int l) { return l == 0 ? kGuideIn : kGuideHidden; }
+__host__ __device__ constexpr int guideLayerOut(int l) { return l == 4 ? kGuideOut : kGuideHidden; }
+__host__ __device__ constexpr int guideWeightOffset(int l) {
+    return l == 0 ? 0 : guideWeightOffset(l - 1) + guideLayerIn(l - 1) * guideLayerOut(l - 1) + guideLayerOut(l - 1);
+}
+__host__ __device__ constexpr int guideBiasOffset(int l) {
+    return guideWeightOffset(l) + guideLayerIn(l) * guideLayerOut(l);
+}
+static_assert(guideBiasOffset(4) + kGuideOut == kGuideParams, "guide parameter layout");
+
+// Feature k of a shading point (pos01 = position normalised to the scene box).
+// Order: axis a (x, y, z), band b, {sin, cos}; then normal, then wo.
+__device__ __forceinline__ float guideFeature(const float p[3], const float n[3], const float wo[3], int k) {
+    if (k < 72) {
+        const int a = k / 24, r = k % 24, b = r >> 1;
+        const float ang = p[a] * float(1 << b) * kPi;
+        return (r & 1) ? cosf(ang) : sinf(ang);
+    }
+    if (k < 75) return n[k - 72];
+    if (k < 78) return wo[k - 75];
+    return 0.0f;
+}
+
+struct GuideMix {
+    V3 mu[kGuideLobes];
+    float kappa[kGuideLobes];
+    float w[kGuideLobes];
+};
+
+__device__ __forceinline__ void decodeMix(const float* raw, GuideMix& m) {
+    float mx = -1e30f;
+    for (int k = 0; k < kGuideLobes; ++k) mx = fmaxf(mx, raw[5 * k + 4]);
+    float s = 0.0f;
+    for (int k = 0; k < kGuideLobes; ++k) {
+        const V3 v = v3(raw[5 * k], raw[5 * k + 1], raw[5 * k + 2]);
+        const float len = sqrtf(dot(v, v));
+        m.mu[k] = len > 1e-12f ? v * (1.0f / len) : v3(0.0f, 0.0f, 1.0f);
+        m.kappa[k] = expf(clampf(raw[5 * k + 3], kLogKappaLo, kLogKappaHi));
+        m.w[k] = expf(raw[5 * k + 4] - mx);
+        s += m.w[k];
+    }
+    const float inv = 1.0f / s;
+    for (int k = 0; k < kGuideLobes; ++k) m.w[k] *= inv;
+}
+
+__device__ __forceinline__ float vmfPdf(V3 mu, float kappa, V3 w) {
+    // kappa / (2 pi (1 - e^{-2 kappa})) without cancellation for small kappa
+    const float norm = kappa / (kTwoPi * -expm1f(-2.0f * kappa));
+    return norm * expf(kappa * (dot(mu, w) - 1.0f));
+}
+
+__device__ __forceinline__ float mixPdf(const GuideMix& m, V3 w) {
+    float p = 0.0f;
+    for (int k = 0; k < kGuideLobes; ++k) p += m.w[k] * vmfPdf(m.mu[k], m.kappa[k], w);
+    return p;
+}
+
+// Lobe by u0 over the cumulative weights, then a vMF direction (Wenzel
+// Jakob, "Numerically stable sampling of the von Mises-Fisher distribution
+// on S^2", 2012) from (u1, u2).
+__device__ __forceinline__ V3 mixSample(const GuideMix& m, float u0, float u1, float u2) {
+    int k = 0;
+    float c = m.w[0];
+    while (k < kGuideLobes - 1 && u0 >= c) {
+        ++k;
+        c += m.w[k];
+    }
+    const float kap = m.kappa[k];
+    float wz = 1.0f + log1pf((1.0f - u1) * expm1f(-2.0f * kap)) / kap;
+    wz = clampf(wz, -1.0f, 1.0f);
+    const float sz = sqrtf(fmaxf(1.0f - wz * wz, 0.0f));
+    float sp, cp;
+    sincosf(kTwoPi * u2, &sp, &cp);
+    V3 t, b;
+    onb(m.mu[k], t, b);
+    return normalize(t * (sz * cp) + b * (sz * sp) + m.mu[k] * wz);
+}
+
+} // namespace dev
+} // namespace mrt

@@ -245,6 +245,76 @@ void mrts_scatter_on_terrain(uint32_t count, float sx, float sz, float amplitude
     }
 }
 
+// ---- room ------------------------------------------------------------------------
+// Closed box room [-hx, hx] x [0, hy] x [-hz, hz], each of the 6 walls an
+// n x n grid of quads (2 triangles each), facing inwards.  Vertices strictly
+// inside a wall are jittered by U[-jitter, jitter] per axis (boundary
+// vertices are shared exactly, so the room stays closed).  A rectangular
+// window covering windowFrac of the +z wall is cut out (its quads are
+// dropped): the only opening.  Returns the triangle count written;
+// positions: 6 (n+1)^2 x 3, indices: up to 6 n^2 x 2 x 3.
+uint32_t mrts_room(uint32_t n, float hx, float hy, float hz, float windowFrac, float jitter, uint64_t seed,
+                   float* positions, uint32_t* indices, uint32_t* vertexCount) {
+    Rng r(seed);
+    uint32_t nv = 0, nt = 0;
+    // wall frames: origin corner, u axis, v axis (u x v points inwards)
+    struct Wall { double o[3], u[3], v[3]; };
+    const Wall walls[6] = {
+        { { -hx, 0, -hz }, { 2 * hx, 0, 0 }, { 0, 0, 2 * hz } },     // floor (y = 0), normal +y
+        { { -hx, hy, -hz }, { 0, 0, 2 * hz }, { 2 * hx, 0, 0 } },    // ceiling, normal -y
+        { { -hx, 0, -hz }, { 0, 0, 2 * hz }, { 0, hy, 0 } },         // x = -hx, normal +x
+        { { hx, 0, -hz }, { 0, hy, 0 }, { 0, 0, 2 * hz } },          // x = +hx, normal -x
+        { { -hx, 0, -hz }, { 0, hy, 0 }, { 2 * hx, 0, 0 } },         // z = -hz, normal +z
+        { { -hx, 0, hz }, { 2 * hx, 0, 0 }, { 0, hy, 0 } },          // z = +hz (window wall), normal -z
+    };
+    // window: centred on the +z wall, aspect 2:1, area windowFrac of the wall
+    const double wallArea = 2.0 * hx * hy;
+    const double ww = std::sqrt(windowFrac * wallArea * 2.0), wh = ww / 2.0;
+    for (int w = 0; w < 6; ++w) {
+        const Wall& f = walls[w];
+        const uint32_t base = nv;
+        // winding so that the face normal (u x v, or v x u) points into the room
+        const double cr[3] = { f.u[1] * f.v[2] - f.u[2] * f.v[1], f.u[2] * f.v[0] - f.u[0] * f.v[2],
+                               f.u[0] * f.v[1] - f.u[1] * f.v[0] };
+        const double toC[3] = { 0.0 - f.o[0], 0.5 * hy - f.o[1], 0.0 - f.o[2] };
+        const bool flip = cr[0] * toC[0] + cr[1] * toC[1] + cr[2] * toC[2] < 0.0;
+        for (uint32_t j = 0; j <= n; ++j)
+            for (uint32_t i = 0; i <= n; ++i) {
+                const double a = double(i) / n, b = double(j) / n;
+                double p[3];
+                for (int k = 0; k < 3; ++k) p[k] = f.o[k] + a * f.u[k] + b * f.v[k];
+                const bool interior = i > 0 && i < n && j > 0 && j < n;
+                for (int k = 0; k < 3; ++k) {
+                    const double jit = (2.0 * r.ud() - 1.0) * jitter;  // drawn for every vertex
+                    if (interior) p[k] += jit;
+                }
+                float* o = positions + size_t(nv) * 3;
+                o[0] = float(p[0]); o[1] = float(p[1]); o[2] = float(p[2]);
+                ++nv;
+            }
+        for (uint32_t j = 0; j < n; ++j)
+            for (uint32_t i = 0; i < n; ++i) {
+                if (w == 5) {
+                    // quad centre in wall coordinates: x in [-hx, hx], y in [0, hy]
+                    const double cx = -hx + (i + 0.5) / n * 2.0 * hx, cy = (j + 0.5) / n * hy;
+                    if (std::fabs(cx) < ww / 2 && std::fabs(cy - hy * 0.55) < wh / 2) continue;
+                }
+                const uint32_t v00 = base + j * (n + 1) + i, v10 = v00 + 1, v01 = v00 + (n + 1), v11 = v01 + 1;
+                uint32_t* t = indices + size_t(nt) * 3;
+                if (!flip) {
+                    t[0] = v00; t[1] = v10; t[2] = v11;
+                    t[3] = v00; t[4] = v11; t[5] = v01;
+                } else {
+                    t[0] = v00; t[1] = v11; t[2] = v10;
+                    t[3] = v00; t[4] = v01; t[5] = v11;
+                }
+                nt += 2;
+            }
+    }
+    *vertexCount = nv;
+    return nt;
+}
+
 // GGX material parameters: roughness U[0.05, 0.8], metallic 0 or 1,
 // albedo U[0.2, 0.9]^3.  out: count x 5 (r, g, b, roughness, metallic).
 void mrts_materials(uint32_t count, uint64_t seed, float* out) {

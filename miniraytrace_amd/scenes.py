@@ -36,6 +36,8 @@ def _bind(h):
     h.mrts_sky.argtypes = [U32, U32, F, F, P]
     h.mrts_scatter_on_terrain.argtypes = [U32, F, F, F, U64, F, F, U32, U64, P, P]
     h.mrts_materials.argtypes = [U32, U64, P]
+    h.mrts_room.argtypes = [U32, F, F, F, F, F, U64, P, P, P]
+    h.mrts_room.restype = U32
 
 
 def terrain(nx: int = 1000, nz: int = 500, sx: float = 20.0, sz: float = 10.0, amplitude: float = 0.2,
@@ -106,6 +108,35 @@ def load_terrain_scene(engine, cfg: dict = CONFIG2, seed: int = 1, sky_img: np.n
     engine.set_camera(**cfg["camera"])
     return {"triangles": tidx.shape[0] + cfg["spheres"] * sidx.shape[0], "instances": cfg["spheres"] + 1,
             "materials": mats}
+
+
+def room(n: int = 204, half=(2.0, 3.0, 2.0), window_frac: float = 0.05, jitter: float = 1e-3,
+         seed: int = 1) -> tuple[np.ndarray, np.ndarray]:
+    """Closed room [-hx,hx] x [0,hy] x [-hz,hz] of 6 n x n-quad walls with a
+    window (windowFrac of the +z wall) cut out; interior vertices jittered."""
+    hx, hy, hz = half
+    pos = np.empty((6 * (n + 1) ** 2, 3), np.float32)
+    idx = np.empty((6 * n * n * 2, 3), np.uint32)
+    nv = C.c_uint32()
+    nt = lib().mrts_room(n, hx, hy, hz, window_frac, jitter, seed, pos.ctypes.data, idx.ctypes.data, C.byref(nv))
+    return pos[: nv.value], idx[:nt]
+
+
+CONFIG4 = dict(n=204, half=(2.0, 3.0, 2.0), window_frac=0.05, jitter=1e-3, albedo=0.7, width=1280, height=720,
+               camera=dict(position=(-1.6, 1.6, -1.7), target=(0.6, 0.7, 1.2), fov_y_deg=60.0))
+
+
+def load_room_scene(engine, cfg: dict = CONFIG4, seed: int = 1, sky_img: np.ndarray | None = None) -> dict:
+    """Diffuse room lit only by the sky through its window (config 4)."""
+    pos, idx = room(cfg["n"], cfg["half"], cfg["window_frac"], cfg["jitter"], seed)
+    mat = engine.add_material(base_color=(cfg["albedo"],) * 3, roughness=1.0, metallic=0.0)
+    m = engine.add_mesh(pos, idx)
+    engine.add_instance(m, None, mat)
+    if sky_img is None:
+        sky_img = sky()
+    engine.set_environment(sky_img, 0.0, 1.0)
+    engine.set_camera(**cfg["camera"])
+    return {"triangles": idx.shape[0], "instances": 1}
 
 
 def soup(count: int = 10_000, seed: int = 1) -> np.ndarray:
