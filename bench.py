@@ -34,6 +34,17 @@ def _log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def _measured_traffic(kernel: str, key: str):
+    """HBM bytes of `kernel` from the committed rocprofv3 counter summary, or
+    None when that exact kernel instantiation was not profiled."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
+            rec = json.load(f)["kernels"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+    return None if rec is None else rec.get(key)
+
+
 # ----------------------------------------------------------------- traversal --
 def traversal_workload(args, torch) -> dict:
     from miniraytrace_amd import scenes
@@ -87,7 +98,9 @@ def traversal_workload(args, torch) -> dict:
         bytes_alg = 32 * n_rays + 8 * n_rays + 32 * nn + 64 * nt
         achieved = bytes_alg / (kernel_ms * 1e-3) / 1e9
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                           "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                           "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                           "traffic": _measured_traffic(f"kTraceClosest<{blk}, {stk}, {loop}>", "bytes_per_launch"),
+                           "kernel": f"kTraceClosest<{blk}, {stk}, {loop}>",
                            "bytes_per_launch": bytes_alg,
                            "nodes_per_ray": nn / n_rays, "tris_per_ray": nt / n_rays}
         # parity spot check against the oracle's exact traversal
@@ -152,6 +165,7 @@ def pt_workload(args, torch) -> dict:
         prof.append(e.frame_stats())
     e.set_profiling(False)
     stage = {k: float(np.mean([p["gpuMs"][k] for p in prof])) for k in prof[0]["gpuMs"]}
+    ext = e.path_options()["extend"]
     rays = [int(x) for x in prof[-1]["rays"]]
     shadows = [int(x) for x in prof[-1]["shadowRays"]]
     out = {
@@ -191,8 +205,9 @@ def pt_workload(args, torch) -> dict:
         bytes_alg = per_query * frame_queries
         achieved = bytes_alg / (stage["extend"] * 1e-3) / 1e9
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                           "kernel": "kExtend (closest hit, all bounces of one frame)",
+                           "frac": achieved / HBM_PEAK_GBS,
+                           "traffic": _measured_traffic(f"kExtend<{ext[0]}, {ext[1]}>", "bytes_per_frame"),
+                           "kernel": f"kExtend<{ext[0]}, {ext[1]}> (closest hit, all bounces of one frame)",
                            "bytes_per_frame": bytes_alg, "queries_per_frame": frame_queries,
                            "nodes_per_query": nn / max(q, 1), "tris_per_query": nt / max(q, 1),
                            "crop_queries_per_path": q / (cw * ch)}

@@ -131,11 +131,12 @@ MRT_API mrtResult mrtxGetPathOptions(mrtEngine* engine, mrtxPathOptions* out);
 
 #define MRTX_MAX_BOUNCE_LEVELS 18
 
-/* Stage order of gpuMs: generate, extend, shade, shadow, accumulate, resolve. */
+/* Stage order of gpuMs: generate, extend, shade, shadow, accumulate, resolve,
+ * guide (prep + network inference), train (record collection + Adam steps). */
 typedef struct mrtxFrameStats {
     size_t   structSize;
     float    frameMs;                                /* host wall time of the last frame */
-    float    gpuMs[6];                               /* per stage, profiling mode only */
+    float    gpuMs[8];                               /* per stage, profiling mode only */
     uint32_t rays[MRTX_MAX_BOUNCE_LEVELS];           /* rays traced per bounce level */
     uint32_t shadowRays[MRTX_MAX_BOUNCE_LEVELS];     /* shadow rays per bounce level */
     uint32_t status;                                 /* bit 0: traversal stack overflow */
@@ -167,6 +168,82 @@ typedef struct mrtxSceneExport {
 } mrtxSceneExport;
 
 MRT_API mrtResult mrtxExportScene(mrtEngine* engine, mrtxSceneExport* inout);
+
+/* ---- 5. path guiding -------------------------------------------------------
+ * No counterpart in the reference: a learned directional distribution (an
+ * MLP producing a mixture of 8 von Mises-Fisher lobes from the position,
+ * shading normal and outgoing direction of a path vertex), trained online
+ * on the engine's own paths and sampled with one-sample MIS against the
+ * BSDF.  Network: 78 inputs (12-band sin/cos encoding of the position in the
+ * scene box, normal, wo) -> 4 x 64 ReLU -> 40 outputs (per lobe: mean
+ * direction xyz, log kappa clamped to [-5, 7], weight logit); BF16 matrix
+ * products with FP32 accumulation, FP32 master weights, Adam. */
+
+#define MRTX_GUIDE_PARAM_COUNT 20136
+#define MRTX_GUIDE_OUTPUTS 40
+
+typedef struct mrtxGuidingOptions {
+    size_t   structSize;
+    uint32_t enableSampling;  /* draw continuation directions from the mixture (default 0) */
+    uint32_t enableTraining;  /* train on this engine's paths every frame (default 0) */
+    float    selectProb;      /* probability of a mixture draw at a guided vertex (0.5) */
+    float    learningRate;    /* Adam step size (1e-3) */
+    uint32_t batchSize;       /* training records per Adam step (65,536) */
+    uint32_t stepsPerFrame;   /* Adam steps per rendered frame (4) */
+    float    minRoughness;    /* less rough or transmissive vertices use the BSDF only (0.2) */
+    uint32_t seed;            /* network initialisation seed (used when the network is created) */
+} mrtxGuidingOptions;
+
+typedef struct mrtxGuidingStats {
+    size_t   structSize;
+    uint32_t steps;           /* Adam steps taken */
+    uint32_t lastBatch;       /* records in the last step's batch */
+    float    lastLoss;        /* weighted negative log-likelihood of that batch */
+    uint32_t poolRecords;     /* training records collected in the last frame */
+    uint32_t candidates;      /* positive-weight path records found in the last frame */
+} mrtxGuidingStats;
+
+/* Changing the options re-captures the frame graph; it does not reset the
+ * accumulation (all sampling strategies are unbiased). */
+MRT_API mrtResult mrtxSetGuiding(mrtEngine* engine, const mrtxGuidingOptions* opts);
+MRT_API mrtResult mrtxGetGuiding(mrtEngine* engine, mrtxGuidingOptions* out);
+MRT_API mrtResult mrtxGetGuidingStats(mrtEngine* engine, mrtxGuidingStats* out);
+/* count must be MRTX_GUIDE_PARAM_COUNT; setting resets the Adam state. */
+MRT_API mrtResult mrtxGetGuideParams(mrtEngine* engine, float* dst, size_t count);
+MRT_API mrtResult mrtxSetGuideParams(mrtEngine* engine, const float* src, size_t count);
+
+/* Standalone network (same kernels as the engine), host buffers in and out.
+ * Record layout, 16 floats per record:
+ *   pos01.xyz, weight, n.xyz, -, wo.xyz, -, wi.xyz, -
+ * (inference reads pos01, n and wo; training also weight and wi). */
+typedef struct mrtxGuideNetwork mrtxGuideNetwork;
+
+MRT_API mrtResult mrtxCreateGuideNetwork(int32_t gpuIndex1, uint32_t maxBatch, uint32_t seed,
+                                         mrtxGuideNetwork** out);
+MRT_API void      mrtxDestroyGuideNetwork(mrtxGuideNetwork* net);
+MRT_API mrtResult mrtxGuideNetworkSetParams(mrtxGuideNetwork* net, const float* src, size_t count);
+MRT_API mrtResult mrtxGuideNetworkGetParams(mrtxGuideNetwork* net, float* dst, size_t count);
+/* Adam state: first and second moments (count floats each, may be NULL) and
+ * the step counter. */
+MRT_API mrtResult mrtxGuideNetworkGetAdam(mrtxGuideNetwork* net, float* m, float* v, size_t count,
+                                          uint32_t* step);
+/* raw: count x MRTX_GUIDE_OUTPUTS network outputs. */
+MRT_API mrtResult mrtxGuideNetworkInfer(mrtxGuideNetwork* net, const float* records, uint32_t count,
+                                        float* raw);
+/* Gradient of loss = sum w (-log q(wi)) / sum w over the records, without an
+ * update; grad may be NULL. */
+MRT_API mrtResult mrtxGuideNetworkGradient(mrtxGuideNetwork* net, const float* records, uint32_t count,
+                                           float* grad, size_t gradCount, float* loss);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the records' gradient. */
+MRT_API mrtResult mrtxGuideNetworkTrain(mrtxGuideNetwork* net, const float* records, uint32_t count,
+                                        float learningRate, float* loss);
+/* Mixture sampling from raw outputs: u holds 3 uniforms per row; out holds
+ * (direction.xyz, mixture pdf) per row. */
+MRT_API mrtResult mrtxGuideNetworkSample(mrtxGuideNetwork* net, const float* raw, const float* u,
+                                         uint32_t count, float* out);
+/* Mixture pdf of unit directions (4 floats per row, xyz used). */
+MRT_API mrtResult mrtxGuideNetworkPdf(mrtxGuideNetwork* net, const float* raw, const float* dirs,
+                                      uint32_t count, float* pdf);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -132,9 +132,24 @@ MRTX_MAX_BOUNCE_LEVELS = 18
 
 
 class mrtxFrameStats(_Sized):
-    _fields_ = [("structSize", C.c_size_t), ("frameMs", C.c_float), ("gpuMs", C.c_float * 6),
+    _fields_ = [("structSize", C.c_size_t), ("frameMs", C.c_float), ("gpuMs", C.c_float * 8),
                 ("rays", C.c_uint32 * MRTX_MAX_BOUNCE_LEVELS),
                 ("shadowRays", C.c_uint32 * MRTX_MAX_BOUNCE_LEVELS), ("status", C.c_uint32)]
+
+
+MRTX_GUIDE_PARAM_COUNT = 20136
+MRTX_GUIDE_OUTPUTS = 40
+
+
+class mrtxGuidingOptions(_Sized):
+    _fields_ = [("structSize", C.c_size_t), ("enableSampling", C.c_uint32), ("enableTraining", C.c_uint32),
+                ("selectProb", C.c_float), ("learningRate", C.c_float), ("batchSize", C.c_uint32),
+                ("stepsPerFrame", C.c_uint32), ("minRoughness", C.c_float), ("seed", C.c_uint32)]
+
+
+class mrtxGuidingStats(_Sized):
+    _fields_ = [("structSize", C.c_size_t), ("steps", C.c_uint32), ("lastBatch", C.c_uint32),
+                ("lastLoss", C.c_float), ("poolRecords", C.c_uint32), ("candidates", C.c_uint32)]
 
 
 class mrtxSceneExport(_Sized):
@@ -201,6 +216,22 @@ _SIGS = {
     "mrtxSetProfiling": (_R, [_P, C.c_int32]),
     "mrtxGetFrameStats": (_R, [_P, C.POINTER(mrtxFrameStats)]),
     "mrtxExportScene": (_R, [_P, C.POINTER(mrtxSceneExport)]),
+    # ext: path guiding
+    "mrtxSetGuiding": (_R, [_P, C.POINTER(mrtxGuidingOptions)]),
+    "mrtxGetGuiding": (_R, [_P, C.POINTER(mrtxGuidingOptions)]),
+    "mrtxGetGuidingStats": (_R, [_P, C.POINTER(mrtxGuidingStats)]),
+    "mrtxGetGuideParams": (_R, [_P, _P, C.c_size_t]),
+    "mrtxSetGuideParams": (_R, [_P, _P, C.c_size_t]),
+    "mrtxCreateGuideNetwork": (_R, [C.c_int32, _U32, _U32, C.POINTER(_P)]),
+    "mrtxDestroyGuideNetwork": (None, [_P]),
+    "mrtxGuideNetworkSetParams": (_R, [_P, _P, C.c_size_t]),
+    "mrtxGuideNetworkGetParams": (_R, [_P, _P, C.c_size_t]),
+    "mrtxGuideNetworkGetAdam": (_R, [_P, _P, _P, C.c_size_t, C.POINTER(_U32)]),
+    "mrtxGuideNetworkInfer": (_R, [_P, _P, _U32, _P]),
+    "mrtxGuideNetworkGradient": (_R, [_P, _P, _U32, _P, C.c_size_t, C.POINTER(C.c_float)]),
+    "mrtxGuideNetworkTrain": (_R, [_P, _P, _U32, C.c_float, C.POINTER(C.c_float)]),
+    "mrtxGuideNetworkSample": (_R, [_P, _P, _P, _U32, _P]),
+    "mrtxGuideNetworkPdf": (_R, [_P, _P, _P, _U32, _P]),
 }
 
 # The 27 entry points of the reference ABI (capi/include/mini_raytrace.h:184-237).

@@ -31,9 +31,14 @@ struct FrameConsts {
     float fireflyClamp;
     uint32_t tonemap, tlasRoot, instanceCount;
     uint32_t seedMix;            // xor-ed into the frame word of the RNG key (0 = reference keys)
-    uint32_t guiding;            // 1 when the guiding distribution is active this frame
-    float guideSelectProb;
-    uint32_t guideTrain;         // 1 when training records are written this frame
+    uint32_t guiding;            // 1 when continuation directions may come from the learned mixture
+    float guideSelectProb;       // probability of drawing from the mixture at a guided vertex
+    uint32_t guideTrain;         // 1 when shade writes training records this frame
+    float guideMinRoughness;     // vertices with smaller roughness are not guided
+    uint32_t guideStamp;         // nonzero, different for every rendered frame of an engine
+    uint32_t pad0, pad1;
+    float4 guideBoxMin;          // scene box: pos01 = (p - min) * scale
+    float4 guideBoxScale;
 };
 
 struct GpuInstance {             // 144 bytes

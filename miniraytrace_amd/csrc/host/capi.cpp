@@ -3,6 +3,7 @@
 // structs are accepted when non-NULL with structSize >= the size this
 // library was built with; Add* return MRT_INVALID_ID on failure; void
 // mutators ignore bad handles; exceptions never cross the boundary.
+#include "capi_handles.h"
 #include "common.h"
 #include "engine.h"
 #include "mini_raytrace.h"
@@ -10,18 +11,10 @@
 
 #include <memory>
 
-struct mrtEngine {
-    std::unique_ptr<mrt::Engine> engine;
-};
-
 namespace {
 
-template <typename T>
-bool validStruct(const T* s) {
-    return s && s->structSize >= sizeof(T);
-}
-
-mrt::Engine* unwrap(mrtEngine* e) { return e ? e->engine.get() : nullptr; }
+using mrt::unwrap;
+using mrt::validStruct;
 
 // Runs a void mutator; failures are recorded as the last error only.
 template <typename F>
@@ -250,7 +243,7 @@ MRT_API mrtResult mrtxGetFrameStats(mrtEngine* e, mrtxFrameStats* out) {
     if (!validStruct(out)) return MRT_ERROR_INVALID_ARGUMENT;
     const mrt::FrameStats& s = eng->stats();
     out->frameMs = s.frameMs;
-    for (int i = 0; i < 6; ++i) out->gpuMs[i] = s.stageMs[i];
+    for (int i = 0; i < 8; ++i) out->gpuMs[i] = s.stageMs[i];
     for (uint32_t b = 0; b < MRTX_MAX_BOUNCE_LEVELS; ++b) {
         out->rays[b] = s.rays[b];
         out->shadowRays[b] = s.shadowRays[b];

@@ -137,6 +137,13 @@ Engine::Engine(const mrtEngineDesc& desc) {
         settings_.width = desc.width;
         settings_.height = desc.height;
     }
+    guide_.structSize = sizeof(guide_);
+    guide_.selectProb = 0.5f;
+    guide_.learningRate = 1e-3f;
+    guide_.batchSize = 65536;
+    guide_.stepsPerFrame = 4;
+    guide_.minRoughness = 0.2f;
+    guide_.seed = 1;
     // Minimal scene: default material, no instances, empty texture table.
     scene_.markDirty(kDirtyMaterials | kDirtyTextures | kDirtyLights | kDirtyGeometry | kDirtyInstances);
     allocFrame();
@@ -169,8 +176,10 @@ void Engine::chooseShapes() {
     extendShape_.grid = uint32_t(std::max(1, perCU) * cuCount_);
     MRT_HIP(dev::persistentGrid(shadowShape_.block, shadowShape_.stack, true, &perCU));
     shadowShape_.grid = uint32_t(std::max(1, perCU) * cuCount_);
-    MRT_HIP(dev::shadeBlocksPerCU(&perCU));
+    MRT_HIP(dev::shadeBlocksPerCU(false, &perCU));
     shadeGrid_ = uint32_t(std::max(1, perCU) * cuCount_);
+    MRT_HIP(dev::shadeBlocksPerCU(true, &perCU));
+    shadeGuideGrid_ = uint32_t(std::max(1, perCU) * cuCount_);
     invalidateGraph();
 }
 
@@ -179,12 +188,17 @@ void Engine::setSettings(const Settings& s) {
     if (s.width == 0 || s.height == 0) throw Error(MRT_ERROR_INVALID_ARGUMENT, "width and height must be > 0");
     const bool resize = s.width != settings_.width || s.height != settings_.height;
     const bool reset = resize || s.maxBounces != settings_.maxBounces || s.fireflyClamp != settings_.fireflyClamp;
-    if (std::min(s.maxBounces, 16u) != std::min(settings_.maxBounces, 16u)) invalidateGraph();
+    const bool levels = std::min(s.maxBounces, 16u) != std::min(settings_.maxBounces, 16u);
+    if (levels) invalidateGraph();
     settings_ = s;
     if (resize) {
         DeviceScope scope(device_);
         MRT_HIP(hipStreamSynchronize(stream_));
         allocFrame();
+    } else if (levels && guideActive()) {
+        DeviceScope scope(device_);
+        MRT_HIP(hipStreamSynchronize(stream_));
+        allocGuide();
     }
     if (reset) resetAccumulation();
 }
@@ -402,6 +416,15 @@ void Engine::packInstances() {
     }
     instanceCount_ = uint32_t(gi.size());
     tlasNodeCount_ = uint32_t(tlas.nodes.size());
+    // Guiding positions: the scene box scaled uniformly into [0, 1].
+    if (tlas.rootBounds.valid()) {
+        float extent = 0.0f;
+        for (int a = 0; a < 3; ++a) {
+            sceneBoxMin_[a] = tlas.rootBounds.lo[a];
+            extent = std::max(extent, tlas.rootBounds.hi[a] - tlas.rootBounds.lo[a]);
+        }
+        sceneBoxScale_ = 1.0f / std::max(extent, 1e-6f);
+    }
     tlasDepth_ = tlas.maxDepth;
     if (tlasDepth_ + maxLeaf + maxBlasDepth_ + 2 > kMaxTraversalStack)
         throw Error(MRT_ERROR_UNKNOWN, "scene BVH too deep for the traversal stack (TLAS depth " +
@@ -592,7 +615,33 @@ void Engine::allocFrame() {
     MRT_HIP(hipStreamSynchronize(stream_));
     hConsts_.ensure(sizeof(FrameConsts));
     hCounters_.ensure(dev::kCounterWords * 4);
+    if (guideActive()) allocGuide();
     invalidateGraph();
+}
+
+// Per-frame guiding buffers, sized for the frame and the bounce levels.
+void Engine::allocGuide() {
+    const uint32_t n = frameW_ * frameH_;
+    const uint32_t levels = guideLevels();
+    dGuideIdx_.allocate(size_t(n) * 4);
+    dGuideRec_.allocate(size_t(n) * 3 * 16);
+    dGuideMix_.allocate(size_t(n) * dev::kGuideMixFloat4 * 16);
+    dPathRec_.allocate(size_t(levels) * n * 4 * 16);
+    dPathRec_.zero(stream_);  // stamp 0 never matches a frame
+    dGuidePool_.allocate(size_t(guide_.batchSize) * guide_.stepsPerFrame * 4 * 16);
+    MRT_HIP(hipStreamSynchronize(stream_));
+    guidePaths_ = n;
+    guideLevels_ = levels;
+    invalidateGraph();
+}
+
+void Engine::ensureGuideNet() {
+    if (!guideNet_) {
+        guideNet_ = std::make_unique<GuideNetwork>(guide_.batchSize);
+        guideNet_->initParams(guide_.seed);
+    } else {
+        guideNet_->ensureCap(guide_.batchSize);
+    }
 }
 
 dev::SceneView Engine::sceneView() const {
@@ -635,6 +684,11 @@ dev::PathView Engine::pathView() const {
     p.accum = dAccum_.as<float4>();
     p.outRgba8 = dOut8_.as<uint32_t>();
     p.outRgba32 = dOut32_.as<float4>();
+    const bool guide = guideActive() && guidePaths_ == p.capacity && guideLevels_ == guideLevels() && guideNet_;
+    p.guideIdx = guide ? dGuideIdx_.as<uint32_t>() : nullptr;
+    p.guideRec = guide ? dGuideRec_.as<float4>() : nullptr;
+    p.guideMix = guide ? dGuideMix_.as<float4>() : nullptr;
+    p.pathRec = guide && guide_.enableTraining ? dPathRec_.as<float4>() : nullptr;
     return p;
 }
 
@@ -692,6 +746,13 @@ void Engine::fillConsts(FrameConsts& g) const {
     g.tlasRoot = tlasRoot_;
     g.instanceCount = instanceCount_;
     g.seedMix = seed_ ? (seed_ * 0x9E3779B9u) ^ 0x85EBCA6Bu : 0u;
+    g.guiding = guide_.enableSampling ? 1u : 0u;
+    g.guideSelectProb = guide_.selectProb;
+    g.guideTrain = guide_.enableTraining ? 1u : 0u;
+    g.guideMinRoughness = guide_.minRoughness;
+    g.guideStamp = guideStamp_;
+    g.guideBoxMin = make_float4(sceneBoxMin_[0], sceneBoxMin_[1], sceneBoxMin_[2], 0.0f);
+    g.guideBoxScale = make_float4(sceneBoxScale_, sceneBoxScale_, sceneBoxScale_, 0.0f);
 }
 
 void Engine::enqueueFrame(hipStream_t s, bool timed) {
@@ -700,29 +761,49 @@ void Engine::enqueueFrame(hipStream_t s, bool timed) {
     const FrameConsts* g = dConsts_.as<const FrameConsts>();
     const uint32_t pixels = frameW_ * frameH_;
     const uint32_t maxB = std::min(settings_.maxBounces, 16u);
+    const bool guide = pv.guideIdx != nullptr;
+    const bool train = pv.pathRec != nullptr;
+    const bool infer = guide && guide_.enableSampling;
+    const uint32_t prepGrid = std::min<uint32_t>((pixels + 255) / 256, uint32_t(cuCount_) * 8);
+    auto elapsed = [&](hipEvent_t a, hipEvent_t b) {
+        float ms = 0.0f;
+        MRT_HIP(hipEventElapsedTime(&ms, a, b));
+        return ms;
+    };
     if (timed) MRT_HIP(hipEventRecord(ev_[0], s));
     MRT_HIP(dev::launchClearWords(pv.counters, dev::kCounterWords, s));
+    if (train) MRT_HIP(dev::launchGuideFrameBegin(guideNet_->deviceWords(), g, s));
     MRT_HIP(dev::launchGenerate(pv, g, pixels, s));
     if (timed) MRT_HIP(hipEventRecord(ev_[1], s));
     for (uint32_t b = 0; b <= maxB; ++b) {
         if (timed) MRT_HIP(hipEventRecord(ev_[2], s));
         MRT_HIP(dev::launchExtend(sc, pv, g, b, extendShape_, s));
         if (timed) MRT_HIP(hipEventRecord(ev_[3], s));
-        MRT_HIP(dev::launchShade(sc, pv, g, b, shadeGrid_, s));
+        if (guide) {
+            MRT_HIP(dev::launchGuidePrep(sc, pv, g, b, prepGrid, s));
+            if (infer) {
+                dev::RecView rv;
+                rv.base = pv.guideRec;
+                rv.stride = 3;
+                MRT_HIP(dev::launchGuideInfer(rv, pv.counters + b * dev::kCntPerBounce + dev::kCntGuideRecords,
+                                              pixels, guideNet_->deviceParams(), pv.guideMix, true, s));
+            }
+        }
+        if (timed) MRT_HIP(hipEventRecord(ev_[9], s));
+        MRT_HIP(dev::launchShade(sc, pv, g, b, guide ? shadeGuideGrid_ : shadeGrid_, guide, s));
         if (timed) MRT_HIP(hipEventRecord(ev_[4], s));
         MRT_HIP(dev::launchShadow(sc, pv, g, b, shadowShape_, s));
         if (timed) {
             MRT_HIP(hipEventRecord(ev_[5], s));
             MRT_HIP(hipEventSynchronize(ev_[5]));
-            float a = 0, bsh = 0, c = 0;
-            MRT_HIP(hipEventElapsedTime(&a, ev_[2], ev_[3]));
-            MRT_HIP(hipEventElapsedTime(&bsh, ev_[3], ev_[4]));
-            MRT_HIP(hipEventElapsedTime(&c, ev_[4], ev_[5]));
-            stats_.stageMs[1] += a;
-            stats_.stageMs[2] += bsh;
-            stats_.stageMs[3] += c;
+            stats_.stageMs[1] += elapsed(ev_[2], ev_[3]);
+            stats_.stageMs[6] += elapsed(ev_[3], ev_[9]);
+            stats_.stageMs[2] += elapsed(ev_[9], ev_[4]);
+            stats_.stageMs[3] += elapsed(ev_[4], ev_[5]);
         }
     }
+    if (timed) MRT_HIP(hipEventRecord(ev_[10], s));
+    if (train) enqueueGuideTraining(s);
     if (timed) MRT_HIP(hipEventRecord(ev_[6], s));
     MRT_HIP(dev::launchAccumulate(pv, g, pixels, s));
     if (timed) MRT_HIP(hipEventRecord(ev_[7], s));
@@ -730,13 +811,41 @@ void Engine::enqueueFrame(hipStream_t s, bool timed) {
     if (timed) {
         MRT_HIP(hipEventRecord(ev_[8], s));
         MRT_HIP(hipEventSynchronize(ev_[8]));
-        float a = 0, b = 0, c = 0;
-        MRT_HIP(hipEventElapsedTime(&a, ev_[0], ev_[1]));
-        MRT_HIP(hipEventElapsedTime(&b, ev_[6], ev_[7]));
-        MRT_HIP(hipEventElapsedTime(&c, ev_[7], ev_[8]));
-        stats_.stageMs[0] = a;
-        stats_.stageMs[4] = b;
-        stats_.stageMs[5] = c;
+        stats_.stageMs[0] = elapsed(ev_[0], ev_[1]);
+        stats_.stageMs[7] = elapsed(ev_[10], ev_[6]);
+        stats_.stageMs[4] = elapsed(ev_[6], ev_[7]);
+        stats_.stageMs[5] = elapsed(ev_[7], ev_[8]);
+    }
+}
+
+// Training records of this frame -> pool -> stepsPerFrame Adam steps, batch t
+// taking pool rows t, t + steps, t + 2 steps, ...
+void Engine::enqueueGuideTraining(hipStream_t s) {
+    const uint32_t steps = guide_.stepsPerFrame;
+    const uint32_t poolCap = guide_.batchSize * steps;
+    dev::GuideCollectArgs ca;
+    ca.pathRec = dPathRec_.as<const float4>();
+    ca.radiance = dRad_.as<const float4>();
+    ca.paths = guidePaths_;
+    ca.levels = guideLevels_;
+    ca.pool = dGuidePool_.as<float4>();
+    ca.poolCap = poolCap;
+    ca.words = guideNet_->deviceWords();
+    ca.g = dConsts_.as<const FrameConsts>();
+    MRT_HIP(dev::launchGuideCollect(ca, s));
+    const dev::GuideNet net = guideNet_->view();
+    dev::GuideAdam h;
+    h.lr = guide_.learningRate;
+    for (uint32_t t = 0; t < steps; ++t) {
+        dev::GuideBatch b;
+        b.pool = dGuidePool_.as<const float4>();
+        b.total = guideNet_->deviceWords() + dev::kGwPool;
+        b.poolCap = poolCap;
+        b.phase = t;
+        b.period = steps;
+        b.cap = guide_.batchSize;
+        MRT_HIP(dev::launchGuideGradient(b, net, true, s));
+        MRT_HIP(dev::launchGuideAdam(net, h, s));
     }
 }
 
@@ -746,6 +855,7 @@ void Engine::renderFrame(mrtFrameInfo* out) {
     DeviceScope scope(device_);
     if (frameIndex_ < settings_.sppLimit) {
         FrameConsts* g = static_cast<FrameConsts*>(hConsts_.get());
+        if (++guideStamp_ == 0) guideStamp_ = 1;
         fillConsts(*g);
         MRT_HIP(hipMemcpyAsync(dConsts_.get(), g, sizeof(FrameConsts), hipMemcpyHostToDevice, stream_));
         if (profiling_) {
@@ -806,6 +916,63 @@ void Engine::readFramebuffer(mrtPixelFormat fmt, void* dst, size_t dstSize) {
     MRT_HIP(dev::launchResolve32(pathView(), uint32_t(pixels), frameIndex_, stream_));
     MRT_HIP(hipMemcpyAsync(dst, dOut32_.get(), pixels * 16, hipMemcpyDeviceToHost, stream_));
     MRT_HIP(hipStreamSynchronize(stream_));
+}
+
+// ------------------------------------------------------------------ guiding --
+void Engine::setGuiding(const mrtxGuidingOptions& in) {
+    mrtxGuidingOptions o = in;
+    o.structSize = sizeof(o);
+    if (o.selectProb == 0.0f) o.selectProb = 0.5f;
+    if (o.learningRate == 0.0f) o.learningRate = 1e-3f;
+    if (o.batchSize == 0) o.batchSize = 65536;
+    if (o.stepsPerFrame == 0) o.stepsPerFrame = 4;
+    if (!(o.selectProb > 0.0f && o.selectProb <= 1.0f))
+        throw Error(MRT_ERROR_INVALID_ARGUMENT, "guiding selectProb must be in (0, 1]");
+    if (!(o.learningRate > 0.0f) || !std::isfinite(o.learningRate))
+        throw Error(MRT_ERROR_INVALID_ARGUMENT, "guiding learningRate must be > 0");
+    if (o.batchSize < 256 || o.batchSize > (1u << 20))
+        throw Error(MRT_ERROR_INVALID_ARGUMENT, "guiding batchSize must be in [256, 1048576]");
+    if (o.stepsPerFrame > 64) throw Error(MRT_ERROR_INVALID_ARGUMENT, "guiding stepsPerFrame must be <= 64");
+    if (!(o.minRoughness >= 0.0f)) throw Error(MRT_ERROR_INVALID_ARGUMENT, "guiding minRoughness must be >= 0");
+    DeviceScope scope(device_);
+    MRT_HIP(hipStreamSynchronize(stream_));
+    const bool realloc = o.batchSize != guide_.batchSize || o.stepsPerFrame != guide_.stepsPerFrame ||
+                         guidePaths_ != frameW_ * frameH_ || guideLevels_ != guideLevels();
+    guide_ = o;
+    if (guideActive()) {
+        ensureGuideNet();
+        if (realloc || !dPathRec_.get()) allocGuide();
+    }
+    invalidateGraph();
+}
+
+void Engine::guidingStats(mrtxGuidingStats& o) {
+    o = mrtxGuidingStats{};
+    o.structSize = sizeof(o);
+    if (!guideNet_) return;
+    DeviceScope scope(device_);
+    MRT_HIP(hipStreamSynchronize(stream_));
+    uint32_t w[dev::kGwWords];
+    guideNet_->readWords(w, stream_);
+    o.steps = w[dev::kGwStep];
+    o.lastBatch = w[dev::kGwLastBatch];
+    std::memcpy(&o.lastLoss, &w[dev::kGwLastLoss], sizeof(float));
+    o.poolRecords = std::min(w[dev::kGwPool], guide_.batchSize * guide_.stepsPerFrame);
+    o.candidates = w[dev::kGwCand0 + (guideStamp_ & 1u)];
+}
+
+void Engine::getGuideParams(float* dst) {
+    DeviceScope scope(device_);
+    MRT_HIP(hipStreamSynchronize(stream_));
+    ensureGuideNet();
+    guideNet_->getParams(dst);
+}
+
+void Engine::setGuideParams(const float* src) {
+    DeviceScope scope(device_);
+    MRT_HIP(hipStreamSynchronize(stream_));
+    ensureGuideNet();
+    guideNet_->setParams(src);
 }
 
 void Engine::exportScene(mrtxSceneExport& o) {

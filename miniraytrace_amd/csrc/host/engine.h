@@ -7,10 +7,12 @@
 #include "bvh_build.h"
 #include "common.h"
 #include "common/gpu_types.h"
+#include "guide.h"
 #include "kernels/pt_launch.h"
 #include "mini_raytrace_ext.h"
 #include "scene.h"
 
+#include <algorithm>
 #include <map>
 #include <memory>
 #include <vector>
@@ -29,7 +31,7 @@ struct Settings {
 
 struct FrameStats {
     float frameMs = 0.0f;
-    float stageMs[6] = {};  // generate, extend, shade, shadow, accumulate, resolve
+    float stageMs[8] = {};  // generate, extend, shade, shadow, accumulate, resolve, guide, train
     uint32_t rays[dev::kMaxBounceLevels] = {};
     uint32_t shadowRays[dev::kMaxBounceLevels] = {};
     uint32_t status = 0;
@@ -57,6 +59,11 @@ public:
     void setProfiling(bool on) { profiling_ = on; }
     const FrameStats& stats() const { return stats_; }
     void exportScene(mrtxSceneExport& out);
+    void setGuiding(const mrtxGuidingOptions& o);
+    void getGuiding(mrtxGuidingOptions& o) const { o = guide_; }
+    void guidingStats(mrtxGuidingStats& o);
+    void getGuideParams(float* dst);
+    void setGuideParams(const float* src);
     int device() const { return device_; }
     hipStream_t stream() const { return stream_; }
 
@@ -83,6 +90,11 @@ private:
     dev::PathView pathView() const;
     void invalidateGraph();
     void chooseShapes();
+    bool guideActive() const { return guide_.enableSampling || guide_.enableTraining; }
+    uint32_t guideLevels() const { return std::min(settings_.maxBounces, 16u) + 1; }
+    void ensureGuideNet();
+    void allocGuide();
+    void enqueueGuideTraining(hipStream_t s);
 
     int device_ = 0;
     int cuCount_ = 0;
@@ -123,6 +135,16 @@ private:
     dev::TraceShape extendShape_{ 128, 16, 0 };
     dev::TraceShape shadowShape_{ 128, 16, 0 };
     uint32_t shadeGrid_ = 0;
+    uint32_t shadeGuideGrid_ = 0;
+
+    // path guiding
+    mrtxGuidingOptions guide_{};
+    std::unique_ptr<GuideNetwork> guideNet_;
+    DeviceBuffer dGuideIdx_, dGuideRec_, dGuideMix_, dPathRec_, dGuidePool_;
+    uint32_t guidePaths_ = 0, guideLevels_ = 0;
+    uint32_t guideStamp_ = 0;
+    float sceneBoxMin_[3] = { 0.0f, 0.0f, 0.0f };
+    float sceneBoxScale_ = 1.0f;
 
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graphExec_ = nullptr;

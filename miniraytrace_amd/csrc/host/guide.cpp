@@ -1,0 +1,100 @@
+// GuideNetwork: device buffers of the path-guiding network.
+#include "guide.h"
+
+#include <cmath>
+#include <random>
+#include <vector>
+
+namespace mrt {
+
+namespace {
+
+constexpr size_t kParamBytes = size_t(dev::kGuideParamCount) * sizeof(float);
+
+// Layer shapes of the flat parameter vector: W [out x in] then b [out].
+constexpr int kLayerIn[5] = { 78, 64, 64, 64, 64 };
+constexpr int kLayerOut[5] = { 64, 64, 64, 64, 40 };
+
+} // namespace
+
+GuideNetwork::GuideNetwork(uint32_t cap) {
+    params_.allocate(kParamBytes);
+    m_.allocate(kParamBytes);
+    v_.allocate(kParamBytes);
+    grad_.allocate(kParamBytes);
+    words_.allocate(dev::kGwWords * sizeof(uint32_t));
+    grad_.zero();
+    words_.zero();
+    allocWorkspace(cap);
+    resetAdam();
+    MRT_HIP(hipDeviceSynchronize());
+}
+
+void GuideNetwork::allocWorkspace(uint32_t cap) {
+    if (cap == 0) throw Error(MRT_ERROR_INVALID_ARGUMENT, "guide batch capacity must be > 0");
+    // bf16 activations: cap x 80 inputs + 4 x cap x 64 hidden.
+    acts_.allocate(size_t(cap) * (dev::kGuideInputsPadded + 4 * 64) * sizeof(uint16_t));
+    raw_.allocate(size_t(cap) * dev::kGuideOutputs * sizeof(float));
+    delta_[0].allocate(size_t(cap) * 64 * sizeof(uint16_t));
+    delta_[1].allocate(size_t(cap) * 64 * sizeof(uint16_t));
+    cap_ = cap;
+}
+
+void GuideNetwork::ensureCap(uint32_t cap) {
+    if (cap > cap_) allocWorkspace(cap);
+}
+
+void GuideNetwork::initParams(uint32_t seed) {
+    std::vector<float> p(dev::kGuideParamCount, 0.0f);
+    std::mt19937_64 rng(0x5EEDull ^ (uint64_t(seed) * 0x9E3779B97F4A7C15ull));
+    size_t off = 0;
+    for (int l = 0; l < 5; ++l) {
+        const float lim = std::sqrt(6.0f / float(kLayerIn[l])) * (l == 4 ? 0.1f : 1.0f);
+        std::uniform_real_distribution<float> u(-lim, lim);
+        for (int i = 0; i < kLayerIn[l] * kLayerOut[l]; ++i) p[off++] = u(rng);
+        off += kLayerOut[l];  // zero biases
+    }
+    setParams(p.data());
+}
+
+void GuideNetwork::setParams(const float* src) {
+    params_.upload(src, kParamBytes);
+    resetAdam();
+}
+
+void GuideNetwork::getParams(float* dst) const { params_.download(dst, kParamBytes); }
+
+void GuideNetwork::getAdam(float* m, float* v, uint32_t* step) const {
+    if (m) m_.download(m, kParamBytes);
+    if (v) v_.download(v, kParamBytes);
+    if (step) words_.download(step, sizeof(uint32_t), dev::kGwStep * sizeof(uint32_t));
+}
+
+void GuideNetwork::resetAdam() {
+    m_.zero();
+    v_.zero();
+    grad_.zero();
+    words_.zero();
+    MRT_HIP(hipStreamSynchronize(nullptr));
+}
+
+void GuideNetwork::readWords(uint32_t out[dev::kGwWords], hipStream_t s) const {
+    words_.download(out, dev::kGwWords * sizeof(uint32_t), 0, s);
+}
+
+dev::GuideNet GuideNetwork::view() const {
+    dev::GuideNet n;
+    n.params = params_.as<float>();
+    n.adamM = m_.as<float>();
+    n.adamV = v_.as<float>();
+    n.grad = grad_.as<float>();
+    n.acts = acts_.as<uint16_t>();
+    n.raw = raw_.as<float>();
+    n.delta[0] = delta_[0].as<uint16_t>();
+    n.delta[1] = delta_[1].as<uint16_t>();
+    n.words = words_.as<uint32_t>();
+    n.cap = cap_;
+    return n;
+}
+
+} // namespace mrt

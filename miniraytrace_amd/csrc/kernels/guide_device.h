@@ -81,9 +81,20 @@ __device__ __forceinline__ float mixPdf(const GuideMix& m, V3 w) {
     return p;
 }
 
-// Lobe by u0 over the cumulative weights, then a vMF direction (Wenzel
-// Jakob, "Numerically stable sampling of the von Mises-Fisher distribution
-// on S^2", 2012) from (u1, u2).
+// Direction from vMF(mu, kappa) by (u1, u2) (Wenzel Jakob, "Numerically
+// stable sampling of the von Mises-Fisher distribution on S^2", 2012).
+__device__ __forceinline__ V3 vmfSample(V3 mu, float kap, float u1, float u2) {
+    float wz = 1.0f + log1pf((1.0f - u1) * expm1f(-2.0f * kap)) / kap;
+    wz = clampf(wz, -1.0f, 1.0f);
+    const float sz = sqrtf(fmaxf(1.0f - wz * wz, 0.0f));
+    float sp, cp;
+    sincosf(kTwoPi * u2, &sp, &cp);
+    V3 t, b;
+    onb(mu, t, b);
+    return normalize(t * (sz * cp) + b * (sz * sp) + mu * wz);
+}
+
+// Lobe by u0 over the cumulative weights, then a vMF direction from (u1, u2).
 __device__ __forceinline__ V3 mixSample(const GuideMix& m, float u0, float u1, float u2) {
     int k = 0;
     float c = m.w[0];
@@ -91,15 +102,55 @@ __device__ __forceinline__ V3 mixSample(const GuideMix& m, float u0, float u1, f
         ++k;
         c += m.w[k];
     }
-    const float kap = m.kappa[k];
-    float wz = 1.0f + log1pf((1.0f - u1) * expm1f(-2.0f * kap)) / kap;
-    wz = clampf(wz, -1.0f, 1.0f);
-    const float sz = sqrtf(fmaxf(1.0f - wz * wz, 0.0f));
-    float sp, cp;
-    sincosf(kTwoPi * u2, &sp, &cp);
-    V3 t, b;
-    onb(m.mu[k], t, b);
-    return normalize(t * (sz * cp) + b * (sz * sp) + m.mu[k] * wz);
+    return vmfSample(m.mu[k], m.kappa[k], u1, u2);
+}
+
+// Decoded mixture as stored for the renderer, kGuideMixF4 float4 per query:
+//   [0..7]   (mu.xyz, kappa) of lobe k
+//   [8..9]   selection weights w_0..w_7
+//   [10..11] pdf prefactors w_k kappa_k / (2 pi (1 - e^{-2 kappa_k}))
+constexpr int kGuideMixF4 = 12;
+
+__device__ __forceinline__ void storeMix(const GuideMix& m, float4* __restrict__ out) {
+    float pre[kGuideLobes];
+    for (int k = 0; k < kGuideLobes; ++k) {
+        out[k] = f4(m.mu[k], m.kappa[k]);
+        pre[k] = m.w[k] * m.kappa[k] / (kTwoPi * -expm1f(-2.0f * m.kappa[k]));
+    }
+    out[8] = make_float4(m.w[0], m.w[1], m.w[2], m.w[3]);
+    out[9] = make_float4(m.w[4], m.w[5], m.w[6], m.w[7]);
+    out[10] = make_float4(pre[0], pre[1], pre[2], pre[3]);
+    out[11] = make_float4(pre[4], pre[5], pre[6], pre[7]);
+}
+
+__device__ __forceinline__ float storedMixPdf(const float4* __restrict__ mix, V3 w) {
+    const float4 a = mix[10], b = mix[11];
+    const float pre[kGuideLobes] = { a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w };
+    float p = 0.0f;
+    for (int k = 0; k < kGuideLobes; ++k) {
+        const float4 l = mix[k];
+        p += pre[k] * expf(l.w * (l.x * w.x + l.y * w.y + l.z * w.z - 1.0f));
+    }
+    return p;
+}
+
+__device__ __forceinline__ V3 storedMixSample(const float4* __restrict__ mix, float u0, float u1, float u2) {
+    const float4 a = mix[8], b = mix[9];
+    const float w[kGuideLobes] = { a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w };
+    // Unrolled scan with constant indices keeps w in registers.
+    int k = kGuideLobes - 1;
+    float c = 0.0f;
+    bool found = false;
+#pragma unroll
+    for (int j = 0; j < kGuideLobes - 1; ++j) {
+        c += w[j];
+        if (!found && u0 < c) {
+            k = j;
+            found = true;
+        }
+    }
+    const float4 l = mix[k];
+    return vmfSample(v3(l), l.w, u1, u2);
 }
 
 } // namespace dev

@@ -1,5 +1,14 @@
 // Host-visible launchers of the path-guiding network kernels (guide.hip).
+//
+// Record formats (float4 rows):
+//   query record     (pos01.xyz, -), (n.xyz, -), (wo.xyz, -)            stride >= 3
+//   training record  (pos01.xyz, weight), (n.xyz, -), (wo.xyz, -), (wi.xyz, -)
+// pos01 is the position normalised to the scene box, n the shading normal
+// facing wo, wo the direction towards the previous vertex and wi the sampled
+// incident direction, all in world space.
 #pragma once
+
+#include "common/gpu_types.h"
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -10,42 +19,100 @@ namespace dev {
 constexpr int kGuideParamCount = 20136;  // flat FP32 parameter vector
 constexpr int kGuideOutputs = 40;        // raw outputs per query
 constexpr int kGuideInputsPadded = 80;   // bf16 features per row (78 used)
+constexpr int kGuideMixFloat4 = 12;      // decoded mixture per query (see guide_device.h)
 
-// Buffers of one training step over `count` records.
-//   rec       recStride float4 per record: (pos01, weight), (n, 0), (wo, 0), (wi, 0)
-//   acts      bf16: count x 80 (encoded inputs) then 4 x count x 64 (hidden activations)
-//   raw       FP32 count x 40 raw outputs
-//   delta[2]  bf16 count x 64 each (ping-pong of the back-propagated error)
-//   grad      FP32 gradient accumulator (zero on entry; Adam clears it)
-//   scratch   FP32 [0] = sum of weights, [1] = loss (both cleared by the caller)
-struct GuideTrainArgs {
-    const float4* rec = nullptr;
-    uint32_t recStride = 4;
-    uint32_t count = 0;
-    uint32_t grid = 1024;
-    float* params = nullptr;
-    float* adamM = nullptr;
-    float* adamV = nullptr;
-    float* grad = nullptr;
-    uint16_t* acts = nullptr;
-    float* raw = nullptr;
-    uint16_t* delta[2] = { nullptr, nullptr };
-    float* scratch = nullptr;
-    float lr = 1e-3f, beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
-    float biasCorr1 = 0.1f, biasCorr2 = 0.001f;  // 1 - beta^t
+// Device state words of one network (uint32 / float bit patterns).
+enum GuideWord : uint32_t {
+    kGwWeightSum = 0,  // f32: sum of the current batch's weights
+    kGwLoss = 1,       // f32: normalised loss of the current batch
+    kGwBatch = 2,      // rows in the current batch
+    kGwStep = 3,       // Adam steps taken (t)
+    kGwLastLoss = 4,   // f32: loss of the last Adam step
+    kGwLastBatch = 5,  // rows of the last Adam step
+    kGwCand0 = 6,      // positive-weight records found, per stamp parity (2 words)
+    kGwPool = 8,       // records appended to the training pool this frame
+    kGwWords = 16,
 };
 
-// Network outputs for the first *countPtr records (countPtr is device memory,
-// at most maxCount).  rec rows: (pos01, .), (n, .), (wo, .).
-hipError_t launchGuideInfer(const float4* rec, uint32_t recStride, const uint32_t* countPtr, uint32_t maxCount,
-                            const float* params, float* out, uint32_t grid, hipStream_t s);
+// Rows i = 0..count-1 of a record array: base + (phase + i * period) * stride.
+struct RecView {
+    const float4* base = nullptr;
+    uint32_t stride = 4;
+    uint32_t phase = 0;
+    uint32_t period = 1;
+};
 
-// Forward + backward + Adam over one batch.
-hipError_t launchGuideTrainStep(const GuideTrainArgs& a, hipStream_t s);
+// One training batch drawn from a pool of training records whose size lives
+// in device memory: rows phase, phase + period, ... of the first
+// min(*total, poolCap) records, at most cap of them.
+struct GuideBatch {
+    const float4* pool = nullptr;
+    const uint32_t* total = nullptr;
+    uint32_t poolCap = 0;
+    uint32_t phase = 0;
+    uint32_t period = 1;
+    uint32_t cap = 0;
+};
 
-// Training-style forward only (activations and raw outputs), for tests.
-hipError_t launchGuideForwardBatch(const float4* rec, uint32_t recStride, uint32_t count, const float* params,
-                                   uint16_t* acts, float* raw, uint32_t grid, hipStream_t s);
+// Network parameters, Adam moments and the workspace for batches of up to
+// `cap` rows.
+struct GuideNet {
+    float* params = nullptr;   // kGuideParamCount
+    float* adamM = nullptr;
+    float* adamV = nullptr;
+    float* grad = nullptr;     // zero between steps (Adam clears it)
+    uint16_t* acts = nullptr;  // bf16: cap x 80 inputs, then 4 x cap x 64 hidden activations
+    float* raw = nullptr;      // cap x 40
+    uint16_t* delta[2] = { nullptr, nullptr };  // bf16 cap x 64 each
+    uint32_t* words = nullptr; // kGwWords
+    uint32_t cap = 0;
+};
+
+struct GuideAdam {
+    float lr = 1e-3f, beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+};
+
+// Network outputs for the first *countPtr rows of rec (countPtr in device
+// memory, at most maxCount).  decoded: kGuideMixFloat4 float4 per row
+// (the renderer's format); otherwise kGuideOutputs raw floats per row.
+hipError_t launchGuideInfer(const RecView& rec, const uint32_t* countPtr, uint32_t maxCount, const float* params,
+                            void* out, bool decoded, hipStream_t s);
+
+// grad += d(loss)/d(params) of one batch, loss = sum w (-log q) / sum w.
+// countStep: advance the Adam step counter when the batch is not empty.
+hipError_t launchGuideGradient(const GuideBatch& b, const GuideNet& net, bool countStep, hipStream_t s);
+
+// Adam update from grad (skipped when the last batch was empty); clears grad.
+hipError_t launchGuideAdam(const GuideNet& net, const GuideAdam& h, hipStream_t s);
+
+// Decodes raw outputs and samples the mixture with the given uniforms:
+// out row i = (direction.xyz, mixture pdf of that direction).
+hipError_t launchGuideSample(const float* raw, const float* u, uint32_t count, float4* out, hipStream_t s);
+
+// Mixture pdf of given directions: raw outputs + dirs (float4 xyz) -> pdf.
+hipError_t launchGuidePdf(const float* raw, const float4* dirs, uint32_t count, float* out, hipStream_t s);
+
+// Per-frame training-pool bookkeeping (clears this frame's counters).
+hipError_t launchGuideFrameBegin(uint32_t* words, const FrameConsts* g, hipStream_t s);
+
+// Path records written by the shade kernel, per bounce level and path:
+//   (pos01.xyz, luminance of the throughput before the vertex),
+//   (n.xyz, luminance of the path radiance when the vertex was shaded),
+//   (wo.xyz, FrameConsts::guideStamp as bits), (wi.xyz, -)
+struct GuideCollectArgs {
+    const float4* pathRec = nullptr;  // [level][path] x 4 float4
+    const float4* radiance = nullptr; // per path: rgb radiance of this frame
+    uint32_t paths = 0;
+    uint32_t levels = 0;
+    float4* pool = nullptr;           // training records
+    uint32_t poolCap = 0;
+    uint32_t* words = nullptr;
+    const FrameConsts* g = nullptr;
+};
+// Turns path records into weighted training records (weight = luminance of
+// the radiance gathered after the vertex / throughput luminance before it),
+// keeping a random subset sized to the pool.
+hipError_t launchGuideCollect(const GuideCollectArgs& a, hipStream_t s);
 
 } // namespace dev
 } // namespace mrt

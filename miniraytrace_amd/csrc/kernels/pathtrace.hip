@@ -17,6 +17,7 @@
 // random numbers follow the reference's draw order exactly (pcg4d keyed by
 // pixel, frame and a per-path counter), so a path consumes the same random
 // numbers as the reference megakernel would.
+#include "guide_device.h"
 #include "pt_device.h"
 #include "pt_launch.h"
 #include "trace_scene.h"
@@ -154,7 +155,54 @@ __global__ __launch_bounds__(BLOCK) void kExtend(SceneView sc, PathView pv,
     if (overflow) atomicOr(pv.counters + kStatusWord, 1u);
 }
 
+// ------------------------------------------------------------- guide prep --
+// For every hit of ray queue `bounce` on a surface that may be guided
+// (opaque, roughness >= guideMinRoughness): a query record (pos01, n, wo) at
+// a compacted row, and the row number in guideIdx (~0u elsewhere).
+__global__ __launch_bounds__(256) void kGuidePrep(SceneView sc, PathView pv, const FrameConsts* __restrict__ gp,
+                                                  uint32_t bounce) {
+    uint32_t* cnt = pv.counters + bounce * kCntPerBounce;
+    const uint32_t count = cnt[kCntRays];
+    const float4* rayD = pv.rayD[bounce & 1];
+    const float minRough = gp->guideMinRoughness;
+    const V3 boxMin = v3(gp->guideBoxMin), boxScale = v3(gp->guideBoxScale);
+    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
+        const uint32_t k = base + threadIdx.x;
+        bool guide = false;
+        V3 pos01, n, wo;
+        if (k < count) {
+            const float4 h = pv.hit[k];
+            const uint32_t tri = __float_as_uint(h.w);
+            if (tri != kNoHit) {
+                const V3 dir = v3(rayD[k]);
+                const Surface s = makeSurface(sc, dir, h.y, h.z, tri, pv.hitInst[k]);
+                const Bsdf b = makeBsdf(sc, s);
+                guide = b.transmission <= 0.0f && b.roughness >= minRough;
+                pos01 = (s.pos - boxMin) * boxScale;
+                n = s.nShade;
+                wo = -dir;
+            }
+        }
+        const uint32_t row = waveAppend(cnt + kCntGuideRecords, guide);
+        if (k < count) pv.guideIdx[k] = guide ? row : kNoHit;
+        if (guide) {
+            float4* r = pv.guideRec + size_t(row) * 3;
+            r[0] = f4(pos01, 0.0f);
+            r[1] = f4(n, 0.0f);
+            r[2] = f4(wo, 0.0f);
+        }
+    }
+}
+
 // ------------------------------------------------------------------- shade --
+// GUIDE: the prep pass ran for this bounce.  At a guided vertex with
+// FrameConsts::guiding set, the continuation direction comes from the learned
+// mixture with probability guideSelectProb and from the BSDF otherwise
+// (one-sample MIS: the path weight uses the combined pdf, which NEE's MIS
+// weight uses too).  With guideTrain set, every guided vertex that continues
+// writes a training record.  Without GUIDE (or with neither flag) the kernel
+// performs exactly the reference's operations.
+template <bool GUIDE>
 __global__ __launch_bounds__(256) void kShade(SceneView sc, PathView pv, const FrameConsts* __restrict__ gp,
                                               uint32_t bounce) {
     const FrameConsts g = *gp;
@@ -213,6 +261,19 @@ __global__ __launch_bounds__(256) void kShade(SceneView sc, PathView pv, const F
                     V3 wo = v3(dot(-dir, T), dot(-dir, B), dot(-dir, s.nShade));
                     if (wo.z < 1e-6f) wo.z = 1e-6f;
                     wo = normalize(wo);
+                    const float4* mix = nullptr;  // learned mixture of this vertex
+                    float sp = 0.0f;
+                    bool record = false;
+                    if (GUIDE) {
+                        const uint32_t gi = pv.guideIdx[k];
+                        if (gi != kNoHit) {
+                            record = g.guideTrain != 0;
+                            if (g.guiding) {
+                                mix = pv.guideMix + size_t(gi) * kGuideMixF4;
+                                sp = g.guideSelectProb;
+                            }
+                        }
+                    }
                     // NEE (pathtrace.comp:181-197)
                     const LightSample ls = sampleOneLight(sc, g, s.pos, rng);
                     if (ls.valid && ls.pdf > 1e-9f) {
@@ -221,7 +282,9 @@ __global__ __launch_bounds__(256) void kShade(SceneView sc, PathView pv, const F
                         float bsdfPdf;
                         bsdfEval(bsdf, wo, wi, f, bsdfPdf);
                         if (maxComp(f) > 0.0f) {
-                            const float w = ls.delta ? 1.0f : balanceHeuristic(ls.pdf, bsdfPdf);
+                            float contPdf = bsdfPdf;
+                            if (GUIDE && mix) contPdf = sp * storedMixPdf(mix, ls.wi) + (1.0f - sp) * bsdfPdf;
+                            const float w = ls.delta ? 1.0f : balanceHeuristic(ls.pdf, contPdf);
                             shO = s.pos + s.nGeom * (wi.z > 0.0f ? kEpsRay : -kEpsRay);
                             shD = ls.wi;
                             shMax = ls.dist - 2.0f * kEpsRay;
@@ -231,11 +294,36 @@ __global__ __launch_bounds__(256) void kShade(SceneView sc, PathView pv, const F
                     }
                     // BSDF bounce and Russian roulette (pathtrace.comp:200-216)
                     if (!last) {
-                        const BsdfSample bs = bsdfSample(bsdf, wo, rng);
-                        if (bs.valid) {
-                            const V3 wiW = normalize(T * bs.wi.x + B * bs.wi.y + s.nShade * bs.wi.z);
-                            beta = beta * bs.f * (fabsf(bs.wi.z) / bs.pdf);
-                            prevPdf = bs.pdf;
+                        V3 wiL, wiW, f;
+                        float pdf = 0.0f;
+                        bool ok;
+                        if (GUIDE && mix && rng.next1() < sp) {
+                            // Draw from the learned mixture.
+                            const float u0 = rng.next1();
+                            float u1, u2;
+                            rng.next2(u1, u2);
+                            wiW = storedMixSample(mix, u0, u1, u2);
+                            wiL = v3(dot(wiW, T), dot(wiW, B), dot(wiW, s.nShade));
+                            float pB;
+                            bsdfEval(bsdf, wo, wiL, f, pB);
+                            pdf = sp * storedMixPdf(mix, wiW) + (1.0f - sp) * pB;
+                            ok = maxComp(f) > 0.0f && pdf > 1e-9f;
+                        } else {
+                            const BsdfSample bs = bsdfSample(bsdf, wo, rng);
+                            ok = bs.valid;
+                            wiL = bs.wi;
+                            f = bs.f;
+                            pdf = bs.pdf;
+                            if (ok) wiW = normalize(T * bs.wi.x + B * bs.wi.y + s.nShade * bs.wi.z);
+                            if (GUIDE && mix && ok) {
+                                pdf = sp * storedMixPdf(mix, wiW) + (1.0f - sp) * bs.pdf;
+                                ok = pdf > 1e-9f;
+                            }
+                        }
+                        if (ok) {
+                            const float lumBeta = luminance(beta);
+                            beta = beta * f * (fabsf(wiL.z) / pdf);
+                            prevPdf = pdf;
                             bool alive = true;
                             if (bounce >= 3) {
                                 const float q = clampf(maxComp(beta), 0.05f, 0.95f);
@@ -243,9 +331,16 @@ __global__ __launch_bounds__(256) void kShade(SceneView sc, PathView pv, const F
                                 else beta = beta * (1.0f / q);
                             }
                             if (alive) {
-                                newO = s.pos + s.nGeom * (bs.wi.z > 0.0f ? kEpsRay : -kEpsRay);
+                                newO = s.pos + s.nGeom * (wiL.z > 0.0f ? kEpsRay : -kEpsRay);
                                 newD = wiW;
                                 emitRay = true;
+                                if (GUIDE && record) {
+                                    float4* r = pv.pathRec + (size_t(bounce) * pv.capacity + p) * 4;
+                                    r[0] = f4((s.pos - v3(g.guideBoxMin)) * v3(g.guideBoxScale), lumBeta);
+                                    r[1] = f4(s.nShade, luminance(L));
+                                    r[2] = f4(-dir, __uint_as_float(g.guideStamp));
+                                    r[3] = f4(wiW, 0.0f);
+                                }
                             }
                         }
                     }
@@ -296,6 +391,12 @@ __global__ __launch_bounds__(BLOCK) void kShadow(SceneView sc, PathView pv,
                 L.y += c.y;
                 L.z += c.z;
                 pv.radiance[p] = L;
+                if (pv.pathRec) {
+                    // NEE from a recorded vertex is not incident along its wi:
+                    // move the record's radiance snapshot past it.
+                    float4* r = pv.pathRec + (size_t(bounce) * pv.capacity + p) * 4;
+                    if (__float_as_uint(r[2].w) == gp->guideStamp) r[1].w += luminance(v3(c));
+                }
             }
         }
     }
@@ -412,8 +513,15 @@ hipError_t launchShadow(const SceneView& sc, const PathView& pv, const FrameCons
 }
 
 hipError_t launchShade(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t bounce,
-                       uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL(kShade, dim3(grid), dim3(256), 0, s, sc, pv, g, bounce);
+                       uint32_t grid, bool guide, hipStream_t s) {
+    if (guide) hipLaunchKernelGGL(kShade<true>, dim3(grid), dim3(256), 0, s, sc, pv, g, bounce);
+    else hipLaunchKernelGGL(kShade<false>, dim3(grid), dim3(256), 0, s, sc, pv, g, bounce);
+    return hipGetLastError();
+}
+
+hipError_t launchGuidePrep(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t bounce,
+                           uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(kGuidePrep, dim3(grid), dim3(256), 0, s, sc, pv, g, bounce);
     return hipGetLastError();
 }
 
@@ -449,8 +557,9 @@ hipError_t persistentGrid(uint32_t block, uint32_t stack, bool shadow, int* bloc
     return hipErrorInvalidValue;
 }
 
-hipError_t shadeBlocksPerCU(int* blocksPerCU) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kShade, 256, 0);
+hipError_t shadeBlocksPerCU(bool guide, int* blocksPerCU) {
+    if (guide) return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kShade<true>, 256, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kShade<false>, 256, 0);
 }
 
 } // namespace dev

@@ -25,8 +25,12 @@ hipError_t launchClearWords(uint32_t* words, uint32_t n, hipStream_t s);
 hipError_t launchGenerate(const PathView& pv, const FrameConsts* g, uint32_t pixels, hipStream_t s);
 hipError_t launchExtend(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t bounce,
                         const TraceShape& shape, hipStream_t s);
+// guide: the guiding variant (needs the prep pass of this bounce).
 hipError_t launchShade(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t bounce,
-                       uint32_t grid, hipStream_t s);
+                       uint32_t grid, bool guide, hipStream_t s);
+// Query records of guidable hits for the guiding network (grid-stride).
+hipError_t launchGuidePrep(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t bounce,
+                           uint32_t grid, hipStream_t s);
 hipError_t launchShadow(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t bounce,
                         const TraceShape& shape, hipStream_t s);
 hipError_t launchAccumulate(const PathView& pv, const FrameConsts* g, uint32_t pixels, hipStream_t s);
@@ -35,7 +39,7 @@ hipError_t launchResolve32(const PathView& pv, uint32_t pixels, uint32_t spp, hi
 
 bool ptShapeSupported(uint32_t block, uint32_t stack);
 hipError_t persistentGrid(uint32_t block, uint32_t stack, bool shadow, int* blocksPerCU);
-hipError_t shadeBlocksPerCU(int* blocksPerCU);
+hipError_t shadeBlocksPerCU(bool guide, int* blocksPerCU);
 
 } // namespace dev
 } // namespace mrt

@@ -58,6 +58,11 @@ struct PathView {
     float4* accum;                   // rgb sum + sample count, per pixel
     uint32_t* outRgba8;              // resolved image
     float4* outRgba32;               // accum / spp, alpha 1
+    // Path guiding (null when off).
+    uint32_t* guideIdx;              // per queue slot: row of guideRec / guideMix, or ~0u
+    float4* guideRec;                // 3 float4 per guided row: (pos01), (n), (wo)
+    float4* guideMix;                // 12 float4 per guided row: decoded mixture
+    float4* pathRec;                 // [level][path] x 4 float4 training records
 };
 
 } // namespace dev

@@ -210,13 +210,47 @@ class Engine:
                               extendStack=extend[1], shadowBlock=shadow[0], shadowStack=shadow[1])
         L.check(L.lib().mrtxSetPathOptions(self._h, C.byref(o)), "mrtxSetPathOptions")
 
+    def path_options(self) -> dict:
+        o = L.mrtxPathOptions()
+        L.check(L.lib().mrtxGetPathOptions(self._h, C.byref(o)), "mrtxGetPathOptions")
+        return {"seed": o.seed, "use_graph": bool(o.useGraph), "extend": (o.extendBlock, o.extendStack),
+                "shadow": (o.shadowBlock, o.shadowStack)}
+
+    # -- path guiding (mini_raytrace_ext.h section 5) --
+    def set_guiding(self, sampling: bool = True, training: bool = True, select_prob: float = 0.5,
+                    learning_rate: float = 1e-3, batch_size: int = 65536, steps_per_frame: int = 4,
+                    min_roughness: float = 0.2, seed: int = 1) -> None:
+        o = L.mrtxGuidingOptions(enableSampling=int(sampling), enableTraining=int(training),
+                                 selectProb=select_prob, learningRate=learning_rate, batchSize=batch_size,
+                                 stepsPerFrame=steps_per_frame, minRoughness=min_roughness, seed=seed)
+        L.check(L.lib().mrtxSetGuiding(self._h, C.byref(o)), "mrtxSetGuiding")
+
+    def guiding(self) -> dict:
+        o = L.mrtxGuidingOptions()
+        L.check(L.lib().mrtxGetGuiding(self._h, C.byref(o)), "mrtxGetGuiding")
+        return {k: getattr(o, k) for k, _ in o._fields_ if k != "structSize"}
+
+    def guiding_stats(self) -> dict:
+        o = L.mrtxGuidingStats()
+        L.check(L.lib().mrtxGetGuidingStats(self._h, C.byref(o)), "mrtxGetGuidingStats")
+        return {k: getattr(o, k) for k, _ in o._fields_ if k != "structSize"}
+
+    def guide_params(self) -> np.ndarray:
+        out = np.empty(L.MRTX_GUIDE_PARAM_COUNT, np.float32)
+        L.check(L.lib().mrtxGetGuideParams(self._h, out.ctypes.data, out.size), "mrtxGetGuideParams")
+        return out
+
+    def set_guide_params(self, params: np.ndarray) -> None:
+        p = np.ascontiguousarray(params, np.float32)
+        L.check(L.lib().mrtxSetGuideParams(self._h, p.ctypes.data, p.size), "mrtxSetGuideParams")
+
     def set_profiling(self, on: bool) -> None:
         L.check(L.lib().mrtxSetProfiling(self._h, int(on)), "mrtxSetProfiling")
 
     def frame_stats(self) -> dict:
         s = L.mrtxFrameStats()
         L.check(L.lib().mrtxGetFrameStats(self._h, C.byref(s)), "mrtxGetFrameStats")
-        names = ["generate", "extend", "shade", "shadow", "accumulate", "resolve"]
+        names = ["generate", "extend", "shade", "shadow", "accumulate", "resolve", "guide", "train"]
         return {"frameMs": s.frameMs, "gpuMs": dict(zip(names, list(s.gpuMs))),
                 "rays": list(s.rays), "shadowRays": list(s.shadowRays), "status": s.status}
 

@@ -57,6 +57,39 @@ def encode(pos01: np.ndarray, nrm: np.ndarray, wo: np.ndarray) -> np.ndarray:
     return np.concatenate([sc, nrm, wo], axis=1)
 
 
+def bf16(a: np.ndarray) -> np.ndarray:
+    """Round float32 values to bfloat16 (nearest even), returned as float64."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def encode_f32(pos01, nrm, wo) -> np.ndarray:
+    """encode() with the angle formed in float32 as p * 2^b * pi (the order
+    the device uses); sin / cos of that float32 angle in float64."""
+    p = np.asarray(pos01, np.float32)
+    freq = (np.float32(2.0) ** np.arange(BANDS, dtype=np.float32))
+    ang = (p[:, :, None] * freq[None, None, :]) * np.float32(np.pi)
+    ang = ang.astype(np.float64)
+    sc = np.stack([np.sin(ang), np.cos(ang)], axis=-1).reshape(p.shape[0], 3 * BANDS * 2)
+    return np.concatenate([sc, np.asarray(nrm, np.float64), np.asarray(wo, np.float64)], axis=1)
+
+
+def forward_bf16(theta, x):
+    """Forward with the device's rounding points: inputs, weights and hidden
+    activations rounded to bfloat16, products accumulated exactly (float64),
+    biases in float32.  Returns the raw outputs (N, 40)."""
+    h = bf16(x)
+    layers = unpack(np.asarray(theta, np.float32))
+    for li, (W, b) in enumerate(layers):
+        z = h @ bf16(W).T + b.astype(np.float64)
+        if li < len(layers) - 1:
+            h = bf16(np.maximum(z, 0.0))
+        else:
+            h = z
+    return h
+
+
 def forward(theta, x):
     """Returns (raw outputs (N,40), activations list [x, a1..a4])."""
     acts = [x]
@@ -114,7 +147,8 @@ def loss_and_grad_raw(raw, w, weight):
     # logits
     g[..., 4] = -W * (resp - pi)
     # log kappa (through the clamp)
-    dlogf_dk = 1.0 / kappa - 2.0 / np.expm1(2 * kappa) + (np.einsum("nkd,nd->nk", mu, w) - 1.0)
+    with np.errstate(over="ignore"):  # expm1(2 kappa) = inf for large kappa: the term is 0
+        dlogf_dk = 1.0 / kappa - 2.0 / np.expm1(2 * kappa) + (np.einsum("nkd,nd->nk", mu, w) - 1.0)
     inside = (aux["lk"] > KAPPA_LO) & (aux["lk"] < KAPPA_HI)
     g[..., 3] = -W * resp * dlogf_dk * kappa * inside
     # mean direction through the normalisation
@@ -141,6 +175,39 @@ def backward(theta, x, w, weight):
     grads.reverse()
     flat = np.concatenate([np.concatenate([gW.reshape(-1), gb]) for gW, gb in grads])
     return L, flat
+
+
+def _onb(n):
+    """Orthonormal basis around unit vectors n (N,3): the branchless
+    construction of Duff et al. 2017 (the device's onb())."""
+    s = np.where(n[:, 2] >= 0.0, 1.0, -1.0)
+    a = -1.0 / (s + n[:, 2])
+    c = n[:, 0] * n[:, 1] * a
+    t = np.stack([1.0 + s * n[:, 0] ** 2 * a, s * c, -s * n[:, 0]], axis=1)
+    b = np.stack([c, s + n[:, 1] ** 2 * a, -n[:, 1]], axis=1)
+    return t, b
+
+
+def vmf_sample(mu, kappa, u1, u2):
+    """vMF directions by Jakob 2012 ("Numerically stable sampling of the von
+    Mises-Fisher distribution on S^2"): cos = 1 + log1p((1-u1) expm1(-2k)) / k."""
+    wz = 1.0 + np.log1p((1.0 - u1) * np.expm1(-2.0 * kappa)) / kappa
+    wz = np.clip(wz, -1.0, 1.0)
+    sz = np.sqrt(np.maximum(1.0 - wz * wz, 0.0))
+    phi = 2.0 * np.pi * u2
+    t, b = _onb(mu)
+    d = t * (sz * np.cos(phi))[:, None] + b * (sz * np.sin(phi))[:, None] + mu * wz[:, None]
+    return d / np.linalg.norm(d, axis=1, keepdims=True)
+
+
+def mixture_sample(raw, u):
+    """Lobe k = first index with u0 < cumulative weight (last lobe otherwise),
+    then vmf_sample with (u1, u2).  Returns (directions (N,3), lobe (N,))."""
+    mu, kappa, pi, _ = decode(raw)
+    cum = np.cumsum(pi, axis=1)
+    k = np.minimum((u[:, :1] >= cum[:, :-1]).sum(axis=1), N_LOBES - 1)
+    rows = np.arange(raw.shape[0])
+    return vmf_sample(mu[rows, k], kappa[rows, k], u[:, 1], u[:, 2]), k
 
 
 def adam_step(theta, m, v, g, t, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):

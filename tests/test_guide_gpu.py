@@ -1,0 +1,283 @@
+"""Path-guiding network and guided rendering on the GPU, against the FP64
+oracle (oracle/guide.py) and against plain path tracing.
+
+Network kernels (through the C ABI's standalone network object):
+* forward: equal to the oracle's bf16-rounding forward to FP32 accumulation
+  precision (pins the MFMA operand layouts); within bf16 error of the exact
+  FP64 network;
+* gradient of the weighted NLL: per layer within bf16 error of the FP64
+  analytic gradient (itself pinned to torch autograd);
+* Adam: the first two updates equal the closed form on the GPU's gradient;
+* training on a synthetic target lowers the loss like FP64 training does;
+* mixture sampling: same directions as the oracle's sampler for the same
+  uniforms, pdf equal to the oracle's mixture pdf.
+Rendering:
+* training without guided sampling renders bit-identically to plain;
+* guided sampling is unbiased (block means agree with plain within noise);
+* guided sampling lowers the error at equal sample count on a room lit
+  through a window.
+"""
+import numpy as np
+import pytest
+
+from oracle import guide as G
+
+pytestmark = pytest.mark.gpu
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def gpu(torch_gpu):
+    return torch_gpu
+
+
+def _unit(rng, n):
+    v = rng.normal(size=(n, 3))
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+def _records(n, seed):
+    from miniraytrace_amd.guide import make_records
+    rng = np.random.default_rng(seed)
+    pos = rng.random((n, 3)).astype(np.float32)
+    nrm, wo, wi = _unit(rng, n), _unit(rng, n), _unit(rng, n)
+    w = rng.exponential(size=n)
+    return make_records(pos, nrm, wo, wi, w)
+
+
+def _theta(seed):
+    th = G.init_params(seed)
+    rng = np.random.default_rng(seed + 99)
+    for W, b in G.unpack(th):
+        b[:] = rng.normal(scale=0.1, size=b.shape)
+    return th.astype(np.float32).astype(np.float64)
+
+
+def _x(rec):
+    return G.encode_f32(rec[:, 0:3], rec[:, 4:7], rec[:, 8:11])
+
+
+@pytest.mark.parametrize("n", [1, 31, 33, 1000, 70_000])
+def test_forward_matches_oracle(gpu, n):
+    from miniraytrace_amd.guide import GuideNetwork
+    _print(f"case: guide forward n={n}")
+    th = _theta(3)
+    rec = _records(n, 4)
+    with GuideNetwork(max_batch=max(n, 256)) as net:
+        net.params = th
+        raw = net.infer(rec).astype(np.float64)
+    ref_bf = G.forward_bf16(th, _x(rec))
+    ref64, _ = G.forward(th, _x(rec))
+    scale = np.abs(ref64).std() + 1e-6
+    d_bf = np.abs(raw - ref_bf) / scale
+    d64 = np.abs(raw - ref64) / scale
+    _print(f"|gpu - bf16 oracle|/std: max {d_bf.max():.2e} p99 {np.percentile(d_bf, 99):.2e}; "
+           f"|gpu - fp64|/std: max {d64.max():.2e}")
+    # Tolerances: FP32 accumulation vs exact sums, plus rare one-ulp bf16
+    # flips of an activation (sinf/cosf vs exact sin of the same argument).
+    assert np.percentile(d_bf, 99) < 2e-3
+    assert d_bf.max() < 3e-2
+    # bf16 operands vs the exact FP64 network
+    assert d64.max() < 0.15 and np.percentile(d64, 99) < 0.05
+
+
+def test_gradient_matches_oracle(gpu):
+    from miniraytrace_amd.guide import GuideNetwork
+    _print("case: guide gradient")
+    th = _theta(5)
+    rec = _records(3000, 6)
+    with GuideNetwork(max_batch=4096) as net:
+        net.params = th
+        loss, g = net.gradient(rec)
+    w = rec[:, 3].astype(np.float64)
+    L, gref = G.backward(th, _x(rec), rec[:, 12:15].astype(np.float64), w)
+    gref /= w.sum()
+    assert loss == pytest.approx(L / w.sum(), rel=2e-2)
+    off = 0
+    for li, (i, o) in enumerate(G.LAYERS):
+        for part, size in (("W", i * o), ("b", o)):
+            a, b = g[off: off + size].astype(np.float64), gref[off: off + size]
+            rel = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
+            cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30)
+            _print(f"layer {li} {part}: rel L2 err {rel:.3e} cos {cos:.6f}")
+            # bf16 activations and deltas: ~1 % relative error per layer
+            assert rel < 0.06 and cos > 0.998, (li, part)
+            off += size
+
+
+def test_adam_matches_closed_form(gpu):
+    from miniraytrace_amd.guide import GuideNetwork
+    _print("case: guide adam")
+    th = _theta(7).astype(np.float32)
+    rec1, rec2 = _records(2048, 8), _records(2048, 9)
+    lr = 1e-3
+    with GuideNetwork(max_batch=2048) as net:
+        net.params = th
+        _, g1 = net.gradient(rec1)
+        net.train_step(rec1, lr)
+        p1 = net.params
+        m1, v1, t1 = net.adam_state()
+        _, g2 = net.gradient(rec2)
+        net.train_step(rec2, lr)
+        p2 = net.params
+        m2, v2, t2 = net.adam_state()
+    g1, g2 = g1.astype(np.float64), g2.astype(np.float64)
+    assert (t1, t2) == (1, 2)
+    np.testing.assert_allclose(m1, 0.1 * g1, rtol=1e-4, atol=1e-12)
+    np.testing.assert_allclose(v1, 0.001 * g1 ** 2, rtol=1e-3, atol=1e-20)
+    e1, _, _ = G.adam_step(th.astype(np.float64), 0, 0, g1, 1, lr=lr)
+    # the update is lr * g / (|g| + eps): compare steps, not parameters
+    np.testing.assert_allclose(p1 - th, e1 - th, rtol=1e-3, atol=2e-7)
+    e2, em, ev = G.adam_step(p1.astype(np.float64), m1.astype(np.float64), v1.astype(np.float64), g2, 2, lr=lr)
+    np.testing.assert_allclose(m2, em, rtol=1e-4, atol=1e-12)
+    np.testing.assert_allclose(p2 - p1, e2 - p1, rtol=2e-3, atol=2e-7)
+
+
+def _synthetic_batch(rng, n):
+    """Incident directions from one vMF lobe whose mean depends on position."""
+    from miniraytrace_amd.guide import make_records
+    pos = rng.random((n, 3))
+    mu = np.stack([np.cos(4 * pos[:, 0]), 0.5 + pos[:, 1], np.sin(4 * pos[:, 2])], axis=1)
+    mu /= np.linalg.norm(mu, axis=1, keepdims=True)
+    wi = G.vmf_sample(mu, np.full(n, 20.0), rng.random(n), rng.random(n))
+    nrm = np.tile([0.0, 1.0, 0.0], (n, 1))
+    wo = _unit(rng, n)
+    return make_records(pos, nrm, wo, wi, np.ones(n))
+
+
+def test_training_matches_oracle_training(gpu):
+    from miniraytrace_amd.guide import GuideNetwork
+    _print("case: guide training vs FP64 training")
+    steps, n, lr = 150, 2048, 3e-3
+    rng = np.random.default_rng(21)
+    batches = [_synthetic_batch(rng, n) for _ in range(steps)]
+    th0 = G.init_params(2)
+    th0[G.param_count() - 40 * 65 : G.param_count() - 40] *= 0.1  # broad initial lobes, as the engine does
+    with GuideNetwork(max_batch=n) as net:
+        net.params = th0
+        gpu_loss = [net.train_step(b, lr) for b in batches]
+    th, m, v = th0.astype(np.float32).astype(np.float64), 0.0, 0.0
+    ref_loss = []
+    for t, b in enumerate(batches, 1):
+        w = b[:, 3].astype(np.float64)
+        L, g = G.backward(th, _x(b), b[:, 12:15].astype(np.float64), w)
+        ref_loss.append(L / w.sum())
+        th, m, v = G.adam_step(th, m, v, g / w.sum(), t, lr=lr)
+    g0, g1 = np.mean(gpu_loss[:5]), np.mean(gpu_loss[-10:])
+    r0, r1 = np.mean(ref_loss[:5]), np.mean(ref_loss[-10:])
+    _print(f"loss first/last: gpu {g0:.4f} -> {g1:.4f}, fp64 {r0:.4f} -> {r1:.4f}")
+    assert g0 == pytest.approx(r0, rel=0.02)
+    assert g1 < g0 - 1.0 and r1 < r0 - 1.0          # both learn the lobe (loss in nats)
+    assert abs(g1 - r1) < 0.1 * abs(r0 - r1)        # and end up within 10 % of the drop
+
+
+def test_sampling_matches_oracle(gpu):
+    from miniraytrace_amd.guide import GuideNetwork
+    _print("case: guide mixture sampling vs oracle")
+    n = 50_000
+    rng = np.random.default_rng(31)
+    raw = rng.normal(size=(n, 40)).astype(np.float32)
+    raw[:, 3::5] = rng.uniform(-6, 8, size=(n, 8))
+    u = rng.random((n, 3)).astype(np.float32)
+    with GuideNetwork(max_batch=256) as net:
+        out = net.sample(raw, u)
+        dirs = _unit(rng, n).astype(np.float32)
+        pdf_gpu = net.pdf(raw, dirs)
+    raw64, u64 = raw.astype(np.float64), u.astype(np.float64)
+    d_ref, _ = G.mixture_sample(raw64, u64)
+    _, _, pi, _ = G.decode(raw64)
+    cum = np.cumsum(pi, 1)[:, :-1]
+    clear = (np.abs(u64[:, :1] - cum) > 1e-5).all(1)   # lobe choice not on a float32 tie
+    err = np.linalg.norm(out[:, :3] - d_ref, axis=1)
+    _print(f"direction error: p99 {np.percentile(err[clear], 99):.2e} max {err[clear].max():.2e}; "
+           f"{(~clear).sum()} ambiguous lobe picks skipped")
+    assert np.percentile(err[clear], 99) < 1e-4 and err[clear].max() < 5e-3
+    pdf_at = G.mixture_pdf(raw64, out[:, :3].astype(np.float64))
+    np.testing.assert_allclose(out[:, 3], pdf_at, rtol=5e-3, atol=1e-6)
+    np.testing.assert_allclose(pdf_gpu, G.mixture_pdf(raw64, dirs.astype(np.float64)), rtol=5e-3, atol=1e-6)
+
+
+# ------------------------------------------------------------------ rendering --
+def _room_engine(w, h, spp_limit, seed=1):
+    from miniraytrace_amd import scenes
+    from miniraytrace_amd.engine import Engine
+    e = Engine(width=w, height=h)
+    e.set_render_settings(width=w, height=h, max_bounces=8, spp_limit=spp_limit, firefly_clamp=0.0,
+                          denoise=False)
+    cfg = dict(scenes.CONFIG4)
+    cfg.update(n=24)
+    scenes.load_room_scene(e, cfg, seed=seed, sky_img=scenes.sky(256, 128))
+    e.commit()
+    return e
+
+
+def _render(e, frames, w, h):
+    for _ in range(frames):
+        e.render_frame()
+    return e.read_rgba32f(w, h)[..., :3].astype(np.float64)
+
+
+def test_training_only_is_plain(gpu):
+    _print("case: training without guided sampling renders like plain")
+    W, H = 64, 36
+    a = _room_engine(W, H, 64)
+    b = _room_engine(W, H, 64)
+    b.set_guiding(sampling=False, training=True, batch_size=4096, steps_per_frame=2)
+    ia, ib = _render(a, 8, W, H), _render(b, 8, W, H)
+    st = b.guiding_stats()
+    _print("stats:", st)
+    assert np.array_equal(ia, ib)
+    assert st["steps"] > 0 and st["candidates"] > 0 and st["poolRecords"] > 0
+    assert np.isfinite(st["lastLoss"])
+
+
+def test_guided_is_unbiased(gpu):
+    _print("case: guided vs plain block means")
+    W, H, F, B = 48, 27, 64, 8
+    plain = _room_engine(W, H, 10**6)
+    guided = _room_engine(W, H, 10**6)
+    guided.set_guiding(sampling=True, training=True, batch_size=4096, steps_per_frame=2)
+    _render(guided, 32, W, H)  # train a little first
+    imgs = {"plain": [], "guided": []}
+    for i in range(B):
+        for name, e in (("plain", plain), ("guided", guided)):
+            e.set_path_options(seed=1000 + 17 * i + (name == "guided"))
+            e.reset_accumulation()
+            imgs[name].append(_render(e, F, W, H).mean(-1))
+    P, Q = np.array(imgs["plain"]), np.array(imgs["guided"])
+    blk = lambda a: a.reshape(B, 9, 3, 12, 4).mean((2, 4))
+    bp, bq = blk(P), blk(Q)
+    z = (bq.mean(0) - bp.mean(0)) / np.sqrt(bq.var(0, ddof=1) / B + bp.var(0, ddof=1) / B + 1e-30)
+    gz = (Q.mean((1, 2)).mean() - P.mean((1, 2)).mean()) / np.sqrt(
+        Q.mean((1, 2)).var(ddof=1) / B + P.mean((1, 2)).var(ddof=1) / B)
+    _print(f"global z {gz:.2f}; block |z| > 4: {(np.abs(z) > 4).sum()} of {z.size}; max {np.abs(z).max():.2f}")
+    assert abs(gz) < 4.5
+    assert (np.abs(z) > 4.5).sum() <= 2
+
+
+def test_guided_lowers_error(gpu):
+    _print("case: guided vs plain error at equal samples")
+    W, H, N = 64, 36, 64
+    ref_e = _room_engine(W, H, 10**6)
+    ref_e.set_guiding(sampling=True, training=True, batch_size=4096, steps_per_frame=2)
+    _render(ref_e, 64, W, H)
+    ref_e.set_path_options(seed=777)
+    ref_e.reset_accumulation()
+    ref = _render(ref_e, 4096, W, H)
+    plain = _room_engine(W, H, 10**6)
+    plain.set_path_options(seed=5)
+    ip = _render(plain, N, W, H)
+    guided = _room_engine(W, H, 10**6)
+    guided.set_guiding(sampling=True, training=True, batch_size=4096, steps_per_frame=2)
+    _render(guided, 64, W, H)  # warm-up training
+    guided.set_path_options(seed=6)
+    guided.reset_accumulation()
+    ig = _render(guided, N, W, H)
+    mse_p = ((ip - ref) ** 2).mean()
+    mse_g = ((ig - ref) ** 2).mean()
+    _print(f"MSE at {N} spp: plain {mse_p:.4e}, guided {mse_g:.4e}, ratio {mse_g / mse_p:.3f}; "
+           f"stats {guided.guiding_stats()}")
+    assert mse_g < mse_p
