@@ -4,8 +4,13 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
 Workloads:
+  pt         (default) plain progressive path tracing at 1920x1080 on a
+             1.256M-triangle terrain scene; one step = one frame (1 spp).
+             The default run appends the traversal and guided lines below.
   traversal  closest-hit queries: 10,000-triangle seeded soup, 1,000,000
              rays (one step = one mrtxTraceClosest launch over all rays).
+  guided     equal-RMSE wall time of guided vs plain path tracing on a room
+             lit through a window (1280x720, 499k triangles).
 
 Prints ONE JSON line.  Inputs are resident in device memory before the timed
 region; the timed region is bracketed by device synchronisation.  The
@@ -227,6 +232,102 @@ def pt_workload(args, torch) -> dict:
     return out
 
 
+# ------------------------------------------------------------- guided vs plain --
+def _room_engine(W, H, seed, sky):
+    from miniraytrace_amd import scenes
+    from miniraytrace_amd.engine import Engine
+    e = Engine(width=W, height=H)
+    e.set_render_settings(width=W, height=H, max_bounces=8, spp_limit=1 << 30, firefly_clamp=0.0, denoise=False)
+    info = scenes.load_room_scene(e, scenes.CONFIG4, seed=1, sky_img=sky)
+    e.set_path_options(seed=seed)
+    e.commit()
+    return e, info
+
+
+def _timed_frames(e, n, torch) -> float:
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        e.render_frame()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def guided_workload(args, torch) -> dict:
+    """Equal-RMSE wall time, guided vs plain: a closed diffuse room (6 walls of
+    204 x 204 quads, 499,392 triangles, albedo 0.7) lit only by the sky and
+    its 0.5 degree sun through a window of 5 % of one wall, 1280x720, max
+    depth 8, no firefly clamp.  Plain renders `spp` frames; guided starts from
+    an untrained network, trains online (batch 65,536, 4 Adam steps per frame)
+    and samples the learned mixture with probability 0.5, and is timed until
+    its RMSE reaches plain's.  RMSE is taken against an independent guided
+    render of `2 spp` samples; its own error adds equally to both sides."""
+    from miniraytrace_amd import scenes
+    W, H = scenes.CONFIG4["width"], scenes.CONFIG4["height"]
+    spp = args.guided_spp
+    sky = scenes.sky()
+    plain, info = _room_engine(W, H, 11, sky)
+    _log(f"guided: room {info['triangles']} triangles, {W}x{H}, plain target {spp} spp")
+    plain.render_frame()
+    plain.reset_accumulation()
+    t_plain = _timed_frames(plain, spp, torch)
+    img_p = plain.read_rgba32f(W, H)[..., :3].astype(np.float64)
+    plain.close()
+
+    ref, _ = _room_engine(W, H, 99, sky)
+    ref.set_guiding(sampling=True, training=True)
+    _timed_frames(ref, 64, torch)
+    ref.reset_accumulation()
+    n_ref = 2 * spp
+    t_ref = _timed_frames(ref, n_ref, torch)
+    img_r = ref.read_rgba32f(W, H)[..., :3].astype(np.float64)
+    ref.close()
+    mse_p = float(((img_p - img_r) ** 2).mean())
+
+    g, _ = _room_engine(W, H, 12, sky)
+    g.set_guiding(sampling=True, training=True)
+    warm = 32
+    t_g = _timed_frames(g, warm, torch)  # training from an untrained network, samples discarded
+    g.reset_accumulation()
+    curve, done, chunk, reached = [], 0, 16, None
+    while done < spp:
+        t_g += _timed_frames(g, chunk, torch)
+        done += chunk
+        img_g = g.read_rgba32f(W, H)[..., :3].astype(np.float64)
+        mse_g = float(((img_g - img_r) ** 2).mean())
+        curve.append((done, mse_g))
+        if mse_g <= mse_p:
+            reached = done
+            break
+    st = g.guiding_stats()
+    g.set_profiling(True)
+    g.render_frame()
+    prof = g.frame_stats()
+    g.close()
+    out = {
+        "workload": f"room {info['triangles']} triangles, 5 % window, sky + 0.5 deg sun, {W}x{H}, depth 8",
+        "plain_spp": spp,
+        "plain_s": t_plain,
+        "plain_ms_per_frame": t_plain * 1e3 / spp,
+        "plain_rmse": mse_p ** 0.5,
+        "guided_spp_to_match": reached,
+        "guided_warmup_frames": warm,
+        "guided_s": t_g,
+        "guided_ms_per_frame": t_g * 1e3 / (warm + done),
+        "guided_rmse": curve[-1][1] ** 0.5,
+        "time_ratio": (t_g / t_plain) if reached else None,
+        "reference_spp": n_ref,
+        "reference_s": t_ref,
+        "rmse_curve": [(n, m ** 0.5) for n, m in curve],
+        "guided_stage_ms": prof["gpuMs"],
+        "train_steps": st["steps"],
+        "last_loss": st["lastLoss"],
+    }
+    _log(f"guided: plain {spp} spp in {t_plain:.2f} s (RMSE {out['plain_rmse']:.4g}); guided reached it at "
+         f"{reached} spp in {t_g:.2f} s -> ratio {out['time_ratio']}")
+    return out
+
+
 def copy_workload(args, torch) -> dict:
     """Plain device copy of a known size (1 GiB), used to calibrate the HBM
     byte counters (FETCH_SIZE / WRITE_SIZE) against known traffic."""
@@ -250,7 +351,10 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="pt", choices=["pt", "traversal", "copy"])
+    ap.add_argument("--workload", default="pt", choices=["pt", "traversal", "guided", "copy"])
+    ap.add_argument("--guided-spp", type=int, default=1024,
+                    help="plain sample count whose RMSE the guided run must reach")
+    ap.add_argument("--no-guided", dest="with_guided", action="store_false", default=True)
     ap.add_argument("--autotune", action="store_true", default=True)
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -271,6 +375,12 @@ def main() -> int:
     if args.workload == "copy":
         res = copy_workload(args, torch)
         print(json.dumps({"workload": "copy", **res}))
+        return 0
+    if args.workload == "guided":
+        res = guided_workload(args, torch)
+        print(json.dumps({"metric": "equal-RMSE wall time, guided / plain", "value": res["time_ratio"],
+                          "unit": "ratio", "n_gpus": 1, "higher_is_better": False, "dtype": "f32",
+                          "data": "synthetic", "config": {"workload": res["workload"]}, "guided": res}))
         return 0
     if args.workload == "traversal":
         res = traversal_workload(args, torch)
@@ -309,6 +419,10 @@ def main() -> int:
     if args.with_traversal:
         trav = traversal_workload(args, torch)
         line["traversal"] = trav
+    if args.with_guided:
+        gd = guided_workload(args, torch)
+        line["guided"] = gd
+        line["equal_rmse_time_ratio"] = gd["time_ratio"]
     print(json.dumps(line))
     return 0
 

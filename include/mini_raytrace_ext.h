@@ -192,6 +192,8 @@ typedef struct mrtxGuidingOptions {
     uint32_t stepsPerFrame;   /* Adam steps per rendered frame (4) */
     float    minRoughness;    /* less rough or transmissive vertices use the BSDF only (0.2) */
     uint32_t seed;            /* network initialisation seed (used when the network is created) */
+    float    positionScale;   /* network position input = (p - scene box min) / largest box extent
+                                 x positionScale, encoded as sin/cos(2^b pi x), b < 12 (0 = 1) */
 } mrtxGuidingOptions;
 
 typedef struct mrtxGuidingStats {

@@ -144,7 +144,8 @@ MRTX_GUIDE_OUTPUTS = 40
 class mrtxGuidingOptions(_Sized):
     _fields_ = [("structSize", C.c_size_t), ("enableSampling", C.c_uint32), ("enableTraining", C.c_uint32),
                 ("selectProb", C.c_float), ("learningRate", C.c_float), ("batchSize", C.c_uint32),
-                ("stepsPerFrame", C.c_uint32), ("minRoughness", C.c_float), ("seed", C.c_uint32)]
+                ("stepsPerFrame", C.c_uint32), ("minRoughness", C.c_float), ("seed", C.c_uint32),
+                ("positionScale", C.c_float)]
 
 
 class mrtxGuidingStats(_Sized):

@@ -144,6 +144,7 @@ Engine::Engine(const mrtEngineDesc& desc) {
     guide_.stepsPerFrame = 4;
     guide_.minRoughness = 0.2f;
     guide_.seed = 1;
+    guide_.positionScale = 1.0f;
     // Minimal scene: default material, no instances, empty texture table.
     scene_.markDirty(kDirtyMaterials | kDirtyTextures | kDirtyLights | kDirtyGeometry | kDirtyInstances);
     allocFrame();
@@ -752,7 +753,8 @@ void Engine::fillConsts(FrameConsts& g) const {
     g.guideMinRoughness = guide_.minRoughness;
     g.guideStamp = guideStamp_;
     g.guideBoxMin = make_float4(sceneBoxMin_[0], sceneBoxMin_[1], sceneBoxMin_[2], 0.0f);
-    g.guideBoxScale = make_float4(sceneBoxScale_, sceneBoxScale_, sceneBoxScale_, 0.0f);
+    const float ps = sceneBoxScale_ * guide_.positionScale;
+    g.guideBoxScale = make_float4(ps, ps, ps, 0.0f);
 }
 
 void Engine::enqueueFrame(hipStream_t s, bool timed) {
@@ -926,6 +928,9 @@ void Engine::setGuiding(const mrtxGuidingOptions& in) {
     if (o.learningRate == 0.0f) o.learningRate = 1e-3f;
     if (o.batchSize == 0) o.batchSize = 65536;
     if (o.stepsPerFrame == 0) o.stepsPerFrame = 4;
+    if (o.positionScale == 0.0f) o.positionScale = 1.0f;
+    if (!(o.positionScale > 0.0f) || !std::isfinite(o.positionScale))
+        throw Error(MRT_ERROR_INVALID_ARGUMENT, "guiding positionScale must be > 0");
     if (!(o.selectProb > 0.0f && o.selectProb <= 1.0f))
         throw Error(MRT_ERROR_INVALID_ARGUMENT, "guiding selectProb must be in (0, 1]");
     if (!(o.learningRate > 0.0f) || !std::isfinite(o.learningRate))

@@ -212,13 +212,23 @@ __global__ void kGuideStepBegin(GuideBatch b, uint32_t* __restrict__ words, int 
     if (countStep && n > 0) words[kGwStep] += 1u;
 }
 
-__global__ __launch_bounds__(256) void kGuideWeightSum(GuideBatch b, uint32_t* __restrict__ words) {
+// Sum of the batch's weights in a fixed order (one block, LDS tree): the
+// sum scales every delta before its bf16 rounding, so an order-dependent sum
+// would make the gradient differ from run to run.
+constexpr int kWeightSumBlock = 1024;
+__global__ __launch_bounds__(kWeightSumBlock) void kGuideWeightSum(GuideBatch b, uint32_t* __restrict__ words) {
+    __shared__ float part[kWeightSumBlock];
     const uint32_t n = words[kGwBatch];
     const RecView rv = batchView(b);
     float s = 0.0f;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += recRow(rv, i)[0].w;
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    if ((threadIdx.x & 63) == 0 && s != 0.0f) atomicAdd(wordF(words, kGwWeightSum), s);
+    for (uint32_t i = threadIdx.x; i < n; i += kWeightSumBlock) s += recRow(rv, i)[0].w;
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int half = kWeightSumBlock / 2; half > 0; half >>= 1) {
+        if (threadIdx.x < half) part[threadIdx.x] += part[threadIdx.x + half];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *wordF(words, kGwWeightSum) = part[0];
 }
 
 // Forward over the batch; keeps the bf16 inputs (cap x 80), the hidden
@@ -535,7 +545,7 @@ hipError_t launchGuideGradient(const GuideBatch& b, const GuideNet& net, bool co
     hipError_t e;
     hipLaunchKernelGGL(kGuideStepBegin, dim3(1), dim3(64), 0, s, b, net.words, countStep ? 1 : 0);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(kGuideWeightSum, dim3(cappedGrid(b.cap, 256, 64)), dim3(256), 0, s, b, net.words);
+    hipLaunchKernelGGL(kGuideWeightSum, dim3(1), dim3(kWeightSumBlock), 0, s, b, net.words);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     GuideBatch ws = b;
     ws.cap = net.cap;

@@ -210,6 +210,36 @@ def mixture_sample(raw, u):
     return vmf_sample(mu[rows, k], kappa[rows, k], u[:, 1], u[:, 2]), k
 
 
+def backward_bf16(theta, x, w, weight):
+    """backward() with the device's rounding points: the forward of
+    forward_bf16 (bf16 inputs, weights and hidden activations), the raw-output
+    gradient rounded to bf16, and every back-propagated delta rounded to bf16
+    after the ReLU mask; products and sums exact.  Returns (loss, grad) of
+    the loss normalised by the batch's total weight."""
+    layers = unpack(np.asarray(theta, np.float32))
+    acts = [bf16(x)]
+    h = acts[0]
+    for li, (W, b) in enumerate(layers):
+        z = h @ bf16(W).T + b.astype(np.float64)
+        if li < len(layers) - 1:
+            h = bf16(np.maximum(z, 0.0))
+            acts.append(h)
+        else:
+            h = z
+    raw = h.astype(np.float32).astype(np.float64)
+    wsum = weight.sum()
+    L, d = loss_and_grad_raw(raw, w, weight / wsum)
+    d = bf16(d)
+    grads = []
+    for li in range(len(layers) - 1, -1, -1):
+        W, _ = layers[li]
+        grads.append((d.T @ acts[li], d.sum(0)))
+        if li > 0:
+            d = bf16((d @ bf16(W)) * (acts[li] > 0))
+    grads.reverse()
+    return L, np.concatenate([np.concatenate([gW.reshape(-1), gb]) for gW, gb in grads])
+
+
 def adam_step(theta, m, v, g, t, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
     """One Adam update (t = 1-based step); returns new (theta, m, v)."""
     m = b1 * m + (1 - b1) * g

@@ -5,8 +5,10 @@ Network kernels (through the C ABI's standalone network object):
 * forward: equal to the oracle's bf16-rounding forward to FP32 accumulation
   precision (pins the MFMA operand layouts); within bf16 error of the exact
   FP64 network;
-* gradient of the weighted NLL: per layer within bf16 error of the FP64
-  analytic gradient (itself pinned to torch autograd);
+* gradient of the weighted NLL: per layer equal to the oracle's backward
+  with the same bf16 rounding points (whose exact-arithmetic form is pinned
+  to torch autograd), and within the bf16 quantisation drift of the exact
+  FP64 gradient;
 * Adam: the first two updates equal the closed form on the GPU's gradient;
 * training on a synthetic target lowers the loss like FP64 training does;
 * mixture sampling: same directions as the oracle's sampler for the same
@@ -93,19 +95,33 @@ def test_gradient_matches_oracle(gpu):
         net.params = th
         loss, g = net.gradient(rec)
     w = rec[:, 3].astype(np.float64)
-    L, gref = G.backward(th, _x(rec), rec[:, 12:15].astype(np.float64), w)
+    wi = rec[:, 12:15].astype(np.float64)
+    L, gref = G.backward(th, _x(rec), wi, w)
     gref /= w.sum()
+    Lb, gbf = G.backward_bf16(th, _x(rec), wi, w)
     assert loss == pytest.approx(L / w.sum(), rel=2e-2)
-    off = 0
+    assert loss == pytest.approx(Lb, rel=1e-4)
+    off, worst = 0, []
     for li, (i, o) in enumerate(G.LAYERS):
         for part, size in (("W", i * o), ("b", o)):
-            a, b = g[off: off + size].astype(np.float64), gref[off: off + size]
-            rel = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
-            cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30)
-            _print(f"layer {li} {part}: rel L2 err {rel:.3e} cos {cos:.6f}")
-            # bf16 activations and deltas: ~1 % relative error per layer
-            assert rel < 0.06 and cos > 0.998, (li, part)
+            a = g[off: off + size].astype(np.float64)
+            b64, bbf = gref[off: off + size], gbf[off: off + size]
+            rel_bf = np.linalg.norm(a - bbf) / (np.linalg.norm(bbf) + 1e-30)
+            rel64 = np.linalg.norm(a - b64) / (np.linalg.norm(b64) + 1e-30)
+            cos64 = a @ b64 / (np.linalg.norm(a) * np.linalg.norm(b64) + 1e-30)
+            _print(f"layer {li} {part}: rel L2 err vs bf16 oracle {rel_bf:.3e}; vs fp64 {rel64:.3e} "
+                   f"(cos {cos64:.5f})")
+            worst.append((li, part, rel_bf, rel64, cos64))
             off += size
+    for li, part, rel_bf, rel64, cos64 in worst:
+        # Same rounding points: FP32 accumulation order and rare ReLU-mask
+        # flips from FP32 vs exact pre-activations remain.
+        assert rel_bf < 1e-2, (li, part, rel_bf)
+        # Exact network: the device differentiates the bf16-quantised network;
+        # quantising W alone moves the exact FP64 gradient of layer 0 by ~9 %
+        # and the inputs alone by ~7 % (ReLU masks and curvature), so this
+        # comparison only bounds the drift.
+        assert rel64 < 0.25 and cos64 > 0.97, (li, part, rel64)
 
 
 def test_adam_matches_closed_form(gpu):
@@ -126,13 +142,15 @@ def test_adam_matches_closed_form(gpu):
         m2, v2, t2 = net.adam_state()
     g1, g2 = g1.astype(np.float64), g2.astype(np.float64)
     assert (t1, t2) == (1, 2)
-    np.testing.assert_allclose(m1, 0.1 * g1, rtol=1e-4, atol=1e-12)
-    np.testing.assert_allclose(v1, 0.001 * g1 ** 2, rtol=1e-3, atol=1e-20)
+    # gradients are FP32 atomic sums: elements near zero differ by summation order
+    gs = np.abs(g1).max()
+    np.testing.assert_allclose(m1, 0.1 * g1, rtol=1e-4, atol=1e-7 * gs)
+    np.testing.assert_allclose(v1, 0.001 * g1 ** 2, rtol=1e-3, atol=1e-10 * gs ** 2)
     e1, _, _ = G.adam_step(th.astype(np.float64), 0, 0, g1, 1, lr=lr)
     # the update is lr * g / (|g| + eps): compare steps, not parameters
     np.testing.assert_allclose(p1 - th, e1 - th, rtol=1e-3, atol=2e-7)
     e2, em, ev = G.adam_step(p1.astype(np.float64), m1.astype(np.float64), v1.astype(np.float64), g2, 2, lr=lr)
-    np.testing.assert_allclose(m2, em, rtol=1e-4, atol=1e-12)
+    np.testing.assert_allclose(m2, em, rtol=1e-4, atol=1e-7 * gs)
     np.testing.assert_allclose(p2 - p1, e2 - p1, rtol=2e-3, atol=2e-7)
 
 

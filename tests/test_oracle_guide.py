@@ -198,3 +198,17 @@ def test_adam_first_steps_closed_form():
     mh = (0.9 * 0.1 * g1 + 0.1 * g2) / (1 - 0.81)
     vh = (0.999 * 0.001 * g1 ** 2 + 0.001 * g2 ** 2) / (1 - 0.999 ** 2)
     np.testing.assert_allclose(t2, t1 - 0.01 * mh / (np.sqrt(vh) + 1e-8), rtol=1e-10)
+
+
+def test_bf16_backward_structure(monkeypatch):
+    """With rounding disabled, backward_bf16 is backward() normalised by the
+    total weight (so only its rounding points differ from the pinned form)."""
+    theta = G.init_params(9)
+    pos, nrm, wo, wi, w = _records(200, 10)
+    x = G.encode(pos, nrm, wo)
+    L, g = G.backward(theta.astype(np.float32).astype(np.float64), x, wi, w)
+    monkeypatch.setattr(G, "bf16", lambda a: np.asarray(a, np.float64))
+    Lb, gb = G.backward_bf16(theta, x, wi, w)
+    assert Lb == pytest.approx(L / w.sum(), rel=1e-6)
+    # raw outputs stay rounded to float32 (as the device stores them)
+    np.testing.assert_allclose(gb, g / w.sum(), rtol=1e-4, atol=1e-6 * np.abs(g).max() / w.sum())
