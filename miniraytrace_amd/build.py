@@ -29,10 +29,12 @@ ARCH = "gfx950"
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
           f"-I{INCLUDE}", f"-I{CSRC}", "-fvisibility=hidden"]
 # Device code: no packed-FP32 instructions; correctly rounded fp32 divide and
-# sqrt (so reciprocal directions match the CPU oracle bit for bit).
+# sqrt (so reciprocal directions match the CPU oracle bit for bit); no
+# type-based alias analysis (lanes hand data to each other through LDS with
+# different access widths, e.g. bf16 stores read back as 16-byte fragments).
 HIP_FLAGS = ["-x", "hip", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-             "-fhip-fp32-correctly-rounded-divide-sqrt", "-munsafe-fp-atomics"]
+             "-fhip-fp32-correctly-rounded-divide-sqrt", "-munsafe-fp-atomics", "-fno-strict-aliasing"]
 # Host code: no FP contraction, so the BVH builder's float arithmetic is the
 # reference's operation sequence.
 HOST_FLAGS = ["-x", "c++", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]

@@ -36,7 +36,8 @@ struct FrameConsts {
     uint32_t guideTrain;         // 1 when shade writes training records this frame
     float guideMinRoughness;     // vertices with smaller roughness are not guided
     uint32_t guideStamp;         // nonzero, different for every rendered frame of an engine
-    uint32_t pad0, pad1;
+    uint32_t guideTrainStride;   // training paths: p % guideTrainStride == guideTrainPhase
+    uint32_t guideTrainPhase;
     float4 guideBoxMin;          // scene box: pos01 = (p - min) * scale
     float4 guideBoxScale;
 };

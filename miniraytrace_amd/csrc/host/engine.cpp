@@ -627,9 +627,14 @@ void Engine::allocGuide() {
     dGuideIdx_.allocate(size_t(n) * 4);
     dGuideRec_.allocate(size_t(n) * 3 * 16);
     dGuideMix_.allocate(size_t(n) * dev::kGuideMixFloat4 * 16);
-    dPathRec_.allocate(size_t(levels) * n * 4 * 16);
+    // Training paths: about four candidate records per path, so every
+    // stride-th path yields roughly the pool's worth of records per frame.
+    const uint64_t poolCap = uint64_t(guide_.batchSize) * guide_.stepsPerFrame;
+    guideTrainStride_ = uint32_t(std::max<uint64_t>(1, uint64_t(n) * 4 / std::max<uint64_t>(poolCap, 1)));
+    const uint32_t trainPaths = (n + guideTrainStride_ - 1) / guideTrainStride_;
+    dPathRec_.allocate(size_t(levels) * trainPaths * 4 * 16);
     dPathRec_.zero(stream_);  // stamp 0 never matches a frame
-    dGuidePool_.allocate(size_t(guide_.batchSize) * guide_.stepsPerFrame * 4 * 16);
+    dGuidePool_.allocate(size_t(poolCap) * 4 * 16);
     MRT_HIP(hipStreamSynchronize(stream_));
     guidePaths_ = n;
     guideLevels_ = levels;
@@ -752,6 +757,8 @@ void Engine::fillConsts(FrameConsts& g) const {
     g.guideTrain = guide_.enableTraining ? 1u : 0u;
     g.guideMinRoughness = guide_.minRoughness;
     g.guideStamp = guideStamp_;
+    g.guideTrainStride = guideTrainStride_;
+    g.guideTrainPhase = guideStamp_ % guideTrainStride_;
     g.guideBoxMin = make_float4(sceneBoxMin_[0], sceneBoxMin_[1], sceneBoxMin_[2], 0.0f);
     const float ps = sceneBoxScale_ * guide_.positionScale;
     g.guideBoxScale = make_float4(ps, ps, ps, 0.0f);
@@ -788,7 +795,7 @@ void Engine::enqueueFrame(hipStream_t s, bool timed) {
                 rv.base = pv.guideRec;
                 rv.stride = 3;
                 MRT_HIP(dev::launchGuideInfer(rv, pv.counters + b * dev::kCntPerBounce + dev::kCntGuideRecords,
-                                              pixels, guideNet_->deviceParams(), pv.guideMix, true, s));
+                                              pixels, guideNet_->deviceImage(), pv.guideMix, true, s));
             }
         }
         if (timed) MRT_HIP(hipEventRecord(ev_[9], s));
@@ -829,6 +836,7 @@ void Engine::enqueueGuideTraining(hipStream_t s) {
     ca.pathRec = dPathRec_.as<const float4>();
     ca.radiance = dRad_.as<const float4>();
     ca.paths = guidePaths_;
+    ca.trainPaths = (guidePaths_ + guideTrainStride_ - 1) / guideTrainStride_;
     ca.levels = guideLevels_;
     ca.pool = dGuidePool_.as<float4>();
     ca.poolCap = poolCap;

@@ -142,6 +142,7 @@ private:
     std::unique_ptr<GuideNetwork> guideNet_;
     DeviceBuffer dGuideIdx_, dGuideRec_, dGuideMix_, dPathRec_, dGuidePool_;
     uint32_t guidePaths_ = 0, guideLevels_ = 0;
+    uint32_t guideTrainStride_ = 1;  // every guideTrainStride_-th path writes training records
     uint32_t guideStamp_ = 0;
     float sceneBoxMin_[3] = { 0.0f, 0.0f, 0.0f };
     float sceneBoxScale_ = 1.0f;

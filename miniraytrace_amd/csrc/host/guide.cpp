@@ -22,22 +22,33 @@ GuideNetwork::GuideNetwork(uint32_t cap) {
     m_.allocate(kParamBytes);
     v_.allocate(kParamBytes);
     grad_.allocate(kParamBytes);
+    image_.allocate(dev::kGuideImageBytes);
     words_.allocate(dev::kGwWords * sizeof(uint32_t));
     grad_.zero();
     words_.zero();
+    params_.zero();
     allocWorkspace(cap);
     resetAdam();
+    MRT_HIP(dev::launchGuidePack(params_.as<float>(), image_.get(), nullptr));
     MRT_HIP(hipDeviceSynchronize());
 }
 
+// Feature-major workspace, rows padded to a multiple of 32.  The input
+// block has 96 feature rows so the weight-gradient tiles of layer 0 read
+// zeros past feature 79 (rows 80..95 are never written).
 void GuideNetwork::allocWorkspace(uint32_t cap) {
     if (cap == 0) throw Error(MRT_ERROR_INVALID_ARGUMENT, "guide batch capacity must be > 0");
-    // bf16 activations: cap x 80 inputs + 4 x cap x 64 hidden.
-    acts_.allocate(size_t(cap) * (dev::kGuideInputsPadded + 4 * 64) * sizeof(uint16_t));
-    raw_.allocate(size_t(cap) * dev::kGuideOutputs * sizeof(float));
-    delta_[0].allocate(size_t(cap) * 64 * sizeof(uint16_t));
-    delta_[1].allocate(size_t(cap) * 64 * sizeof(uint16_t));
+    const uint32_t stride = dev::guideRowStride(cap);
+    actIn_.allocate(size_t(96) * stride * sizeof(uint16_t));
+    actIn_.zero();
+    act_.allocate(size_t(4) * 64 * stride * sizeof(uint16_t));
+    raw_.allocate(size_t(dev::kGuideOutputs) * stride * sizeof(float));
+    delta_[0].allocate(size_t(64) * stride * sizeof(uint16_t));
+    delta_[1].allocate(size_t(64) * stride * sizeof(uint16_t));
+    partial_.allocate(size_t(dev::guideGradChunks(cap)) * dev::kGuideParamCount * sizeof(float));
+    MRT_HIP(hipStreamSynchronize(nullptr));
     cap_ = cap;
+    stride_ = stride;
 }
 
 void GuideNetwork::ensureCap(uint32_t cap) {
@@ -59,6 +70,7 @@ void GuideNetwork::initParams(uint32_t seed) {
 
 void GuideNetwork::setParams(const float* src) {
     params_.upload(src, kParamBytes);
+    MRT_HIP(dev::launchGuidePack(params_.as<float>(), image_.get(), nullptr));
     resetAdam();
 }
 
@@ -88,12 +100,16 @@ dev::GuideNet GuideNetwork::view() const {
     n.adamM = m_.as<float>();
     n.adamV = v_.as<float>();
     n.grad = grad_.as<float>();
-    n.acts = acts_.as<uint16_t>();
+    n.image = image_.get();
+    n.actIn = actIn_.as<uint16_t>();
+    n.act = act_.as<uint16_t>();
     n.raw = raw_.as<float>();
     n.delta[0] = delta_[0].as<uint16_t>();
     n.delta[1] = delta_[1].as<uint16_t>();
+    n.partial = partial_.as<float>();
     n.words = words_.as<uint32_t>();
     n.cap = cap_;
+    n.stride = stride_;
     return n;
 }
 

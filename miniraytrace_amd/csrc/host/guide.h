@@ -29,14 +29,16 @@ public:
     void readWords(uint32_t out[dev::kGwWords], hipStream_t s) const;
 
     dev::GuideNet view() const;
-    const float* deviceParams() const { return params_.as<const float>(); }
+    const void* deviceImage() const { return image_.get(); }
     uint32_t* deviceWords() const { return words_.as<uint32_t>(); }
+    float* deviceGrad() const { return grad_.as<float>(); }
 
 private:
     void allocWorkspace(uint32_t cap);
 
-    uint32_t cap_ = 0;
-    DeviceBuffer params_, m_, v_, grad_, acts_, raw_, delta_[2], words_;
+    uint32_t cap_ = 0, stride_ = 0;
+    DeviceBuffer params_, m_, v_, grad_, image_, words_;
+    DeviceBuffer actIn_, act_, raw_, delta_[2], partial_;
 };
 
 } // namespace mrt

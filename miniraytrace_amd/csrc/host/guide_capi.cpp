@@ -181,7 +181,7 @@ MRT_API mrtResult mrtxGuideNetworkInfer(mrtxGuideNetwork* n, const float* record
         mrt::DeviceScope scope(n->device);
         const mrt::dev::RecView v = stageRecords(n, records, count);
         n->out.ensure(size_t(count) * MRTX_GUIDE_OUTPUTS * sizeof(float));
-        MRT_HIP(mrt::dev::launchGuideInfer(v, n->count.as<const uint32_t>(), count, n->net->deviceParams(),
+        MRT_HIP(mrt::dev::launchGuideInfer(v, n->count.as<const uint32_t>(), count, n->net->deviceImage(),
                                            n->out.get(), false, n->stream));
         n->out.download(raw, size_t(count) * MRTX_GUIDE_OUTPUTS * sizeof(float), 0, n->stream);
         return MRT_SUCCESS;
@@ -202,7 +202,6 @@ MRT_API mrtResult mrtxGuideNetworkGradient(mrtxGuideNetwork* n, const float* rec
             MRT_HIP(hipMemcpyAsync(grad, net.grad, MRTX_GUIDE_PARAM_COUNT * sizeof(float), hipMemcpyDeviceToHost,
                                    n->stream));
         }
-        MRT_HIP(hipMemsetAsync(net.grad, 0, MRTX_GUIDE_PARAM_COUNT * sizeof(float), n->stream));
         uint32_t w[mrt::dev::kGwWords];
         n->net->readWords(w, n->stream);
         if (loss) *loss = wordAsFloat(w, mrt::dev::kGwLoss);

@@ -3,21 +3,30 @@
 // Network: 78 (padded 80) -> 64 -> 64 -> 64 -> 64 -> 40, ReLU on the hidden
 // layers (guide_device.h holds the encoding and the decoding of the outputs).
 //
-// Every matrix product uses v_mfma_f32_32x32x8bf16_1k.  Operand fragments of
-// one 32x32 output tile C = A (32 x 8) * B (8 x 32) for lane l (0..63):
-//   A: row m = l % 32, k = 4 (l / 32) + {0..3}        (4 bf16)
-//   B: col n = l % 32, k = 4 (l / 32) + {0..3}        (4 bf16)
-//   C: col n = l % 32, rows m = 8 j + 4 (l / 32) + i in acc[4 j + i]
-// A wave owns 32 rows (records); its activations live in LDS between layers.
+// Every matrix product uses v_mfma_f32_32x32x16_bf16.  Fragments of one
+// 32x32 output tile C = A (32 x 16) * B (16 x 32) for lane l (0..63),
+// h = l / 32:
+//   A: row m = l % 32, k = 8 h + {0..7}        (8 bf16, one 16-byte load)
+//   B: col n = l % 32, k = 8 h + {0..7}
+//   C: col n = l % 32, rows m = 8 j + 4 h + i in acc[4 j + i]
+// A and B fill their k slots from the same k, so the product does not depend
+// on how the instruction orders k internally; only the C layout matters.
+//
+// Inference keeps one wave's 32 rows in LDS between layers (row-major).
+// Training stores activations and deltas feature-major ([feature][row],
+// rows padded to 32) so every operand of the backward and weight-gradient
+// products is a contiguous 16-byte fragment in HBM.
 //
 // Kernels:
+//   kGuidePack        FP32 params -> packed bf16 forward image
 //   kGuideInfer       fused forward for rendering (features -> mixture)
 //   kGuideStepBegin   batch size and counters of one training step
-//   kGuideWeightSum   sum of the batch's training weights
+//   kGuideWeightSum   sum of the batch's training weights (fixed order)
 //   kGuideFwdTrain    forward over the batch, keeping the activations
 //   kGuideOutGrad     d(weighted NLL)/d(raw outputs) per row
-//   kGuideBackLayer   delta_{l-1} = (delta_l W_l) * relu'(a_{l-1})
-//   kGuideWeightGrad  grad W_l += delta_l^T a_{l-1}, grad b_l += sum delta_l
+//   kGuideBackLayer   delta_{l-1} = (W_l^T delta_l) * relu'(a_{l-1})
+//   kGuideWeightGrad  per row chunk: grad W_l = delta_l a_{l-1}^T, grad b_l
+//   kGuideGradReduce  sum of the chunk partials (fixed order, no atomics)
 //   kGuideAdam        Adam update of the FP32 master weights
 //   kGuideCollect     path records -> weighted training records
 // All counts are read from device memory, so a captured graph replays the
@@ -28,8 +37,9 @@
 namespace mrt {
 namespace dev {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // Round to nearest even (inputs are finite).
 __device__ __forceinline__ uint16_t toBf16(float f) {
@@ -38,19 +48,14 @@ __device__ __forceinline__ uint16_t toBf16(float f) {
     return uint16_t(u >> 16);
 }
 __device__ __forceinline__ float fromBf16(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+__device__ __forceinline__ uint32_t pack2(uint16_t lo, uint16_t hi) { return uint32_t(lo) | (uint32_t(hi) << 16); }
 
-// Four consecutive bf16 at an 8-byte aligned address.
-__device__ __forceinline__ bf16x4 ld4(const uint16_t* p) {
-    const uint2 v = *reinterpret_cast<const uint2*>(p);
-    bf16x4 r;
-    r[0] = short(v.x & 0xFFFFu);
-    r[1] = short(v.x >> 16);
-    r[2] = short(v.y & 0xFFFFu);
-    r[3] = short(v.y >> 16);
-    return r;
+// Eight consecutive bf16 at a 16-byte aligned address (LDS or global).
+__device__ __forceinline__ bf16x8 ld8(const uint16_t* p) {
+    return __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(p));
 }
-__device__ __forceinline__ f32x16 mfma(bf16x4 a, bf16x4 b, f32x16 c) {
-    return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
+__device__ __forceinline__ f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
 __device__ __forceinline__ f32x16 zero16() {
     f32x16 z;
@@ -58,7 +63,14 @@ __device__ __forceinline__ f32x16 zero16() {
     return z;
 }
 
+// Lanes of one wave hand rows to each other through LDS (written by some
+// lanes, read by others, often with another access width).  A wavefront-
+// scope fence keeps the compiler from moving those LDS accesses across the
+// hand-off; the hardware executes one wave's LDS operations in order.
+__device__ __forceinline__ void waveHandoff() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); }
+
 __device__ __forceinline__ float* wordF(uint32_t* w, uint32_t i) { return reinterpret_cast<float*>(w + i); }
+__device__ __forceinline__ uint32_t pad32(uint32_t n) { return (n + 31u) & ~31u; }
 
 __device__ __forceinline__ const float4* recRow(const RecView& v, uint32_t i) {
     return v.base + (size_t(v.phase) + size_t(i) * v.period) * v.stride;
@@ -78,42 +90,57 @@ __device__ __forceinline__ RecView batchView(const GuideBatch& b) {
     return v;
 }
 
-// LDS image of the forward weights: W_l output-major, layer 0 rows padded to
-// 80 inputs, the output layer padded to 64 rows.
+// ------------------------------------------------------------ forward image --
+// W_l output-major, layer 0 rows padded to 80 inputs, the output layer padded
+// to 64 rows: 21,504 bf16, then 5 x 64 FP32 biases.
 constexpr int kWOff1 = 64 * kGuideInPad;
-constexpr int kWTotal = kWOff1 + 4 * 64 * 64;  // 21,504 bf16
+constexpr int kWTotal = kWOff1 + 4 * 64 * 64;
+constexpr int kImageU4 = (kWTotal * 2 + 320 * 4) / 16;  // 2,768
+constexpr int kImageWU4 = kWTotal * 2 / 16;              // 2,688
+static_assert(kWTotal * 2 + 320 * 4 == int(kGuideImageBytes), "image size");
 __host__ __device__ constexpr int wOff(int l) { return l == 0 ? 0 : kWOff1 + (l - 1) * 64 * 64; }
 __host__ __device__ constexpr int wStride(int l) { return l == 0 ? kGuideInPad : 64; }
 
-__device__ void loadForwardWeights(const float* __restrict__ params, uint16_t* W, float* bias) {
-    for (int i = threadIdx.x; i < kWTotal; i += blockDim.x) {
+__global__ __launch_bounds__(256) void kGuidePack(const float* __restrict__ params, uint16_t* __restrict__ image) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kWTotal) {
         const int l = i < kWOff1 ? 0 : 1 + (i - kWOff1) / (64 * 64);
         const int r = (i - wOff(l)) / wStride(l), c = (i - wOff(l)) % wStride(l);
         float v = 0.0f;
         if (r < guideLayerOut(l) && c < guideLayerIn(l)) v = params[guideWeightOffset(l) + r * guideLayerIn(l) + c];
-        W[i] = toBf16(v);
-    }
-    for (int i = threadIdx.x; i < 5 * 64; i += blockDim.x) {
-        const int l = i / 64, o = i % 64;
-        bias[i] = o < guideLayerOut(l) ? params[guideBiasOffset(l) + o] : 0.0f;
+        image[i] = toBf16(v);
+    } else if (i < kWTotal + 320) {
+        const int j = i - kWTotal, l = j / 64, o = j % 64;
+        reinterpret_cast<float*>(image + kWTotal)[j] = o < guideLayerOut(l) ? params[guideBiasOffset(l) + o] : 0.0f;
     }
 }
 
-// One layer over the wave's 32 rows in act (row stride kGuideInPad).
+__device__ __forceinline__ void loadImage(const u32x4* __restrict__ image, uint16_t* W, float* bias) {
+    u32x4* w4 = reinterpret_cast<u32x4*>(W);
+    u32x4* b4 = reinterpret_cast<u32x4*>(bias);
+    for (int i = threadIdx.x; i < kImageU4; i += blockDim.x) {
+        const u32x4 v = image[i];
+        if (i < kImageWU4) w4[i] = v;
+        else b4[i - kImageWU4] = v;
+    }
+}
+
+// One layer over the wave's 32 rows in act (row-major, stride kGuideInPad).
 // Hidden layers write ReLU(W a + b) back into act as bf16; the output layer
 // leaves W a + b in the two accumulator tiles (columns 0..31, 32..63).
 __device__ __forceinline__ void layerForward(int l, uint16_t* act, const uint16_t* W, const float* bias,
                                              f32x16& c0, f32x16& c1) {
-    const int lane = threadIdx.x & 63, r = lane & 31, kq = 4 * (lane >> 5);
+    const int lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
     const int K = l == 0 ? kGuideInPad : 64;
     const uint16_t* Wl = W + wOff(l);
     const int ws = wStride(l);
     c0 = zero16();
     c1 = zero16();
-    for (int kb = 0; kb < K; kb += 8) {
-        const bf16x4 a = ld4(act + r * kGuideInPad + kb + kq);
-        const bf16x4 b0 = ld4(Wl + r * ws + kb + kq);
-        const bf16x4 b1 = ld4(Wl + (32 + r) * ws + kb + kq);
+    waveHandoff();
+    for (int kb = 0; kb < K; kb += 16) {
+        const bf16x8 a = ld8(act + r * kGuideInPad + kb + 8 * h);
+        const bf16x8 b0 = ld8(Wl + r * ws + kb + 8 * h);
+        const bf16x8 b1 = ld8(Wl + (32 + r) * ws + kb + 8 * h);
         c0 = mfma(a, b0, c0);
         c1 = mfma(a, b1, c1);
     }
@@ -125,61 +152,93 @@ __device__ __forceinline__ void layerForward(int l, uint16_t* act, const uint16_
     if (l < 4) {
         for (int j = 0; j < 4; ++j)
             for (int i = 0; i < 4; ++i) {
-                const int m = 8 * j + kq + i;
+                const int m = 8 * j + 4 * h + i;
                 act[m * kGuideInPad + r] = toBf16(fmaxf(c0[4 * j + i], 0.0f));
                 act[m * kGuideInPad + 32 + r] = toBf16(fmaxf(c1[4 * j + i], 0.0f));
             }
     }
+    waveHandoff();
 }
 
-// Encodes rows [row0, row0 + 32) into act: lanes 0..31 write features 0..39
-// of row lane, lanes 32..63 features 40..79.  Rows past count are zero.
+// Encodes rows [row0, row0 + 32) into act.  Feature pair j = 12 a + b is
+// (sin, cos)(pi 2^b p_a); lanes 0..31 write pairs 0..17 of row lane, lanes
+// 32..63 pairs 18..35, the normal, wo and two zero pads.  Rows past count
+// are zero.
 __device__ __forceinline__ void encodeRows(const RecView& rv, uint32_t row0, uint32_t count, uint16_t* act) {
-    const int lane = threadIdx.x & 63, r = lane & 31, half = lane >> 5;
+    const int lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
     const uint32_t row = row0 + r;
-    float p[3] = { 0, 0, 0 }, n[3] = { 0, 0, 0 }, wo[3] = { 0, 0, 0 };
     const bool valid = row < count;
+    float4 a = make_float4(0, 0, 0, 0), b = a, c = a;
     if (valid) {
         const float4* q = recRow(rv, row);
-        const float4 a = q[0], b = q[1], c = q[2];
-        p[0] = a.x; p[1] = a.y; p[2] = a.z;
-        n[0] = b.x; n[1] = b.y; n[2] = b.z;
-        wo[0] = c.x; wo[1] = c.y; wo[2] = c.z;
+        a = q[0];
+        b = q[1];
+        c = q[2];
     }
-    for (int k = half * 40; k < half * 40 + 40; ++k)
-        act[r * kGuideInPad + k] = valid ? toBf16(guideFeature(p, n, wo, k)) : uint16_t(0);
+    const float p[3] = { a.x, a.y, a.z };
+    uint32_t* dst = reinterpret_cast<uint32_t*>(act + r * kGuideInPad);
+    waveHandoff();  // the previous tile's reads of act come first
+#pragma unroll
+    for (int jj = 0; jj < 18; ++jj) {
+        const int j = jj + 18 * h;
+        const int ax = j / 12, band = j % 12;
+        float s, cs;
+        sincospif(p[ax] * float(1 << band), &s, &cs);
+        dst[j] = valid ? pack2(toBf16(s), toBf16(cs)) : 0u;
+    }
+    if (h == 1) {
+        dst[36] = valid ? pack2(toBf16(b.x), toBf16(b.y)) : 0u;
+        dst[37] = valid ? pack2(toBf16(b.z), toBf16(c.x)) : 0u;
+        dst[38] = valid ? pack2(toBf16(c.y), toBf16(c.z)) : 0u;
+        dst[39] = 0u;
+    }
+    waveHandoff();
 }
 
-// Writes the output tiles of rows [row0, row0 + 32) as FP32 into act (reused
-// as float[32][40]).
+// Output tiles of rows [row0, row0 + 32) as FP32 into act (reused as
+// float[32][40]).
 __device__ __forceinline__ void stashOutputs(const f32x16& c0, const f32x16& c1, uint16_t* act) {
-    const int lane = threadIdx.x & 63, r = lane & 31, kq = 4 * (lane >> 5);
+    const int lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
     float* fo = reinterpret_cast<float*>(act);
     for (int j = 0; j < 4; ++j)
         for (int i = 0; i < 4; ++i) {
-            const int m = 8 * j + kq + i;
+            const int m = 8 * j + 4 * h + i;
             fo[m * kGuideOut + r] = c0[4 * j + i];
             if (r < kGuideOut - 32) fo[m * kGuideOut + 32 + r] = c1[4 * j + i];
         }
+    waveHandoff();
+}
+
+// Feature-major copy of the wave's 32 rows: features [0, nfeat) of act to
+// dst[f * stride + row0 + 0..31] (64 contiguous bytes per feature).
+__device__ __forceinline__ void storeFeatureMajor(const uint16_t* act, int nfeat, uint16_t* __restrict__ dst,
+                                                  uint32_t stride, uint32_t row0) {
+    const int lane = threadIdx.x & 63;
+    for (int f = lane; f < nfeat; f += 64) {
+        uint32_t w[16];
+        for (int m = 0; m < 16; ++m) w[m] = pack2(act[(2 * m) * kGuideInPad + f], act[(2 * m + 1) * kGuideInPad + f]);
+        u32x4* d = reinterpret_cast<u32x4*>(dst + size_t(f) * stride + row0);
+        for (int q = 0; q < 4; ++q) d[q] = u32x4{ w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3] };
+    }
 }
 
 // ------------------------------------------------------------------ inference --
-constexpr int kInferBlock = 256;  // 4 waves
-__global__ __launch_bounds__(kInferBlock) void kGuideInfer(RecView rv, const uint32_t* __restrict__ countPtr,
-                                                           const float* __restrict__ params, void* __restrict__ out,
-                                                           int decoded) {
+constexpr int kFwdWaves = 16;  // one 1,024-thread block per CU: 43 KB weights + 16 x 5 KB rows
+__global__ __launch_bounds__(kFwdWaves * 64) void kGuideInfer(RecView rv, const uint32_t* __restrict__ countPtr,
+                                                              const u32x4* __restrict__ image, void* __restrict__ out,
+                                                              int decoded) {
     __shared__ __attribute__((aligned(16))) uint16_t W[kWTotal];
-    __shared__ float bias[5 * 64];
-    __shared__ __attribute__((aligned(16))) uint16_t actAll[4][32 * kGuideInPad];
+    __shared__ __attribute__((aligned(16))) float bias[5 * 64];
+    __shared__ __attribute__((aligned(16))) uint16_t actAll[kFwdWaves][32 * kGuideInPad];
     const uint32_t count = *countPtr;
-    if (blockIdx.x * 128 >= count) return;  // uniform per block
-    loadForwardWeights(params, W, bias);
+    if (blockIdx.x * kFwdWaves * 32 >= count) return;  // uniform per block
+    loadImage(image, W, bias);
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint16_t* act = actAll[wave];
     const float* fo = reinterpret_cast<const float*>(act);
-    const uint32_t totalWaves = gridDim.x * 4;
-    for (uint32_t blk = blockIdx.x * 4 + wave; blk * 32 < count; blk += totalWaves) {
+    const uint32_t totalWaves = gridDim.x * kFwdWaves;
+    for (uint32_t blk = blockIdx.x * kFwdWaves + wave; blk * 32 < count; blk += totalWaves) {
         const uint32_t row0 = blk * 32;
         encodeRows(rv, row0, count, act);
         f32x16 c0, c1;
@@ -231,199 +290,205 @@ __global__ __launch_bounds__(kWeightSumBlock) void kGuideWeightSum(GuideBatch b,
     if (threadIdx.x == 0) *wordF(words, kGwWeightSum) = part[0];
 }
 
-// Forward over the batch; keeps the bf16 inputs (cap x 80), the hidden
-// activations (4 x cap x 64) and the raw FP32 outputs (cap x 40), where
-// cap = b.cap is the workspace's row capacity.
-__global__ __launch_bounds__(256) void kGuideFwdTrain(GuideBatch b, const uint32_t* __restrict__ words,
-                                                      const float* __restrict__ params, uint16_t* __restrict__ acts,
-                                                      float* __restrict__ raw) {
+// Forward over the batch's padded rows [0, pad32(count)): keeps the encoded
+// inputs (features 0..79 of actIn), the hidden outputs and the raw outputs,
+// feature-major.  Padding rows encode to zero inputs (finite activations).
+__global__ __launch_bounds__(kFwdWaves * 64) void kGuideFwdTrain(GuideBatch b, const uint32_t* __restrict__ words,
+                                                                 const u32x4* __restrict__ image, GuideNet net) {
     __shared__ __attribute__((aligned(16))) uint16_t W[kWTotal];
-    __shared__ float bias[5 * 64];
-    __shared__ __attribute__((aligned(16))) uint16_t actAll[4][32 * kGuideInPad];
+    __shared__ __attribute__((aligned(16))) float bias[5 * 64];
+    __shared__ __attribute__((aligned(16))) uint16_t actAll[kFwdWaves][32 * kGuideInPad];
     const uint32_t count = words[kGwBatch];
-    if (blockIdx.x * 128 >= count) return;  // uniform per block
-    loadForwardWeights(params, W, bias);
+    const uint32_t rows = pad32(count);
+    if (blockIdx.x * kFwdWaves * 32 >= rows) return;  // uniform per block
+    loadImage(image, W, bias);
     __syncthreads();
     const RecView rv = batchView(b);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint16_t* act = actAll[wave];
     const float* fo = reinterpret_cast<const float*>(act);
-    const uint32_t totalWaves = gridDim.x * 4;
-    for (uint32_t blk = blockIdx.x * 4 + wave; blk * 32 < count; blk += totalWaves) {
+    const uint32_t stride = net.stride;
+    const uint32_t totalWaves = gridDim.x * kFwdWaves;
+    for (uint32_t blk = blockIdx.x * kFwdWaves + wave; blk * 32 < rows; blk += totalWaves) {
         const uint32_t row0 = blk * 32;
         encodeRows(rv, row0, count, act);
-        for (int k = lane; k < 32 * kGuideInPad; k += 64) {
-            const uint32_t row = row0 + k / kGuideInPad;
-            if (row < count) acts[size_t(row) * kGuideInPad + k % kGuideInPad] = act[k];
-        }
+        storeFeatureMajor(act, kGuideInPad, net.actIn, stride, row0);
         f32x16 c0, c1;
         for (int l = 0; l < 5; ++l) {
             layerForward(l, act, W, bias, c0, c1);
-            if (l < 4) {
-                uint16_t* dst = acts + size_t(b.cap) * kGuideInPad + size_t(l) * b.cap * 64;
-                for (int k = lane; k < 32 * 64; k += 64) {
-                    const uint32_t row = row0 + k / 64;
-                    if (row < count) dst[size_t(row) * 64 + k % 64] = act[(k / 64) * kGuideInPad + k % 64];
-                }
-            }
+            if (l < 4) storeFeatureMajor(act, 64, net.act + size_t(l) * 64 * stride, stride, row0);
         }
         stashOutputs(c0, c1, act);
-        for (int i = lane; i < 32 * kGuideOut; i += 64) {
-            const uint32_t row = row0 + i / kGuideOut;
-            if (row < count) raw[size_t(row) * kGuideOut + i % kGuideOut] = fo[i];
+        if (lane < kGuideOut) {
+            float* d = net.raw + size_t(lane) * stride + row0;
+            for (int m = 0; m < 32; m += 4)
+                *reinterpret_cast<float4*>(d + m) = make_float4(fo[m * kGuideOut + lane], fo[(m + 1) * kGuideOut + lane],
+                                                                fo[(m + 2) * kGuideOut + lane],
+                                                                fo[(m + 3) * kGuideOut + lane]);
         }
     }
 }
 
-// delta (bf16, 64 columns, 40..63 zero) = d L / d raw with
-// L = sum_j w_j (-log q(wi_j)) / sum_j w_j; the loss is accumulated too.
-__global__ __launch_bounds__(256) void kGuideOutGrad(GuideBatch b, uint32_t* __restrict__ words,
-                                                     const float* __restrict__ raw, uint16_t* __restrict__ delta) {
+// delta (feature-major bf16, 64 features, 40..63 zero) = d L / d raw with
+// L = sum_j w_j (-log q(wi_j)) / sum_j w_j; padding rows get zero deltas.
+__global__ __launch_bounds__(256) void kGuideOutGrad(GuideBatch b, uint32_t* __restrict__ words, GuideNet net) {
     const uint32_t count = words[kGwBatch];
     const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t stride = net.stride;
     float loss = 0.0f;
-    if (row < count) {
-        const float scale = 1.0f / fmaxf(*wordF(words, kGwWeightSum), 1e-30f);
-        const float4* q = recRow(batchView(b), row);
-        const float wgt = q[0].w * scale;
-        const V3 wi = v3(q[3]);
-        const float* o = raw + size_t(row) * kGuideOut;
-        GuideMix m;
-        decodeMix(o, m);
-        float lf[kGuideLobes], mx = -1e30f;
-        for (int k = 0; k < kGuideLobes; ++k) {
-            const float c = dot(m.mu[k], wi);
-            lf[k] = logf(m.w[k]) + logf(m.kappa[k]) - logf(kTwoPi) - logf(-expm1f(-2.0f * m.kappa[k])) +
-                    m.kappa[k] * (c - 1.0f);
-            mx = fmaxf(mx, lf[k]);
+    if (row < pad32(count)) {
+        uint16_t* dcol = net.delta[0] + row;
+        if (row < count) {
+            const float scale = 1.0f / fmaxf(*wordF(words, kGwWeightSum), 1e-30f);
+            const float4* q = recRow(batchView(b), row);
+            const float wgt = q[0].w * scale;
+            const V3 wi = v3(q[3]);
+            float o[kGuideOut];
+            for (int k = 0; k < kGuideOut; ++k) o[k] = net.raw[size_t(k) * stride + row];
+            GuideMix m;
+            decodeMix(o, m);
+            float lf[kGuideLobes], mx = -1e30f;
+            for (int k = 0; k < kGuideLobes; ++k) {
+                const float c = dot(m.mu[k], wi);
+                lf[k] = logf(m.w[k]) + logf(m.kappa[k]) - logf(kTwoPi) - logf(-expm1f(-2.0f * m.kappa[k])) +
+                        m.kappa[k] * (c - 1.0f);
+                mx = fmaxf(mx, lf[k]);
+            }
+            float s = 0.0f;
+            for (int k = 0; k < kGuideLobes; ++k) s += expf(lf[k] - mx);
+            const float logq = mx + logf(s);
+            loss = -wgt * logq;
+            for (int k = 0; k < kGuideLobes; ++k) {
+                const float resp = expf(lf[k] - logq);
+                const float kap = m.kappa[k];
+                const float c = dot(m.mu[k], wi);
+                const float lk = o[5 * k + 3];
+                const bool inside = lk > kLogKappaLo && lk < kLogKappaHi;
+                const float dlogf = 1.0f / kap - 2.0f / expm1f(2.0f * kap) + (c - 1.0f);
+                const V3 v = v3(o[5 * k], o[5 * k + 1], o[5 * k + 2]);
+                const float vn = fmaxf(sqrtf(dot(v, v)), 1e-12f);
+                const float gmu = -wgt * resp * kap / vn;
+                dcol[size_t(5 * k + 0) * stride] = toBf16(gmu * (wi.x - c * m.mu[k].x));
+                dcol[size_t(5 * k + 1) * stride] = toBf16(gmu * (wi.y - c * m.mu[k].y));
+                dcol[size_t(5 * k + 2) * stride] = toBf16(gmu * (wi.z - c * m.mu[k].z));
+                dcol[size_t(5 * k + 3) * stride] = toBf16(inside ? -wgt * resp * dlogf * kap : 0.0f);
+                dcol[size_t(5 * k + 4) * stride] = toBf16(-wgt * (resp - m.w[k]));
+            }
+        } else {
+            for (int k = 0; k < kGuideOut; ++k) dcol[size_t(k) * stride] = 0;
         }
-        float s = 0.0f;
-        for (int k = 0; k < kGuideLobes; ++k) s += expf(lf[k] - mx);
-        const float logq = mx + logf(s);
-        loss = -wgt * logq;
-        uint16_t* d = delta + size_t(row) * 64;
-        for (int k = 0; k < kGuideLobes; ++k) {
-            const float resp = expf(lf[k] - logq);
-            const float kap = m.kappa[k];
-            const float c = dot(m.mu[k], wi);
-            const float lk = o[5 * k + 3];
-            const bool inside = lk > kLogKappaLo && lk < kLogKappaHi;
-            const float dlogf = 1.0f / kap - 2.0f / expm1f(2.0f * kap) + (c - 1.0f);
-            const V3 v = v3(o[5 * k], o[5 * k + 1], o[5 * k + 2]);
-            const float vn = fmaxf(sqrtf(dot(v, v)), 1e-12f);
-            const float gmu = -wgt * resp * kap / vn;
-            d[5 * k + 0] = toBf16(gmu * (wi.x - c * m.mu[k].x));
-            d[5 * k + 1] = toBf16(gmu * (wi.y - c * m.mu[k].y));
-            d[5 * k + 2] = toBf16(gmu * (wi.z - c * m.mu[k].z));
-            d[5 * k + 3] = toBf16(inside ? -wgt * resp * dlogf * kap : 0.0f);
-            d[5 * k + 4] = toBf16(-wgt * (resp - m.w[k]));
-        }
-        for (int k = kGuideOut; k < 64; ++k) d[k] = 0;
+        for (int k = kGuideOut; k < 64; ++k) dcol[size_t(k) * stride] = 0;
     }
     for (int off = 32; off > 0; off >>= 1) loss += __shfl_down(loss, off, 64);
     if ((threadIdx.x & 63) == 0 && loss != 0.0f) atomicAdd(wordF(words, kGwLoss), loss);
 }
 
-// deltaPrev (count x 64) = (delta . W_l) * (actPrev > 0), l >= 1.  W_l is
-// staged transposed in LDS (WT[in][out]) so B fragments are contiguous.
+// deltaPrev[in][row] = (sum_o W_l[o][in] delta[o][row]) * (actPrev[in][row] > 0),
+// l >= 1, one wave per 32 rows.  W_l^T is staged in LDS as WT[in][o]; the
+// wave's delta tile is staged as sD[o][row] (row stride 34 against bank
+// conflicts between the two lane halves).
+constexpr int kDeltaPad = 34;
 __global__ __launch_bounds__(256) void kGuideBackLayer(int l, const uint32_t* __restrict__ words,
                                                        const float* __restrict__ params,
                                                        const uint16_t* __restrict__ delta,
                                                        const uint16_t* __restrict__ actPrev,
-                                                       uint16_t* __restrict__ deltaPrev) {
+                                                       uint16_t* __restrict__ deltaPrev, uint32_t stride) {
     __shared__ __attribute__((aligned(16))) uint16_t WT[64 * 64];
-    const uint32_t count = words[kGwBatch];
-    if (blockIdx.x * 128 >= count) return;
+    __shared__ __attribute__((aligned(16))) uint16_t sDAll[4][64 * kDeltaPad];
+    const uint32_t rows = pad32(words[kGwBatch]);
+    if (blockIdx.x * 128 >= rows) return;  // uniform per block
     const int nin = guideLayerIn(l), nout = guideLayerOut(l);
     for (int i = threadIdx.x; i < 64 * 64; i += blockDim.x) {
         const int in = i / 64, o = i % 64;
         WT[i] = (o < nout && in < nin) ? toBf16(params[guideWeightOffset(l) + o * nin + in]) : uint16_t(0);
     }
     __syncthreads();
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 31, kq = 4 * (lane >> 5);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
+    uint16_t* sD = sDAll[wave];
     const uint32_t totalWaves = gridDim.x * 4;
-    for (uint32_t blk = blockIdx.x * 4 + wave; blk * 32 < count; blk += totalWaves) {
+    for (uint32_t blk = blockIdx.x * 4 + wave; blk * 32 < rows; blk += totalWaves) {
         const uint32_t row0 = blk * 32;
-        const uint32_t ra = min(row0 + r, count - 1);
-        f32x16 c0 = zero16(), c1 = zero16();
-        for (int kb = 0; kb < 64; kb += 8) {
-            const bf16x4 a = ld4(delta + size_t(ra) * 64 + kb + kq);
-            const bf16x4 b0 = ld4(WT + r * 64 + kb + kq);
-            const bf16x4 b1 = ld4(WT + (32 + r) * 64 + kb + kq);
-            c0 = mfma(a, b0, c0);
-            c1 = mfma(a, b1, c1);
+        // stage delta[0..63][row0 .. row0 + 31]: 256 16-byte pieces, 4 per lane
+        for (int q = lane; q < 256; q += 64) {
+            const int o = q >> 2, part = q & 3;
+            const u32x4 v = *reinterpret_cast<const u32x4*>(delta + size_t(o) * stride + row0 + 8 * part);
+            uint32_t* d = reinterpret_cast<uint32_t*>(sD + o * kDeltaPad + 8 * part);
+            d[0] = v.x;
+            d[1] = v.y;
+            d[2] = v.z;
+            d[3] = v.w;
         }
-        for (int j = 0; j < 4; ++j)
-            for (int i = 0; i < 4; ++i) {
-                const uint32_t row = row0 + 8 * j + kq + i;
-                if (row >= count) continue;
-                const uint16_t p0 = actPrev[size_t(row) * 64 + r], p1 = actPrev[size_t(row) * 64 + 32 + r];
-                deltaPrev[size_t(row) * 64 + r] = fromBf16(p0) > 0.0f ? toBf16(c0[4 * j + i]) : uint16_t(0);
-                deltaPrev[size_t(row) * 64 + 32 + r] = fromBf16(p1) > 0.0f ? toBf16(c1[4 * j + i]) : uint16_t(0);
+        waveHandoff();
+        for (int mt = 0; mt < 2; ++mt) {
+            f32x16 acc = zero16();
+            for (int kb = 0; kb < 64; kb += 16) {
+                const bf16x8 a = ld8(WT + (32 * mt + r) * 64 + kb + 8 * h);
+                const uint16_t* col = sD + (kb + 8 * h) * kDeltaPad + r;
+                const u32x4 bv = u32x4{ pack2(col[0], col[kDeltaPad]), pack2(col[2 * kDeltaPad], col[3 * kDeltaPad]),
+                                        pack2(col[4 * kDeltaPad], col[5 * kDeltaPad]),
+                                        pack2(col[6 * kDeltaPad], col[7 * kDeltaPad]) };
+                acc = mfma(a, __builtin_bit_cast(bf16x8, bv), acc);
             }
+            for (int j = 0; j < 4; ++j)
+                for (int i = 0; i < 4; ++i) {
+                    const int in = 32 * mt + 8 * j + 4 * h + i;
+                    const size_t at = size_t(in) * stride + row0 + r;
+                    deltaPrev[at] = fromBf16(actPrev[at]) > 0.0f ? toBf16(acc[4 * j + i]) : uint16_t(0);
+                }
+        }
+        waveHandoff();  // the next tile's staging overwrites sD
     }
 }
 
-// grad W_l += delta^T actPrev and grad b_l += column sums of delta over rows
-// [blockIdx.x * kGradRowsPerBlock, ...); FP32 atomics into grad.  Output
-// tiles (32 outputs x 32 inputs) go to the 4 waves round-robin (4 or 6 tiles).
-constexpr uint32_t kGradRowsPerBlock = 512;
+// Partial weight and bias gradients of layer l over one chunk of rows:
+// wave job = (tile, chunk); tile (mt, nt) covers outputs 32 mt.. and inputs
+// 32 nt.. (layer 0: 3 input tiles over the 96 zero-padded input features).
+// Tiles with nt == 0 also form the bias sums with an all-ones B operand.
 __global__ __launch_bounds__(256) void kGuideWeightGrad(int l, const uint32_t* __restrict__ words,
                                                         const uint16_t* __restrict__ delta,
-                                                        const uint16_t* __restrict__ actPrev, uint32_t actStride,
-                                                        float* __restrict__ grad) {
-    __shared__ __attribute__((aligned(16))) uint16_t dT[64 * 32];  // [out][row]
-    __shared__ __attribute__((aligned(16))) uint16_t aT[96 * 32];  // [in][row]
-    const uint32_t count = words[kGwBatch];
-    const uint32_t begin = blockIdx.x * kGradRowsPerBlock;
-    if (begin >= count) return;
-    const uint32_t end = min(begin + kGradRowsPerBlock, count);
+                                                        const uint16_t* __restrict__ actPrev, uint32_t stride,
+                                                        float* __restrict__ partial) {
+    const uint32_t rows = pad32(words[kGwBatch]);
     const int nin = guideLayerIn(l), nout = guideLayerOut(l);
-    const int ntIn = (nin + 31) / 32;  // 3 for layer 0, else 2
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 31, kq = 4 * (lane >> 5);
-    f32x16 acc[2] = { zero16(), zero16() };
-    float bsum = 0.0f;
-    for (uint32_t row0 = begin; row0 < end; row0 += 32) {
-        __syncthreads();
-        for (int i = threadIdx.x; i < 64 * 32; i += blockDim.x) {
-            const int o = i / 32, rr = i % 32;
-            const uint32_t row = row0 + rr;
-            dT[i] = row < end ? delta[size_t(row) * 64 + o] : uint16_t(0);
-        }
-        for (int i = threadIdx.x; i < 96 * 32; i += blockDim.x) {
-            const int in = i / 32, rr = i % 32;
-            const uint32_t row = row0 + rr;
-            aT[i] = (row < end && in < nin) ? actPrev[size_t(row) * actStride + in] : uint16_t(0);
-        }
-        __syncthreads();
-        for (int t = 0; t < 2; ++t) {
-            const int tile = wave + 4 * t;
-            if (tile >= 2 * ntIn) break;
-            const int mt = tile / ntIn, nt = tile % ntIn;
-            for (int kb = 0; kb < 32; kb += 8) {
-                const bf16x4 a = ld4(dT + (32 * mt + r) * 32 + kb + kq);
-                const bf16x4 bb = ld4(aT + (32 * nt + r) * 32 + kb + kq);
-                acc[t] = mfma(a, bb, acc[t]);
-            }
-        }
-        if (wave == 0)
-            for (int rr = 0; rr < 32; ++rr) bsum += fromBf16(dT[lane * 32 + rr]);
+    const int ntIn = l == 0 ? 3 : 2;
+    const uint32_t job = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t tile = job % uint32_t(2 * ntIn), chunk = job / uint32_t(2 * ntIn);
+    const uint32_t r0 = chunk * kGuideGradChunk;
+    if (r0 >= rows) return;  // whole wave
+    const uint32_t r1 = min(r0 + kGuideGradChunk, rows);
+    const int mt = int(tile) / ntIn, nt = int(tile) % ntIn;
+    const int lane = threadIdx.x & 63, lr = lane & 31, h = lane >> 5;
+    const uint16_t* aRow = delta + size_t(32 * mt + lr) * stride + 8 * h;
+    const uint16_t* bRow = actPrev + size_t(32 * nt + lr) * stride + 8 * h;
+    const uint32_t one2 = 0x3F803F80u;  // two bf16 1.0
+    const bf16x8 ones = __builtin_bit_cast(bf16x8, u32x4{ one2, one2, one2, one2 });
+    f32x16 acc = zero16(), accB = zero16();
+    for (uint32_t r = r0; r < r1; r += 16) {
+        const bf16x8 a = ld8(aRow + r);
+        acc = mfma(a, ld8(bRow + r), acc);
+        if (nt == 0) accB = mfma(a, ones, accB);
     }
-    for (int t = 0; t < 2; ++t) {
-        const int tile = wave + 4 * t;
-        if (tile >= 2 * ntIn) break;
-        const int mt = tile / ntIn, nt = tile % ntIn;
-        for (int j = 0; j < 4; ++j)
-            for (int i = 0; i < 4; ++i) {
-                const int o = 32 * mt + 8 * j + kq + i, in = 32 * nt + r;
-                if (o < nout && in < nin) atomicAdd(&grad[guideWeightOffset(l) + o * nin + in], acc[t][4 * j + i]);
-            }
-    }
-    if (wave == 0 && lane < nout) atomicAdd(&grad[guideBiasOffset(l) + lane], bsum);
+    float* dst = partial + size_t(chunk) * kGuideParams;
+    for (int j = 0; j < 4; ++j)
+        for (int i = 0; i < 4; ++i) {
+            const int o = 32 * mt + 8 * j + 4 * h + i, in = 32 * nt + lr;
+            if (o < nout && in < nin) dst[guideWeightOffset(l) + o * nin + in] = acc[4 * j + i];
+            if (nt == 0 && lr == 0 && o < nout) dst[guideBiasOffset(l) + o] = accB[4 * j + i];
+        }
+}
+
+__global__ __launch_bounds__(256) void kGuideGradReduce(const uint32_t* __restrict__ words,
+                                                        const float* __restrict__ partial, float* __restrict__ grad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kGuideParams) return;
+    const uint32_t chunks = (pad32(words[kGwBatch]) + kGuideGradChunk - 1) / kGuideGradChunk;
+    float s = 0.0f;
+    for (uint32_t c = 0; c < chunks; ++c) s += partial[size_t(c) * kGuideParams + i];
+    grad[i] = s;
 }
 
 __global__ __launch_bounds__(256) void kGuideAdam(float* __restrict__ params, float* __restrict__ m,
-                                                  float* __restrict__ v, float* __restrict__ grad,
+                                                  float* __restrict__ v, const float* __restrict__ grad,
                                                   uint32_t* __restrict__ words, GuideAdam h) {
     const uint32_t n = words[kGwBatch];
     if (n == 0) return;
@@ -441,7 +506,6 @@ __global__ __launch_bounds__(256) void kGuideAdam(float* __restrict__ params, fl
     m[i] = mi;
     v[i] = vi;
     params[i] -= h.lr * (mi / c1) / (sqrtf(vi / c2) + h.eps);
-    grad[i] = 0.0f;
 }
 
 // ------------------------------------------------------- sampling queries --
@@ -486,21 +550,23 @@ __device__ __forceinline__ float hashUnit(uint32_t a, uint32_t b, uint32_t c) {
     return float(h >> 8) * (1.0f / 16777216.0f);
 }
 
-// One thread per path, levels visited in order; every lane of a wave runs
-// the same number of iterations (the wave-wide appends need all lanes).
+// One thread per training path, levels visited in order; every lane of a
+// wave runs the same number of iterations (the wave-wide appends need all
+// lanes).
 __global__ __launch_bounds__(256) void kGuideCollect(GuideCollectArgs a) {
     const uint32_t stamp = a.g->guideStamp;
-    const uint32_t frame = stamp;
-    uint32_t* cand = a.words + kGwCand0 + (frame & 1u);
-    const uint32_t prev = a.words[kGwCand0 + ((frame + 1u) & 1u)];
+    const uint32_t kStride = a.g->guideTrainStride, phase = a.g->guideTrainPhase;
+    uint32_t* cand = a.words + kGwCand0 + (stamp & 1u);
+    const uint32_t prev = a.words[kGwCand0 + ((stamp + 1u) & 1u)];
     const float keep = prev > a.poolCap ? float(a.poolCap) / float(prev) : 1.0f;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < a.paths; base += stride) {
-        const uint32_t p = base + threadIdx.x;
-        const bool valid = p < a.paths;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < a.trainPaths; base += stride) {
+        const uint32_t q = base + threadIdx.x;
+        const uint32_t p = q * kStride + phase;
+        const bool valid = q < a.trainPaths && p < a.paths;
         const float lf = valid ? luminance(v3(a.radiance[p])) : 0.0f;
         for (uint32_t b = 0; b < a.levels; ++b) {
-            const float4* rec = a.pathRec + (size_t(b) * a.paths + (valid ? p : 0u)) * 4;
+            const float4* rec = a.pathRec + (size_t(b) * a.trainPaths + (valid ? q : 0u)) * 4;
             bool take = false;
             float w = 0.0f;
             if (valid) {
@@ -511,7 +577,7 @@ __global__ __launch_bounds__(256) void kGuideCollect(GuideCollectArgs a) {
                 }
             }
             (void)waveAppend(cand, take);
-            const bool kept = take && (keep >= 1.0f || hashUnit(p, b, frame) < keep);
+            const bool kept = take && (keep >= 1.0f || hashUnit(p, b, stamp) < keep);
             const uint32_t slot = waveAppend(a.words + kGwPool, kept);
             if (kept && slot < a.poolCap) {
                 const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
@@ -532,51 +598,59 @@ static uint32_t cappedGrid(uint32_t rows, uint32_t rowsPerBlock, uint32_t cap) {
     return g ? g : 1;
 }
 
-hipError_t launchGuideInfer(const RecView& rec, const uint32_t* countPtr, uint32_t maxCount, const float* params,
-                            void* out, bool decoded, hipStream_t s) {
-    hipLaunchKernelGGL(kGuideInfer, dim3(cappedGrid(maxCount, 128, 512)), dim3(kInferBlock), 0, s, rec, countPtr,
-                       params, out, decoded ? 1 : 0);
+hipError_t launchGuidePack(const float* params, void* image, hipStream_t s) {
+    hipLaunchKernelGGL(kGuidePack, dim3((kWTotal + 320 + 255) / 256), dim3(256), 0, s, params,
+                       static_cast<uint16_t*>(image));
     return hipGetLastError();
 }
 
-// b.cap bounds the batch; the workspace stride is net.cap.
+hipError_t launchGuideInfer(const RecView& rec, const uint32_t* countPtr, uint32_t maxCount, const void* image,
+                            void* out, bool decoded, hipStream_t s) {
+    hipLaunchKernelGGL(kGuideInfer, dim3(cappedGrid(maxCount, kFwdWaves * 32, 512)), dim3(kFwdWaves * 64), 0, s, rec,
+                       countPtr, static_cast<const u32x4*>(image), out, decoded ? 1 : 0);
+    return hipGetLastError();
+}
+
+// b.cap bounds the batch; the workspace is sized for net.cap rows.
 hipError_t launchGuideGradient(const GuideBatch& b, const GuideNet& net, bool countStep, hipStream_t s) {
-    if (b.cap > net.cap || b.cap == 0) return hipErrorInvalidValue;
+    if (b.cap > net.cap || b.cap == 0 || net.stride != guideRowStride(net.cap)) return hipErrorInvalidValue;
+    const uint32_t rows = guideRowStride(b.cap);
     hipError_t e;
     hipLaunchKernelGGL(kGuideStepBegin, dim3(1), dim3(64), 0, s, b, net.words, countStep ? 1 : 0);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(kGuideWeightSum, dim3(1), dim3(kWeightSumBlock), 0, s, b, net.words);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    GuideBatch ws = b;
-    ws.cap = net.cap;
-    hipLaunchKernelGGL(kGuideFwdTrain, dim3(cappedGrid(b.cap, 128, 1024)), dim3(256), 0, s, ws, net.words,
-                       net.params, net.acts, net.raw);
+    hipLaunchKernelGGL(kGuideFwdTrain, dim3(cappedGrid(rows, kFwdWaves * 32, 1024)), dim3(kFwdWaves * 64), 0, s, b,
+                       net.words, static_cast<const u32x4*>(net.image), net);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(kGuideOutGrad, dim3(cappedGrid(b.cap, 256, 1u << 20)), dim3(256), 0, s, b, net.words, net.raw,
-                       net.delta[0]);
+    hipLaunchKernelGGL(kGuideOutGrad, dim3(cappedGrid(rows, 256, 1u << 20)), dim3(256), 0, s, b, net.words, net);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    const size_t actBase = size_t(net.cap) * kGuideInPad;
+    const uint32_t chunks = guideGradChunks(b.cap);
     int cur = 0;
     for (int l = 4; l >= 0; --l) {
-        const uint16_t* prev = l == 0 ? net.acts : net.acts + actBase + size_t(l - 1) * net.cap * 64;
-        const uint32_t stride = l == 0 ? kGuideInPad : 64;
-        hipLaunchKernelGGL(kGuideWeightGrad, dim3(cappedGrid(b.cap, kGradRowsPerBlock, 1u << 20)), dim3(256), 0, s, l,
-                           net.words, net.delta[cur], prev, stride, net.grad);
+        const uint16_t* prev = l == 0 ? net.actIn : net.act + size_t(l - 1) * 64 * net.stride;
+        const uint32_t jobs = chunks * (l == 0 ? 6u : 4u);
+        hipLaunchKernelGGL(kGuideWeightGrad, dim3((jobs + 3) / 4), dim3(256), 0, s, l, net.words, net.delta[cur], prev,
+                           net.stride, net.partial);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if (l > 0) {
-            hipLaunchKernelGGL(kGuideBackLayer, dim3(cappedGrid(b.cap, 128, 1024)), dim3(256), 0, s, l, net.words,
-                               net.params, net.delta[cur], prev, net.delta[cur ^ 1]);
+            hipLaunchKernelGGL(kGuideBackLayer, dim3(cappedGrid(rows, 128, 1024)), dim3(256), 0, s, l, net.words,
+                               net.params, net.delta[cur], prev, net.delta[cur ^ 1], net.stride);
             if ((e = hipGetLastError()) != hipSuccess) return e;
             cur ^= 1;
         }
     }
-    return hipSuccess;
+    hipLaunchKernelGGL(kGuideGradReduce, dim3((kGuideParams + 255) / 256), dim3(256), 0, s, net.words, net.partial,
+                       net.grad);
+    return hipGetLastError();
 }
 
 hipError_t launchGuideAdam(const GuideNet& net, const GuideAdam& h, hipStream_t s) {
     hipLaunchKernelGGL(kGuideAdam, dim3((kGuideParams + 255) / 256), dim3(256), 0, s, net.params, net.adamM,
                        net.adamV, net.grad, net.words, h);
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launchGuidePack(net.params, net.image, s);
 }
 
 hipError_t launchGuideSample(const float* raw, const float* u, uint32_t count, float4* out, hipStream_t s) {
@@ -597,8 +671,8 @@ hipError_t launchGuideFrameBegin(uint32_t* words, const FrameConsts* g, hipStrea
 }
 
 hipError_t launchGuideCollect(const GuideCollectArgs& a, hipStream_t s) {
-    if (a.paths == 0) return hipSuccess;
-    hipLaunchKernelGGL(kGuideCollect, dim3(cappedGrid(a.paths, 256, 2048)), dim3(256), 0, s, a);
+    if (a.trainPaths == 0) return hipSuccess;
+    hipLaunchKernelGGL(kGuideCollect, dim3(cappedGrid(a.trainPaths, 256, 2048)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

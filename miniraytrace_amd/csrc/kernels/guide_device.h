@@ -34,18 +34,10 @@ __host__ __device__ constexpr int guideBiasOffset(int l) {
 }
 static_assert(guideBiasOffset(4) + kGuideOut == kGuideParams, "guide parameter layout");
 
-// Feature k of a shading point (pos01 = position normalised to the scene box).
-// Order: axis a (x, y, z), band b, {sin, cos}; then normal, then wo.
-__device__ __forceinline__ float guideFeature(const float p[3], const float n[3], const float wo[3], int k) {
-    if (k < 72) {
-        const int a = k / 24, r = k % 24, b = r >> 1;
-        const float ang = p[a] * float(1 << b) * kPi;
-        return (r & 1) ? cosf(ang) : sinf(ang);
-    }
-    if (k < 75) return n[k - 72];
-    if (k < 78) return wo[k - 75];
-    return 0.0f;
-}
+// Features of a shading point (pos01 = position normalised to the scene
+// box): for axis a (x, y, z) and band b < 12 the pair sin(pi 2^b p_a),
+// cos(pi 2^b p_a) at 24 a + 2 b; then the normal (72..74) and wo (75..77);
+// 78 and 79 are zero pads.  guide.hip's encodeRows forms them.
 
 struct GuideMix {
     V3 mu[kGuideLobes];

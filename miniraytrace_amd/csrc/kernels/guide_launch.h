@@ -20,6 +20,14 @@ constexpr int kGuideParamCount = 20136;  // flat FP32 parameter vector
 constexpr int kGuideOutputs = 40;        // raw outputs per query
 constexpr int kGuideInputsPadded = 80;   // bf16 features per row (78 used)
 constexpr int kGuideMixFloat4 = 12;      // decoded mixture per query (see guide_device.h)
+// Packed forward image: bf16 weights (21,504) then FP32 biases (5 x 64), in bytes.
+constexpr size_t kGuideImageBytes = 21504 * 2 + 320 * 4;
+constexpr uint32_t kGuideGradChunk = 2048;  // training rows per weight-gradient partial
+
+__host__ __device__ constexpr uint32_t guideRowStride(uint32_t cap) { return (cap + 31u) & ~31u; }
+__host__ __device__ constexpr uint32_t guideGradChunks(uint32_t cap) {
+    return (guideRowStride(cap) + kGuideGradChunk - 1) / kGuideGradChunk;
+}
 
 // Device state words of one network (uint32 / float bit patterns).
 enum GuideWord : uint32_t {
@@ -54,35 +62,45 @@ struct GuideBatch {
     uint32_t cap = 0;
 };
 
-// Network parameters, Adam moments and the workspace for batches of up to
-// `cap` rows.
+// Network parameters, Adam moments and the training workspace for batches of
+// up to `cap` rows.  Activations and deltas are stored feature-major
+// ([feature][row], row stride `stride` = cap rounded up to 32) so the matrix
+// kernels read 16-byte fragments.
 struct GuideNet {
-    float* params = nullptr;   // kGuideParamCount
+    float* params = nullptr;     // kGuideParamCount
     float* adamM = nullptr;
     float* adamV = nullptr;
-    float* grad = nullptr;     // zero between steps (Adam clears it)
-    uint16_t* acts = nullptr;  // bf16: cap x 80 inputs, then 4 x cap x 64 hidden activations
-    float* raw = nullptr;      // cap x 40
-    uint16_t* delta[2] = { nullptr, nullptr };  // bf16 cap x 64 each
-    uint32_t* words = nullptr; // kGwWords
+    float* grad = nullptr;       // written by the gradient reduction
+    void* image = nullptr;       // packed forward image (kGuideImageBytes), kept in step with params
+    uint16_t* actIn = nullptr;   // bf16 [96][stride] encoded inputs (rows 78..95 zero)
+    uint16_t* act = nullptr;     // bf16 [4][64][stride] hidden-layer outputs
+    float* raw = nullptr;        // FP32 [40][stride] raw outputs
+    uint16_t* delta[2] = { nullptr, nullptr };  // bf16 [64][stride] back-propagated error
+    float* partial = nullptr;    // FP32 [guideGradChunks(cap)][kGuideParamCount]
+    uint32_t* words = nullptr;   // kGwWords
     uint32_t cap = 0;
+    uint32_t stride = 0;
 };
 
 struct GuideAdam {
     float lr = 1e-3f, beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
 };
 
+// Packs params into the forward image (bf16 weights, padded, and biases).
+hipError_t launchGuidePack(const float* params, void* image, hipStream_t s);
+
 // Network outputs for the first *countPtr rows of rec (countPtr in device
 // memory, at most maxCount).  decoded: kGuideMixFloat4 float4 per row
 // (the renderer's format); otherwise kGuideOutputs raw floats per row.
-hipError_t launchGuideInfer(const RecView& rec, const uint32_t* countPtr, uint32_t maxCount, const float* params,
+hipError_t launchGuideInfer(const RecView& rec, const uint32_t* countPtr, uint32_t maxCount, const void* image,
                             void* out, bool decoded, hipStream_t s);
 
-// grad += d(loss)/d(params) of one batch, loss = sum w (-log q) / sum w.
+// grad = d(loss)/d(params) of one batch, loss = sum w (-log q) / sum w.
 // countStep: advance the Adam step counter when the batch is not empty.
 hipError_t launchGuideGradient(const GuideBatch& b, const GuideNet& net, bool countStep, hipStream_t s);
 
-// Adam update from grad (skipped when the last batch was empty); clears grad.
+// Adam update from grad (skipped when the last batch was empty), then repacks
+// the forward image.
 hipError_t launchGuideAdam(const GuideNet& net, const GuideAdam& h, hipStream_t s);
 
 // Decodes raw outputs and samples the mixture with the given uniforms:
@@ -95,14 +113,17 @@ hipError_t launchGuidePdf(const float* raw, const float4* dirs, uint32_t count, 
 // Per-frame training-pool bookkeeping (clears this frame's counters).
 hipError_t launchGuideFrameBegin(uint32_t* words, const FrameConsts* g, hipStream_t s);
 
-// Path records written by the shade kernel, per bounce level and path:
+// Path records written by the shade kernel for the frame's training paths
+// (paths p with p % guideTrainStride == guideTrainPhase), per bounce level
+// and training path q = p / guideTrainStride:
 //   (pos01.xyz, luminance of the throughput before the vertex),
 //   (n.xyz, luminance of the path radiance when the vertex was shaded),
 //   (wo.xyz, FrameConsts::guideStamp as bits), (wi.xyz, -)
 struct GuideCollectArgs {
-    const float4* pathRec = nullptr;  // [level][path] x 4 float4
+    const float4* pathRec = nullptr;  // [level][trainPaths] x 4 float4
     const float4* radiance = nullptr; // per path: rgb radiance of this frame
-    uint32_t paths = 0;
+    uint32_t paths = 0;               // all paths of the frame
+    uint32_t trainPaths = 0;          // ceil(paths / guideTrainStride)
     uint32_t levels = 0;
     float4* pool = nullptr;           // training records
     uint32_t poolCap = 0;

@@ -267,7 +267,7 @@ __global__ __launch_bounds__(256) void kShade(SceneView sc, PathView pv, const F
                     if (GUIDE) {
                         const uint32_t gi = pv.guideIdx[k];
                         if (gi != kNoHit) {
-                            record = g.guideTrain != 0;
+                            record = g.guideTrain != 0 && p % g.guideTrainStride == g.guideTrainPhase;
                             if (g.guiding) {
                                 mix = pv.guideMix + size_t(gi) * kGuideMixF4;
                                 sp = g.guideSelectProb;
@@ -335,7 +335,8 @@ __global__ __launch_bounds__(256) void kShade(SceneView sc, PathView pv, const F
                                 newD = wiW;
                                 emitRay = true;
                                 if (GUIDE && record) {
-                                    float4* r = pv.pathRec + (size_t(bounce) * pv.capacity + p) * 4;
+                                    const uint32_t trainPaths = (pv.capacity + g.guideTrainStride - 1) / g.guideTrainStride;
+                                    float4* r = pv.pathRec + (size_t(bounce) * trainPaths + p / g.guideTrainStride) * 4;
                                     r[0] = f4((s.pos - v3(g.guideBoxMin)) * v3(g.guideBoxScale), lumBeta);
                                     r[1] = f4(s.nShade, luminance(L));
                                     r[2] = f4(-dir, __uint_as_float(g.guideStamp));
@@ -391,10 +392,11 @@ __global__ __launch_bounds__(BLOCK) void kShadow(SceneView sc, PathView pv,
                 L.y += c.y;
                 L.z += c.z;
                 pv.radiance[p] = L;
-                if (pv.pathRec) {
+                const uint32_t ts = gp->guideTrainStride;
+                if (pv.pathRec && p % ts == gp->guideTrainPhase) {
                     // NEE from a recorded vertex is not incident along its wi:
                     // move the record's radiance snapshot past it.
-                    float4* r = pv.pathRec + (size_t(bounce) * pv.capacity + p) * 4;
+                    float4* r = pv.pathRec + (size_t(bounce) * ((pv.capacity + ts - 1) / ts) + p / ts) * 4;
                     if (__float_as_uint(r[2].w) == gp->guideStamp) r[1].w += luminance(v3(c));
                 }
             }

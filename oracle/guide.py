@@ -65,12 +65,12 @@ def bf16(a: np.ndarray) -> np.ndarray:
 
 
 def encode_f32(pos01, nrm, wo) -> np.ndarray:
-    """encode() with the angle formed in float32 as p * 2^b * pi (the order
-    the device uses); sin / cos of that float32 angle in float64."""
+    """encode() of float32 positions the way the device forms it: x = p * 2^b
+    (exact in float32), then sin / cos of pi x (sinpi / cospi)."""
     p = np.asarray(pos01, np.float32)
     freq = (np.float32(2.0) ** np.arange(BANDS, dtype=np.float32))
-    ang = (p[:, :, None] * freq[None, None, :]) * np.float32(np.pi)
-    ang = ang.astype(np.float64)
+    x = (p[:, :, None] * freq[None, None, :]).astype(np.float64)
+    ang = np.pi * np.fmod(x, 2.0)
     sc = np.stack([np.sin(ang), np.cos(ang)], axis=-1).reshape(p.shape[0], 3 * BANDS * 2)
     return np.concatenate([sc, np.asarray(nrm, np.float64), np.asarray(wo, np.float64)], axis=1)
 
