@@ -289,42 +289,53 @@ def guided_workload(args, torch) -> dict:
     warm = 32
     t_g = _timed_frames(g, warm, torch)  # training from an untrained network, samples discarded
     g.reset_accumulation()
-    curve, done, chunk, reached = [], 0, 16, None
-    while done < spp:
+    # Render until guided has both reached plain's error and run plain's wall
+    # time (at most 2 x spp frames); checkpoints every 16 frames (readbacks untimed).
+    curve, done, chunk = [], 0, 16
+    while done < 2 * spp:
         t_g += _timed_frames(g, chunk, torch)
         done += chunk
         img_g = g.read_rgba32f(W, H)[..., :3].astype(np.float64)
-        mse_g = float(((img_g - img_r) ** 2).mean())
-        curve.append((done, mse_g))
-        if mse_g <= mse_p:
-            reached = done
+        curve.append((done, t_g, float(((img_g - img_r) ** 2).mean())))
+        if curve[-1][2] <= mse_p and t_g >= t_plain:
             break
     st = g.guiding_stats()
     g.set_profiling(True)
     g.render_frame()
     prof = g.frame_stats()
     g.close()
+
+    def crossing(key_idx, target, out_idx):
+        """Power-law interpolation of column out_idx where column key_idx meets target."""
+        for a, b in zip(curve, curve[1:]):
+            if (a[key_idx] - target) * (b[key_idx] - target) <= 0 and a[key_idx] != b[key_idx]:
+                f = np.log(target / a[key_idx]) / np.log(b[key_idx] / a[key_idx])
+                return float(a[out_idx] * (b[out_idx] / a[out_idx]) ** f)
+        return None
+
+    t_equal_rmse = crossing(2, mse_p, 1) if curve[0][2] > mse_p else curve[0][1]
+    mse_equal_time = crossing(1, t_plain, 2)
     out = {
         "workload": f"room {info['triangles']} triangles, 5 % window, sky + 0.5 deg sun, {W}x{H}, depth 8",
         "plain_spp": spp,
         "plain_s": t_plain,
         "plain_ms_per_frame": t_plain * 1e3 / spp,
         "plain_rmse": mse_p ** 0.5,
-        "guided_spp_to_match": reached,
         "guided_warmup_frames": warm,
-        "guided_s": t_g,
-        "guided_ms_per_frame": t_g * 1e3 / (warm + done),
-        "guided_rmse": curve[-1][1] ** 0.5,
-        "time_ratio": (t_g / t_plain) if reached else None,
+        "guided_ms_per_frame": curve[-1][1] * 1e3 / (warm + curve[-1][0]),
+        "guided_s_to_plain_rmse": t_equal_rmse,
+        "time_ratio": (t_equal_rmse / t_plain) if t_equal_rmse else None,
+        "guided_rmse_at_plain_time": mse_equal_time ** 0.5 if mse_equal_time else None,
+        "equal_time_rmse_ratio": (mse_equal_time / mse_p) ** 0.5 if mse_equal_time else None,
         "reference_spp": n_ref,
         "reference_s": t_ref,
-        "rmse_curve": [(n, m ** 0.5) for n, m in curve],
+        "curve_frames_s_rmse": [(n, t, m ** 0.5) for n, t, m in curve],
         "guided_stage_ms": prof["gpuMs"],
         "train_steps": st["steps"],
         "last_loss": st["lastLoss"],
     }
-    _log(f"guided: plain {spp} spp in {t_plain:.2f} s (RMSE {out['plain_rmse']:.4g}); guided reached it at "
-         f"{reached} spp in {t_g:.2f} s -> ratio {out['time_ratio']}")
+    _log(f"guided: plain {spp} spp in {t_plain:.2f} s (RMSE {out['plain_rmse']:.4g}); guided reaches it in "
+         f"{t_equal_rmse} s -> ratio {out['time_ratio']}; equal-time RMSE ratio {out['equal_time_rmse_ratio']}")
     return out
 
 

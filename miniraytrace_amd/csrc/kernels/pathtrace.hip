@@ -239,7 +239,22 @@ __global__ __launch_bounds__(256) void kShade(SceneView sc, PathView pv, const F
                 const V3 env = envRadiance(sc, g, dir);
                 if (maxComp(env) > 0.0f) {
                     float w = 1.0f;
-                    if (bounce > 0) w = balanceHeuristic(prevPdf, envPdf(sc, g, dir) * envPickPdf(g));
+                    if (bounce > 0) {
+                        const float pl = envPdf(sc, g, dir) * envPickPdf(g);
+                        w = balanceHeuristic(prevPdf, pl);
+                        if (GUIDE && g.guideTrain && p % g.guideTrainStride == g.guideTrainPhase) {
+                            // Training target of the previous vertex: its own
+                            // continuation's environment hit counts with the MIS
+                            // weight of the BSDF pdf (L4.w), not of the guided
+                            // mixture, so light that NEE already samples well
+                            // (the sun) does not pull the mixture towards itself.
+                            const float wTrain = balanceHeuristic(L4.w, pl);
+                            const uint32_t ts = g.guideTrainStride;
+                            float4* r = pv.pathRec + (size_t(bounce - 1) * ((pv.capacity + ts - 1) / ts) + p / ts) * 4;
+                            if (wTrain != w && __float_as_uint(r[2].w) == g.guideStamp)
+                                r[1].w -= luminance(beta * env) * (wTrain - w);
+                        }
+                    }
                     L = L + beta * env * w;
                 }
             } else {
@@ -282,8 +297,12 @@ __global__ __launch_bounds__(256) void kShade(SceneView sc, PathView pv, const F
                         float bsdfPdf;
                         bsdfEval(bsdf, wo, wi, f, bsdfPdf);
                         if (maxComp(f) > 0.0f) {
+                            // MIS against the continuation's pdf.  At the last bounce
+                            // no continuation is drawn and the reference weights NEE
+                            // against the BSDF pdf alone; guided shading does the same,
+                            // so both tracers converge to the same image.
                             float contPdf = bsdfPdf;
-                            if (GUIDE && mix) contPdf = sp * storedMixPdf(mix, ls.wi) + (1.0f - sp) * bsdfPdf;
+                            if (GUIDE && mix && !last) contPdf = sp * storedMixPdf(mix, ls.wi) + (1.0f - sp) * bsdfPdf;
                             const float w = ls.delta ? 1.0f : balanceHeuristic(ls.pdf, contPdf);
                             shO = s.pos + s.nGeom * (wi.z > 0.0f ? kEpsRay : -kEpsRay);
                             shD = ls.wi;
@@ -295,7 +314,7 @@ __global__ __launch_bounds__(256) void kShade(SceneView sc, PathView pv, const F
                     // BSDF bounce and Russian roulette (pathtrace.comp:200-216)
                     if (!last) {
                         V3 wiL, wiW, f;
-                        float pdf = 0.0f;
+                        float pdf = 0.0f, pBsdf = 0.0f;
                         bool ok;
                         if (GUIDE && mix && rng.next1() < sp) {
                             // Draw from the learned mixture.
@@ -304,9 +323,8 @@ __global__ __launch_bounds__(256) void kShade(SceneView sc, PathView pv, const F
                             rng.next2(u1, u2);
                             wiW = storedMixSample(mix, u0, u1, u2);
                             wiL = v3(dot(wiW, T), dot(wiW, B), dot(wiW, s.nShade));
-                            float pB;
-                            bsdfEval(bsdf, wo, wiL, f, pB);
-                            pdf = sp * storedMixPdf(mix, wiW) + (1.0f - sp) * pB;
+                            bsdfEval(bsdf, wo, wiL, f, pBsdf);
+                            pdf = sp * storedMixPdf(mix, wiW) + (1.0f - sp) * pBsdf;
                             ok = maxComp(f) > 0.0f && pdf > 1e-9f;
                         } else {
                             const BsdfSample bs = bsdfSample(bsdf, wo, rng);
@@ -314,6 +332,7 @@ __global__ __launch_bounds__(256) void kShade(SceneView sc, PathView pv, const F
                             wiL = bs.wi;
                             f = bs.f;
                             pdf = bs.pdf;
+                            pBsdf = bs.pdf;
                             if (ok) wiW = normalize(T * bs.wi.x + B * bs.wi.y + s.nShade * bs.wi.z);
                             if (GUIDE && mix && ok) {
                                 pdf = sp * storedMixPdf(mix, wiW) + (1.0f - sp) * bs.pdf;
@@ -324,6 +343,7 @@ __global__ __launch_bounds__(256) void kShade(SceneView sc, PathView pv, const F
                             const float lumBeta = luminance(beta);
                             beta = beta * f * (fabsf(wiL.z) / pdf);
                             prevPdf = pdf;
+                            if (GUIDE) L4.w = pBsdf;  // BSDF pdf of this continuation
                             bool alive = true;
                             if (bounce >= 3) {
                                 const float q = clampf(maxComp(beta), 0.05f, 0.95f);

@@ -45,7 +45,7 @@ enum CounterSlot : uint32_t {
 struct PathView {
     uint32_t capacity;               // paths (= pixels) per frame
     float4* beta;                    // rgb throughput + previous BSDF pdf
-    float4* radiance;                // rgb radiance + flag bits
+    float4* radiance;                // rgb radiance + BSDF pdf of the last continuation (guided shading)
     uint32_t* rngCounter;            // draws consumed so far
     float4* rayO[2];                 // queued ray origin xyz + path index bits
     float4* rayD[2];                 // queued ray direction xyz
