@@ -11,6 +11,8 @@ Workloads:
              rays (one step = one mrtxTraceClosest launch over all rays).
   guided     equal-RMSE wall time of guided vs plain path tracing on a room
              lit through a window (1280x720, 499k triangles).
+  batch      64 uneven requests rendered by one batched graph launch vs one
+             frame call at a time (images must be identical).
 
 Prints ONE JSON line.  Inputs are resident in device memory before the timed
 region; the timed region is bracketed by device synchronisation.  The
@@ -339,6 +341,78 @@ def guided_workload(args, torch) -> dict:
     return out
 
 
+# -------------------------------------------------------------------- batched --
+def batch_workload(args, torch) -> dict:
+    """Batched plain rendering: 64 seeded requests (resolution log-uniform
+    from 256x256 to 1920x1080, terrain-plus-spheres scenes of 10k-2M
+    triangles, 1-16 spp, depth 8), launch shapes picked per request by the
+    autotuner, all rendered by one mrtxRenderBatch call (one graph launch).
+    The same engines then render the same frames one mrtRenderFrame call at a
+    time; the images must be identical."""
+    from miniraytrace_amd import scenes
+    from miniraytrace_amd.batch import render_batch
+    from miniraytrace_amd.engine import Engine
+    reqs = scenes.batch_requests(args.batch_requests, 1000)
+    sky = scenes.sky()
+    t0 = time.perf_counter()
+    engines = []
+    for r in reqs:
+        e = Engine(width=r["width"], height=r["height"])
+        e.set_render_settings(width=r["width"], height=r["height"], max_bounces=8, spp_limit=1 << 20,
+                              firefly_clamp=0.0, denoise=False)
+        scenes.load_terrain_scene(e, scenes.terrain_config(r["triangles"]), seed=r["seed"], sky_img=sky)
+        e.commit()
+        engines.append(e)
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    shapes = [e.autotune_path() for e in engines]
+    t_tune = time.perf_counter() - t0
+    _log(f"batch: {len(reqs)} requests, setup {t_setup:.1f} s, autotune {t_tune:.1f} s")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = render_batch(engines, [r["spp"] for r in reqs])
+    t_batch = time.perf_counter() - t0
+    imgs = [e.read_rgba32f(r["width"], r["height"]) for e, r in zip(engines, reqs)]
+    for e in engines:
+        e.reset_accumulation()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e, r in zip(engines, reqs):
+        for _ in range(r["spp"]):
+            e.render_frame()
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    identical = all(np.array_equal(e.read_rgba32f(r["width"], r["height"]), im)
+                    for e, r, im in zip(engines, reqs, imgs))
+    for e in engines:
+        e.close()
+    samples = sum(r["width"] * r["height"] * r["spp"] for r in reqs)
+    shape_counts = {}
+    for sh in shapes:
+        k = f"extend {sh['extend'][0]}x{sh['extend'][1]} / shadow {sh['shadow'][0]}x{sh['shadow'][1]}"
+        shape_counts[k] = shape_counts.get(k, 0) + 1
+    out = {
+        "workload": f"{len(reqs)} requests, 256x256..1920x1080, 10k..2M triangles, 1..16 spp, depth 8",
+        "requests": len(reqs),
+        "samples": samples,
+        "triangles_total": sum(r["triangles"] for r in reqs),
+        "batch_s": t_batch,
+        "batch_capture_ms": st["captureMs"],
+        "batch_launch_ms": st["launchMs"],
+        "msamples_s_batch": samples / t_batch / 1e6,
+        "sequential_s": t_seq,
+        "msamples_s_sequential": samples / t_seq / 1e6,
+        "speedup_vs_sequential": t_seq / t_batch,
+        "identical_to_sequential": identical,
+        "autotune_s": t_tune,
+        "setup_s": t_setup,
+        "shapes": shape_counts,
+    }
+    _log(f"batch: {samples / 1e6:.1f} Msamples in {t_batch:.2f} s (capture {st['captureMs']:.0f} ms) vs "
+         f"sequential {t_seq:.2f} s; identical {identical}")
+    return out
+
+
 def copy_workload(args, torch) -> dict:
     """Plain device copy of a known size (1 GiB), used to calibrate the HBM
     byte counters (FETCH_SIZE / WRITE_SIZE) against known traffic."""
@@ -362,10 +436,12 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="pt", choices=["pt", "traversal", "guided", "copy"])
+    ap.add_argument("--workload", default="pt", choices=["pt", "traversal", "guided", "batch", "copy"])
     ap.add_argument("--guided-spp", type=int, default=1024,
                     help="plain sample count whose RMSE the guided run must reach")
     ap.add_argument("--no-guided", dest="with_guided", action="store_false", default=True)
+    ap.add_argument("--batch-requests", type=int, default=64)
+    ap.add_argument("--no-batch", dest="with_batch", action="store_false", default=True)
     ap.add_argument("--autotune", action="store_true", default=True)
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -386,6 +462,12 @@ def main() -> int:
     if args.workload == "copy":
         res = copy_workload(args, torch)
         print(json.dumps({"workload": "copy", **res}))
+        return 0
+    if args.workload == "batch":
+        res = batch_workload(args, torch)
+        print(json.dumps({"metric": "batched Msamples/s", "value": res["msamples_s_batch"], "unit": "Msamples/s",
+                          "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+                          "config": {"workload": res["workload"]}, "batch": res}))
         return 0
     if args.workload == "guided":
         res = guided_workload(args, torch)
@@ -430,6 +512,8 @@ def main() -> int:
     if args.with_traversal:
         trav = traversal_workload(args, torch)
         line["traversal"] = trav
+    if args.with_batch:
+        line["batch"] = batch_workload(args, torch)
     if args.with_guided:
         gd = guided_workload(args, torch)
         line["guided"] = gd

@@ -131,8 +131,9 @@ MRT_API mrtResult mrtxGetPathOptions(mrtEngine* engine, mrtxPathOptions* out);
 
 #define MRTX_MAX_BOUNCE_LEVELS 18
 
-/* Stage order of gpuMs: generate, extend, shade, shadow, accumulate, resolve,
- * guide (prep + network inference), train (record collection + Adam steps). */
+/* Stage order of gpuMs: generate, extend, shade, shadow, accumulate (with the
+ * RGBA8 resolve, one kernel), resolve (0: part of accumulate), guide (prep +
+ * network inference), train (record collection + Adam steps). */
 typedef struct mrtxFrameStats {
     size_t   structSize;
     float    frameMs;                                /* host wall time of the last frame */
@@ -169,7 +170,43 @@ typedef struct mrtxSceneExport {
 
 MRT_API mrtResult mrtxExportScene(mrtEngine* engine, mrtxSceneExport* inout);
 
-/* ---- 5. path guiding -------------------------------------------------------
+/* ---- 5. batched rendering -----------------------------------------------------
+ * No counterpart in the reference ABI: many engines (render requests of any
+ * resolution and scene) progress together in one HIP graph launch.  Every
+ * engine ends in exactly the state that `frames` calls of mrtRenderFrame
+ * would leave (same images, bit for bit), clamped to its spp limit.  All
+ * engines must be on one device and appear once.
+ * Mode 0 (up to 64 requests, none guided): one set of persistent kernels per
+ * batched frame works on the queues of all requests at once (each wave takes
+ * 64-item chunks of any request's queue).  Mode 1 (otherwise): each request's
+ * frames are captured as a chain and the chains are independent nodes of the
+ * graph. */
+typedef struct mrtxBatchRequest {
+    mrtEngine* engine;
+    uint32_t   frames;        /* samples per pixel to add */
+    uint32_t   framesDone;    /* out: samples added (after the spp-limit clamp) */
+} mrtxBatchRequest;
+
+typedef struct mrtxBatchStats {
+    size_t   structSize;
+    float    captureMs;       /* host time to capture and instantiate the graph */
+    float    launchMs;        /* graph launch to completion */
+    uint32_t requests;        /* requests with work */
+    uint32_t frames;          /* frames rendered in total */
+    uint32_t mode;            /* 0 = shared persistent kernels, 1 = per-request graph chains */
+    uint32_t extendBlock, extendStack, shadowBlock, shadowStack;  /* mode 0 launch shapes */
+} mrtxBatchStats;
+
+MRT_API mrtResult mrtxRenderBatch(mrtxBatchRequest* requests, uint32_t count, mrtxBatchStats* stats);
+
+/* Path-tracer launch-shape autotuner: one profiled trial frame per candidate
+ * (block 64/128/256 x LDS stack 8/16/32); the fastest shape per stage
+ * (closest hit, shadow) is kept and reported.  Results are cached per
+ * (triangle count, pixel count, depth) class.  Images do not depend on the
+ * shape; the trial frames are discarded (accumulation restarts). */
+MRT_API mrtResult mrtxAutotunePath(mrtEngine* engine, mrtxPathOptions* chosen);
+
+/* ---- 6. path guiding -------------------------------------------------------
  * No counterpart in the reference: a learned directional distribution (an
  * MLP producing a mixture of 8 von Mises-Fisher lobes from the position,
  * shading normal and outgoing direction of a path vertex), trained online

@@ -137,6 +137,17 @@ class mrtxFrameStats(_Sized):
                 ("shadowRays", C.c_uint32 * MRTX_MAX_BOUNCE_LEVELS), ("status", C.c_uint32)]
 
 
+class mrtxBatchRequest(C.Structure):
+    _fields_ = [("engine", C.c_void_p), ("frames", C.c_uint32), ("framesDone", C.c_uint32)]
+
+
+class mrtxBatchStats(_Sized):
+    _fields_ = [("structSize", C.c_size_t), ("captureMs", C.c_float), ("launchMs", C.c_float),
+                ("requests", C.c_uint32), ("frames", C.c_uint32), ("mode", C.c_uint32),
+                ("extendBlock", C.c_uint32), ("extendStack", C.c_uint32), ("shadowBlock", C.c_uint32),
+                ("shadowStack", C.c_uint32)]
+
+
 MRTX_GUIDE_PARAM_COUNT = 20136
 MRTX_GUIDE_OUTPUTS = 40
 
@@ -217,6 +228,9 @@ _SIGS = {
     "mrtxSetProfiling": (_R, [_P, C.c_int32]),
     "mrtxGetFrameStats": (_R, [_P, C.POINTER(mrtxFrameStats)]),
     "mrtxExportScene": (_R, [_P, C.POINTER(mrtxSceneExport)]),
+    # ext: batching
+    "mrtxRenderBatch": (_R, [C.POINTER(mrtxBatchRequest), _U32, C.POINTER(mrtxBatchStats)]),
+    "mrtxAutotunePath": (_R, [_P, C.POINTER(mrtxPathOptions)]),
     # ext: path guiding
     "mrtxSetGuiding": (_R, [_P, C.POINTER(mrtxGuidingOptions)]),
     "mrtxGetGuiding": (_R, [_P, C.POINTER(mrtxGuidingOptions)]),

@@ -1,0 +1,218 @@
+// Batched rendering and the path-tracer autotuner (mini_raytrace_ext.h,
+// section 5).
+#include "capi_handles.h"
+#include "common.h"
+#include "engine.h"
+#include "mini_raytrace.h"
+#include "mini_raytrace_ext.h"
+
+#include <algorithm>
+#include <chrono>
+#include <map>
+#include <set>
+#include <vector>
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+float msSince(Clock::time_point t0) {
+    return std::chrono::duration<float, std::milli>(Clock::now() - t0).count();
+}
+
+// Owns the graphs of one batch call.
+struct BatchGraphs {
+    std::vector<hipGraph_t> chains;
+    hipGraph_t parent = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~BatchGraphs() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (parent) (void)hipGraphDestroy(parent);
+        for (hipGraph_t g : chains)
+            if (g) (void)hipGraphDestroy(g);
+    }
+};
+
+// The launch shape that most of the batch's work was tuned for (weighted by
+// pixels x frames of each request).
+mrt::dev::TraceShape voteShape(const std::vector<mrt::Engine*>& engines, const std::vector<uint32_t>& frames,
+                               bool shadow) {
+    std::map<std::pair<uint32_t, uint32_t>, double> weight;
+    for (size_t i = 0; i < engines.size(); ++i) {
+        const mrt::dev::TraceShape& s = shadow ? engines[i]->shadowShape() : engines[i]->extendShape();
+        weight[{ s.block, s.stack }] += double(engines[i]->pixels()) * frames[i];
+    }
+    auto best = std::max_element(weight.begin(), weight.end(),
+                                 [](const auto& a, const auto& b) { return a.second < b.second; });
+    mrt::dev::TraceShape t{ best->first.first, best->first.second, 0 };
+    int perCU = 0;
+    MRT_HIP(mrt::dev::batchTraceBlocksPerCU(t.block, t.stack, shadow, &perCU));
+    t.grid = uint32_t(std::max(1, perCU) * engines[0]->cuCount());
+    return t;
+}
+
+// Mode 0: all requests' queues served by one set of persistent kernels per
+// batched frame.
+void renderMerged(const std::vector<mrt::Engine*>& engines, std::vector<uint32_t>& frames, mrtxBatchStats& st) {
+    const size_t n = engines.size();
+    std::vector<const mrt::FrameConsts*> consts(n, nullptr);
+    for (size_t i = 0; i < n; ++i) frames[i] = engines[i]->prepareFrames(frames[i], &consts[i]);
+    const uint32_t F = *std::max_element(frames.begin(), frames.end());
+    if (F == 0) return;
+    std::vector<mrt::dev::BatchEntry> table(size_t(F) * n);
+    uint32_t slots = 0, levels = 0;
+    std::vector<uint32_t> base(n);
+    for (size_t i = 0; i < n; ++i) {
+        base[i] = slots;
+        slots += (engines[i]->pixels() + 63u) & ~63u;
+        levels = std::max(levels, engines[i]->bounceLevels());
+    }
+    for (uint32_t f = 0; f < F; ++f)
+        for (size_t i = 0; i < n; ++i) {
+            mrt::dev::BatchEntry& e = table[size_t(f) * n + i];
+            e.sc = engines[i]->sceneView();
+            e.pv = engines[i]->pathView();
+            e.g = frames[i] > 0 ? consts[i] + std::min(f, frames[i] - 1) : consts[0];
+            e.pixels = engines[i]->pixels();
+            e.pixelBase = base[i];
+            e.active = f < frames[i] ? 1u : 0u;
+            e.pad = 0;
+        }
+    mrt::DeviceBuffer dTable, dCursors;
+    dTable.allocate(table.size() * sizeof(mrt::dev::BatchEntry));
+    dTable.upload(table.data(), table.size() * sizeof(mrt::dev::BatchEntry));
+    dCursors.allocate(mrt::dev::kMaxBounceLevels * 4 * sizeof(uint32_t));
+    const mrt::dev::TraceShape ext = voteShape(engines, frames, false);
+    const mrt::dev::TraceShape sha = voteShape(engines, frames, true);
+    int perCU = 0;
+    MRT_HIP(mrt::dev::batchShadeBlocksPerCU(&perCU));
+    const uint32_t shadeGrid = uint32_t(std::max(1, perCU) * engines[0]->cuCount());
+    hipStream_t s = engines[0]->stream();
+    BatchGraphs g;
+    g.chains.assign(1, nullptr);
+    MRT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+        for (uint32_t f = 0; f < F; ++f) {
+            mrt::dev::BatchView bv;
+            bv.entries = dTable.as<const mrt::dev::BatchEntry>() + size_t(f) * n;
+            bv.count = uint32_t(n);
+            bv.pixelSlots = slots;
+            bv.cursors = dCursors.as<uint32_t>();
+            MRT_HIP(mrt::dev::launchClearBatch(bv, s));
+            MRT_HIP(mrt::dev::launchGenerateBatch(bv, s));
+            for (uint32_t b = 0; b < levels; ++b) {
+                MRT_HIP(mrt::dev::launchExtendBatch(bv, b, ext, s));
+                MRT_HIP(mrt::dev::launchShadeBatch(bv, b, shadeGrid, s));
+                MRT_HIP(mrt::dev::launchShadowBatch(bv, b, sha, s));
+            }
+            MRT_HIP(mrt::dev::launchFinishBatch(bv, s));
+        }
+    } catch (...) {
+        hipGraph_t g2 = nullptr;
+        (void)hipStreamEndCapture(s, &g2);
+        if (g2) (void)hipGraphDestroy(g2);
+        throw;
+    }
+    MRT_HIP(hipStreamEndCapture(s, &g.chains[0]));
+    MRT_HIP(hipGraphInstantiate(&g.exec, g.chains[0], nullptr, nullptr, 0));
+    const auto t1 = Clock::now();
+    MRT_HIP(hipGraphLaunch(g.exec, s));
+    MRT_HIP(hipStreamSynchronize(s));
+    st.launchMs = msSince(t1);
+    st.mode = 0;
+    st.extendBlock = ext.block;
+    st.extendStack = ext.stack;
+    st.shadowBlock = sha.block;
+    st.shadowStack = sha.stack;
+}
+
+// Mode 1: one captured chain per request, chains as independent graph nodes.
+void renderChains(const std::vector<mrt::Engine*>& engines, std::vector<uint32_t>& frames, mrtxBatchStats& st) {
+    const size_t n = engines.size();
+    BatchGraphs g;
+    g.chains.assign(n, nullptr);
+    for (size_t i = 0; i < n; ++i) frames[i] = engines[i]->captureFrames(frames[i], &g.chains[i]);
+    MRT_HIP(hipGraphCreate(&g.parent, 0));
+    bool any = false;
+    for (size_t i = 0; i < n; ++i) {
+        if (!g.chains[i]) continue;
+        hipGraphNode_t node = nullptr;
+        MRT_HIP(hipGraphAddChildGraphNode(&node, g.parent, nullptr, 0, g.chains[i]));
+        any = true;
+    }
+    st.mode = 1;
+    if (!any) return;
+    MRT_HIP(hipGraphInstantiate(&g.exec, g.parent, nullptr, nullptr, 0));
+    hipStream_t s = engines[0]->stream();
+    const auto t1 = Clock::now();
+    MRT_HIP(hipGraphLaunch(g.exec, s));
+    MRT_HIP(hipStreamSynchronize(s));
+    st.launchMs = msSince(t1);
+}
+
+} // namespace
+
+extern "C" {
+
+MRT_API mrtResult mrtxRenderBatch(mrtxBatchRequest* reqs, uint32_t count, mrtxBatchStats* stats) {
+    if (count > 0 && !reqs) return MRT_ERROR_INVALID_ARGUMENT;
+    if (stats && stats->structSize < sizeof(mrtxBatchStats)) return MRT_ERROR_INVALID_ARGUMENT;
+    std::set<const mrtEngine*> seen;
+    int device = -1;
+    for (uint32_t i = 0; i < count; ++i) {
+        mrt::Engine* e = mrt::unwrap(reqs[i].engine);
+        if (!e) return MRT_ERROR_INVALID_HANDLE;
+        if (!seen.insert(reqs[i].engine).second) {
+            mrt::setLastErrorMessage("an engine appears twice in the batch");
+            return MRT_ERROR_INVALID_ARGUMENT;
+        }
+        if (device >= 0 && e->device() != device) {
+            mrt::setLastErrorMessage("batched engines must share one device");
+            return MRT_ERROR_INVALID_ARGUMENT;
+        }
+        device = e->device();
+        reqs[i].framesDone = 0;
+    }
+    return mrt::guarded([&] {
+        mrtxBatchStats st{};
+        st.structSize = sizeof(st);
+        if (count > 0) {
+            mrt::DeviceScope scope(device);
+            const auto t0 = Clock::now();
+            std::vector<mrt::Engine*> engines(count);
+            std::vector<uint32_t> frames(count);
+            bool guided = false;
+            for (uint32_t i = 0; i < count; ++i) {
+                engines[i] = mrt::unwrap(reqs[i].engine);
+                frames[i] = reqs[i].frames;
+                guided = guided || engines[i]->guideActive();
+            }
+            if (!guided && count <= mrt::dev::kMaxBatchRequests) renderMerged(engines, frames, st);
+            else renderChains(engines, frames, st);
+            st.captureMs = msSince(t0) - st.launchMs;
+            for (uint32_t i = 0; i < count; ++i) {
+                engines[i]->framesDone(frames[i]);
+                reqs[i].framesDone = frames[i];
+                if (frames[i] > 0) ++st.requests;
+                st.frames += frames[i];
+            }
+        }
+        if (stats) *stats = st;
+        return MRT_SUCCESS;
+    });
+}
+
+MRT_API mrtResult mrtxAutotunePath(mrtEngine* e, mrtxPathOptions* chosen) {
+    mrt::Engine* eng = mrt::unwrap(e);
+    if (!eng) return MRT_ERROR_INVALID_HANDLE;
+    if (chosen && chosen->structSize < sizeof(mrtxPathOptions)) return MRT_ERROR_INVALID_ARGUMENT;
+    return mrt::guarded([&] {
+        mrtxPathOptions o{};
+        o.structSize = sizeof(o);
+        eng->autotuneShapes(o);
+        if (chosen) *chosen = o;
+        return MRT_SUCCESS;
+    });
+}
+
+} // extern "C"

@@ -7,7 +7,9 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <thread>
+#include <tuple>
 
 namespace mrt {
 
@@ -171,17 +173,33 @@ void Engine::invalidateGraph() {
     graph_ = nullptr;
 }
 
+// Persistent grids: as many workgroups as fit on the device at once.
 void Engine::chooseShapes() {
     int perCU = 0;
-    MRT_HIP(dev::persistentGrid(extendShape_.block, extendShape_.stack, false, &perCU));
+    MRT_HIP(dev::batchTraceBlocksPerCU(extendShape_.block, extendShape_.stack, false, &perCU));
     extendShape_.grid = uint32_t(std::max(1, perCU) * cuCount_);
-    MRT_HIP(dev::persistentGrid(shadowShape_.block, shadowShape_.stack, true, &perCU));
+    MRT_HIP(dev::batchTraceBlocksPerCU(shadowShape_.block, shadowShape_.stack, true, &perCU));
     shadowShape_.grid = uint32_t(std::max(1, perCU) * cuCount_);
-    MRT_HIP(dev::shadeBlocksPerCU(false, &perCU));
+    MRT_HIP(dev::batchShadeBlocksPerCU(&perCU));
     shadeGrid_ = uint32_t(std::max(1, perCU) * cuCount_);
-    MRT_HIP(dev::shadeBlocksPerCU(true, &perCU));
+    MRT_HIP(dev::guidedShadeBlocksPerCU(&perCU));
     shadeGuideGrid_ = uint32_t(std::max(1, perCU) * cuCount_);
     invalidateGraph();
+}
+
+// Batch-table entries of this engine for frames using consts g0, g0 + 1, ...
+// (an engine's own frame runs the batch kernels as a batch of one).
+void Engine::fillEntries(dev::BatchEntry* out, const FrameConsts* g0, uint32_t frames) const {
+    for (uint32_t i = 0; i < frames; ++i) {
+        dev::BatchEntry& e = out[i];
+        std::memset(&e, 0, sizeof(e));
+        e.sc = sceneView();
+        e.pv = pathView();
+        e.g = g0 + i;
+        e.pixels = frameW_ * frameH_;
+        e.pixelBase = 0;
+        e.active = 1;
+    }
 }
 
 // ------------------------------------------------------------------- settings --
@@ -607,6 +625,9 @@ void Engine::allocFrame() {
     dShD_.allocate(n * 16);
     dShC_.allocate(n * 16);
     dCounters_.ensure(dev::kCounterWords * 4);
+    dCursors_.ensure(dev::kMaxBounceLevels * 4 * 4);
+    dEntry_.ensure(sizeof(dev::BatchEntry));
+    hEntry_.ensure(sizeof(dev::BatchEntry));
     dAccum_.allocate(n * 16);
     dOut8_.allocate(n * 4);
     dOut32_.allocate(n * 16);
@@ -764,29 +785,33 @@ void Engine::fillConsts(FrameConsts& g) const {
     g.guideBoxScale = make_float4(ps, ps, ps, 0.0f);
 }
 
-void Engine::enqueueFrame(hipStream_t s, bool timed) {
+void Engine::enqueueFrame(hipStream_t s, bool timed, const FrameConsts* g, const dev::BatchEntry* entry) {
     const dev::SceneView sc = sceneView();
     const dev::PathView pv = pathView();
-    const FrameConsts* g = dConsts_.as<const FrameConsts>();
     const uint32_t pixels = frameW_ * frameH_;
     const uint32_t maxB = std::min(settings_.maxBounces, 16u);
     const bool guide = pv.guideIdx != nullptr;
     const bool train = pv.pathRec != nullptr;
     const bool infer = guide && guide_.enableSampling;
     const uint32_t prepGrid = std::min<uint32_t>((pixels + 255) / 256, uint32_t(cuCount_) * 8);
+    dev::BatchView bv;
+    bv.entries = entry;
+    bv.count = 1;
+    bv.pixelSlots = (pixels + 63u) & ~63u;
+    bv.cursors = dCursors_.as<uint32_t>();
     auto elapsed = [&](hipEvent_t a, hipEvent_t b) {
         float ms = 0.0f;
         MRT_HIP(hipEventElapsedTime(&ms, a, b));
         return ms;
     };
     if (timed) MRT_HIP(hipEventRecord(ev_[0], s));
-    MRT_HIP(dev::launchClearWords(pv.counters, dev::kCounterWords, s));
+    MRT_HIP(dev::launchClearBatch(bv, s));
     if (train) MRT_HIP(dev::launchGuideFrameBegin(guideNet_->deviceWords(), g, s));
-    MRT_HIP(dev::launchGenerate(pv, g, pixels, s));
+    MRT_HIP(dev::launchGenerateBatch(bv, s));
     if (timed) MRT_HIP(hipEventRecord(ev_[1], s));
     for (uint32_t b = 0; b <= maxB; ++b) {
         if (timed) MRT_HIP(hipEventRecord(ev_[2], s));
-        MRT_HIP(dev::launchExtend(sc, pv, g, b, extendShape_, s));
+        MRT_HIP(dev::launchExtendBatch(bv, b, extendShape_, s));
         if (timed) MRT_HIP(hipEventRecord(ev_[3], s));
         if (guide) {
             MRT_HIP(dev::launchGuidePrep(sc, pv, g, b, prepGrid, s));
@@ -799,9 +824,10 @@ void Engine::enqueueFrame(hipStream_t s, bool timed) {
             }
         }
         if (timed) MRT_HIP(hipEventRecord(ev_[9], s));
-        MRT_HIP(dev::launchShade(sc, pv, g, b, guide ? shadeGuideGrid_ : shadeGrid_, guide, s));
+        if (guide) MRT_HIP(dev::launchShadeGuided(sc, pv, g, b, shadeGuideGrid_, s));
+        else MRT_HIP(dev::launchShadeBatch(bv, b, shadeGrid_, s));
         if (timed) MRT_HIP(hipEventRecord(ev_[4], s));
-        MRT_HIP(dev::launchShadow(sc, pv, g, b, shadowShape_, s));
+        MRT_HIP(dev::launchShadowBatch(bv, b, shadowShape_, s));
         if (timed) {
             MRT_HIP(hipEventRecord(ev_[5], s));
             MRT_HIP(hipEventSynchronize(ev_[5]));
@@ -812,24 +838,22 @@ void Engine::enqueueFrame(hipStream_t s, bool timed) {
         }
     }
     if (timed) MRT_HIP(hipEventRecord(ev_[10], s));
-    if (train) enqueueGuideTraining(s);
+    if (train) enqueueGuideTraining(s, g);
     if (timed) MRT_HIP(hipEventRecord(ev_[6], s));
-    MRT_HIP(dev::launchAccumulate(pv, g, pixels, s));
-    if (timed) MRT_HIP(hipEventRecord(ev_[7], s));
-    MRT_HIP(dev::launchResolve8(pv, g, pixels, s));
+    MRT_HIP(dev::launchFinishBatch(bv, s));  // accumulate + resolve
     if (timed) {
-        MRT_HIP(hipEventRecord(ev_[8], s));
-        MRT_HIP(hipEventSynchronize(ev_[8]));
+        MRT_HIP(hipEventRecord(ev_[7], s));
+        MRT_HIP(hipEventSynchronize(ev_[7]));
         stats_.stageMs[0] = elapsed(ev_[0], ev_[1]);
         stats_.stageMs[7] = elapsed(ev_[10], ev_[6]);
         stats_.stageMs[4] = elapsed(ev_[6], ev_[7]);
-        stats_.stageMs[5] = elapsed(ev_[7], ev_[8]);
+        stats_.stageMs[5] = 0.0f;
     }
 }
 
 // Training records of this frame -> pool -> stepsPerFrame Adam steps, batch t
 // taking pool rows t, t + steps, t + 2 steps, ...
-void Engine::enqueueGuideTraining(hipStream_t s) {
+void Engine::enqueueGuideTraining(hipStream_t s, const FrameConsts* g) {
     const uint32_t steps = guide_.stepsPerFrame;
     const uint32_t poolCap = guide_.batchSize * steps;
     dev::GuideCollectArgs ca;
@@ -841,7 +865,7 @@ void Engine::enqueueGuideTraining(hipStream_t s) {
     ca.pool = dGuidePool_.as<float4>();
     ca.poolCap = poolCap;
     ca.words = guideNet_->deviceWords();
-    ca.g = dConsts_.as<const FrameConsts>();
+    ca.g = g;
     MRT_HIP(dev::launchGuideCollect(ca, s));
     const dev::GuideNet net = guideNet_->view();
     dev::GuideAdam h;
@@ -868,9 +892,13 @@ void Engine::renderFrame(mrtFrameInfo* out) {
         if (++guideStamp_ == 0) guideStamp_ = 1;
         fillConsts(*g);
         MRT_HIP(hipMemcpyAsync(dConsts_.get(), g, sizeof(FrameConsts), hipMemcpyHostToDevice, stream_));
+        dev::BatchEntry* he = static_cast<dev::BatchEntry*>(hEntry_.get());
+        fillEntries(he, dConsts_.as<const FrameConsts>(), 1);
+        MRT_HIP(hipMemcpyAsync(dEntry_.get(), he, sizeof(dev::BatchEntry), hipMemcpyHostToDevice, stream_));
+        const dev::BatchEntry* entry = dEntry_.as<const dev::BatchEntry>();
         if (profiling_) {
             std::memset(stats_.stageMs, 0, sizeof(stats_.stageMs));
-            enqueueFrame(stream_, true);
+            enqueueFrame(stream_, true, dConsts_.as<const FrameConsts>(), entry);
             MRT_HIP(hipMemcpyAsync(hCounters_.get(), dCounters_.get(), dev::kCounterWords * 4,
                                    hipMemcpyDeviceToHost, stream_));
             MRT_HIP(hipStreamSynchronize(stream_));
@@ -884,7 +912,7 @@ void Engine::renderFrame(mrtFrameInfo* out) {
             if (!graphExec_) {
                 MRT_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
                 try {
-                    enqueueFrame(stream_, false);
+                    enqueueFrame(stream_, false, dConsts_.as<const FrameConsts>(), entry);
                 } catch (...) {
                     hipGraph_t g2 = nullptr;
                     (void)hipStreamEndCapture(stream_, &g2);
@@ -897,7 +925,7 @@ void Engine::renderFrame(mrtFrameInfo* out) {
             MRT_HIP(hipGraphLaunch(graphExec_, stream_));
             MRT_HIP(hipStreamSynchronize(stream_));
         } else {
-            enqueueFrame(stream_, false);
+            enqueueFrame(stream_, false, dConsts_.as<const FrameConsts>(), entry);
             MRT_HIP(hipStreamSynchronize(stream_));
         }
         ++frameIndex_;
@@ -926,6 +954,128 @@ void Engine::readFramebuffer(mrtPixelFormat fmt, void* dst, size_t dstSize) {
     MRT_HIP(dev::launchResolve32(pathView(), uint32_t(pixels), frameIndex_, stream_));
     MRT_HIP(hipMemcpyAsync(dst, dOut32_.get(), pixels * 16, hipMemcpyDeviceToHost, stream_));
     MRT_HIP(hipStreamSynchronize(stream_));
+}
+
+// -------------------------------------------------------------------- batching --
+// Captures the next `frames` frames (clamped to the spp limit) as one graph:
+// their constants are uploaded to a device array first, so every frame's
+// kernels read their own slot and the chain needs no host step between frames.
+// Commits, then uploads the constants of the next `frames` frames (clamped to
+// the spp limit) to a device array; returns the frame count.
+uint32_t Engine::prepareFrames(uint32_t frames, const FrameConsts** consts) {
+    *consts = nullptr;
+    commit(nullptr);
+    DeviceScope scope(device_);
+    MRT_HIP(hipStreamSynchronize(stream_));
+    const uint32_t left = frameIndex_ < settings_.sppLimit ? settings_.sppLimit - frameIndex_ : 0;
+    frames = std::min(frames, left);
+    if (frames == 0) return 0;
+    dBatchConsts_.ensure(size_t(frames) * sizeof(FrameConsts));
+    hBatchConsts_.ensure(size_t(frames) * sizeof(FrameConsts));
+    FrameConsts* h = static_cast<FrameConsts*>(hBatchConsts_.get());
+    const uint32_t f0 = frameIndex_, s0 = guideStamp_;
+    for (uint32_t i = 0; i < frames; ++i) {
+        frameIndex_ = f0 + i;
+        if (++guideStamp_ == 0) guideStamp_ = 1;
+        fillConsts(h[i]);
+    }
+    frameIndex_ = f0;
+    guideStamp_ = s0;
+    MRT_HIP(hipMemcpyAsync(dBatchConsts_.get(), h, size_t(frames) * sizeof(FrameConsts), hipMemcpyHostToDevice,
+                           stream_));
+    MRT_HIP(hipStreamSynchronize(stream_));
+    *consts = dBatchConsts_.as<const FrameConsts>();
+    return frames;
+}
+
+uint32_t Engine::captureFrames(uint32_t frames, hipGraph_t* out) {
+    *out = nullptr;
+    const FrameConsts* consts = nullptr;
+    frames = prepareFrames(frames, &consts);
+    if (frames == 0) return 0;
+    DeviceScope scope(device_);
+    std::vector<dev::BatchEntry> entries(frames);
+    fillEntries(entries.data(), consts, frames);
+    dBatchEntries_.ensure(frames * sizeof(dev::BatchEntry));
+    dBatchEntries_.upload(entries.data(), frames * sizeof(dev::BatchEntry), 0, stream_);
+    const dev::BatchEntry* dEntries = dBatchEntries_.as<const dev::BatchEntry>();
+    MRT_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    try {
+        for (uint32_t i = 0; i < frames; ++i) enqueueFrame(stream_, false, consts + i, dEntries + i);
+    } catch (...) {
+        hipGraph_t g2 = nullptr;
+        (void)hipStreamEndCapture(stream_, &g2);
+        if (g2) (void)hipGraphDestroy(g2);
+        throw;
+    }
+    MRT_HIP(hipStreamEndCapture(stream_, out));
+    return frames;
+}
+
+// The captured frames ran: advance the frame counters as renderFrame would.
+void Engine::framesDone(uint32_t frames) {
+    for (uint32_t i = 0; i < frames; ++i)
+        if (++guideStamp_ == 0) guideStamp_ = 1;
+    frameIndex_ += frames;
+}
+
+// Launch-shape autotuner for the path tracer: one profiled frame per
+// candidate (extend and shadow share the candidate), best stage time wins for
+// each stage.  Traversal results do not depend on the shape, so neither does
+// the image; the trial frames are discarded (accumulation restarts).  Results
+// are cached per (triangle count, pixel count, depth) class.
+void Engine::autotuneShapes(mrtxPathOptions& chosen) {
+    struct Key {
+        int tris, pixels, depth;
+        bool operator<(const Key& o) const {
+            return std::tie(tris, pixels, depth) < std::tie(o.tris, o.pixels, o.depth);
+        }
+    };
+    static std::mutex cacheMutex;
+    static std::map<Key, std::pair<dev::TraceShape, dev::TraceShape>> cache;
+    commit(nullptr);
+    const Key key{ int(std::lround(std::log2(double(std::max<size_t>(totalTris_, 1))) * 2.0)),
+                   int(std::lround(std::log2(double(frameW_) * frameH_) * 2.0)),
+                   int(std::min(settings_.maxBounces, 16u)) };
+    bool hit = false;
+    {
+        std::lock_guard<std::mutex> lock(cacheMutex);
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+            extendShape_.block = it->second.first.block;
+            extendShape_.stack = it->second.first.stack;
+            shadowShape_.block = it->second.second.block;
+            shadowShape_.stack = it->second.second.stack;
+            hit = true;
+        }
+    }
+    if (!hit) {
+        static const uint32_t kCand[][2] = { { 64, 16 }, { 128, 8 }, { 128, 16 }, { 128, 32 }, { 256, 8 }, { 256, 16 } };
+        const bool prof = profiling_;
+        const uint32_t sppLimit = settings_.sppLimit;
+        settings_.sppLimit = std::max(sppLimit, 1u << 30);
+        profiling_ = true;
+        float bestE = 1e30f, bestS = 1e30f;
+        dev::TraceShape be = extendShape_, bs = shadowShape_;
+        for (const auto& c : kCand) {
+            extendShape_.block = shadowShape_.block = c[0];
+            extendShape_.stack = shadowShape_.stack = c[1];
+            chooseShapes();
+            resetAccumulation();
+            renderFrame(nullptr);
+            if (stats_.stageMs[1] < bestE) { bestE = stats_.stageMs[1]; be = extendShape_; }
+            if (stats_.stageMs[3] < bestS) { bestS = stats_.stageMs[3]; bs = shadowShape_; }
+        }
+        profiling_ = prof;
+        settings_.sppLimit = sppLimit;
+        extendShape_ = be;
+        shadowShape_ = bs;
+        std::lock_guard<std::mutex> lock(cacheMutex);
+        cache[key] = { be, bs };
+    }
+    chooseShapes();
+    resetAccumulation();
+    getPathOptions(chosen);
 }
 
 // ------------------------------------------------------------------ guiding --

@@ -31,7 +31,7 @@ struct Settings {
 
 struct FrameStats {
     float frameMs = 0.0f;
-    float stageMs[8] = {};  // generate, extend, shade, shadow, accumulate, resolve, guide, train
+    float stageMs[8] = {};  // generate, extend, shade, shadow, accumulate + resolve, (0), guide, train
     uint32_t rays[dev::kMaxBounceLevels] = {};
     uint32_t shadowRays[dev::kMaxBounceLevels] = {};
     uint32_t status = 0;
@@ -64,6 +64,19 @@ public:
     void guidingStats(mrtxGuidingStats& o);
     void getGuideParams(float* dst);
     void setGuideParams(const float* src);
+    // batching
+    uint32_t captureFrames(uint32_t frames, hipGraph_t* out);
+    uint32_t prepareFrames(uint32_t frames, const FrameConsts** consts);
+    dev::SceneView sceneView() const;
+    dev::PathView pathView() const;
+    bool guideActive() const { return guide_.enableSampling || guide_.enableTraining; }
+    const dev::TraceShape& extendShape() const { return extendShape_; }
+    const dev::TraceShape& shadowShape() const { return shadowShape_; }
+    uint32_t pixels() const { return frameW_ * frameH_; }
+    uint32_t bounceLevels() const { return std::min(settings_.maxBounces, 16u) + 1; }
+    int cuCount() const { return cuCount_; }
+    void framesDone(uint32_t frames);
+    void autotuneShapes(mrtxPathOptions& chosen);
     int device() const { return device_; }
     hipStream_t stream() const { return stream_; }
 
@@ -85,16 +98,14 @@ private:
     void uploadTextures();
     void allocFrame();
     void fillConsts(FrameConsts& g) const;
-    void enqueueFrame(hipStream_t s, bool timed);
-    dev::SceneView sceneView() const;
-    dev::PathView pathView() const;
+    void enqueueFrame(hipStream_t s, bool timed, const FrameConsts* g, const dev::BatchEntry* entry);
+    void fillEntries(dev::BatchEntry* out, const FrameConsts* g0, uint32_t frames) const;
     void invalidateGraph();
     void chooseShapes();
-    bool guideActive() const { return guide_.enableSampling || guide_.enableTraining; }
     uint32_t guideLevels() const { return std::min(settings_.maxBounces, 16u) + 1; }
     void ensureGuideNet();
     void allocGuide();
-    void enqueueGuideTraining(hipStream_t s);
+    void enqueueGuideTraining(hipStream_t s, const FrameConsts* g);
 
     int device_ = 0;
     int cuCount_ = 0;
@@ -129,7 +140,8 @@ private:
     // frame buffers
     DeviceBuffer dConsts_, dBeta_, dRad_, dRng_, dRayO_[2], dRayD_[2], dHit_, dHitInst_;
     DeviceBuffer dShO_, dShD_, dShC_, dCounters_, dAccum_, dOut8_, dOut32_;
-    PinnedBuffer hConsts_, hReadback_, hCounters_;
+    PinnedBuffer hConsts_, hReadback_, hCounters_, hBatchConsts_, hEntry_;
+    DeviceBuffer dBatchConsts_, dBatchEntries_, dEntry_, dCursors_;
     uint32_t frameW_ = 0, frameH_ = 0;
 
     dev::TraceShape extendShape_{ 128, 16, 0 };

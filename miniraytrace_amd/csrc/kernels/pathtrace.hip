@@ -97,62 +97,129 @@ __device__ __forceinline__ Bsdf makeBsdf(const SceneView& sc, const Surface& s) 
 }
 
 // ---------------------------------------------------------------- generate --
-__global__ __launch_bounds__(256) void kGenerate(PathView pv, const FrameConsts* __restrict__ gp) {
-    const FrameConsts g = *gp;
-    const uint32_t n = g.width * g.height;
-    if (blockIdx.x == 0 && threadIdx.x == 0) pv.counters[kCntRays] = n;
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-        const uint32_t px = p % g.width, py = p / g.width;
-        Rng rng{ px, py, g.frameIndex ^ g.seedMix, 0u };
-        float jx, jy;
-        rng.next2(jx, jy);
-        const float ux = (float(px) + jx) / float(g.width);
-        const float uy = (float(py) + jy) / float(g.height);
-        const float sx = g.camFrustum.x * (1.0f - ux) + g.camFrustum.y * ux;
-        const float sy = g.camFrustum.w * (1.0f - uy) + g.camFrustum.z * uy;
-        V3 o, d;
-        if (g.flags & kFlagOrthographic) {
-            o = v3(g.camPos) + v3(g.camRight) * sx + v3(g.camUp) * sy;
-            d = v3(g.camForward);
-        } else {
-            o = v3(g.camPos);
-            d = normalize(v3(g.camForward) + v3(g.camRight) * sx + v3(g.camUp) * sy);
-        }
-        pv.rayO[0][p] = f4(o, __uint_as_float(p));
-        pv.rayD[0][p] = f4(d, 0.0f);
-        pv.beta[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-        pv.radiance[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        pv.rngCounter[p] = rng.counter;
+__device__ __forceinline__ void generatePixel(const PathView& pv, const FrameConsts& g, uint32_t p) {
+    const uint32_t px = p % g.width, py = p / g.width;
+    Rng rng{ px, py, g.frameIndex ^ g.seedMix, 0u };
+    float jx, jy;
+    rng.next2(jx, jy);
+    const float ux = (float(px) + jx) / float(g.width);
+    const float uy = (float(py) + jy) / float(g.height);
+    const float sx = g.camFrustum.x * (1.0f - ux) + g.camFrustum.y * ux;
+    const float sy = g.camFrustum.w * (1.0f - uy) + g.camFrustum.z * uy;
+    V3 o, d;
+    if (g.flags & kFlagOrthographic) {
+        o = v3(g.camPos) + v3(g.camRight) * sx + v3(g.camUp) * sy;
+        d = v3(g.camForward);
+    } else {
+        o = v3(g.camPos);
+        d = normalize(v3(g.camForward) + v3(g.camRight) * sx + v3(g.camUp) * sy);
+    }
+    pv.rayO[0][p] = f4(o, __uint_as_float(p));
+    pv.rayD[0][p] = f4(d, 0.0f);
+    pv.beta[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    pv.radiance[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    pv.rngCounter[p] = rng.counter;
+}
+
+// ------------------------------------------------------------ batch helpers --
+// Pixel slot -> request: segments start at multiples of 64, so a wave's 64
+// slots belong to one request (the last one whose pixelBase <= slot).
+__device__ __forceinline__ uint32_t batchRequestOfSlot(const BatchView& bv, uint32_t waveFirstSlot) {
+    const uint32_t j = laneId();
+    const bool le = j < bv.count && bv.entries[j].pixelBase <= waveFirstSlot;
+    const uint64_t m = __ballot(le);
+    return m ? 63u - uint32_t(__builtin_clzll(m)) : 0u;
+}
+
+// Work split of one queue stage over all requests: lane j holds request j's
+// queue length and the inclusive prefix of 64-item chunks.
+struct BatchChunks {
+    uint32_t n;      // lane j: items in request j's queue
+    uint32_t incl;   // lane j: chunks of requests 0..j
+    uint32_t total;
+};
+
+__device__ __forceinline__ BatchChunks batchChunks(const BatchView& bv, uint32_t bounce, uint32_t slot) {
+    const uint32_t j = laneId();
+    BatchChunks bc;
+    bc.n = 0;
+    if (j < bv.count && bv.entries[j].active) bc.n = bv.entries[j].pv.counters[bounce * kCntPerBounce + slot];
+    uint32_t c = (bc.n + 63u) / 64u;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(c, off, 64);
+        if (j >= uint32_t(off)) c += t;
+    }
+    bc.incl = c;
+    bc.total = __shfl(c, 63, 64);
+    return bc;
+}
+
+// Next chunk of the stage for this wave: request r (wave-uniform), the
+// request's queue length and the chunk's first item.  False when done.
+__device__ __forceinline__ bool batchNext(const BatchChunks& bc, uint32_t* cursor, uint32_t& r, uint32_t& count,
+                                          uint32_t& base) {
+    uint32_t chunk = 0;
+    if (laneId() == 0) chunk = atomicAdd(cursor, 1u);
+    chunk = __builtin_amdgcn_readfirstlane(chunk);
+    if (chunk >= bc.total) return false;
+    const uint64_t m = __ballot(bc.incl > chunk);
+    r = __builtin_amdgcn_readfirstlane(uint32_t(__builtin_ctzll(m)));
+    const uint32_t prev = __shfl(bc.incl, r == 0 ? 0 : r - 1, 64);
+    base = (chunk - (r == 0 ? 0u : prev)) * 64u;
+    count = __builtin_amdgcn_readfirstlane(__shfl(bc.n, r, 64));
+    return true;
+}
+
+// Per frame: every request's counters and the batch's work cursors to zero.
+__global__ __launch_bounds__(256) void kClearBatch(BatchView bv) {
+    const uint32_t words = bv.count * kCounterWords + kMaxBounceLevels * 4;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x) {
+        if (i < bv.count * kCounterWords) bv.entries[i / kCounterWords].pv.counters[i % kCounterWords] = 0u;
+        else bv.cursors[i - bv.count * kCounterWords] = 0u;
+    }
+}
+
+__global__ __launch_bounds__(256) void kGenerateBatch(BatchView bv) {
+    for (uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; w < bv.pixelSlots; w += gridDim.x * blockDim.x) {
+        const uint32_t r = batchRequestOfSlot(bv, w);
+        const BatchEntry& e = bv.entries[r];
+        if (!e.active) continue;
+        const uint32_t p = w - e.pixelBase + laneId();
+        if (p == 0) e.pv.counters[kCntRays] = e.pixels;
+        if (p < e.pixels) generatePixel(e.pv, *e.g, p);
     }
 }
 
 // ------------------------------------------------------------------ extend --
+// Closest hits for queue slots [base, base + 64) of one request's ray queue.
 template <int BLOCK, int S>
-__global__ __launch_bounds__(BLOCK) void kExtend(SceneView sc, PathView pv,
-                                                 const FrameConsts* __restrict__ gp, uint32_t bounce) {
-    __shared__ uint2 ldsStack[S * BLOCK];
-    const uint32_t tlasRoot = gp->tlasRoot, instCount = gp->instanceCount;
-    uint32_t* cnt = pv.counters + bounce * kCntPerBounce;
-    const uint32_t count = cnt[kCntRays];
-    const float4* rayO = pv.rayO[bounce & 1];
-    const float4* rayD = pv.rayD[bounce & 1];
+__device__ __forceinline__ void extendChunk(const SceneView& sc, const PathView& pv, uint32_t tlasRoot,
+                                            uint32_t instCount, uint32_t bounce, uint32_t count, uint32_t base,
+                                            uint2* lds) {
+    const uint32_t k = base + laneId();
     bool overflow = false;
-    while (true) {
-        const uint32_t base = waveFetch(cnt + kCntExtendFetch);
-        if (base >= count) break;
-        const uint32_t k = base + laneId();
-        if (k < count) {
-            const float4 o = rayO[k], d = rayD[k];
-            SceneHit h;
-            const bool hit = traceScene<false, S, BLOCK>(sc, tlasRoot, instCount, make_float3(o.x, o.y, o.z),
-                                                         make_float3(d.x, d.y, d.z), kEpsRay, 1e30f,
-                                                         ldsStack + threadIdx.x, h, overflow);
-            pv.hit[k] = hit ? make_float4(h.t, h.b1, h.b2, __uint_as_float(h.tri))
-                            : make_float4(1e30f, 0.0f, 0.0f, __uint_as_float(kNoHit));
-            pv.hitInst[k] = hit ? h.inst : kNoHit;
-        }
+    if (k < count) {
+        const float4 o = pv.rayO[bounce & 1][k], d = pv.rayD[bounce & 1][k];
+        SceneHit h;
+        const bool hit = traceScene<false, S, BLOCK>(sc, tlasRoot, instCount, make_float3(o.x, o.y, o.z),
+                                                     make_float3(d.x, d.y, d.z), kEpsRay, 1e30f, lds, h, overflow);
+        pv.hit[k] = hit ? make_float4(h.t, h.b1, h.b2, __uint_as_float(h.tri))
+                        : make_float4(1e30f, 0.0f, 0.0f, __uint_as_float(kNoHit));
+        pv.hitInst[k] = hit ? h.inst : kNoHit;
     }
     if (overflow) atomicOr(pv.counters + kStatusWord, 1u);
+}
+
+template <int BLOCK, int S>
+__global__ __launch_bounds__(BLOCK) void kExtendBatch(BatchView bv, uint32_t bounce) {
+    __shared__ uint2 ldsStack[S * BLOCK];
+    const BatchChunks bc = batchChunks(bv, bounce, kCntRays);
+    uint32_t r, count, base;
+    while (batchNext(bc, bv.cursors + bounce * 4 + 0, r, count, base)) {
+        const BatchEntry& e = bv.entries[r];
+        extendChunk<BLOCK, S>(e.sc, e.pv, e.g->tlasRoot, e.g->instanceCount, bounce, count, base,
+                              ldsStack + threadIdx.x);
+    }
 }
 
 // ------------------------------------------------------------- guide prep --
@@ -203,20 +270,16 @@ __global__ __launch_bounds__(256) void kGuidePrep(SceneView sc, PathView pv, con
 // writes a training record.  Without GUIDE (or with neither flag) the kernel
 // performs exactly the reference's operations.
 template <bool GUIDE>
-__global__ __launch_bounds__(256) void kShade(SceneView sc, PathView pv, const FrameConsts* __restrict__ gp,
-                                              uint32_t bounce) {
-    const FrameConsts g = *gp;
+__device__ __forceinline__ void shadeChunk(const SceneView& sc, const PathView& pv, const FrameConsts& g,
+                                           uint32_t bounce, uint32_t count, uint32_t base) {
     uint32_t* cnt = pv.counters + bounce * kCntPerBounce;
     uint32_t* next = pv.counters + (bounce + 1) * kCntPerBounce;
-    const uint32_t count = cnt[kCntRays];
     const float4* rayO = pv.rayO[bounce & 1];
     const float4* rayD = pv.rayD[bounce & 1];
     float4* nrayO = pv.rayO[(bounce + 1) & 1];
     float4* nrayD = pv.rayD[(bounce + 1) & 1];
     const bool last = bounce >= g.maxBounces;
-    while (true) {
-        const uint32_t base = waveFetch(cnt + kCntShadeFetch);
-        if (base >= count) break;
+    {
         const uint32_t k = base + laneId();
         bool emitRay = false, emitShadow = false;
         V3 newO, newD, shO, shD, shC;
@@ -385,61 +448,88 @@ __global__ __launch_bounds__(256) void kShade(SceneView sc, PathView pv, const F
     }
 }
 
-// ------------------------------------------------------------------ shadow --
-template <int BLOCK, int S>
-__global__ __launch_bounds__(BLOCK) void kShadow(SceneView sc, PathView pv,
-                                                 const FrameConsts* __restrict__ gp, uint32_t bounce) {
-    __shared__ uint2 ldsStack[S * BLOCK];
-    const uint32_t tlasRoot = gp->tlasRoot, instCount = gp->instanceCount;
+// Guided shading of one engine's frame (guided requests are never merged
+// into a shared batch: each has its own network).
+__global__ __launch_bounds__(256) void kShadeGuided(SceneView sc, PathView pv, const FrameConsts* __restrict__ gp,
+                                                    uint32_t bounce) {
+    const FrameConsts g = *gp;
     uint32_t* cnt = pv.counters + bounce * kCntPerBounce;
-    const uint32_t count = cnt[kCntShadows];
-    bool overflow = false;
+    const uint32_t count = cnt[kCntRays];
     while (true) {
-        const uint32_t base = waveFetch(cnt + kCntShadowFetch);
+        const uint32_t base = waveFetch(cnt + kCntShadeFetch);
         if (base >= count) break;
-        const uint32_t k = base + laneId();
-        if (k < count) {
-            const float4 o = pv.shadowO[k], d = pv.shadowD[k];
-            SceneHit h;
-            const bool occluded = traceScene<true, S, BLOCK>(sc, tlasRoot, instCount, make_float3(o.x, o.y, o.z),
-                                                             make_float3(d.x, d.y, d.z), kEpsRay, o.w,
-                                                             ldsStack + threadIdx.x, h, overflow);
-            if (!occluded) {
-                const uint32_t p = __float_as_uint(d.w);
-                const float4 c = pv.shadowC[k];
-                float4 L = pv.radiance[p];
-                L.x += c.x;
-                L.y += c.y;
-                L.z += c.z;
-                pv.radiance[p] = L;
-                const uint32_t ts = gp->guideTrainStride;
-                if (pv.pathRec && p % ts == gp->guideTrainPhase) {
-                    // NEE from a recorded vertex is not incident along its wi:
-                    // move the record's radiance snapshot past it.
-                    float4* r = pv.pathRec + (size_t(bounce) * ((pv.capacity + ts - 1) / ts) + p / ts) * 4;
-                    if (__float_as_uint(r[2].w) == gp->guideStamp) r[1].w += luminance(v3(c));
-                }
+        shadeChunk<true>(sc, pv, g, bounce, count, base);
+    }
+}
+
+// Plain shading of every active request of a batched frame.
+__global__ __launch_bounds__(256) void kShadeBatch(BatchView bv, uint32_t bounce) {
+    const BatchChunks bc = batchChunks(bv, bounce, kCntRays);
+    uint32_t r, count, base;
+    while (batchNext(bc, bv.cursors + bounce * 4 + 1, r, count, base)) {
+        const BatchEntry& e = bv.entries[r];
+        const FrameConsts g = *e.g;
+        shadeChunk<false>(e.sc, e.pv, g, bounce, count, base);
+    }
+}
+
+
+// ------------------------------------------------------------------ shadow --
+// Any hit for shadow slots [base, base + 64) of one request; unoccluded
+// slots add their contribution to the path's radiance.
+template <int BLOCK, int S>
+__device__ __forceinline__ void shadowChunk(const SceneView& sc, const PathView& pv, const FrameConsts* __restrict__ gp,
+                                            uint32_t bounce, uint32_t count, uint32_t base, uint2* lds) {
+    const uint32_t k = base + laneId();
+    bool overflow = false;
+    if (k < count) {
+        const float4 o = pv.shadowO[k], d = pv.shadowD[k];
+        SceneHit h;
+        const bool occluded = traceScene<true, S, BLOCK>(sc, gp->tlasRoot, gp->instanceCount,
+                                                         make_float3(o.x, o.y, o.z), make_float3(d.x, d.y, d.z),
+                                                         kEpsRay, o.w, lds, h, overflow);
+        if (!occluded) {
+            const uint32_t p = __float_as_uint(d.w);
+            const float4 c = pv.shadowC[k];
+            float4 L = pv.radiance[p];
+            L.x += c.x;
+            L.y += c.y;
+            L.z += c.z;
+            pv.radiance[p] = L;
+            const uint32_t ts = gp->guideTrainStride;
+            if (pv.pathRec && p % ts == gp->guideTrainPhase) {
+                // NEE from a recorded vertex is not incident along its wi:
+                // move the record's radiance snapshot past it.
+                float4* r = pv.pathRec + (size_t(bounce) * ((pv.capacity + ts - 1) / ts) + p / ts) * 4;
+                if (__float_as_uint(r[2].w) == gp->guideStamp) r[1].w += luminance(v3(c));
             }
         }
     }
     if (overflow) atomicOr(pv.counters + kStatusWord, 1u);
 }
 
+template <int BLOCK, int S>
+__global__ __launch_bounds__(BLOCK) void kShadowBatch(BatchView bv, uint32_t bounce) {
+    __shared__ uint2 ldsStack[S * BLOCK];
+    const BatchChunks bc = batchChunks(bv, bounce, kCntShadows);
+    uint32_t r, count, base;
+    while (batchNext(bc, bv.cursors + bounce * 4 + 2, r, count, base)) {
+        const BatchEntry& e = bv.entries[r];
+        shadowChunk<BLOCK, S>(e.sc, e.pv, e.g, bounce, count, base, ldsStack + threadIdx.x);
+    }
+}
+
 // -------------------------------------------------------------- accumulate --
 // pathtrace.comp:219-228
-__global__ __launch_bounds__(256) void kAccumulate(PathView pv, const FrameConsts* __restrict__ gp) {
-    const FrameConsts g = *gp;
-    const uint32_t n = g.width * g.height;
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-        V3 L = v3(pv.radiance[p]);
-        if (g.fireflyClamp > 0.0f) {
-            const float m = maxComp(L);
-            if (m > g.fireflyClamp) L = L * (g.fireflyClamp / m);
-        }
-        if (!isfinite(L.x) || !isfinite(L.y) || !isfinite(L.z)) L = v3(0.0f, 0.0f, 0.0f);
-        const float4 prev = g.frameIndex == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : pv.accum[p];
-        pv.accum[p] = make_float4(prev.x + L.x, prev.y + L.y, prev.z + L.z, prev.w + 1.0f);
+__device__ __forceinline__ void accumulatePixel(const PathView& pv, const FrameConsts& g, uint32_t p) {
+    V3 L = v3(pv.radiance[p]);
+    if (g.fireflyClamp > 0.0f) {
+        const float m = maxComp(L);
+        if (m > g.fireflyClamp) L = L * (g.fireflyClamp / m);
     }
+    if (!isfinite(L.x) || !isfinite(L.y) || !isfinite(L.z)) L = v3(0.0f, 0.0f, 0.0f);
+    const float4 prev = g.frameIndex == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : pv.accum[p];
+    pv.accum[p] = make_float4(prev.x + L.x, prev.y + L.y, prev.z + L.z, prev.w + 1.0f);
 }
 
 // ----------------------------------------------------------------- resolve --
@@ -455,21 +545,32 @@ __device__ __forceinline__ uint32_t unorm8(float c) {
     return uint32_t(rintf(clampf(c, 0.0f, 1.0f) * 255.0f));
 }
 
-__global__ __launch_bounds__(256) void kResolve8(PathView pv, const FrameConsts* __restrict__ gp) {
-    const FrameConsts g = *gp;
-    const uint32_t n = g.width * g.height;
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-        const float4 acc = pv.accum[p];
-        float c[3] = { 0.0f, 0.0f, 0.0f };
-        if (acc.w > 0.0f) { c[0] = acc.x / acc.w; c[1] = acc.y / acc.w; c[2] = acc.z / acc.w; }
-        uint32_t px = 0xFF000000u;
-        for (int i = 0; i < 3; ++i) {
-            float x = c[i] * g.exposure;
-            if (g.tonemap == 1) x = aces(x);
-            x = linearToSrgb(clampf(x, 0.0f, 1e4f));
-            px |= unorm8(x) << (8 * i);
+__device__ __forceinline__ void resolvePixel(const PathView& pv, const FrameConsts& g, uint32_t p) {
+    const float4 acc = pv.accum[p];
+    float c[3] = { 0.0f, 0.0f, 0.0f };
+    if (acc.w > 0.0f) { c[0] = acc.x / acc.w; c[1] = acc.y / acc.w; c[2] = acc.z / acc.w; }
+    uint32_t px = 0xFF000000u;
+    for (int i = 0; i < 3; ++i) {
+        float x = c[i] * g.exposure;
+        if (g.tonemap == 1) x = aces(x);
+        x = linearToSrgb(clampf(x, 0.0f, 1e4f));
+        px |= unorm8(x) << (8 * i);
+    }
+    pv.outRgba8[p] = px;
+}
+
+// Accumulate and resolve of every active request of a batched frame.
+__global__ __launch_bounds__(256) void kFinishBatch(BatchView bv) {
+    for (uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; w < bv.pixelSlots; w += gridDim.x * blockDim.x) {
+        const uint32_t r = batchRequestOfSlot(bv, w);
+        const BatchEntry& e = bv.entries[r];
+        if (!e.active) continue;
+        const uint32_t p = w - e.pixelBase + laneId();
+        if (p < e.pixels) {
+            const FrameConsts& g = *e.g;
+            accumulatePixel(e.pv, g, p);
+            resolvePixel(e.pv, g, p);
         }
-        pv.outRgba8[p] = px;
     }
 }
 
@@ -498,46 +599,11 @@ hipError_t launchClearWords(uint32_t* words, uint32_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launchGenerate(const PathView& pv, const FrameConsts* g, uint32_t pixels, hipStream_t s) {
-    hipLaunchKernelGGL(kGenerate, dim3(gridFor(pixels, 256, 8192)), dim3(256), 0, s, pv, g);
-    return hipGetLastError();
-}
-
-template <int B, int S>
-static hipError_t extendT(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t b,
-                          uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL((kExtend<B, S>), dim3(grid), dim3(B), 0, s, sc, pv, g, b);
-    return hipGetLastError();
-}
-template <int B, int S>
-static hipError_t shadowT(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t b,
-                          uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL((kShadow<B, S>), dim3(grid), dim3(B), 0, s, sc, pv, g, b);
-    return hipGetLastError();
-}
-
 #define MRT_PT_SHAPES(X) X(64, 16) X(128, 8) X(128, 16) X(128, 32) X(256, 8) X(256, 16)
 
-hipError_t launchExtend(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t bounce,
-                        const TraceShape& shape, hipStream_t s) {
-#define X(B, S) if (shape.block == B && shape.stack == S) return extendT<B, S>(sc, pv, g, bounce, shape.grid, s);
-    MRT_PT_SHAPES(X)
-#undef X
-    return hipErrorInvalidValue;
-}
-
-hipError_t launchShadow(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t bounce,
-                        const TraceShape& shape, hipStream_t s) {
-#define X(B, S) if (shape.block == B && shape.stack == S) return shadowT<B, S>(sc, pv, g, bounce, shape.grid, s);
-    MRT_PT_SHAPES(X)
-#undef X
-    return hipErrorInvalidValue;
-}
-
-hipError_t launchShade(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t bounce,
-                       uint32_t grid, bool guide, hipStream_t s) {
-    if (guide) hipLaunchKernelGGL(kShade<true>, dim3(grid), dim3(256), 0, s, sc, pv, g, bounce);
-    else hipLaunchKernelGGL(kShade<false>, dim3(grid), dim3(256), 0, s, sc, pv, g, bounce);
+hipError_t launchShadeGuided(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t bounce,
+                             uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(kShadeGuided, dim3(grid), dim3(256), 0, s, sc, pv, g, bounce);
     return hipGetLastError();
 }
 
@@ -547,19 +613,70 @@ hipError_t launchGuidePrep(const SceneView& sc, const PathView& pv, const FrameC
     return hipGetLastError();
 }
 
-hipError_t launchAccumulate(const PathView& pv, const FrameConsts* g, uint32_t pixels, hipStream_t s) {
-    hipLaunchKernelGGL(kAccumulate, dim3(gridFor(pixels, 256, 8192)), dim3(256), 0, s, pv, g);
-    return hipGetLastError();
-}
-
-hipError_t launchResolve8(const PathView& pv, const FrameConsts* g, uint32_t pixels, hipStream_t s) {
-    hipLaunchKernelGGL(kResolve8, dim3(gridFor(pixels, 256, 8192)), dim3(256), 0, s, pv, g);
-    return hipGetLastError();
-}
-
 hipError_t launchResolve32(const PathView& pv, uint32_t pixels, uint32_t spp, hipStream_t s) {
     hipLaunchKernelGGL(kResolve32, dim3(gridFor(pixels, 256, 8192)), dim3(256), 0, s, pv, pixels, spp);
     return hipGetLastError();
+}
+
+// ------------------------------------------------------------ batched frame --
+template <int B, int S>
+static hipError_t extendBatchT(const BatchView& bv, uint32_t b, uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL((kExtendBatch<B, S>), dim3(grid), dim3(B), 0, s, bv, b);
+    return hipGetLastError();
+}
+template <int B, int S>
+static hipError_t shadowBatchT(const BatchView& bv, uint32_t b, uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL((kShadowBatch<B, S>), dim3(grid), dim3(B), 0, s, bv, b);
+    return hipGetLastError();
+}
+
+hipError_t launchClearBatch(const BatchView& bv, hipStream_t s) {
+    const uint32_t words = bv.count * kCounterWords + kMaxBounceLevels * 4;
+    hipLaunchKernelGGL(kClearBatch, dim3(gridFor(words, 256, 1024)), dim3(256), 0, s, bv);
+    return hipGetLastError();
+}
+
+hipError_t launchGenerateBatch(const BatchView& bv, hipStream_t s) {
+    hipLaunchKernelGGL(kGenerateBatch, dim3(gridFor(bv.pixelSlots, 256, 8192)), dim3(256), 0, s, bv);
+    return hipGetLastError();
+}
+
+hipError_t launchExtendBatch(const BatchView& bv, uint32_t bounce, const TraceShape& shape, hipStream_t s) {
+#define X(B, S) if (shape.block == B && shape.stack == S) return extendBatchT<B, S>(bv, bounce, shape.grid, s);
+    MRT_PT_SHAPES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launchShadeBatch(const BatchView& bv, uint32_t bounce, uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(kShadeBatch, dim3(grid), dim3(256), 0, s, bv, bounce);
+    return hipGetLastError();
+}
+
+hipError_t launchShadowBatch(const BatchView& bv, uint32_t bounce, const TraceShape& shape, hipStream_t s) {
+#define X(B, S) if (shape.block == B && shape.stack == S) return shadowBatchT<B, S>(bv, bounce, shape.grid, s);
+    MRT_PT_SHAPES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launchFinishBatch(const BatchView& bv, hipStream_t s) {
+    hipLaunchKernelGGL(kFinishBatch, dim3(gridFor(bv.pixelSlots, 256, 8192)), dim3(256), 0, s, bv);
+    return hipGetLastError();
+}
+
+hipError_t batchShadeBlocksPerCU(int* blocksPerCU) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kShadeBatch, 256, 0);
+}
+
+hipError_t batchTraceBlocksPerCU(uint32_t block, uint32_t stack, bool shadow, int* blocksPerCU) {
+#define X(B, S)                                                                                              \
+    if (block == B && stack == S)                                                                            \
+        return shadow ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kShadowBatch<B, S>, B, 0)  \
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kExtendBatch<B, S>, B, 0);
+    MRT_PT_SHAPES(X)
+#undef X
+    return hipErrorInvalidValue;
 }
 
 bool ptShapeSupported(uint32_t block, uint32_t stack) {
@@ -569,19 +686,8 @@ bool ptShapeSupported(uint32_t block, uint32_t stack) {
     return false;
 }
 
-hipError_t persistentGrid(uint32_t block, uint32_t stack, bool shadow, int* blocksPerCU) {
-#define X(B, S)                                                                                           \
-    if (block == B && stack == S)                                                                        \
-        return shadow ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kShadow<B, S>, B, 0)    \
-                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kExtend<B, S>, B, 0);
-    MRT_PT_SHAPES(X)
-#undef X
-    return hipErrorInvalidValue;
-}
-
-hipError_t shadeBlocksPerCU(bool guide, int* blocksPerCU) {
-    if (guide) return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kShade<true>, 256, 0);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kShade<false>, 256, 0);
+hipError_t guidedShadeBlocksPerCU(int* blocksPerCU) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kShadeGuided, 256, 0);
 }
 
 } // namespace dev

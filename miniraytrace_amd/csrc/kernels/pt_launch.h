@@ -22,24 +22,28 @@ struct TraceShape {
 };
 
 hipError_t launchClearWords(uint32_t* words, uint32_t n, hipStream_t s);
-hipError_t launchGenerate(const PathView& pv, const FrameConsts* g, uint32_t pixels, hipStream_t s);
-hipError_t launchExtend(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t bounce,
-                        const TraceShape& shape, hipStream_t s);
-// guide: the guiding variant (needs the prep pass of this bounce).
-hipError_t launchShade(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t bounce,
-                       uint32_t grid, bool guide, hipStream_t s);
+hipError_t launchResolve32(const PathView& pv, uint32_t pixels, uint32_t spp, hipStream_t s);
+// Guided shading of one engine's frame (needs the guide prep pass of this bounce).
+hipError_t launchShadeGuided(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t bounce,
+                             uint32_t grid, hipStream_t s);
 // Query records of guidable hits for the guiding network (grid-stride).
 hipError_t launchGuidePrep(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t bounce,
                            uint32_t grid, hipStream_t s);
-hipError_t launchShadow(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t bounce,
-                        const TraceShape& shape, hipStream_t s);
-hipError_t launchAccumulate(const PathView& pv, const FrameConsts* g, uint32_t pixels, hipStream_t s);
-hipError_t launchResolve8(const PathView& pv, const FrameConsts* g, uint32_t pixels, hipStream_t s);
-hipError_t launchResolve32(const PathView& pv, uint32_t pixels, uint32_t spp, hipStream_t s);
+hipError_t guidedShadeBlocksPerCU(int* blocksPerCU);
+
+// A frame over one or more requests (BatchView; an engine's own frame is a
+// batch of one): clear, generate, per bounce extend / shade / shadow, then
+// accumulate + resolve.  Plain shading (guided engines use launchShadeGuided).
+hipError_t launchClearBatch(const BatchView& bv, hipStream_t s);
+hipError_t launchGenerateBatch(const BatchView& bv, hipStream_t s);
+hipError_t launchExtendBatch(const BatchView& bv, uint32_t bounce, const TraceShape& shape, hipStream_t s);
+hipError_t launchShadeBatch(const BatchView& bv, uint32_t bounce, uint32_t grid, hipStream_t s);
+hipError_t launchShadowBatch(const BatchView& bv, uint32_t bounce, const TraceShape& shape, hipStream_t s);
+hipError_t launchFinishBatch(const BatchView& bv, hipStream_t s);
+hipError_t batchShadeBlocksPerCU(int* blocksPerCU);
+hipError_t batchTraceBlocksPerCU(uint32_t block, uint32_t stack, bool shadow, int* blocksPerCU);
 
 bool ptShapeSupported(uint32_t block, uint32_t stack);
-hipError_t persistentGrid(uint32_t block, uint32_t stack, bool shadow, int* blocksPerCU);
-hipError_t shadeBlocksPerCU(bool guide, int* blocksPerCU);
 
 } // namespace dev
 } // namespace mrt

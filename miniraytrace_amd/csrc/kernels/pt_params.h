@@ -65,5 +65,26 @@ struct PathView {
     float4* pathRec;                 // [level][path] x 4 float4 training records
 };
 
+// One request of a batched frame (requests keep their own queues, counters
+// and frame buffers; the batch kernels only spread their waves over them).
+struct BatchEntry {
+    SceneView sc;
+    PathView pv;
+    const FrameConsts* g;            // this request's constants for this frame
+    uint32_t pixels;                 // width * height
+    uint32_t pixelBase;              // first slot in the batch pixel space (multiple of 64)
+    uint32_t active;                 // 1 when the request renders this frame
+    uint32_t pad;
+};
+
+constexpr uint32_t kMaxBatchRequests = 64;  // one wave lane per request in the work split
+
+struct BatchView {
+    const BatchEntry* entries;       // count entries, pixelBase ascending
+    uint32_t count;                  // <= kMaxBatchRequests
+    uint32_t pixelSlots;             // padded pixel space of all requests
+    uint32_t* cursors;               // [bounce level][4] work cursors, cleared per frame
+};
+
 } // namespace dev
 } // namespace mrt

@@ -245,6 +245,12 @@ class Engine:
         p = np.ascontiguousarray(params, np.float32)
         L.check(L.lib().mrtxSetGuideParams(self._h, p.ctypes.data, p.size), "mrtxSetGuideParams")
 
+    def autotune_path(self) -> dict:
+        """Chooses the closest-hit and shadow launch shapes (restarts accumulation)."""
+        o = L.mrtxPathOptions()
+        L.check(L.lib().mrtxAutotunePath(self._h, C.byref(o)), "mrtxAutotunePath")
+        return {"extend": (o.extendBlock, o.extendStack), "shadow": (o.shadowBlock, o.shadowStack)}
+
     def set_profiling(self, on: bool) -> None:
         L.check(L.lib().mrtxSetProfiling(self._h, int(on)), "mrtxSetProfiling")
 

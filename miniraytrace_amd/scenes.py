@@ -110,6 +110,35 @@ def load_terrain_scene(engine, cfg: dict = CONFIG2, seed: int = 1, sky_img: np.n
             "materials": mats}
 
 
+def terrain_config(triangles: int) -> dict:
+    """CONFIG2-style terrain-plus-spheres scene of about `triangles`
+    triangles: spheres scale with the budget (200 at 1.256M, at least 1),
+    the 2:1 terrain grid takes the rest."""
+    cfg = dict(CONFIG2)
+    spheres = int(min(200, max(1, round(200 * triangles / 1_256_000))))
+    terrain_tris = max(2 * 4 * 2, triangles - spheres * 1280)
+    nz = max(2, int(round((terrain_tris / 4) ** 0.5)))
+    cfg.update(nx=2 * nz, nz=nz, spheres=spheres)
+    return cfg
+
+
+def batch_requests(count: int = 64, seed: int = 1000) -> list[dict]:
+    """Seeded request list of the batched workload: resolution log-uniform
+    from 256x256 to 1920x1080 (one draw scales both axes), scene size
+    log-uniform in [10k, 2M] triangles, spp uniform in 1..16; request k uses
+    scene seed seed + k."""
+    import math
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        t = rng.random()
+        w, h = int(round(256 * (1920 / 256) ** t)), int(round(256 * (1080 / 256) ** t))
+        tris = int(round(math.exp(math.log(10_000) + rng.random() * (math.log(2_000_000) - math.log(10_000)))))
+        out.append({"width": w, "height": h, "triangles": tris, "spp": int(rng.integers(1, 17)),
+                    "seed": seed + k})
+    return out
+
+
 def room(n: int = 204, half=(2.0, 3.0, 2.0), window_frac: float = 0.05, jitter: float = 1e-3,
          seed: int = 1) -> tuple[np.ndarray, np.ndarray]:
     """Closed room [-hx,hx] x [0,hy] x [-hz,hz] of 6 n x n-quad walls with a
