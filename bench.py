@@ -203,20 +203,29 @@ def pt_workload(args, torch) -> dict:
                                   threads=threads, count_fetches=True)
         t_count = time.perf_counter() - t1
         q, nn, nt = int(counts[0]), int(counts[1]), int(counts[2])
-        # extend-stage algorithmic bytes per frame: per closest-hit query a
-        # 32-byte ray read, a 20-byte hit record write, 32 B per node popped
-        # and 64 B per triangle tested (reference order), scaled from the
-        # crop's queries per path to the frame's queue sizes.
+        qs, ns, ts = int(counts[3]), int(counts[4]), int(counts[5])
+        # Trace-launch algorithmic bytes per frame (closest hits of every
+        # bounce and the shadow rays, which share the launches): per
+        # closest-hit query a 32-byte ray read and a 20-byte hit record
+        # write, per shadow query a 48-byte shadow-ray read, and for both
+        # 32 B per node popped and 64 B per triangle tested in the reference
+        # order; per-query figures from the crop, scaled to the frame's queue
+        # sizes.
         per_query = 32 + 20 + 32 * nn / max(q, 1) + 64 * nt / max(q, 1)
-        frame_queries = sum(rays)
-        bytes_alg = per_query * frame_queries
-        achieved = bytes_alg / (stage["extend"] * 1e-3) / 1e9
+        per_shadow = 48 + 32 * ns / max(qs, 1) + 64 * ts / max(qs, 1)
+        frame_queries, frame_shadows = sum(rays), sum(shadows)
+        bytes_alg = per_query * frame_queries + per_shadow * frame_shadows
+        trace_ms = stage["extend"] + stage["shadow"]
+        achieved = bytes_alg / (trace_ms * 1e-3) / 1e9
+        kname = f"kTraceBatch<{ext[0]}, {ext[1]}>"
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": achieved / HBM_PEAK_GBS,
-                           "traffic": _measured_traffic(f"kExtend<{ext[0]}, {ext[1]}>", "bytes_per_frame"),
-                           "kernel": f"kExtend<{ext[0]}, {ext[1]}> (closest hit, all bounces of one frame)",
-                           "bytes_per_frame": bytes_alg, "queries_per_frame": frame_queries,
+                           "traffic": _measured_traffic(kname, "bytes_per_frame"),
+                           "kernel": f"{kname} (closest hit + any hit, all trace launches of one frame)",
+                           "bytes_per_frame": bytes_alg, "trace_ms_per_frame": trace_ms,
+                           "queries_per_frame": frame_queries, "shadow_queries_per_frame": frame_shadows,
                            "nodes_per_query": nn / max(q, 1), "tris_per_query": nt / max(q, 1),
+                           "nodes_per_shadow_query": ns / max(qs, 1), "tris_per_shadow_query": ts / max(qs, 1),
                            "crop_queries_per_path": q / (cw * ch)}
         # CPU baseline: the FP64 path tracer timed on the same crop without counting.
         reps, t_cpu = 0, 0.0

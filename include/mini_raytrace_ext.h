@@ -118,9 +118,9 @@ typedef struct mrtxPathOptions {
     size_t   structSize;
     uint32_t seed;          /* 0 = the reference's RNG keys; else mixed into the frame key */
     uint32_t useGraph;      /* 1 (default) = replay the frame as a captured HIP graph */
-    uint32_t extendBlock;   /* closest-hit stage launch shape (0 = default / tuned) */
-    uint32_t extendStack;
-    uint32_t shadowBlock;   /* shadow stage launch shape */
+    uint32_t extendBlock;   /* shape of the per-bounce trace launches: closest hits of a */
+    uint32_t extendStack;   /* bounce with the previous bounce's shadow rays (0 = default / tuned) */
+    uint32_t shadowBlock;   /* shape of the last launch: the deepest bounce's shadow rays */
     uint32_t shadowStack;
 } mrtxPathOptions;
 
@@ -133,7 +133,9 @@ MRT_API mrtResult mrtxGetPathOptions(mrtEngine* engine, mrtxPathOptions* out);
 
 /* Stage order of gpuMs: generate, extend, shade, shadow, accumulate (with the
  * RGBA8 resolve, one kernel), resolve (0: part of accumulate), guide (prep +
- * network inference), train (record collection + Adam steps). */
+ * network inference), train (record collection + Adam steps).  Each bounce's
+ * closest-hit launch also traces the previous bounce's shadow rays, so
+ * "extend" holds those and "shadow" only the deepest bounce's shadow rays. */
 typedef struct mrtxFrameStats {
     size_t   structSize;
     float    frameMs;                                /* host wall time of the last frame */
@@ -200,10 +202,10 @@ typedef struct mrtxBatchStats {
 MRT_API mrtResult mrtxRenderBatch(mrtxBatchRequest* requests, uint32_t count, mrtxBatchStats* stats);
 
 /* Path-tracer launch-shape autotuner: one profiled trial frame per candidate
- * (block 64/128/256 x LDS stack 8/16/32); the fastest shape per stage
- * (closest hit, shadow) is kept and reported.  Results are cached per
- * (triangle count, pixel count, depth) class.  Images do not depend on the
- * shape; the trial frames are discarded (accumulation restarts). */
+ * (block 64/128/256 x LDS stack 8/16/32); the shape with the least trace time
+ * over the frame is kept for both trace launch kinds and reported.  Results
+ * are cached per (triangle count, pixel count, depth) class.  Images do not
+ * depend on the shape; the trial frames are discarded (accumulation restarts). */
 MRT_API mrtResult mrtxAutotunePath(mrtEngine* engine, mrtxPathOptions* chosen);
 
 /* ---- 6. path guiding -------------------------------------------------------

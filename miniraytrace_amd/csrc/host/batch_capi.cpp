@@ -46,7 +46,7 @@ mrt::dev::TraceShape voteShape(const std::vector<mrt::Engine*>& engines, const s
                                  [](const auto& a, const auto& b) { return a.second < b.second; });
     mrt::dev::TraceShape t{ best->first.first, best->first.second, 0 };
     int perCU = 0;
-    MRT_HIP(mrt::dev::batchTraceBlocksPerCU(t.block, t.stack, shadow, &perCU));
+    MRT_HIP(mrt::dev::batchTraceBlocksPerCU(t.block, t.stack, &perCU));
     t.grid = uint32_t(std::max(1, perCU) * engines[0]->cuCount());
     return t;
 }
@@ -101,10 +101,10 @@ void renderMerged(const std::vector<mrt::Engine*>& engines, std::vector<uint32_t
             MRT_HIP(mrt::dev::launchClearBatch(bv, s));
             MRT_HIP(mrt::dev::launchGenerateBatch(bv, s));
             for (uint32_t b = 0; b < levels; ++b) {
-                MRT_HIP(mrt::dev::launchExtendBatch(bv, b, ext, s));
+                MRT_HIP(mrt::dev::launchTraceBatch(bv, b, b == 0 ? mrt::dev::kNoLevel : b - 1, ext, s));
                 MRT_HIP(mrt::dev::launchShadeBatch(bv, b, shadeGrid, s));
-                MRT_HIP(mrt::dev::launchShadowBatch(bv, b, sha, s));
             }
+            MRT_HIP(mrt::dev::launchTraceBatch(bv, mrt::dev::kNoLevel, levels - 1, sha, s));
             MRT_HIP(mrt::dev::launchFinishBatch(bv, s));
         }
     } catch (...) {

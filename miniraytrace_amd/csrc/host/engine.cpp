@@ -25,6 +25,8 @@ float msSince(Clock::time_point t0) {
 // TLAS leaf + BLAS depth.  The kernels hold 64 scratch entries plus the LDS
 // part (at least 8).
 constexpr uint32_t kMaxTraversalStack = 64 + 8;
+// Traversal stack entries hold node, triangle and instance indices in 27 bits.
+constexpr size_t kMaxTraversalIndex = (size_t(1) << 27) - 1;
 
 // A node whose inverted box fails every slab test (reference Renderer.cpp:51-58).
 BvhNode sentinelNode() {
@@ -176,9 +178,9 @@ void Engine::invalidateGraph() {
 // Persistent grids: as many workgroups as fit on the device at once.
 void Engine::chooseShapes() {
     int perCU = 0;
-    MRT_HIP(dev::batchTraceBlocksPerCU(extendShape_.block, extendShape_.stack, false, &perCU));
+    MRT_HIP(dev::batchTraceBlocksPerCU(extendShape_.block, extendShape_.stack, &perCU));
     extendShape_.grid = uint32_t(std::max(1, perCU) * cuCount_);
-    MRT_HIP(dev::batchTraceBlocksPerCU(shadowShape_.block, shadowShape_.stack, true, &perCU));
+    MRT_HIP(dev::batchTraceBlocksPerCU(shadowShape_.block, shadowShape_.stack, &perCU));
     shadowShape_.grid = uint32_t(std::max(1, perCU) * cuCount_);
     MRT_HIP(dev::batchShadeBlocksPerCU(&perCU));
     shadeGrid_ = uint32_t(std::max(1, perCU) * cuCount_);
@@ -345,10 +347,14 @@ uint32_t Engine::packGeometry() {
     tris.reserve(nt);
     triVerts.reserve(nt * 12);
     blasNodes_.clear();
-    blasNodes_.reserve(nn);
+    blasNodes_.reserve(nn + meshes.size() + 1);
     maxBlasDepth_ = 0;
     for (auto& [id, m] : meshes) {
         Blas& b = blas_[id];
+        // Every BVH root at an odd index, behind an unused node where needed:
+        // child pairs (root + 1 + 2k) then start on 64-byte boundaries, so the
+        // two children a traversal step reads share one cache line.
+        if (blasNodes_.size() % 2 == 0) blasNodes_.push_back(sentinelNode());
         const uint32_t vb = uint32_t(pos.size() / 4), tb = uint32_t(tris.size()), nb = uint32_t(blasNodes_.size());
         b.rootGlobal = nb;
         maxBlasDepth_ = std::max(maxBlasDepth_, b.depth);
@@ -376,8 +382,11 @@ uint32_t Engine::packGeometry() {
             blasNodes_.push_back(n);
         }
     }
+    if (blasNodes_.size() % 2 == 0) blasNodes_.push_back(sentinelNode());
     tlasRoot_ = uint32_t(blasNodes_.size());
     totalTris_ = tris.size();
+    if (totalTris_ > kMaxTraversalIndex)
+        throw Error(MRT_ERROR_OUT_OF_MEMORY, "scene holds more than 2^27 - 1 triangles");
     totalVerts_ = pos.size() / 4;
     uploadVec(dPos_, pos.data(), pos.size() * 4);
     uploadVec(dNrm_, nrm.data(), nrm.size() * 4);
@@ -435,6 +444,8 @@ void Engine::packInstances() {
     }
     instanceCount_ = uint32_t(gi.size());
     tlasNodeCount_ = uint32_t(tlas.nodes.size());
+    if (size_t(tlasRoot_) + tlas.nodes.size() > kMaxTraversalIndex || gi.size() > kMaxTraversalIndex)
+        throw Error(MRT_ERROR_OUT_OF_MEMORY, "scene BVH holds more than 2^27 - 1 nodes or instances");
     // Guiding positions: the scene box scaled uniformly into [0, 1].
     if (tlas.rootBounds.valid()) {
         float extent = 0.0f;
@@ -809,9 +820,12 @@ void Engine::enqueueFrame(hipStream_t s, bool timed, const FrameConsts* g, const
     if (train) MRT_HIP(dev::launchGuideFrameBegin(guideNet_->deviceWords(), g, s));
     MRT_HIP(dev::launchGenerateBatch(bv, s));
     if (timed) MRT_HIP(hipEventRecord(ev_[1], s));
+    // Per bounce b: one trace launch (closest hits of level b with the shadow
+    // rays of level b - 1), the guide passes, shade; the deepest level's
+    // shadow rays get a trace launch of their own.
     for (uint32_t b = 0; b <= maxB; ++b) {
         if (timed) MRT_HIP(hipEventRecord(ev_[2], s));
-        MRT_HIP(dev::launchExtendBatch(bv, b, extendShape_, s));
+        MRT_HIP(dev::launchTraceBatch(bv, b, b == 0 ? dev::kNoLevel : b - 1, extendShape_, s));
         if (timed) MRT_HIP(hipEventRecord(ev_[3], s));
         if (guide) {
             MRT_HIP(dev::launchGuidePrep(sc, pv, g, b, prepGrid, s));
@@ -826,16 +840,20 @@ void Engine::enqueueFrame(hipStream_t s, bool timed, const FrameConsts* g, const
         if (timed) MRT_HIP(hipEventRecord(ev_[9], s));
         if (guide) MRT_HIP(dev::launchShadeGuided(sc, pv, g, b, shadeGuideGrid_, s));
         else MRT_HIP(dev::launchShadeBatch(bv, b, shadeGrid_, s));
-        if (timed) MRT_HIP(hipEventRecord(ev_[4], s));
-        MRT_HIP(dev::launchShadowBatch(bv, b, shadowShape_, s));
         if (timed) {
-            MRT_HIP(hipEventRecord(ev_[5], s));
-            MRT_HIP(hipEventSynchronize(ev_[5]));
+            MRT_HIP(hipEventRecord(ev_[4], s));
+            MRT_HIP(hipEventSynchronize(ev_[4]));
             stats_.stageMs[1] += elapsed(ev_[2], ev_[3]);
             stats_.stageMs[6] += elapsed(ev_[3], ev_[9]);
             stats_.stageMs[2] += elapsed(ev_[9], ev_[4]);
-            stats_.stageMs[3] += elapsed(ev_[4], ev_[5]);
         }
+    }
+    if (timed) MRT_HIP(hipEventRecord(ev_[4], s));
+    MRT_HIP(dev::launchTraceBatch(bv, dev::kNoLevel, maxB, shadowShape_, s));
+    if (timed) {
+        MRT_HIP(hipEventRecord(ev_[5], s));
+        MRT_HIP(hipEventSynchronize(ev_[5]));
+        stats_.stageMs[3] += elapsed(ev_[4], ev_[5]);
     }
     if (timed) MRT_HIP(hipEventRecord(ev_[10], s));
     if (train) enqueueGuideTraining(s, g);
@@ -1055,7 +1073,9 @@ void Engine::autotuneShapes(mrtxPathOptions& chosen) {
         const uint32_t sppLimit = settings_.sppLimit;
         settings_.sppLimit = std::max(sppLimit, 1u << 30);
         profiling_ = true;
-        float bestE = 1e30f, bestS = 1e30f;
+        // One shape for both trace launches: the one with the least trace
+        // time over a whole frame.
+        float best = 1e30f;
         dev::TraceShape be = extendShape_, bs = shadowShape_;
         for (const auto& c : kCand) {
             extendShape_.block = shadowShape_.block = c[0];
@@ -1063,8 +1083,12 @@ void Engine::autotuneShapes(mrtxPathOptions& chosen) {
             chooseShapes();
             resetAccumulation();
             renderFrame(nullptr);
-            if (stats_.stageMs[1] < bestE) { bestE = stats_.stageMs[1]; be = extendShape_; }
-            if (stats_.stageMs[3] < bestS) { bestS = stats_.stageMs[3]; bs = shadowShape_; }
+            const float t = stats_.stageMs[1] + stats_.stageMs[3];
+            if (t < best) {
+                best = t;
+                be = extendShape_;
+                bs = shadowShape_;
+            }
         }
         profiling_ = prof;
         settings_.sppLimit = sppLimit;

@@ -144,8 +144,8 @@ private:
     DeviceBuffer dBatchConsts_, dBatchEntries_, dEntry_, dCursors_;
     uint32_t frameW_ = 0, frameH_ = 0;
 
-    dev::TraceShape extendShape_{ 128, 16, 0 };
-    dev::TraceShape shadowShape_{ 128, 16, 0 };
+    dev::TraceShape extendShape_{ 256, 8, 0 };
+    dev::TraceShape shadowShape_{ 256, 8, 0 };
     uint32_t shadeGrid_ = 0;
     uint32_t shadeGuideGrid_ = 0;
 

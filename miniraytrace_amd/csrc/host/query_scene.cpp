@@ -34,6 +34,8 @@ using mrt::Error;
 // Deepest stack a depth-d tree needs is d entries; the kernels hold the top
 // `stack` entries in LDS and spill up to 64 more to scratch.
 constexpr uint32_t kMaxTraversalDepth = 64 + 8;
+// Traversal stack entries hold node and triangle indices in 27 bits.
+constexpr size_t kMaxTraversalIndex = (size_t(1) << 27) - 1;
 
 mrtxQueryScene* createScene(const float* positions, uint32_t vertexCount, const uint32_t* indices,
                             uint32_t triCount, uint32_t gpuIndex1) {
@@ -94,8 +96,15 @@ mrtxQueryScene* createScene(const float* positions, uint32_t vertexCount, const 
         }
         sc->nodes = std::move(b.nodes);
         sc->primOrder = std::move(b.primOrder);
-        sc->dNodes.allocate(sc->nodes.size() * sizeof(mrt::BvhNode));
-        sc->dNodes.upload(sc->nodes.data(), sc->nodes.size() * sizeof(mrt::BvhNode));
+        if (sc->nodes.size() > kMaxTraversalIndex || triCount > kMaxTraversalIndex)
+            throw Error(MRT_ERROR_INVALID_ARGUMENT, "too many triangles for one query scene");
+        // Device copy behind one unused leading node: the root then sits at
+        // byte 32 and every child pair (indices 2k+1, 2k+2) on a 64-byte
+        // boundary, so the two children a traversal step reads share a line.
+        std::vector<mrt::BvhNode> dev(sc->nodes.size() + 1);
+        std::memcpy(dev.data() + 1, sc->nodes.data(), sc->nodes.size() * sizeof(mrt::BvhNode));
+        sc->dNodes.allocate(dev.size() * sizeof(mrt::BvhNode));
+        sc->dNodes.upload(dev.data(), dev.size() * sizeof(mrt::BvhNode));
         sc->dTris.allocate(tris.size() * sizeof(float));
         sc->dTris.upload(tris.data(), tris.size() * sizeof(float));
         sc->dStatus.allocate(sizeof(uint32_t));
@@ -110,7 +119,8 @@ mrtxQueryScene* createScene(const float* positions, uint32_t vertexCount, const 
 
 mrt::dev::QueryLaunch makeLaunch(mrtxQueryScene* s, const mrtxRay* rays, uint32_t n, void* out) {
     mrt::dev::QueryLaunch q;
-    q.nodes = s->dNodes.as<const mrt::dev::Node>();
+    // skip the alignment node
+    q.nodes = reinterpret_cast<const mrt::dev::Node*>(s->dNodes.as<const char>() + sizeof(mrt::BvhNode));
     q.tris = s->dTris.as<const float4>();
     q.rays = reinterpret_cast<const float4*>(rays);
     q.n = n;

@@ -154,19 +154,31 @@ __device__ __forceinline__ BatchChunks batchChunks(const BatchView& bv, uint32_t
     return bc;
 }
 
-// Next chunk of the stage for this wave: request r (wave-uniform), the
-// request's queue length and the chunk's first item.  False when done.
-__device__ __forceinline__ bool batchNext(const BatchChunks& bc, uint32_t* cursor, uint32_t& r, uint32_t& count,
-                                          uint32_t& base) {
-    uint32_t chunk = 0;
-    if (laneId() == 0) chunk = atomicAdd(cursor, 1u);
-    chunk = __builtin_amdgcn_readfirstlane(chunk);
-    if (chunk >= bc.total) return false;
+// Request r (wave-uniform), its queue length and the first item of chunk
+// `chunk` of a stage's work split.
+__device__ __forceinline__ void batchLocate(const BatchChunks& bc, uint32_t chunk, uint32_t& r, uint32_t& count,
+                                            uint32_t& base) {
     const uint64_t m = __ballot(bc.incl > chunk);
     r = __builtin_amdgcn_readfirstlane(uint32_t(__builtin_ctzll(m)));
     const uint32_t prev = __shfl(bc.incl, r == 0 ? 0 : r - 1, 64);
     base = (chunk - (r == 0 ? 0u : prev)) * 64u;
     count = __builtin_amdgcn_readfirstlane(__shfl(bc.n, r, 64));
+}
+
+// Next chunk number from a stage's work cursor (wave-uniform).
+__device__ __forceinline__ uint32_t batchFetch(uint32_t* cursor) {
+    uint32_t chunk = 0;
+    if (laneId() == 0) chunk = atomicAdd(cursor, 1u);
+    return __builtin_amdgcn_readfirstlane(chunk);
+}
+
+// Next chunk of the stage for this wave: request r (wave-uniform), the
+// request's queue length and the chunk's first item.  False when done.
+__device__ __forceinline__ bool batchNext(const BatchChunks& bc, uint32_t* cursor, uint32_t& r, uint32_t& count,
+                                          uint32_t& base) {
+    const uint32_t chunk = batchFetch(cursor);
+    if (chunk >= bc.total) return false;
+    batchLocate(bc, chunk, r, count, base);
     return true;
 }
 
@@ -208,18 +220,6 @@ __device__ __forceinline__ void extendChunk(const SceneView& sc, const PathView&
         pv.hitInst[k] = hit ? h.inst : kNoHit;
     }
     if (overflow) atomicOr(pv.counters + kStatusWord, 1u);
-}
-
-template <int BLOCK, int S>
-__global__ __launch_bounds__(BLOCK) void kExtendBatch(BatchView bv, uint32_t bounce) {
-    __shared__ uint2 ldsStack[S * BLOCK];
-    const BatchChunks bc = batchChunks(bv, bounce, kCntRays);
-    uint32_t r, count, base;
-    while (batchNext(bc, bv.cursors + bounce * 4 + 0, r, count, base)) {
-        const BatchEntry& e = bv.entries[r];
-        extendChunk<BLOCK, S>(e.sc, e.pv, e.g->tlasRoot, e.g->instanceCount, bounce, count, base,
-                              ldsStack + threadIdx.x);
-    }
 }
 
 // ------------------------------------------------------------- guide prep --
@@ -508,14 +508,33 @@ __device__ __forceinline__ void shadowChunk(const SceneView& sc, const PathView&
     if (overflow) atomicOr(pv.counters + kStatusWord, 1u);
 }
 
+// Closest hits of ray queue `extendLevel` and any hits of shadow queue
+// `shadowLevel` (either may be kNoLevel) over all requests in one launch.
+// The two are independent -- shadow rays only add to path radiance, closest
+// hits only write hit records -- so the slowest rays of one overlap with the
+// other's work instead of each ending a launch of its own.  Chunks
+// [0, extend chunks) of the shared cursor are closest hit, the rest any hit.
 template <int BLOCK, int S>
-__global__ __launch_bounds__(BLOCK) void kShadowBatch(BatchView bv, uint32_t bounce) {
+__global__ __launch_bounds__(BLOCK) void kTraceBatch(BatchView bv, uint32_t extendLevel, uint32_t shadowLevel) {
     __shared__ uint2 ldsStack[S * BLOCK];
-    const BatchChunks bc = batchChunks(bv, bounce, kCntShadows);
-    uint32_t r, count, base;
-    while (batchNext(bc, bv.cursors + bounce * 4 + 2, r, count, base)) {
-        const BatchEntry& e = bv.entries[r];
-        shadowChunk<BLOCK, S>(e.sc, e.pv, e.g, bounce, count, base, ldsStack + threadIdx.x);
+    const BatchChunks be = extendLevel != kNoLevel ? batchChunks(bv, extendLevel, kCntRays) : BatchChunks{ 0, 0, 0 };
+    const BatchChunks bs = shadowLevel != kNoLevel ? batchChunks(bv, shadowLevel, kCntShadows) : BatchChunks{ 0, 0, 0 };
+    uint32_t* cursor = bv.cursors + (extendLevel != kNoLevel ? extendLevel * 4 + 0 : shadowLevel * 4 + 2);
+    while (true) {
+        const uint32_t chunk = batchFetch(cursor);
+        uint32_t r, count, base;
+        if (chunk < be.total) {
+            batchLocate(be, chunk, r, count, base);
+            const BatchEntry& e = bv.entries[r];
+            extendChunk<BLOCK, S>(e.sc, e.pv, e.g->tlasRoot, e.g->instanceCount, extendLevel, count, base,
+                                  ldsStack + threadIdx.x);
+        } else if (chunk - be.total < bs.total) {
+            batchLocate(bs, chunk - be.total, r, count, base);
+            const BatchEntry& e = bv.entries[r];
+            shadowChunk<BLOCK, S>(e.sc, e.pv, e.g, shadowLevel, count, base, ldsStack + threadIdx.x);
+        } else {
+            break;
+        }
     }
 }
 
@@ -620,13 +639,9 @@ hipError_t launchResolve32(const PathView& pv, uint32_t pixels, uint32_t spp, hi
 
 // ------------------------------------------------------------ batched frame --
 template <int B, int S>
-static hipError_t extendBatchT(const BatchView& bv, uint32_t b, uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL((kExtendBatch<B, S>), dim3(grid), dim3(B), 0, s, bv, b);
-    return hipGetLastError();
-}
-template <int B, int S>
-static hipError_t shadowBatchT(const BatchView& bv, uint32_t b, uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL((kShadowBatch<B, S>), dim3(grid), dim3(B), 0, s, bv, b);
+static hipError_t traceBatchT(const BatchView& bv, uint32_t extendLevel, uint32_t shadowLevel, uint32_t grid,
+                              hipStream_t s) {
+    hipLaunchKernelGGL((kTraceBatch<B, S>), dim3(grid), dim3(B), 0, s, bv, extendLevel, shadowLevel);
     return hipGetLastError();
 }
 
@@ -641,8 +656,14 @@ hipError_t launchGenerateBatch(const BatchView& bv, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launchExtendBatch(const BatchView& bv, uint32_t bounce, const TraceShape& shape, hipStream_t s) {
-#define X(B, S) if (shape.block == B && shape.stack == S) return extendBatchT<B, S>(bv, bounce, shape.grid, s);
+hipError_t launchTraceBatch(const BatchView& bv, uint32_t extendLevel, uint32_t shadowLevel, const TraceShape& shape,
+                            hipStream_t s) {
+    if (extendLevel == kNoLevel && shadowLevel == kNoLevel) return hipSuccess;
+    if ((extendLevel != kNoLevel && extendLevel >= kMaxBounceLevels) ||
+        (shadowLevel != kNoLevel && shadowLevel >= kMaxBounceLevels))
+        return hipErrorInvalidValue;
+#define X(B, S)                                                                                  \
+    if (shape.block == B && shape.stack == S) return traceBatchT<B, S>(bv, extendLevel, shadowLevel, shape.grid, s);
     MRT_PT_SHAPES(X)
 #undef X
     return hipErrorInvalidValue;
@@ -651,13 +672,6 @@ hipError_t launchExtendBatch(const BatchView& bv, uint32_t bounce, const TraceSh
 hipError_t launchShadeBatch(const BatchView& bv, uint32_t bounce, uint32_t grid, hipStream_t s) {
     hipLaunchKernelGGL(kShadeBatch, dim3(grid), dim3(256), 0, s, bv, bounce);
     return hipGetLastError();
-}
-
-hipError_t launchShadowBatch(const BatchView& bv, uint32_t bounce, const TraceShape& shape, hipStream_t s) {
-#define X(B, S) if (shape.block == B && shape.stack == S) return shadowBatchT<B, S>(bv, bounce, shape.grid, s);
-    MRT_PT_SHAPES(X)
-#undef X
-    return hipErrorInvalidValue;
 }
 
 hipError_t launchFinishBatch(const BatchView& bv, hipStream_t s) {
@@ -669,11 +683,9 @@ hipError_t batchShadeBlocksPerCU(int* blocksPerCU) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kShadeBatch, 256, 0);
 }
 
-hipError_t batchTraceBlocksPerCU(uint32_t block, uint32_t stack, bool shadow, int* blocksPerCU) {
-#define X(B, S)                                                                                              \
-    if (block == B && stack == S)                                                                            \
-        return shadow ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kShadowBatch<B, S>, B, 0)  \
-                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kExtendBatch<B, S>, B, 0);
+hipError_t batchTraceBlocksPerCU(uint32_t block, uint32_t stack, int* blocksPerCU) {
+#define X(B, S) \
+    if (block == B && stack == S) return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kTraceBatch<B, S>, B, 0);
     MRT_PT_SHAPES(X)
 #undef X
     return hipErrorInvalidValue;

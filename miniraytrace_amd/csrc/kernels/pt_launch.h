@@ -14,6 +14,7 @@ namespace dev {
 constexpr uint32_t kMaxBounceLevels = 18;
 constexpr uint32_t kStatusWord = kMaxBounceLevels * kCntPerBounce;  // bit 0: stack overflow
 constexpr uint32_t kCounterWords = kStatusWord + 8;
+constexpr uint32_t kNoLevel = 0xFFFFFFFFu;  // "no queue" for launchTraceBatch
 
 struct TraceShape {
     uint32_t block;  // threads per workgroup
@@ -32,16 +33,19 @@ hipError_t launchGuidePrep(const SceneView& sc, const PathView& pv, const FrameC
 hipError_t guidedShadeBlocksPerCU(int* blocksPerCU);
 
 // A frame over one or more requests (BatchView; an engine's own frame is a
-// batch of one): clear, generate, per bounce extend / shade / shadow, then
-// accumulate + resolve.  Plain shading (guided engines use launchShadeGuided).
+// batch of one): clear, generate, then per bounce b one trace launch (closest
+// hits of level b together with the shadow rays of level b - 1) and shade,
+// a last trace launch for the deepest level's shadow rays, then accumulate +
+// resolve.  Plain shading (guided engines use launchShadeGuided).
 hipError_t launchClearBatch(const BatchView& bv, hipStream_t s);
 hipError_t launchGenerateBatch(const BatchView& bv, hipStream_t s);
-hipError_t launchExtendBatch(const BatchView& bv, uint32_t bounce, const TraceShape& shape, hipStream_t s);
+// extendLevel / shadowLevel: queue levels, either may be kNoLevel.
+hipError_t launchTraceBatch(const BatchView& bv, uint32_t extendLevel, uint32_t shadowLevel, const TraceShape& shape,
+                            hipStream_t s);
 hipError_t launchShadeBatch(const BatchView& bv, uint32_t bounce, uint32_t grid, hipStream_t s);
-hipError_t launchShadowBatch(const BatchView& bv, uint32_t bounce, const TraceShape& shape, hipStream_t s);
 hipError_t launchFinishBatch(const BatchView& bv, hipStream_t s);
 hipError_t batchShadeBlocksPerCU(int* blocksPerCU);
-hipError_t batchTraceBlocksPerCU(uint32_t block, uint32_t stack, bool shadow, int* blocksPerCU);
+hipError_t batchTraceBlocksPerCU(uint32_t block, uint32_t stack, int* blocksPerCU);
 
 bool ptShapeSupported(uint32_t block, uint32_t stack);
 

@@ -1,10 +1,11 @@
 // Ray-query kernels over one BVH (closest hit / any hit) for gfx950.
 //
-// One lane per ray.  A lane keeps the current node in registers; at an
-// internal node it fetches both children (adjacent 32-byte records), tests
-// both boxes, continues into the nearer hit child and pushes the farther one
-// with its entry distance; a popped entry is skipped without a fetch when its
-// entry distance is beyond the current limit.  Closest-hit ties in t resolve
+// One lane per ray.  A lane keeps the cursor of the current node in
+// registers; at an internal node it fetches both children (adjacent 32-byte
+// records, one 64-byte line), tests both boxes, continues into the nearer hit
+// child and pushes the farther one's cursor with its entry distance; a popped
+// entry is skipped when its entry distance is beyond the current limit, and
+// otherwise continues without a node fetch.  Closest-hit ties in t resolve
 // to the lowest triangle id.
 //
 // Two loop structures (chosen per scene by the autotuner):
@@ -43,24 +44,24 @@ __device__ __forceinline__ void leafClosest(const float4* __restrict__ tris, con
 // Pops until an entry within the limit is found; returns false when empty.
 template <int S, int BLOCK>
 __device__ __forceinline__ bool popNext(ShortStack<S, BLOCK>& stack, const Node* __restrict__ nodes,
-                                        float limit, Node& cur) {
+                                        float limit, Cursor& cur) {
     while (!stack.empty()) {
         const uint2 e = stack.pop();
         if (__uint_as_float(e.y) <= limit) {
-            cur = loadNode(nodes, e.x);
+            cur = entryCursor(nodes, e);
             return true;
         }
     }
     return false;
 }
 
-// Inner-node step: returns true and sets cur to the next node to visit
-// (pushing the farther child when both are hit); false when neither child
-// is hit.
+// Inner-node step: fetches both children of `cur`, returns true and sets cur
+// to the next node to visit (pushing the farther child when both are hit);
+// false when neither child is hit.
 template <int S, int BLOCK>
 __device__ __forceinline__ bool innerStep(const Node* __restrict__ nodes, const RayK& r, float lim,
-                                          ShortStack<S, BLOCK>& stack, Node& cur) {
-    const uint32_t lp = nodeLeft(cur);
+                                          ShortStack<S, BLOCK>& stack, Cursor& cur) {
+    const uint32_t lp = cur.index;
     const Node c0 = loadNode(nodes, lp);
     const Node c1 = loadNode(nodes, lp + 1);
     float t0, t1;
@@ -68,12 +69,12 @@ __device__ __forceinline__ bool innerStep(const Node* __restrict__ nodes, const 
     const bool h1 = slab(c1, r, lim, t1);
     if (h0 && h1) {
         const bool firstNear = t0 <= t1;
-        stack.push(make_uint2(firstNear ? lp + 1 : lp, __float_as_uint(firstNear ? t1 : t0)));
-        cur = firstNear ? c0 : c1;
+        stack.push(firstNear ? nodeEntry(c1, lp + 1, t1) : nodeEntry(c0, lp, t0));
+        cur = cursorOf(firstNear ? c0 : c1);
         return true;
     }
     if (h0 || h1) {
-        cur = h0 ? c0 : c1;
+        cur = cursorOf(h0 ? c0 : c1);
         return true;
     }
     return false;
@@ -95,28 +96,28 @@ __global__ __launch_bounds__(BLOCK) void kTraceClosest(const Node* __restrict__ 
     ClosestState st{ r1.w, kNoHit, false };
     uint2 spill[kSpill];
     ShortStack<S, BLOCK> stack(ldsStack + threadIdx.x, spill);
-    Node cur = loadNode(nodes, 0);
+    const Node root = loadNode(nodes, 0);
+    Cursor cur = cursorOf(root);
     float te;
-    bool have = slab(cur, r, st.best * kLimitWiden, te);
+    bool have = slab(root, r, st.best * kLimitWiden, te);
     if (LOOP == 0) {
         while (have) {
-            const uint32_t cnt = nodeCount(cur);
-            if (cnt == 0) {
+            if (cur.count == 0) {
                 if (innerStep(nodes, r, st.best * kLimitWiden, stack, cur)) continue;
             } else {
-                leafClosest(tris, r, nodeLeft(cur), cnt, st);
+                leafClosest(tris, r, cur.index, cur.count, st);
             }
             have = popNext(stack, nodes, st.best * kLimitWiden, cur);
         }
     } else {
         while (have) {
             // descend to a leaf
-            while (have && nodeCount(cur) == 0) {
+            while (have && cur.count == 0) {
                 if (!innerStep(nodes, r, st.best * kLimitWiden, stack, cur))
                     have = popNext(stack, nodes, st.best * kLimitWiden, cur);
             }
             if (!have) break;
-            leafClosest(tris, r, nodeLeft(cur), nodeCount(cur), st);
+            leafClosest(tris, r, cur.index, cur.count, st);
             have = popNext(stack, nodes, st.best * kLimitWiden, cur);
         }
     }
@@ -142,21 +143,22 @@ __global__ __launch_bounds__(BLOCK) void kTraceAny(const Node* __restrict__ node
     uint2 spill[kSpill];
     ShortStack<S, BLOCK> stack(ldsStack + threadIdx.x, spill);
     uint32_t occ = 0;
-    Node cur = loadNode(nodes, 0);
+    const Node root = loadNode(nodes, 0);
+    Cursor cur = cursorOf(root);
     float te;
-    bool have = slab(cur, r, lim, te);
+    bool have = slab(root, r, lim, te);
     while (have) {
         if (LOOP != 0) {
-            while (have && nodeCount(cur) == 0) {
+            while (have && cur.count == 0) {
                 if (!innerStep(nodes, r, lim, stack, cur)) have = popNext(stack, nodes, lim, cur);
             }
             if (!have) break;
         }
-        const uint32_t cnt = nodeCount(cur);
+        const uint32_t cnt = cur.count;
         if (cnt == 0) {
             if (innerStep(nodes, r, lim, stack, cur)) continue;
         } else {
-            const uint32_t lp = nodeLeft(cur);
+            const uint32_t lp = cur.index;
             for (uint32_t k = lp; k < lp + cnt; ++k) {
                 float t, b1, b2;
                 if (isectTri(r, tris[3 * size_t(k)], tris[3 * size_t(k) + 1], tris[3 * size_t(k) + 2], tmax, t,

@@ -3,7 +3,9 @@
 // Node layout: the reference's 32-byte BvhNode (core/src/accel/Bvh.hpp:15-26)
 // read as two float4: a = (lo.xyz, leftOrPrim bits), b = (hi.xyz, packed bits),
 // packed = (count << 2) | axis, count == 0 -> internal node whose children sit
-// at leftOrPrim and leftOrPrim + 1.
+// at leftOrPrim and leftOrPrim + 1.  The host places node arrays so that every
+// child pair starts on a 64-byte boundary: the two children a step fetches
+// then share one cache line.
 //
 // Triangles are read from a de-indexed, leaf-ordered array of 3 float4 per
 // triangle (v0.xyz + id bits, v1.xyz, v2.xyz): one 48-byte contiguous read per
@@ -40,11 +42,47 @@ __device__ __forceinline__ Node loadNode(const Node* __restrict__ nodes, uint32_
 __device__ __forceinline__ uint32_t nodeLeft(const Node& n) { return __float_as_uint(n.a.w); }
 __device__ __forceinline__ uint32_t nodeCount(const Node& n) { return __float_as_uint(n.b.w) >> 2; }
 
-// Per-ray constants: origin, direction, reciprocal direction and the shear
-// of the watertight test.
+// What traversal needs of a node whose box it has entered: the first child
+// (inner node, count 0) or the first primitive and the primitive count.
+struct Cursor {
+    uint32_t index, count;
+};
+__device__ __forceinline__ Cursor cursorOf(const Node& n) { return Cursor{ nodeLeft(n), nodeCount(n) }; }
+
+// Stack entries carry the cursor of the node they stand for, so a pop needs
+// no node fetch: x = (code << 27) | index, y = entry distance bits, with
+//   code 0       inner node, index = first child (children at index, index + 1)
+//   code 1..29   leaf of `code` primitives starting at index
+//   code 30      instance to enter (two-level traversal), index = instance
+//   code 31      leaf of 30 or more primitives, index = the node (fetched again)
+// Node, primitive and instance indices stay below 2^27 (checked on the host).
+constexpr uint32_t kEntryIndexBits = 27;
+constexpr uint32_t kEntryIndexMask = (1u << kEntryIndexBits) - 1u;
+constexpr uint32_t kEntryInstance = 30;
+constexpr uint32_t kEntryRefetch = 31;
+constexpr uint32_t kEntryMaxLeaf = 29;
+
+__device__ __forceinline__ uint2 nodeEntry(const Node& n, uint32_t nodeIdx, float tEnter) {
+    const uint32_t cnt = nodeCount(n);
+    const uint32_t x = cnt <= kEntryMaxLeaf ? (cnt << kEntryIndexBits) | nodeLeft(n)
+                                            : (kEntryRefetch << kEntryIndexBits) | nodeIdx;
+    return make_uint2(x, __float_as_uint(tEnter));
+}
+__device__ __forceinline__ uint2 instanceEntry(uint32_t inst) {
+    return make_uint2((kEntryInstance << kEntryIndexBits) | inst, 0u);
+}
+__device__ __forceinline__ uint32_t entryCode(uint2 e) { return e.x >> kEntryIndexBits; }
+// Cursor of a node entry (code != kEntryInstance).
+__device__ __forceinline__ Cursor entryCursor(const Node* __restrict__ nodes, uint2 e) {
+    const uint32_t code = entryCode(e), idx = e.x & kEntryIndexMask;
+    if (code == kEntryRefetch) return cursorOf(nodes[idx]);
+    return Cursor{ idx, code };
+}
+
+// Per-ray constants: origin, reciprocal direction and the shear of the
+// watertight test.
 struct RayK {
     float ox, oy, oz;
-    float dx, dy, dz;
     float ix, iy, iz;
     float sx, sy, sz;
     int kx, ky, kz;
@@ -60,7 +98,6 @@ __device__ __forceinline__ RayK setupRay(float ox, float oy, float oz, float dx,
 #pragma clang fp contract(off)
     RayK r;
     r.ox = ox; r.oy = oy; r.oz = oz;
-    r.dx = dx; r.dy = dy; r.dz = dz;
     r.ix = 1.0f / dx; r.iy = 1.0f / dy; r.iz = 1.0f / dz;
     const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
     int kz = 0;
@@ -140,8 +177,8 @@ __device__ __forceinline__ bool isectTri(const RayK& r, const float4& v0, const 
 
 // Short traversal stack: the top S entries of each lane live in LDS (entry s
 // of lane `tid` at lds[s * BLOCK + tid], so a wave's accesses to one slot are
-// conflict free); deeper entries spill to a private scratch array.  An entry
-// is (node index, entry distance bits).
+// conflict free); deeper entries spill to a private scratch array.  Entries
+// are encoded as described at nodeEntry.
 constexpr int kSpill = 64;
 
 //

@@ -23,8 +23,6 @@ struct SceneHit {
     uint32_t tri, inst;
 };
 
-constexpr uint32_t kInstEntry = 0x80000000u;  // stack entry tag: "enter instance"
-
 __device__ __forceinline__ void toObject(const TraceInstance& ti, float ox, float oy, float oz,
                                          float dx, float dy, float dz, float3& o, float3& d) {
     o.x = ((ti.row0.x * ox + ti.row0.y * oy) + ti.row0.z * oz) + ti.row0.w;
@@ -42,8 +40,9 @@ __device__ bool traceScene(const SceneView& sc, uint32_t tlasRoot, uint32_t inst
                            float3 o, float3 d, float tmin, float tmax, uint2* ldsLane,
                            SceneHit& out, bool& overflow) {
     if (instanceCount == 0) return false;
-    const RayK world = setupRay(o.x, o.y, o.z, d.x, d.y, d.z, tmin);
-    RayK r = world;
+    // The world ray's constants are rebuilt when traversal returns from an
+    // instance rather than kept, which keeps register pressure down.
+    RayK r = setupRay(o.x, o.y, o.z, d.x, d.y, d.z, tmin);
     uint2 spill[kSpill];
     ShortStack<S, BLOCK> stack(ldsLane, spill);
     bool inBlas = false;
@@ -54,9 +53,10 @@ __device__ bool traceScene(const SceneView& sc, uint32_t tlasRoot, uint32_t inst
     uint32_t bestTri = kNoHit, bestInst = kNoHit;
     float bestB1 = 0.0f, bestB2 = 0.0f;
 
-    Node cur = loadNode(sc.nodes, tlasRoot);
+    Node root = loadNode(sc.nodes, tlasRoot);
+    Cursor cur = cursorOf(root);
     float te;
-    bool have = slab(cur, r, best * kLimitWiden, te);
+    bool have = slab(root, r, best * kLimitWiden, te);
     bool pendingInst = false;
     uint32_t pendingIdx = 0;
 
@@ -66,17 +66,18 @@ __device__ bool traceScene(const SceneView& sc, uint32_t tlasRoot, uint32_t inst
             pendingInst = false;
             const TraceInstance ti = sc.traceInst[pendingIdx];
             float3 oo, dd;
-            toObject(ti, world.ox, world.oy, world.oz, world.dx, world.dy, world.dz, oo, dd);
+            toObject(ti, o.x, o.y, o.z, d.x, d.y, d.z, oo, dd);
             r = setupRay(oo.x, oo.y, oo.z, dd.x, dd.y, dd.z, tmin);
             inBlas = true;
             blasBase = stack.sp;
             curInst = pendingIdx;
-            cur = loadNode(sc.nodes, ti.blasRoot);
-            have = slab(cur, r, best * kLimitWiden, te);
+            root = loadNode(sc.nodes, ti.blasRoot);
+            cur = cursorOf(root);
+            have = slab(root, r, best * kLimitWiden, te);
         }
         if (have) {
-            const uint32_t lp = nodeLeft(cur);
-            const uint32_t cnt = nodeCount(cur);
+            const uint32_t lp = cur.index;
+            const uint32_t cnt = cur.count;
             if (cnt == 0) {
                 const Node c0 = loadNode(sc.nodes, lp);
                 const Node c1 = loadNode(sc.nodes, lp + 1);
@@ -86,12 +87,12 @@ __device__ bool traceScene(const SceneView& sc, uint32_t tlasRoot, uint32_t inst
                 const bool h1 = slab(c1, r, lim, t1);
                 if (h0 && h1) {
                     const bool firstNear = t0 <= t1;
-                    stack.push(make_uint2(firstNear ? lp + 1 : lp, __float_as_uint(firstNear ? t1 : t0)));
-                    cur = firstNear ? c0 : c1;
+                    stack.push(firstNear ? nodeEntry(c1, lp + 1, t1) : nodeEntry(c0, lp, t0));
+                    cur = cursorOf(firstNear ? c0 : c1);
                     continue;
                 }
                 if (h0 || h1) {
-                    cur = h0 ? c0 : c1;
+                    cur = cursorOf(h0 ? c0 : c1);
                     continue;
                 }
             } else if (inBlas) {
@@ -119,7 +120,7 @@ __device__ bool traceScene(const SceneView& sc, uint32_t tlasRoot, uint32_t inst
                 }
             } else {
                 // TLAS leaf: one or more instances [lp, lp + cnt).
-                for (uint32_t j = cnt - 1; j >= 1; --j) stack.push(make_uint2(kInstEntry | (lp + j), 0u));
+                for (uint32_t j = cnt - 1; j >= 1; --j) stack.push(instanceEntry(lp + j));
                 pendingInst = true;
                 pendingIdx = lp;
                 continue;
@@ -129,17 +130,17 @@ __device__ bool traceScene(const SceneView& sc, uint32_t tlasRoot, uint32_t inst
         while (true) {
             if (inBlas && stack.sp == blasBase) {
                 inBlas = false;
-                r = world;
+                r = setupRay(o.x, o.y, o.z, d.x, d.y, d.z, tmin);
             }
             if (stack.empty()) break;
             const uint2 e = stack.pop();
-            if (e.x & kInstEntry) {
+            if (entryCode(e) == kEntryInstance) {
                 pendingInst = true;
-                pendingIdx = e.x & ~kInstEntry;
+                pendingIdx = e.x & kEntryIndexMask;
                 break;
             }
             if (__uint_as_float(e.y) <= best * kLimitWiden) {
-                cur = loadNode(sc.nodes, e.x);
+                cur = entryCursor(sc.nodes, e);
                 have = true;
                 break;
             }
