@@ -215,13 +215,15 @@ def pt_workload(args, torch) -> dict:
         per_shadow = 48 + 32 * ns / max(qs, 1) + 64 * ts / max(qs, 1)
         frame_queries, frame_shadows = sum(rays), sum(shadows)
         bytes_alg = per_query * frame_queries + per_shadow * frame_shadows
-        trace_ms = stage["extend"] + stage["shadow"]
+        # the tail kernel traces the last few paths' rays (and shades them)
+        trace_ms = stage["extend"] + stage["shadow"] + stage["tail"]
         achieved = bytes_alg / (trace_ms * 1e-3) / 1e9
         kname = f"kTraceBatch<{ext[0]}, {ext[1]}>"
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": achieved / HBM_PEAK_GBS,
                            "traffic": _measured_traffic(kname, "bytes_per_frame"),
-                           "kernel": f"{kname} (closest hit + any hit, all trace launches of one frame)",
+                           "kernel": f"{kname} (closest hit + any hit, all trace launches of one frame, "
+                                     "with the tail kernel)",
                            "bytes_per_frame": bytes_alg, "trace_ms_per_frame": trace_ms,
                            "queries_per_frame": frame_queries, "shadow_queries_per_frame": frame_shadows,
                            "nodes_per_query": nn / max(q, 1), "tris_per_query": nt / max(q, 1),

@@ -122,7 +122,13 @@ typedef struct mrtxPathOptions {
     uint32_t extendStack;   /* bounce with the previous bounce's shadow rays (0 = default / tuned) */
     uint32_t shadowBlock;   /* shape of the last launch: the deepest bounce's shadow rays */
     uint32_t shadowStack;
+    uint32_t tailPaths;     /* once a bounce level holds at most this many rays (all batched
+                             * requests together), one kernel finishes those paths instead of a
+                             * launch per level; 0 = default (1280 per compute unit),
+                             * MRTX_TAIL_DISABLED = never.  Images do not depend on it. */
 } mrtxPathOptions;
+
+#define MRTX_TAIL_DISABLED 0xFFFFFFFFu
 
 MRT_API mrtResult mrtxSetPathOptions(mrtEngine* engine, const mrtxPathOptions* opts);
 MRT_API mrtResult mrtxGetPathOptions(mrtEngine* engine, mrtxPathOptions* out);
@@ -132,10 +138,13 @@ MRT_API mrtResult mrtxGetPathOptions(mrtEngine* engine, mrtxPathOptions* out);
 #define MRTX_MAX_BOUNCE_LEVELS 18
 
 /* Stage order of gpuMs: generate, extend, shade, shadow, accumulate (with the
- * RGBA8 resolve, one kernel), resolve (0: part of accumulate), guide (prep +
- * network inference), train (record collection + Adam steps).  Each bounce's
- * closest-hit launch also traces the previous bounce's shadow rays, so
- * "extend" holds those and "shadow" only the deepest bounce's shadow rays. */
+ * RGBA8 resolve, one kernel), tail, guide (prep + network inference), train
+ * (record collection + Adam steps).  Each bounce's closest-hit launch also
+ * traces the previous bounce's shadow rays, so "extend" holds those and
+ * "shadow" only the deepest bounce's shadow rays; "tail" is the kernel that
+ * finishes the last few paths (their shading, closest hits and shadow rays;
+ * see mrtxPathOptions::tailPaths).  rays / shadowRays count every ray traced
+ * at a level, by the per-level launches or the tail kernel. */
 typedef struct mrtxFrameStats {
     size_t   structSize;
     float    frameMs;                                /* host wall time of the last frame */

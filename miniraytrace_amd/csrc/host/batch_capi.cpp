@@ -87,6 +87,9 @@ void renderMerged(const std::vector<mrt::Engine*>& engines, std::vector<uint32_t
     int perCU = 0;
     MRT_HIP(mrt::dev::batchShadeBlocksPerCU(&perCU));
     const uint32_t shadeGrid = uint32_t(std::max(1, perCU) * engines[0]->cuCount());
+    const mrt::dev::TraceShape tail = engines[0]->tailShape();
+    // The tail threshold of the first request (the batch shares one tail).
+    const uint32_t tailPaths = engines[0]->tailPaths();
     hipStream_t s = engines[0]->stream();
     BatchGraphs g;
     g.chains.assign(1, nullptr);
@@ -102,7 +105,8 @@ void renderMerged(const std::vector<mrt::Engine*>& engines, std::vector<uint32_t
             MRT_HIP(mrt::dev::launchGenerateBatch(bv, s));
             for (uint32_t b = 0; b < levels; ++b) {
                 MRT_HIP(mrt::dev::launchTraceBatch(bv, b, b == 0 ? mrt::dev::kNoLevel : b - 1, ext, s));
-                MRT_HIP(mrt::dev::launchShadeBatch(bv, b, shadeGrid, s));
+                MRT_HIP(mrt::dev::launchShadeBatch(bv, b, tailPaths, shadeGrid, s));
+                MRT_HIP(mrt::dev::launchTailBatch(bv, b, tailPaths, tail, s));
             }
             MRT_HIP(mrt::dev::launchTraceBatch(bv, mrt::dev::kNoLevel, levels - 1, sha, s));
             MRT_HIP(mrt::dev::launchFinishBatch(bv, s));

@@ -184,6 +184,11 @@ void Engine::chooseShapes() {
     shadowShape_.grid = uint32_t(std::max(1, perCU) * cuCount_);
     MRT_HIP(dev::batchShadeBlocksPerCU(&perCU));
     shadeGrid_ = uint32_t(std::max(1, perCU) * cuCount_);
+    MRT_HIP(dev::batchTailBlocksPerCU(tailBlock_, tailStack_, &perCU));
+    tailGrid_ = uint32_t(std::max(1, perCU) * cuCount_);
+    tailPaths_ = tailOption_ == MRTX_TAIL_DISABLED ? 0u
+                 : tailOption_ != 0                ? tailOption_
+                                                   : kTailPathsPerCU * uint32_t(cuCount_);
     MRT_HIP(dev::guidedShadeBlocksPerCU(&perCU));
     shadeGuideGrid_ = uint32_t(std::max(1, perCU) * cuCount_);
     invalidateGraph();
@@ -227,14 +232,17 @@ void Engine::setSettings(const Settings& s) {
 void Engine::setPathOptions(const mrtxPathOptions& o) {
     seed_ = o.seed;
     useGraph_ = o.useGraph != 0;
-    const uint32_t eb = o.extendBlock ? o.extendBlock : 128, es = o.extendStack ? o.extendStack : 16;
-    const uint32_t sb = o.shadowBlock ? o.shadowBlock : 128, ss = o.shadowStack ? o.shadowStack : 16;
+    const uint32_t eb = o.extendBlock ? o.extendBlock : kDefaultTraceBlock;
+    const uint32_t es = o.extendStack ? o.extendStack : kDefaultTraceStack;
+    const uint32_t sb = o.shadowBlock ? o.shadowBlock : kDefaultTraceBlock;
+    const uint32_t ss = o.shadowStack ? o.shadowStack : kDefaultTraceStack;
     if (!dev::ptShapeSupported(eb, es) || !dev::ptShapeSupported(sb, ss))
         throw Error(MRT_ERROR_INVALID_ARGUMENT, "unsupported traversal launch shape");
     extendShape_.block = eb;
     extendShape_.stack = es;
     shadowShape_.block = sb;
     shadowShape_.stack = ss;
+    tailOption_ = o.tailPaths;
     DeviceScope scope(device_);
     chooseShapes();
 }
@@ -246,6 +254,7 @@ void Engine::getPathOptions(mrtxPathOptions& o) const {
     o.extendStack = extendShape_.stack;
     o.shadowBlock = shadowShape_.block;
     o.shadowStack = shadowShape_.stack;
+    o.tailPaths = tailOption_;
 }
 
 // --------------------------------------------------------------------- commit --
@@ -839,13 +848,16 @@ void Engine::enqueueFrame(hipStream_t s, bool timed, const FrameConsts* g, const
         }
         if (timed) MRT_HIP(hipEventRecord(ev_[9], s));
         if (guide) MRT_HIP(dev::launchShadeGuided(sc, pv, g, b, shadeGuideGrid_, s));
-        else MRT_HIP(dev::launchShadeBatch(bv, b, shadeGrid_, s));
+        else MRT_HIP(dev::launchShadeBatch(bv, b, tailPaths_, shadeGrid_, s));
+        if (timed) MRT_HIP(hipEventRecord(ev_[4], s));
+        if (!guide) MRT_HIP(dev::launchTailBatch(bv, b, tailPaths_, tailShape(), s));
         if (timed) {
-            MRT_HIP(hipEventRecord(ev_[4], s));
-            MRT_HIP(hipEventSynchronize(ev_[4]));
+            MRT_HIP(hipEventRecord(ev_[8], s));
+            MRT_HIP(hipEventSynchronize(ev_[8]));
             stats_.stageMs[1] += elapsed(ev_[2], ev_[3]);
             stats_.stageMs[6] += elapsed(ev_[3], ev_[9]);
             stats_.stageMs[2] += elapsed(ev_[9], ev_[4]);
+            stats_.stageMs[5] += elapsed(ev_[4], ev_[8]);
         }
     }
     if (timed) MRT_HIP(hipEventRecord(ev_[4], s));
@@ -865,7 +877,6 @@ void Engine::enqueueFrame(hipStream_t s, bool timed, const FrameConsts* g, const
         stats_.stageMs[0] = elapsed(ev_[0], ev_[1]);
         stats_.stageMs[7] = elapsed(ev_[10], ev_[6]);
         stats_.stageMs[4] = elapsed(ev_[6], ev_[7]);
-        stats_.stageMs[5] = 0.0f;
     }
 }
 
@@ -922,8 +933,10 @@ void Engine::renderFrame(mrtFrameInfo* out) {
             MRT_HIP(hipStreamSynchronize(stream_));
             const uint32_t* c = static_cast<const uint32_t*>(hCounters_.get());
             for (uint32_t b = 0; b < dev::kMaxBounceLevels; ++b) {
-                stats_.rays[b] = c[b * dev::kCntPerBounce + dev::kCntRays];
-                stats_.shadowRays[b] = c[b * dev::kCntPerBounce + dev::kCntShadows];
+                stats_.rays[b] = c[b * dev::kCntPerBounce + dev::kCntRays] +
+                                 c[b * dev::kCntPerBounce + dev::kCntTailRays];
+                stats_.shadowRays[b] = c[b * dev::kCntPerBounce + dev::kCntShadows] +
+                                       c[b * dev::kCntPerBounce + dev::kCntTailShadows];
             }
             stats_.status |= c[dev::kStatusWord];
         } else if (useGraph_) {
@@ -1073,8 +1086,8 @@ void Engine::autotuneShapes(mrtxPathOptions& chosen) {
         const uint32_t sppLimit = settings_.sppLimit;
         settings_.sppLimit = std::max(sppLimit, 1u << 30);
         profiling_ = true;
-        // One shape for both trace launches: the one with the least trace
-        // time over a whole frame.
+        // One shape for both trace launch kinds and the tail kernel: the one
+        // with the least time in those kernels over a whole frame.
         float best = 1e30f;
         dev::TraceShape be = extendShape_, bs = shadowShape_;
         for (const auto& c : kCand) {
@@ -1083,7 +1096,7 @@ void Engine::autotuneShapes(mrtxPathOptions& chosen) {
             chooseShapes();
             resetAccumulation();
             renderFrame(nullptr);
-            const float t = stats_.stageMs[1] + stats_.stageMs[3];
+            const float t = stats_.stageMs[1] + stats_.stageMs[3] + stats_.stageMs[5];
             if (t < best) {
                 best = t;
                 be = extendShape_;

@@ -19,6 +19,20 @@
 
 namespace mrt {
 
+// Default launch shape of the path tracer's trace and tail kernels (threads
+// per workgroup, LDS traversal-stack entries per lane).
+constexpr uint32_t kDefaultTraceBlock = 256;
+constexpr uint32_t kDefaultTraceStack = 8;
+// Tail kernel shape: its registers (shading and traversal in one kernel)
+// allow 2 waves per SIMD, so a deep LDS stack costs no occupancy.
+constexpr uint32_t kTailBlock = 128;
+constexpr uint32_t kTailStack = 32;
+// Default tail threshold: the tail kernel takes over a bounce level once all
+// requests together hold at most this many rays per compute unit there
+// (measured on the 1.256M-triangle terrain at 1080p: the tail then covers the
+// levels whose launches are dominated by their slowest rays).
+constexpr uint32_t kTailPathsPerCU = 1280;
+
 struct Settings {
     uint32_t width = 1280, height = 720;
     uint32_t maxBounces = 8;
@@ -72,6 +86,8 @@ public:
     bool guideActive() const { return guide_.enableSampling || guide_.enableTraining; }
     const dev::TraceShape& extendShape() const { return extendShape_; }
     const dev::TraceShape& shadowShape() const { return shadowShape_; }
+    uint32_t tailPaths() const { return tailPaths_; }
+    dev::TraceShape tailShape() const { return dev::TraceShape{ tailBlock_, tailStack_, tailGrid_ }; }
     uint32_t pixels() const { return frameW_ * frameH_; }
     uint32_t bounceLevels() const { return std::min(settings_.maxBounces, 16u) + 1; }
     int cuCount() const { return cuCount_; }
@@ -144,8 +160,12 @@ private:
     DeviceBuffer dBatchConsts_, dBatchEntries_, dEntry_, dCursors_;
     uint32_t frameW_ = 0, frameH_ = 0;
 
-    dev::TraceShape extendShape_{ 256, 8, 0 };
-    dev::TraceShape shadowShape_{ 256, 8, 0 };
+    dev::TraceShape extendShape_{ kDefaultTraceBlock, kDefaultTraceStack, 0 };
+    dev::TraceShape shadowShape_{ kDefaultTraceBlock, kDefaultTraceStack, 0 };
+    uint32_t tailOption_ = 0;  // mrtxPathOptions::tailPaths as set
+    uint32_t tailPaths_ = 0;   // tail kernel threshold passed to the kernels (0 = never)
+    uint32_t tailGrid_ = 0;
+    uint32_t tailBlock_ = kTailBlock, tailStack_ = kTailStack;
     uint32_t shadeGrid_ = 0;
     uint32_t shadeGuideGrid_ = 0;
 

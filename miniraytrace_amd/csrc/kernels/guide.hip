@@ -440,22 +440,28 @@ __global__ __launch_bounds__(256) void kGuideBackLayer(int l, const uint32_t* __
     }
 }
 
-// Partial weight and bias gradients of layer l over one chunk of rows:
-// wave job = (tile, chunk); tile (mt, nt) covers outputs 32 mt.. and inputs
+// Partial weight and bias gradients of layer l over one chunk of rows.
+// Block job = (tile, chunk); tile (mt, nt) covers outputs 32 mt.. and inputs
 // 32 nt.. (layer 0: 3 input tiles over the 96 zero-padded input features).
-// Tiles with nt == 0 also form the bias sums with an all-ones B operand.
+// The block's 4 waves take consecutive quarters of the chunk's rows, and
+// wave 0 adds their accumulators in wave order (fixed summation order, no
+// atomics).  Tiles with nt == 0 also form the bias sums with an all-ones B
+// operand.
+constexpr uint32_t kGradSlice = kGuideGradChunk / 4;  // rows per wave
 __global__ __launch_bounds__(256) void kGuideWeightGrad(int l, const uint32_t* __restrict__ words,
                                                         const uint16_t* __restrict__ delta,
                                                         const uint16_t* __restrict__ actPrev, uint32_t stride,
                                                         float* __restrict__ partial) {
+    __shared__ float sAcc[4][16][64];
+    __shared__ float sBias[4][16][64];
     const uint32_t rows = pad32(words[kGwBatch]);
     const int nin = guideLayerIn(l), nout = guideLayerOut(l);
     const int ntIn = l == 0 ? 3 : 2;
-    const uint32_t job = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const uint32_t tile = job % uint32_t(2 * ntIn), chunk = job / uint32_t(2 * ntIn);
+    const uint32_t tile = blockIdx.x % uint32_t(2 * ntIn), chunk = blockIdx.x / uint32_t(2 * ntIn);
     const uint32_t r0 = chunk * kGuideGradChunk;
-    if (r0 >= rows) return;  // whole wave
-    const uint32_t r1 = min(r0 + kGuideGradChunk, rows);
+    if (r0 >= rows) return;  // whole block
+    const uint32_t w = threadIdx.x >> 6;
+    const uint32_t s0 = r0 + w * kGradSlice, s1 = min(s0 + kGradSlice, rows);
     const int mt = int(tile) / ntIn, nt = int(tile) % ntIn;
     const int lane = threadIdx.x & 63, lr = lane & 31, h = lane >> 5;
     const uint16_t* aRow = delta + size_t(32 * mt + lr) * stride + 8 * h;
@@ -463,10 +469,20 @@ __global__ __launch_bounds__(256) void kGuideWeightGrad(int l, const uint32_t* _
     const uint32_t one2 = 0x3F803F80u;  // two bf16 1.0
     const bf16x8 ones = __builtin_bit_cast(bf16x8, u32x4{ one2, one2, one2, one2 });
     f32x16 acc = zero16(), accB = zero16();
-    for (uint32_t r = r0; r < r1; r += 16) {
+    for (uint32_t r = s0; r < s1; r += 16) {
         const bf16x8 a = ld8(aRow + r);
         acc = mfma(a, ld8(bRow + r), acc);
         if (nt == 0) accB = mfma(a, ones, accB);
+    }
+    for (int i = 0; i < 16; ++i) {
+        sAcc[w][i][lane] = acc[i];
+        sBias[w][i][lane] = accB[i];
+    }
+    __syncthreads();
+    if (w != 0) return;
+    for (int i = 0; i < 16; ++i) {
+        acc[i] = ((sAcc[0][i][lane] + sAcc[1][i][lane]) + sAcc[2][i][lane]) + sAcc[3][i][lane];
+        accB[i] = ((sBias[0][i][lane] + sBias[1][i][lane]) + sBias[2][i][lane]) + sBias[3][i][lane];
     }
     float* dst = partial + size_t(chunk) * kGuideParams;
     for (int j = 0; j < 4; ++j)
@@ -630,7 +646,7 @@ hipError_t launchGuideGradient(const GuideBatch& b, const GuideNet& net, bool co
     for (int l = 4; l >= 0; --l) {
         const uint16_t* prev = l == 0 ? net.actIn : net.act + size_t(l - 1) * 64 * net.stride;
         const uint32_t jobs = chunks * (l == 0 ? 6u : 4u);
-        hipLaunchKernelGGL(kGuideWeightGrad, dim3((jobs + 3) / 4), dim3(256), 0, s, l, net.words, net.delta[cur], prev,
+        hipLaunchKernelGGL(kGuideWeightGrad, dim3(jobs), dim3(256), 0, s, l, net.words, net.delta[cur], prev,
                            net.stride, net.partial);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if (l > 0) {

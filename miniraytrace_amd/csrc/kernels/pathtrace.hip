@@ -17,6 +17,12 @@
 // random numbers follow the reference's draw order exactly (pcg4d keyed by
 // pixel, frame and a per-path counter), so a path consumes the same random
 // numbers as the reference megakernel would.
+//
+// No floating-point contraction anywhere in this file (the pragma precedes
+// the includes, so it covers the inlined device helpers too): a path's
+// arithmetic is then the same whichever kernel shades it (per-level, tail,
+// batched or alone), which keeps images independent of how work is split.
+#pragma clang fp contract(off)
 #include "guide_device.h"
 #include "pt_device.h"
 #include "pt_launch.h"
@@ -262,189 +268,209 @@ __global__ __launch_bounds__(256) void kGuidePrep(SceneView sc, PathView pv, con
 }
 
 // ------------------------------------------------------------------- shade --
+// Outputs of shading one path vertex: the continuation ray and the shadow
+// ray (NEE), each when emitted.
+struct ShadeOut {
+    bool emitRay, emitShadow;
+    V3 newO, newD, shO, shD, shC;
+    float shMax;
+};
+
+// Shades path p at the hit `h` (t, b1, b2, triangle bits; instance hitInst)
+// of its bounce-`bounce` ray with direction `dir`: reads and updates the
+// path's throughput, radiance and random-number counter.  guideRow: the
+// vertex's row of decoded guiding mixtures (GUIDE), else kNoHit.
+//
 // GUIDE: the prep pass ran for this bounce.  At a guided vertex with
 // FrameConsts::guiding set, the continuation direction comes from the learned
 // mixture with probability guideSelectProb and from the BSDF otherwise
 // (one-sample MIS: the path weight uses the combined pdf, which NEE's MIS
 // weight uses too).  With guideTrain set, every guided vertex that continues
-// writes a training record.  Without GUIDE (or with neither flag) the kernel
+// writes a training record.  Without GUIDE (or with neither flag) this
 // performs exactly the reference's operations.
+template <bool GUIDE>
+__device__ __forceinline__ ShadeOut shadePath(const SceneView& sc, const PathView& pv, const FrameConsts& g,
+                                              uint32_t bounce, uint32_t p, V3 dir, float4 h, uint32_t hitInst,
+                                              uint32_t guideRow) {
+    const bool last = bounce >= g.maxBounces;
+    ShadeOut o;
+    o.emitRay = false;
+    o.emitShadow = false;
+    o.shMax = 0.0f;
+    const uint32_t tri = __float_as_uint(h.w);
+    float4 beta4 = pv.beta[p];
+    V3 beta = v3(beta4);
+    float prevPdf = beta4.w;
+    float4 L4 = pv.radiance[p];
+    V3 L = v3(L4);
+    Rng rng{ p % g.width, p / g.width, g.frameIndex ^ g.seedMix, pv.rngCounter[p] };
+    if (tri == kNoHit) {
+        // pathtrace.comp:135-147: environment, MIS against NEE after a BSDF bounce.
+        const V3 env = envRadiance(sc, g, dir);
+        if (maxComp(env) > 0.0f) {
+            float w = 1.0f;
+            if (bounce > 0) {
+                const float pl = envPdf(sc, g, dir) * envPickPdf(g);
+                w = balanceHeuristic(prevPdf, pl);
+                if (GUIDE && g.guideTrain && p % g.guideTrainStride == g.guideTrainPhase) {
+                    // Training target of the previous vertex: its own
+                    // continuation's environment hit counts with the MIS
+                    // weight of the BSDF pdf (L4.w), not of the guided
+                    // mixture, so light that NEE already samples well
+                    // (the sun) does not pull the mixture towards itself.
+                    const float wTrain = balanceHeuristic(L4.w, pl);
+                    const uint32_t ts = g.guideTrainStride;
+                    float4* r = pv.pathRec + (size_t(bounce - 1) * ((pv.capacity + ts - 1) / ts) + p / ts) * 4;
+                    if (wTrain != w && __float_as_uint(r[2].w) == g.guideStamp)
+                        r[1].w -= luminance(beta * env) * (wTrain - w);
+                }
+            }
+            L = L + beta * env * w;
+        }
+    } else {
+        const Surface s = makeSurface(sc, dir, h.y, h.z, tri, hitInst);
+        const float opacity = s.mat.baseColorOpacity.w;
+        if (opacity < 1.0f && rng.next1() >= opacity) {
+            // Stochastic alpha: pass through, consuming one bounce.
+            o.newO = s.pos + dir * kEpsRay;
+            o.newD = dir;
+            o.emitRay = !last;
+        } else {
+            const V3 emission = s.mat.emissionTex >= 0
+                                    ? v3(s.mat.emissionRoughness) * v3(sampleTexture(sc, s.mat.emissionTex, s.u, s.v))
+                                    : v3(s.mat.emissionRoughness);
+            if (maxComp(emission) > 0.0f && s.front) L = L + beta * emission;
+            const Bsdf bsdf = makeBsdf(sc, s);
+            V3 T, B;
+            onb(s.nShade, T, B);
+            V3 wo = v3(dot(-dir, T), dot(-dir, B), dot(-dir, s.nShade));
+            if (wo.z < 1e-6f) wo.z = 1e-6f;
+            wo = normalize(wo);
+            const float4* mix = nullptr;  // learned mixture of this vertex
+            float sp = 0.0f;
+            bool record = false;
+            if (GUIDE) {
+                const uint32_t gi = guideRow;
+                if (gi != kNoHit) {
+                    record = g.guideTrain != 0 && p % g.guideTrainStride == g.guideTrainPhase;
+                    if (g.guiding) {
+                        mix = pv.guideMix + size_t(gi) * kGuideMixF4;
+                        sp = g.guideSelectProb;
+                    }
+                }
+            }
+            // NEE (pathtrace.comp:181-197)
+            const LightSample ls = sampleOneLight(sc, g, s.pos, rng);
+            if (ls.valid && ls.pdf > 1e-9f) {
+                const V3 wi = v3(dot(ls.wi, T), dot(ls.wi, B), dot(ls.wi, s.nShade));
+                V3 f;
+                float bsdfPdf;
+                bsdfEval(bsdf, wo, wi, f, bsdfPdf);
+                if (maxComp(f) > 0.0f) {
+                    // MIS against the continuation's pdf.  At the last bounce
+                    // no continuation is drawn and the reference weights NEE
+                    // against the BSDF pdf alone; guided shading does the same,
+                    // so both tracers converge to the same image.
+                    float contPdf = bsdfPdf;
+                    if (GUIDE && mix && !last) contPdf = sp * storedMixPdf(mix, ls.wi) + (1.0f - sp) * bsdfPdf;
+                    const float w = ls.delta ? 1.0f : balanceHeuristic(ls.pdf, contPdf);
+                    o.shO = s.pos + s.nGeom * (wi.z > 0.0f ? kEpsRay : -kEpsRay);
+                    o.shD = ls.wi;
+                    o.shMax = ls.dist - 2.0f * kEpsRay;
+                    o.shC = beta * f * ls.radiance * (fabsf(wi.z) * w / ls.pdf);
+                    o.emitShadow = true;
+                }
+            }
+            // BSDF bounce and Russian roulette (pathtrace.comp:200-216)
+            if (!last) {
+                V3 wiL, wiW, f;
+                float pdf = 0.0f, pBsdf = 0.0f;
+                bool ok;
+                if (GUIDE && mix && rng.next1() < sp) {
+                    // Draw from the learned mixture.
+                    const float u0 = rng.next1();
+                    float u1, u2;
+                    rng.next2(u1, u2);
+                    wiW = storedMixSample(mix, u0, u1, u2);
+                    wiL = v3(dot(wiW, T), dot(wiW, B), dot(wiW, s.nShade));
+                    bsdfEval(bsdf, wo, wiL, f, pBsdf);
+                    pdf = sp * storedMixPdf(mix, wiW) + (1.0f - sp) * pBsdf;
+                    ok = maxComp(f) > 0.0f && pdf > 1e-9f;
+                } else {
+                    const BsdfSample bs = bsdfSample(bsdf, wo, rng);
+                    ok = bs.valid;
+                    wiL = bs.wi;
+                    f = bs.f;
+                    pdf = bs.pdf;
+                    pBsdf = bs.pdf;
+                    if (ok) wiW = normalize(T * bs.wi.x + B * bs.wi.y + s.nShade * bs.wi.z);
+                    if (GUIDE && mix && ok) {
+                        pdf = sp * storedMixPdf(mix, wiW) + (1.0f - sp) * bs.pdf;
+                        ok = pdf > 1e-9f;
+                    }
+                }
+                if (ok) {
+                    const float lumBeta = luminance(beta);
+                    beta = beta * f * (fabsf(wiL.z) / pdf);
+                    prevPdf = pdf;
+                    if (GUIDE) L4.w = pBsdf;  // BSDF pdf of this continuation
+                    bool alive = true;
+                    if (bounce >= 3) {
+                        const float q = clampf(maxComp(beta), 0.05f, 0.95f);
+                        if (rng.next1() >= q) alive = false;
+                        else beta = beta * (1.0f / q);
+                    }
+                    if (alive) {
+                        o.newO = s.pos + s.nGeom * (wiL.z > 0.0f ? kEpsRay : -kEpsRay);
+                        o.newD = wiW;
+                        o.emitRay = true;
+                        if (GUIDE && record) {
+                            const uint32_t trainPaths = (pv.capacity + g.guideTrainStride - 1) / g.guideTrainStride;
+                            float4* r = pv.pathRec + (size_t(bounce) * trainPaths + p / g.guideTrainStride) * 4;
+                            r[0] = f4((s.pos - v3(g.guideBoxMin)) * v3(g.guideBoxScale), lumBeta);
+                            r[1] = f4(s.nShade, luminance(L));
+                            r[2] = f4(-dir, __uint_as_float(g.guideStamp));
+                            r[3] = f4(wiW, 0.0f);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    pv.beta[p] = f4(beta, prevPdf);
+    pv.radiance[p] = f4(L, L4.w);
+    pv.rngCounter[p] = rng.counter;
+    return o;
+}
+
+// Shading of queue slots [base, base + 64) of ray queue `bounce`: new rays
+// to queue bounce + 1, shadow rays to the shadow queue.
 template <bool GUIDE>
 __device__ __forceinline__ void shadeChunk(const SceneView& sc, const PathView& pv, const FrameConsts& g,
                                            uint32_t bounce, uint32_t count, uint32_t base) {
     uint32_t* cnt = pv.counters + bounce * kCntPerBounce;
     uint32_t* next = pv.counters + (bounce + 1) * kCntPerBounce;
-    const float4* rayO = pv.rayO[bounce & 1];
-    const float4* rayD = pv.rayD[bounce & 1];
-    float4* nrayO = pv.rayO[(bounce + 1) & 1];
-    float4* nrayD = pv.rayD[(bounce + 1) & 1];
-    const bool last = bounce >= g.maxBounces;
-    {
-        const uint32_t k = base + laneId();
-        bool emitRay = false, emitShadow = false;
-        V3 newO, newD, shO, shD, shC;
-        float shMax = 0.0f;
-        uint32_t p = 0;
-        if (k < count) {
-            const float4 ro = rayO[k], rd = rayD[k];
-            p = __float_as_uint(ro.w);
-            const V3 dir = v3(rd);
-            const float4 h = pv.hit[k];
-            const uint32_t tri = __float_as_uint(h.w);
-            float4 beta4 = pv.beta[p];
-            V3 beta = v3(beta4);
-            float prevPdf = beta4.w;
-            float4 L4 = pv.radiance[p];
-            V3 L = v3(L4);
-            Rng rng{ p % g.width, p / g.width, g.frameIndex ^ g.seedMix, pv.rngCounter[p] };
-            if (tri == kNoHit) {
-                // pathtrace.comp:135-147: environment, MIS against NEE after a BSDF bounce.
-                const V3 env = envRadiance(sc, g, dir);
-                if (maxComp(env) > 0.0f) {
-                    float w = 1.0f;
-                    if (bounce > 0) {
-                        const float pl = envPdf(sc, g, dir) * envPickPdf(g);
-                        w = balanceHeuristic(prevPdf, pl);
-                        if (GUIDE && g.guideTrain && p % g.guideTrainStride == g.guideTrainPhase) {
-                            // Training target of the previous vertex: its own
-                            // continuation's environment hit counts with the MIS
-                            // weight of the BSDF pdf (L4.w), not of the guided
-                            // mixture, so light that NEE already samples well
-                            // (the sun) does not pull the mixture towards itself.
-                            const float wTrain = balanceHeuristic(L4.w, pl);
-                            const uint32_t ts = g.guideTrainStride;
-                            float4* r = pv.pathRec + (size_t(bounce - 1) * ((pv.capacity + ts - 1) / ts) + p / ts) * 4;
-                            if (wTrain != w && __float_as_uint(r[2].w) == g.guideStamp)
-                                r[1].w -= luminance(beta * env) * (wTrain - w);
-                        }
-                    }
-                    L = L + beta * env * w;
-                }
-            } else {
-                const Surface s = makeSurface(sc, dir, h.y, h.z, tri, pv.hitInst[k]);
-                const float opacity = s.mat.baseColorOpacity.w;
-                if (opacity < 1.0f && rng.next1() >= opacity) {
-                    // Stochastic alpha: pass through, consuming one bounce.
-                    newO = s.pos + dir * kEpsRay;
-                    newD = dir;
-                    emitRay = !last;
-                } else {
-                    const V3 emission = s.mat.emissionTex >= 0
-                                            ? v3(s.mat.emissionRoughness) * v3(sampleTexture(sc, s.mat.emissionTex, s.u, s.v))
-                                            : v3(s.mat.emissionRoughness);
-                    if (maxComp(emission) > 0.0f && s.front) L = L + beta * emission;
-                    const Bsdf bsdf = makeBsdf(sc, s);
-                    V3 T, B;
-                    onb(s.nShade, T, B);
-                    V3 wo = v3(dot(-dir, T), dot(-dir, B), dot(-dir, s.nShade));
-                    if (wo.z < 1e-6f) wo.z = 1e-6f;
-                    wo = normalize(wo);
-                    const float4* mix = nullptr;  // learned mixture of this vertex
-                    float sp = 0.0f;
-                    bool record = false;
-                    if (GUIDE) {
-                        const uint32_t gi = pv.guideIdx[k];
-                        if (gi != kNoHit) {
-                            record = g.guideTrain != 0 && p % g.guideTrainStride == g.guideTrainPhase;
-                            if (g.guiding) {
-                                mix = pv.guideMix + size_t(gi) * kGuideMixF4;
-                                sp = g.guideSelectProb;
-                            }
-                        }
-                    }
-                    // NEE (pathtrace.comp:181-197)
-                    const LightSample ls = sampleOneLight(sc, g, s.pos, rng);
-                    if (ls.valid && ls.pdf > 1e-9f) {
-                        const V3 wi = v3(dot(ls.wi, T), dot(ls.wi, B), dot(ls.wi, s.nShade));
-                        V3 f;
-                        float bsdfPdf;
-                        bsdfEval(bsdf, wo, wi, f, bsdfPdf);
-                        if (maxComp(f) > 0.0f) {
-                            // MIS against the continuation's pdf.  At the last bounce
-                            // no continuation is drawn and the reference weights NEE
-                            // against the BSDF pdf alone; guided shading does the same,
-                            // so both tracers converge to the same image.
-                            float contPdf = bsdfPdf;
-                            if (GUIDE && mix && !last) contPdf = sp * storedMixPdf(mix, ls.wi) + (1.0f - sp) * bsdfPdf;
-                            const float w = ls.delta ? 1.0f : balanceHeuristic(ls.pdf, contPdf);
-                            shO = s.pos + s.nGeom * (wi.z > 0.0f ? kEpsRay : -kEpsRay);
-                            shD = ls.wi;
-                            shMax = ls.dist - 2.0f * kEpsRay;
-                            shC = beta * f * ls.radiance * (fabsf(wi.z) * w / ls.pdf);
-                            emitShadow = true;
-                        }
-                    }
-                    // BSDF bounce and Russian roulette (pathtrace.comp:200-216)
-                    if (!last) {
-                        V3 wiL, wiW, f;
-                        float pdf = 0.0f, pBsdf = 0.0f;
-                        bool ok;
-                        if (GUIDE && mix && rng.next1() < sp) {
-                            // Draw from the learned mixture.
-                            const float u0 = rng.next1();
-                            float u1, u2;
-                            rng.next2(u1, u2);
-                            wiW = storedMixSample(mix, u0, u1, u2);
-                            wiL = v3(dot(wiW, T), dot(wiW, B), dot(wiW, s.nShade));
-                            bsdfEval(bsdf, wo, wiL, f, pBsdf);
-                            pdf = sp * storedMixPdf(mix, wiW) + (1.0f - sp) * pBsdf;
-                            ok = maxComp(f) > 0.0f && pdf > 1e-9f;
-                        } else {
-                            const BsdfSample bs = bsdfSample(bsdf, wo, rng);
-                            ok = bs.valid;
-                            wiL = bs.wi;
-                            f = bs.f;
-                            pdf = bs.pdf;
-                            pBsdf = bs.pdf;
-                            if (ok) wiW = normalize(T * bs.wi.x + B * bs.wi.y + s.nShade * bs.wi.z);
-                            if (GUIDE && mix && ok) {
-                                pdf = sp * storedMixPdf(mix, wiW) + (1.0f - sp) * bs.pdf;
-                                ok = pdf > 1e-9f;
-                            }
-                        }
-                        if (ok) {
-                            const float lumBeta = luminance(beta);
-                            beta = beta * f * (fabsf(wiL.z) / pdf);
-                            prevPdf = pdf;
-                            if (GUIDE) L4.w = pBsdf;  // BSDF pdf of this continuation
-                            bool alive = true;
-                            if (bounce >= 3) {
-                                const float q = clampf(maxComp(beta), 0.05f, 0.95f);
-                                if (rng.next1() >= q) alive = false;
-                                else beta = beta * (1.0f / q);
-                            }
-                            if (alive) {
-                                newO = s.pos + s.nGeom * (wiL.z > 0.0f ? kEpsRay : -kEpsRay);
-                                newD = wiW;
-                                emitRay = true;
-                                if (GUIDE && record) {
-                                    const uint32_t trainPaths = (pv.capacity + g.guideTrainStride - 1) / g.guideTrainStride;
-                                    float4* r = pv.pathRec + (size_t(bounce) * trainPaths + p / g.guideTrainStride) * 4;
-                                    r[0] = f4((s.pos - v3(g.guideBoxMin)) * v3(g.guideBoxScale), lumBeta);
-                                    r[1] = f4(s.nShade, luminance(L));
-                                    r[2] = f4(-dir, __uint_as_float(g.guideStamp));
-                                    r[3] = f4(wiW, 0.0f);
-                                }
-                            }
-                        }
-                    }
-                }
-            }
-            pv.beta[p] = f4(beta, prevPdf);
-            pv.radiance[p] = f4(L, L4.w);
-            pv.rngCounter[p] = rng.counter;
-        }
-        const uint32_t rs = waveAppend(next + kCntRays, emitRay);
-        if (emitRay) {
-            nrayO[rs] = f4(newO, __uint_as_float(p));
-            nrayD[rs] = f4(newD, 0.0f);
-        }
-        const uint32_t ss = waveAppend(cnt + kCntShadows, emitShadow);
-        if (emitShadow) {
-            pv.shadowO[ss] = f4(shO, shMax);
-            pv.shadowD[ss] = f4(shD, __uint_as_float(p));
-            pv.shadowC[ss] = f4(shC, 0.0f);
-        }
+    const uint32_t k = base + laneId();
+    ShadeOut o;
+    o.emitRay = false;
+    o.emitShadow = false;
+    uint32_t p = 0;
+    if (k < count) {
+        const float4 ro = pv.rayO[bounce & 1][k], rd = pv.rayD[bounce & 1][k];
+        p = __float_as_uint(ro.w);
+        o = shadePath<GUIDE>(sc, pv, g, bounce, p, v3(rd), pv.hit[k], pv.hitInst[k],
+                             GUIDE ? pv.guideIdx[k] : kNoHit);
+    }
+    const uint32_t rs = waveAppend(next + kCntRays, o.emitRay);
+    if (o.emitRay) {
+        pv.rayO[(bounce + 1) & 1][rs] = f4(o.newO, __uint_as_float(p));
+        pv.rayD[(bounce + 1) & 1][rs] = f4(o.newD, 0.0f);
+    }
+    const uint32_t ss = waveAppend(cnt + kCntShadows, o.emitShadow);
+    if (o.emitShadow) {
+        pv.shadowO[ss] = f4(o.shO, o.shMax);
+        pv.shadowD[ss] = f4(o.shD, __uint_as_float(p));
+        pv.shadowC[ss] = f4(o.shC, 0.0f);
     }
 }
 
@@ -462,9 +488,19 @@ __global__ __launch_bounds__(256) void kShadeGuided(SceneView sc, PathView pv, c
     }
 }
 
-// Plain shading of every active request of a batched frame.
-__global__ __launch_bounds__(256) void kShadeBatch(BatchView bv, uint32_t bounce) {
+// Items of a stage over all requests of a batch (wave-uniform).
+__device__ __forceinline__ uint32_t batchItems(const BatchChunks& bc) {
+    uint32_t n = bc.n;
+    for (int off = 32; off >= 1; off >>= 1) n += __shfl_xor(n, off, 64);
+    return __builtin_amdgcn_readfirstlane(n);
+}
+
+// Plain shading of every active request of a batched frame, unless the
+// tail kernel takes this level over (all requests together hold at most
+// tailPaths rays here).
+__global__ __launch_bounds__(256) void kShadeBatch(BatchView bv, uint32_t bounce, uint32_t tailPaths) {
     const BatchChunks bc = batchChunks(bv, bounce, kCntRays);
+    if (batchItems(bc) <= tailPaths) return;
     uint32_t r, count, base;
     while (batchNext(bc, bv.cursors + bounce * 4 + 1, r, count, base)) {
         const BatchEntry& e = bv.entries[r];
@@ -473,8 +509,25 @@ __global__ __launch_bounds__(256) void kShadeBatch(BatchView bv, uint32_t bounce
     }
 }
 
-
 // ------------------------------------------------------------------ shadow --
+// An unoccluded shadow ray of path p (bounce `bounce`) adds its contribution
+// to the path's radiance.
+__device__ __forceinline__ void applyShadow(const PathView& pv, const FrameConsts& g, uint32_t bounce, uint32_t p,
+                                            V3 c) {
+    float4 L = pv.radiance[p];
+    L.x += c.x;
+    L.y += c.y;
+    L.z += c.z;
+    pv.radiance[p] = L;
+    const uint32_t ts = g.guideTrainStride;
+    if (pv.pathRec && p % ts == g.guideTrainPhase) {
+        // NEE from a recorded vertex is not incident along its wi: move the
+        // record's radiance snapshot past it.
+        float4* r = pv.pathRec + (size_t(bounce) * ((pv.capacity + ts - 1) / ts) + p / ts) * 4;
+        if (__float_as_uint(r[2].w) == g.guideStamp) r[1].w += luminance(c);
+    }
+}
+
 // Any hit for shadow slots [base, base + 64) of one request; unoccluded
 // slots add their contribution to the path's radiance.
 template <int BLOCK, int S>
@@ -488,22 +541,7 @@ __device__ __forceinline__ void shadowChunk(const SceneView& sc, const PathView&
         const bool occluded = traceScene<true, S, BLOCK>(sc, gp->tlasRoot, gp->instanceCount,
                                                          make_float3(o.x, o.y, o.z), make_float3(d.x, d.y, d.z),
                                                          kEpsRay, o.w, lds, h, overflow);
-        if (!occluded) {
-            const uint32_t p = __float_as_uint(d.w);
-            const float4 c = pv.shadowC[k];
-            float4 L = pv.radiance[p];
-            L.x += c.x;
-            L.y += c.y;
-            L.z += c.z;
-            pv.radiance[p] = L;
-            const uint32_t ts = gp->guideTrainStride;
-            if (pv.pathRec && p % ts == gp->guideTrainPhase) {
-                // NEE from a recorded vertex is not incident along its wi:
-                // move the record's radiance snapshot past it.
-                float4* r = pv.pathRec + (size_t(bounce) * ((pv.capacity + ts - 1) / ts) + p / ts) * 4;
-                if (__float_as_uint(r[2].w) == gp->guideStamp) r[1].w += luminance(v3(c));
-            }
-        }
+        if (!occluded) applyShadow(pv, *gp, bounce, __float_as_uint(d.w), v3(pv.shadowC[k]));
     }
     if (overflow) atomicOr(pv.counters + kStatusWord, 1u);
 }
@@ -535,6 +573,70 @@ __global__ __launch_bounds__(BLOCK) void kTraceBatch(BatchView bv, uint32_t exte
         } else {
             break;
         }
+    }
+}
+
+// ------------------------------------------------------------------- tail --
+// The rest of the frame for the paths of ray queue `bounce` (whose closest
+// hits the trace launch of this level found), one lane per path: shade, its
+// shadow ray, then closest hit and shading of every further bounce until the
+// path ends.  Per path this is the same sequence of operations as the
+// per-level kernels (shade b, shadow b, closest hit b + 1, ...), and no
+// floating-point contraction is applied in this file, so the result does not
+// depend on which kernel shaded a vertex.  Used when few paths are left: a
+// per-level launch then mostly waits for its slowest ray, while here each
+// wave only waits for its own paths.  Ray and shadow-ray counts of the levels
+// it covers go to the kCntTail* words (statistics; the queues stay empty).
+template <int BLOCK, int S>
+__device__ __forceinline__ void tailChunk(const SceneView& sc, const PathView& pv, const FrameConsts& g,
+                                          uint32_t bounce, uint32_t count, uint32_t base, uint2* lds) {
+    const uint32_t k = base + laneId();
+    bool overflow = false;
+    if (k < count) {
+        const uint32_t p = __float_as_uint(pv.rayO[bounce & 1][k].w);
+        V3 dir = v3(pv.rayD[bounce & 1][k]);
+        float4 h = pv.hit[k];
+        uint32_t inst = pv.hitInst[k];
+        for (uint32_t b = bounce;; ++b) {
+            const ShadeOut o = shadePath<false>(sc, pv, g, b, p, dir, h, inst, kNoHit);
+            waveAppend(pv.counters + b * kCntPerBounce + kCntTailShadows, o.emitShadow);
+            if (o.emitShadow) {
+                SceneHit sh;
+                const bool occluded = traceScene<true, S, BLOCK>(sc, g.tlasRoot, g.instanceCount,
+                                                                 make_float3(o.shO.x, o.shO.y, o.shO.z),
+                                                                 make_float3(o.shD.x, o.shD.y, o.shD.z), kEpsRay,
+                                                                 o.shMax, lds, sh, overflow);
+                if (!occluded) applyShadow(pv, g, b, p, o.shC);
+            }
+            if (!o.emitRay) break;
+            waveAppend(pv.counters + (b + 1) * kCntPerBounce + kCntTailRays, true);
+            SceneHit hh;
+            const bool hit = traceScene<false, S, BLOCK>(sc, g.tlasRoot, g.instanceCount,
+                                                         make_float3(o.newO.x, o.newO.y, o.newO.z),
+                                                         make_float3(o.newD.x, o.newD.y, o.newD.z), kEpsRay, 1e30f,
+                                                         lds, hh, overflow);
+            h = hit ? make_float4(hh.t, hh.b1, hh.b2, __uint_as_float(hh.tri))
+                    : make_float4(1e30f, 0.0f, 0.0f, __uint_as_float(kNoHit));
+            inst = hit ? hh.inst : kNoHit;
+            dir = o.newD;
+        }
+    }
+    if (overflow) atomicOr(pv.counters + kStatusWord, 1u);
+}
+
+// Runs when all requests together hold 1..tailPaths rays at level `bounce`
+// (kShadeBatch runs otherwise).
+template <int BLOCK, int S>
+__global__ __launch_bounds__(BLOCK) void kTailBatch(BatchView bv, uint32_t bounce, uint32_t tailPaths) {
+    __shared__ uint2 ldsStack[S * BLOCK];
+    const BatchChunks bc = batchChunks(bv, bounce, kCntRays);
+    const uint32_t items = batchItems(bc);
+    if (items == 0 || items > tailPaths) return;
+    uint32_t r, count, base;
+    while (batchNext(bc, bv.cursors + bounce * 4 + 3, r, count, base)) {
+        const BatchEntry& e = bv.entries[r];
+        const FrameConsts g = *e.g;
+        tailChunk<BLOCK, S>(e.sc, e.pv, g, bounce, count, base, ldsStack + threadIdx.x);
     }
 }
 
@@ -669,9 +771,33 @@ hipError_t launchTraceBatch(const BatchView& bv, uint32_t extendLevel, uint32_t 
     return hipErrorInvalidValue;
 }
 
-hipError_t launchShadeBatch(const BatchView& bv, uint32_t bounce, uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL(kShadeBatch, dim3(grid), dim3(256), 0, s, bv, bounce);
+hipError_t launchShadeBatch(const BatchView& bv, uint32_t bounce, uint32_t tailPaths, uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(kShadeBatch, dim3(grid), dim3(256), 0, s, bv, bounce, tailPaths);
     return hipGetLastError();
+}
+
+template <int B, int S>
+static hipError_t tailBatchT(const BatchView& bv, uint32_t bounce, uint32_t tailPaths, uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL((kTailBatch<B, S>), dim3(grid), dim3(B), 0, s, bv, bounce, tailPaths);
+    return hipGetLastError();
+}
+
+hipError_t launchTailBatch(const BatchView& bv, uint32_t bounce, uint32_t tailPaths, const TraceShape& shape,
+                           hipStream_t s) {
+    if (tailPaths == 0) return hipSuccess;
+    if (bounce >= kMaxBounceLevels) return hipErrorInvalidValue;
+#define X(B, S) if (shape.block == B && shape.stack == S) return tailBatchT<B, S>(bv, bounce, tailPaths, shape.grid, s);
+    MRT_PT_SHAPES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t batchTailBlocksPerCU(uint32_t block, uint32_t stack, int* blocksPerCU) {
+#define X(B, S) \
+    if (block == B && stack == S) return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kTailBatch<B, S>, B, 0);
+    MRT_PT_SHAPES(X)
+#undef X
+    return hipErrorInvalidValue;
 }
 
 hipError_t launchFinishBatch(const BatchView& bv, hipStream_t s) {

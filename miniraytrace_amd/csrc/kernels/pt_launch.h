@@ -34,15 +34,22 @@ hipError_t guidedShadeBlocksPerCU(int* blocksPerCU);
 
 // A frame over one or more requests (BatchView; an engine's own frame is a
 // batch of one): clear, generate, then per bounce b one trace launch (closest
-// hits of level b together with the shadow rays of level b - 1) and shade,
-// a last trace launch for the deepest level's shadow rays, then accumulate +
-// resolve.  Plain shading (guided engines use launchShadeGuided).
+// hits of level b together with the shadow rays of level b - 1), shade and
+// tail, a last trace launch for the deepest level's shadow rays, then
+// accumulate + resolve.  Plain shading (guided engines use launchShadeGuided
+// and no tail).
 hipError_t launchClearBatch(const BatchView& bv, hipStream_t s);
 hipError_t launchGenerateBatch(const BatchView& bv, hipStream_t s);
 // extendLevel / shadowLevel: queue levels, either may be kNoLevel.
 hipError_t launchTraceBatch(const BatchView& bv, uint32_t extendLevel, uint32_t shadowLevel, const TraceShape& shape,
                             hipStream_t s);
-hipError_t launchShadeBatch(const BatchView& bv, uint32_t bounce, uint32_t grid, hipStream_t s);
+// Plain shading of level `bounce`; skipped when all requests together hold at
+// most tailPaths rays there, in which case launchTailBatch finishes the
+// frame for them (both are launched for every level; 0 = never the tail).
+hipError_t launchShadeBatch(const BatchView& bv, uint32_t bounce, uint32_t tailPaths, uint32_t grid, hipStream_t s);
+hipError_t launchTailBatch(const BatchView& bv, uint32_t bounce, uint32_t tailPaths, const TraceShape& shape,
+                           hipStream_t s);
+hipError_t batchTailBlocksPerCU(uint32_t block, uint32_t stack, int* blocksPerCU);
 hipError_t launchFinishBatch(const BatchView& bv, hipStream_t s);
 hipError_t batchShadeBlocksPerCU(int* blocksPerCU);
 hipError_t batchTraceBlocksPerCU(uint32_t block, uint32_t stack, int* blocksPerCU);

@@ -35,10 +35,10 @@ struct SceneView {
 enum CounterSlot : uint32_t {
     kCntRays = 0,        // rays queued for this bounce
     kCntShadows = 1,     // shadow rays queued at this bounce
-    kCntExtendFetch = 2, // work-fetch cursor of the extend kernel
-    kCntShadeFetch = 3,
-    kCntShadowFetch = 4,
+    kCntShadeFetch = 3,  // work-fetch cursor of guided shading
     kCntGuideRecords = 5,
+    kCntTailRays = 6,    // rays traced at this bounce by the tail kernel (statistics)
+    kCntTailShadows = 7, // shadow rays traced at this bounce by the tail kernel (statistics)
     kCntPerBounce = 8,
 };
 

@@ -205,16 +205,18 @@ class Engine:
         L.lib().mrtResetAccumulation(self._h)
 
     # -- extensions --------------------------------------------------------------------
-    def set_path_options(self, seed=0, use_graph=True, extend=(0, 0), shadow=(0, 0)) -> None:
+    def set_path_options(self, seed=0, use_graph=True, extend=(0, 0), shadow=(0, 0), tail_paths=0) -> None:
+        """tail_paths: 0 = default, L.TAIL_DISABLED = never use the tail kernel."""
         o = L.mrtxPathOptions(seed=seed, useGraph=int(use_graph), extendBlock=extend[0],
-                              extendStack=extend[1], shadowBlock=shadow[0], shadowStack=shadow[1])
+                              extendStack=extend[1], shadowBlock=shadow[0], shadowStack=shadow[1],
+                              tailPaths=tail_paths)
         L.check(L.lib().mrtxSetPathOptions(self._h, C.byref(o)), "mrtxSetPathOptions")
 
     def path_options(self) -> dict:
         o = L.mrtxPathOptions()
         L.check(L.lib().mrtxGetPathOptions(self._h, C.byref(o)), "mrtxGetPathOptions")
         return {"seed": o.seed, "use_graph": bool(o.useGraph), "extend": (o.extendBlock, o.extendStack),
-                "shadow": (o.shadowBlock, o.shadowStack)}
+                "shadow": (o.shadowBlock, o.shadowStack), "tail_paths": o.tailPaths}
 
     # -- path guiding (mini_raytrace_ext.h section 5) --
     def set_guiding(self, sampling: bool = True, training: bool = True, select_prob: float = 0.5,
@@ -257,7 +259,7 @@ class Engine:
     def frame_stats(self) -> dict:
         s = L.mrtxFrameStats()
         L.check(L.lib().mrtxGetFrameStats(self._h, C.byref(s)), "mrtxGetFrameStats")
-        names = ["generate", "extend", "shade", "shadow", "accumulate", "resolve", "guide", "train"]
+        names = ["generate", "extend", "shade", "shadow", "accumulate", "tail", "guide", "train"]
         return {"frameMs": s.frameMs, "gpuMs": dict(zip(names, list(s.gpuMs))),
                 "rays": list(s.rays), "shadowRays": list(s.shadowRays), "status": s.status}
 

@@ -113,3 +113,30 @@ def test_autotune_keeps_images(gpu):
         for _ in range(4):
             e.render_frame()
     assert np.array_equal(a.read_rgba32f(r["width"], r["height"]), b.read_rgba32f(r["width"], r["height"]))
+
+
+@pytest.mark.parametrize("tail_paths", [1 << 30, 3000])
+def test_tail_kernel_equals_per_level(gpu, tail_paths):
+    """The tail kernel (one lane per path for the rest of the frame) gives
+    the same image, bit for bit, and the same per-level ray counts as the
+    per-level launches: from the first level (tail_paths above the frame's
+    paths) and from a level in the middle of the frame."""
+    from miniraytrace_amd import _lib as L, scenes
+    _print(f"case: tail kernel vs per-level launches, tail_paths={tail_paths}")
+    sky = scenes.sky(256, 128)
+    r = REQS[2]
+    a, b = _engine(r, sky), _engine(r, sky)
+    a.set_path_options(tail_paths=tail_paths)
+    b.set_path_options(tail_paths=L.TAIL_DISABLED)
+    for e in (a, b):
+        for _ in range(3):
+            e.render_frame()
+        e.set_profiling(True)
+        e.render_frame()
+    sa, sb = a.frame_stats(), b.frame_stats()
+    _print("rays per level:", list(sa["rays"])[:10], "tail ms", sa["gpuMs"]["tail"], "vs", sb["gpuMs"]["tail"])
+    assert list(sa["rays"]) == list(sb["rays"]) and list(sa["shadowRays"]) == list(sb["shadowRays"])
+    assert sa["gpuMs"]["tail"] > 0.0 and sb["gpuMs"]["tail"] >= 0.0
+    ia, ib = a.read_rgba32f(r["width"], r["height"]), b.read_rgba32f(r["width"], r["height"])
+    assert np.array_equal(ia, ib)
+    assert np.isfinite(ia).all() and ia[..., :3].mean() > 0
