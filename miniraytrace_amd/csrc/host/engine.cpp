@@ -814,6 +814,9 @@ void Engine::enqueueFrame(hipStream_t s, bool timed, const FrameConsts* g, const
     const bool train = pv.pathRec != nullptr;
     const bool infer = guide && guide_.enableSampling;
     const uint32_t prepGrid = std::min<uint32_t>((pixels + 255) / 256, uint32_t(cuCount_) * 8);
+    // Vertices the tail kernel shades are not guided, so a guided frame hands
+    // over only its last few paths (at most 1/8 of its pixels).
+    const uint32_t tail = guide ? std::min(tailPaths_, pixels / 8) : tailPaths_;
     dev::BatchView bv;
     bv.entries = entry;
     bv.count = 1;
@@ -837,7 +840,7 @@ void Engine::enqueueFrame(hipStream_t s, bool timed, const FrameConsts* g, const
         MRT_HIP(dev::launchTraceBatch(bv, b, b == 0 ? dev::kNoLevel : b - 1, extendShape_, s));
         if (timed) MRT_HIP(hipEventRecord(ev_[3], s));
         if (guide) {
-            MRT_HIP(dev::launchGuidePrep(sc, pv, g, b, prepGrid, s));
+            MRT_HIP(dev::launchGuidePrep(sc, pv, g, b, tail, prepGrid, s));
             if (infer) {
                 dev::RecView rv;
                 rv.base = pv.guideRec;
@@ -847,10 +850,10 @@ void Engine::enqueueFrame(hipStream_t s, bool timed, const FrameConsts* g, const
             }
         }
         if (timed) MRT_HIP(hipEventRecord(ev_[9], s));
-        if (guide) MRT_HIP(dev::launchShadeGuided(sc, pv, g, b, shadeGuideGrid_, s));
-        else MRT_HIP(dev::launchShadeBatch(bv, b, tailPaths_, shadeGrid_, s));
+        if (guide) MRT_HIP(dev::launchShadeGuided(sc, pv, g, b, tail, shadeGuideGrid_, s));
+        else MRT_HIP(dev::launchShadeBatch(bv, b, tail, shadeGrid_, s));
         if (timed) MRT_HIP(hipEventRecord(ev_[4], s));
-        if (!guide) MRT_HIP(dev::launchTailBatch(bv, b, tailPaths_, tailShape(), s));
+        MRT_HIP(dev::launchTailBatch(bv, b, tail, tailShape(), s));
         if (timed) {
             MRT_HIP(hipEventRecord(ev_[8], s));
             MRT_HIP(hipEventSynchronize(ev_[8]));

@@ -103,8 +103,16 @@ __device__ __forceinline__ Bsdf makeBsdf(const SceneView& sc, const Surface& s) 
 }
 
 // ---------------------------------------------------------------- generate --
-__device__ __forceinline__ void generatePixel(const PathView& pv, const FrameConsts& g, uint32_t p) {
-    const uint32_t px = p % g.width, py = p / g.width;
+// Camera ray of queue slot k.  Slots walk the image in bands of 8 rows,
+// column by column inside a band, so 64 consecutive slots (one wave) cover
+// an 8 x 8 pixel tile: the wave's rays stay close through the BVH.  A
+// bijection of [0, width * height) (the last band may be shorter).
+__device__ __forceinline__ void generatePixel(const PathView& pv, const FrameConsts& g, uint32_t k) {
+    const uint32_t band = k / (8u * g.width);
+    const uint32_t rows = min(8u, g.height - 8u * band);
+    const uint32_t j = k - band * 8u * g.width;
+    const uint32_t px = j / rows, py = 8u * band + j % rows;
+    const uint32_t p = py * g.width + px;
     Rng rng{ px, py, g.frameIndex ^ g.seedMix, 0u };
     float jx, jy;
     rng.next2(jx, jy);
@@ -120,8 +128,8 @@ __device__ __forceinline__ void generatePixel(const PathView& pv, const FrameCon
         o = v3(g.camPos);
         d = normalize(v3(g.camForward) + v3(g.camRight) * sx + v3(g.camUp) * sy);
     }
-    pv.rayO[0][p] = f4(o, __uint_as_float(p));
-    pv.rayD[0][p] = f4(d, 0.0f);
+    pv.rayO[0][k] = f4(o, __uint_as_float(p));
+    pv.rayD[0][k] = f4(d, 0.0f);
     pv.beta[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     pv.radiance[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     pv.rngCounter[p] = rng.counter;
@@ -233,9 +241,10 @@ __device__ __forceinline__ void extendChunk(const SceneView& sc, const PathView&
 // (opaque, roughness >= guideMinRoughness): a query record (pos01, n, wo) at
 // a compacted row, and the row number in guideIdx (~0u elsewhere).
 __global__ __launch_bounds__(256) void kGuidePrep(SceneView sc, PathView pv, const FrameConsts* __restrict__ gp,
-                                                  uint32_t bounce) {
+                                                  uint32_t bounce, uint32_t tailPaths) {
     uint32_t* cnt = pv.counters + bounce * kCntPerBounce;
     const uint32_t count = cnt[kCntRays];
+    if (count <= tailPaths) return;  // the tail kernel shades this level
     const float4* rayD = pv.rayD[bounce & 1];
     const float minRough = gp->guideMinRoughness;
     const V3 boxMin = v3(gp->guideBoxMin), boxScale = v3(gp->guideBoxScale);
@@ -312,7 +321,7 @@ __device__ __forceinline__ ShadeOut shadePath(const SceneView& sc, const PathVie
             if (bounce > 0) {
                 const float pl = envPdf(sc, g, dir) * envPickPdf(g);
                 w = balanceHeuristic(prevPdf, pl);
-                if (GUIDE && g.guideTrain && p % g.guideTrainStride == g.guideTrainPhase) {
+                if (pv.pathRec && g.guideTrain && p % g.guideTrainStride == g.guideTrainPhase) {
                     // Training target of the previous vertex: its own
                     // continuation's environment hit counts with the MIS
                     // weight of the BSDF pdf (L4.w), not of the guided
@@ -476,11 +485,15 @@ __device__ __forceinline__ void shadeChunk(const SceneView& sc, const PathView& 
 
 // Guided shading of one engine's frame (guided requests are never merged
 // into a shared batch: each has its own network).
+// Skipped when the level holds at most tailPaths rays: the tail kernel then
+// finishes the frame, shading those vertices from the BSDF alone (unguided
+// one-sample MIS with selection probability 0, still unbiased).
 __global__ __launch_bounds__(256) void kShadeGuided(SceneView sc, PathView pv, const FrameConsts* __restrict__ gp,
-                                                    uint32_t bounce) {
+                                                    uint32_t bounce, uint32_t tailPaths) {
     const FrameConsts g = *gp;
     uint32_t* cnt = pv.counters + bounce * kCntPerBounce;
     const uint32_t count = cnt[kCntRays];
+    if (count <= tailPaths) return;
     while (true) {
         const uint32_t base = waveFetch(cnt + kCntShadeFetch);
         if (base >= count) break;
@@ -723,14 +736,14 @@ hipError_t launchClearWords(uint32_t* words, uint32_t n, hipStream_t s) {
 #define MRT_PT_SHAPES(X) X(64, 16) X(128, 8) X(128, 16) X(128, 32) X(256, 8) X(256, 16)
 
 hipError_t launchShadeGuided(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t bounce,
-                             uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL(kShadeGuided, dim3(grid), dim3(256), 0, s, sc, pv, g, bounce);
+                             uint32_t tailPaths, uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(kShadeGuided, dim3(grid), dim3(256), 0, s, sc, pv, g, bounce, tailPaths);
     return hipGetLastError();
 }
 
 hipError_t launchGuidePrep(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t bounce,
-                           uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL(kGuidePrep, dim3(grid), dim3(256), 0, s, sc, pv, g, bounce);
+                           uint32_t tailPaths, uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(kGuidePrep, dim3(grid), dim3(256), 0, s, sc, pv, g, bounce, tailPaths);
     return hipGetLastError();
 }
 

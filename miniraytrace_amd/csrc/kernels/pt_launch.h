@@ -24,20 +24,22 @@ struct TraceShape {
 
 hipError_t launchClearWords(uint32_t* words, uint32_t n, hipStream_t s);
 hipError_t launchResolve32(const PathView& pv, uint32_t pixels, uint32_t spp, hipStream_t s);
-// Guided shading of one engine's frame (needs the guide prep pass of this bounce).
+// Guided shading of one engine's frame (needs the guide prep pass of this
+// bounce); both skip a level that holds at most tailPaths rays (the tail
+// kernel shades it, unguided).
 hipError_t launchShadeGuided(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t bounce,
-                             uint32_t grid, hipStream_t s);
+                             uint32_t tailPaths, uint32_t grid, hipStream_t s);
 // Query records of guidable hits for the guiding network (grid-stride).
 hipError_t launchGuidePrep(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t bounce,
-                           uint32_t grid, hipStream_t s);
+                           uint32_t tailPaths, uint32_t grid, hipStream_t s);
 hipError_t guidedShadeBlocksPerCU(int* blocksPerCU);
 
 // A frame over one or more requests (BatchView; an engine's own frame is a
 // batch of one): clear, generate, then per bounce b one trace launch (closest
 // hits of level b together with the shadow rays of level b - 1), shade and
 // tail, a last trace launch for the deepest level's shadow rays, then
-// accumulate + resolve.  Plain shading (guided engines use launchShadeGuided
-// and no tail).
+// accumulate + resolve.  Plain shading (guided engines shade with
+// launchShadeGuided and use the same tail).
 hipError_t launchClearBatch(const BatchView& bv, hipStream_t s);
 hipError_t launchGenerateBatch(const BatchView& bv, hipStream_t s);
 // extendLevel / shadowLevel: queue levels, either may be kNoLevel.
