@@ -46,7 +46,7 @@ mrt::dev::TraceShape voteShape(const std::vector<mrt::Engine*>& engines, const s
                                  [](const auto& a, const auto& b) { return a.second < b.second; });
     mrt::dev::TraceShape t{ best->first.first, best->first.second, 0 };
     int perCU = 0;
-    MRT_HIP(mrt::dev::batchTraceBlocksPerCU(t.block, t.stack, &perCU));
+    MRT_HIP(mrt::dev::batchTraceBlocksPerCU(t.block, t.stack, false, &perCU));
     t.grid = uint32_t(std::max(1, perCU) * engines[0]->cuCount());
     return t;
 }

@@ -178,9 +178,11 @@ void Engine::invalidateGraph() {
 // Persistent grids: as many workgroups as fit on the device at once.
 void Engine::chooseShapes() {
     int perCU = 0;
-    MRT_HIP(dev::batchTraceBlocksPerCU(extendShape_.block, extendShape_.stack, &perCU));
+    MRT_HIP(dev::batchTraceBlocksPerCU(extendShape_.block, extendShape_.stack, false, &perCU));
     extendShape_.grid = uint32_t(std::max(1, perCU) * cuCount_);
-    MRT_HIP(dev::batchTraceBlocksPerCU(shadowShape_.block, shadowShape_.stack, &perCU));
+    MRT_HIP(dev::batchTraceBlocksPerCU(extendShape_.block, extendShape_.stack, true, &perCU));
+    prepTraceGrid_ = uint32_t(std::max(1, perCU) * cuCount_);
+    MRT_HIP(dev::batchTraceBlocksPerCU(shadowShape_.block, shadowShape_.stack, false, &perCU));
     shadowShape_.grid = uint32_t(std::max(1, perCU) * cuCount_);
     MRT_HIP(dev::batchShadeBlocksPerCU(&perCU));
     shadeGrid_ = uint32_t(std::max(1, perCU) * cuCount_);
@@ -813,10 +815,11 @@ void Engine::enqueueFrame(hipStream_t s, bool timed, const FrameConsts* g, const
     const bool guide = pv.guideIdx != nullptr;
     const bool train = pv.pathRec != nullptr;
     const bool infer = guide && guide_.enableSampling;
-    const uint32_t prepGrid = std::min<uint32_t>((pixels + 255) / 256, uint32_t(cuCount_) * 8);
     // Vertices the tail kernel shades are not guided, so a guided frame hands
     // over only its last few paths (at most 1/8 of its pixels).
     const uint32_t tail = guide ? std::min(tailPaths_, pixels / 8) : tailPaths_;
+    dev::TraceShape traceShape = extendShape_;
+    if (guide) traceShape.grid = prepTraceGrid_;  // the query-record variant holds more registers
     dev::BatchView bv;
     bv.entries = entry;
     bv.count = 1;
@@ -837,10 +840,9 @@ void Engine::enqueueFrame(hipStream_t s, bool timed, const FrameConsts* g, const
     // shadow rays get a trace launch of their own.
     for (uint32_t b = 0; b <= maxB; ++b) {
         if (timed) MRT_HIP(hipEventRecord(ev_[2], s));
-        MRT_HIP(dev::launchTraceBatch(bv, b, b == 0 ? dev::kNoLevel : b - 1, extendShape_, s));
+        MRT_HIP(dev::launchTraceBatch(bv, b, b == 0 ? dev::kNoLevel : b - 1, traceShape, s, guide, tail));
         if (timed) MRT_HIP(hipEventRecord(ev_[3], s));
         if (guide) {
-            MRT_HIP(dev::launchGuidePrep(sc, pv, g, b, tail, prepGrid, s));
             if (infer) {
                 dev::RecView rv;
                 rv.base = pv.guideRec;

@@ -165,6 +165,7 @@ private:
     uint32_t tailOption_ = 0;  // mrtxPathOptions::tailPaths as set
     uint32_t tailPaths_ = 0;   // tail kernel threshold passed to the kernels (0 = never)
     uint32_t tailGrid_ = 0;
+    uint32_t prepTraceGrid_ = 0;  // grid of the trace launches that also write guide query records
     uint32_t tailBlock_ = kTailBlock, tailStack_ = kTailStack;
     uint32_t shadeGrid_ = 0;
     uint32_t shadeGuideGrid_ = 0;

@@ -160,11 +160,42 @@ __device__ __forceinline__ void layerForward(int l, uint16_t* act, const uint16_
     waveHandoff();
 }
 
+// sin(pi t) and cos(pi t) for |t| <= 1: reflection to a = |t| <= 1/2 (or
+// 1 - |t|), then the Taylor series of sin(pi a) to degree 11 and of
+// cos(pi a) to degree 12 (truncation error below 1e-7), evaluated without
+// contraction so that the oracle repeats it bit for bit.
+__device__ __forceinline__ void sinCosPiSeries(float t, float& s, float& c) {
+#pragma clang fp contract(off)
+    const float a0 = fabsf(t);
+    const bool reflect = a0 > 0.5f;
+    const float a = reflect ? 1.0f - a0 : a0;
+    const float z = a * a;
+    float ps = -0x1.e3075p-8f;  // -pi^11 / 11!
+    ps = ps * z + 0x1.507834p-4f;
+    ps = ps * z + -0x1.32d2ccp-1f;
+    ps = ps * z + 0x1.466bc6p+1f;
+    ps = ps * z + -0x1.4abbcep+2f;
+    ps = ps * z + 0x1.921fb6p+1f;  // pi
+    float pc = 0x1.f9d38ap-10f;  // pi^12 / 12!
+    pc = pc * z + -0x1.a6d1f2p-6f;
+    pc = pc * z + 0x1.e1f506p-3f;
+    pc = pc * z + -0x1.55d3c8p+0f;
+    pc = pc * z + 0x1.03c1fp+2f;
+    pc = pc * z + -0x1.3bd3ccp+2f;  // -pi^2 / 2
+    s = ps * a;
+    c = pc * z + 1.0f;
+    if (reflect) c = -c;
+    if (t < 0.0f) s = -s;
+}
+
 // Encodes rows [row0, row0 + 32) into act.  Feature pair j = 12 a + b is
-// (sin, cos)(pi 2^b p_a); lanes 0..31 write pairs 0..17 of row lane, lanes
-// 32..63 pairs 18..35, the normal, wo and two zero pads.  Rows past count
-// are zero.
+// (sin, cos)(pi 2^b p_a): band 0 from the series after exact reduction of
+// p_a to [-1, 1], each further band by the double-angle step
+// (s, c) -> (2 s c, (c - s)(c + s)).  Lanes 0..31 write pairs 0..17 of row
+// lane, lanes 32..63 pairs 18..35, the normal, wo and two zero pads.  Rows
+// past count are zero.
 __device__ __forceinline__ void encodeRows(const RecView& rv, uint32_t row0, uint32_t count, uint16_t* act) {
+#pragma clang fp contract(off)
     const int lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
     const uint32_t row = row0 + r;
     const bool valid = row < count;
@@ -179,12 +210,19 @@ __device__ __forceinline__ void encodeRows(const RecView& rv, uint32_t row0, uin
     uint32_t* dst = reinterpret_cast<uint32_t*>(act + r * kGuideInPad);
     waveHandoff();  // the previous tile's reads of act come first
 #pragma unroll
-    for (int jj = 0; jj < 18; ++jj) {
-        const int j = jj + 18 * h;
-        const int ax = j / 12, band = j % 12;
-        float s, cs;
-        sincospif(p[ax] * float(1 << band), &s, &cs);
-        dst[j] = valid ? pack2(toBf16(s), toBf16(cs)) : 0u;
+    for (int ax = 0; ax < 3; ++ax) {
+        float sn, cs;
+        sinCosPiSeries(p[ax] - 2.0f * rintf(0.5f * p[ax]), sn, cs);
+#pragma unroll
+        for (int band = 0; band < kGuideBands; ++band) {
+            if (band > 0) {
+                const float s2 = (2.0f * sn) * cs;
+                cs = (cs - sn) * (cs + sn);
+                sn = s2;
+            }
+            const int j = 12 * ax + band;  // feature pair; this lane half writes 18 h .. 18 h + 17
+            if (j / 18 == h) dst[j] = valid ? pack2(toBf16(sn), toBf16(cs)) : 0u;
+        }
     }
     if (h == 1) {
         dst[36] = valid ? pack2(toBf16(b.x), toBf16(b.y)) : 0u;
@@ -245,12 +283,11 @@ __global__ __launch_bounds__(kFwdWaves * 64) void kGuideInfer(RecView rv, const 
         for (int l = 0; l < 5; ++l) layerForward(l, act, W, bias, c0, c1);
         stashOutputs(c0, c1, act);
         if (decoded) {
-            const uint32_t row = row0 + lane;
-            if (lane < 32 && row < count) {
-                GuideMix m;
-                decodeMix(fo + lane * kGuideOut, m);
-                storeMix(m, static_cast<float4*>(out) + size_t(row) * kGuideMixF4);
-            }
+            // lanes r and r + 32 decode lobes 0..3 and 4..7 of row r
+            const int r = lane & 31;
+            const uint32_t row = row0 + r;
+            storeMixHalf(fo + r * kGuideOut, lane >> 5, row < count,
+                         static_cast<float4*>(out) + size_t(row) * kGuideMixF4);
         } else {
             float* o = static_cast<float*>(out);
             for (int i = lane; i < 32 * kGuideOut; i += 64) {

@@ -45,19 +45,26 @@ struct GuideMix {
     float w[kGuideLobes];
 };
 
+// Mean direction and concentration of lobe k from its raw outputs.
+__device__ __forceinline__ void decodeLobe(const float* raw, int k, V3& mu, float& kappa) {
+    const V3 v = v3(raw[5 * k], raw[5 * k + 1], raw[5 * k + 2]);
+    const float len = sqrtf(dot(v, v));
+    mu = len > 1e-12f ? v * (1.0f / len) : v3(0.0f, 0.0f, 1.0f);
+    kappa = expf(clampf(raw[5 * k + 3], kLogKappaLo, kLogKappaHi));
+}
+
+// The softmax sums lobes 0..3 and 4..7 separately and then adds the two, the
+// order storeMixHalf uses, so both decode the same weights.
 __device__ __forceinline__ void decodeMix(const float* raw, GuideMix& m) {
     float mx = -1e30f;
     for (int k = 0; k < kGuideLobes; ++k) mx = fmaxf(mx, raw[5 * k + 4]);
-    float s = 0.0f;
+    float s[2] = { 0.0f, 0.0f };
     for (int k = 0; k < kGuideLobes; ++k) {
-        const V3 v = v3(raw[5 * k], raw[5 * k + 1], raw[5 * k + 2]);
-        const float len = sqrtf(dot(v, v));
-        m.mu[k] = len > 1e-12f ? v * (1.0f / len) : v3(0.0f, 0.0f, 1.0f);
-        m.kappa[k] = expf(clampf(raw[5 * k + 3], kLogKappaLo, kLogKappaHi));
+        decodeLobe(raw, k, m.mu[k], m.kappa[k]);
         m.w[k] = expf(raw[5 * k + 4] - mx);
-        s += m.w[k];
+        s[k / 4] += m.w[k];
     }
-    const float inv = 1.0f / s;
+    const float inv = 1.0f / (s[0] + s[1]);
     for (int k = 0; k < kGuideLobes; ++k) m.w[k] *= inv;
 }
 
@@ -113,6 +120,34 @@ __device__ __forceinline__ void storeMix(const GuideMix& m, float4* __restrict__
     out[9] = make_float4(m.w[4], m.w[5], m.w[6], m.w[7]);
     out[10] = make_float4(pre[0], pre[1], pre[2], pre[3]);
     out[11] = make_float4(pre[4], pre[5], pre[6], pre[7]);
+}
+
+// storeMix of lobes 4 h .. 4 h + 3 of one row by one lane; the lanes of the
+// other half (lane ^ 32) hold the other four lobes of the same row.  All
+// lanes of the wave must call it (the softmax is shared through shuffles).
+__device__ __forceinline__ void storeMixHalf(const float* raw, int h, bool store, float4* __restrict__ out) {
+    float mx = -1e30f;
+    for (int k = 4 * h; k < 4 * h + 4; ++k) mx = fmaxf(mx, raw[5 * k + 4]);
+    mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
+    V3 mu[4];
+    float kappa[4], w[4], s = 0.0f;
+    for (int i = 0; i < 4; ++i) {
+        const int k = 4 * h + i;
+        decodeLobe(raw, k, mu[i], kappa[i]);
+        w[i] = expf(raw[5 * k + 4] - mx);
+        s += w[i];
+    }
+    const float other = __shfl_xor(s, 32, 64);
+    const float inv = 1.0f / (h == 0 ? s + other : other + s);
+    float pre[4];
+    for (int i = 0; i < 4; ++i) {
+        w[i] *= inv;
+        pre[i] = w[i] * kappa[i] / (kTwoPi * -expm1f(-2.0f * kappa[i]));
+    }
+    if (!store) return;
+    for (int i = 0; i < 4; ++i) out[4 * h + i] = f4(mu[i], kappa[i]);
+    out[8 + h] = make_float4(w[0], w[1], w[2], w[3]);
+    out[10 + h] = make_float4(pre[0], pre[1], pre[2], pre[3]);
 }
 
 __device__ __forceinline__ float storedMixPdf(const float4* __restrict__ mix, V3 w) {

@@ -218,60 +218,48 @@ __global__ __launch_bounds__(256) void kGenerateBatch(BatchView bv) {
 
 // ------------------------------------------------------------------ extend --
 // Closest hits for queue slots [base, base + 64) of one request's ray queue.
-template <int BLOCK, int S>
-__device__ __forceinline__ void extendChunk(const SceneView& sc, const PathView& pv, uint32_t tlasRoot,
-                                            uint32_t instCount, uint32_t bounce, uint32_t count, uint32_t base,
-                                            uint2* lds) {
+// PREP (guided engines, when count > prepAbove): the same wave then forms the
+// guiding network's query record of every hit on a surface that may be
+// guided (opaque, roughness >= guideMinRoughness) -- (pos01, n, wo) at a
+// compacted row -- and the row number in guideIdx (~0u elsewhere), while
+// the hit is still in registers.  A level at or below prepAbove rays is
+// shaded by the tail kernel, which needs no records.
+template <int BLOCK, int S, bool PREP>
+__device__ __forceinline__ void extendChunk(const SceneView& sc, const PathView& pv, const FrameConsts* __restrict__ gp,
+                                            uint32_t bounce, uint32_t count, uint32_t base, uint2* lds,
+                                            uint32_t prepAbove) {
     const uint32_t k = base + laneId();
     bool overflow = false;
+    bool hit = false;
+    SceneHit h;
+    V3 dir = v3(0.0f, 0.0f, 0.0f);
     if (k < count) {
         const float4 o = pv.rayO[bounce & 1][k], d = pv.rayD[bounce & 1][k];
-        SceneHit h;
-        const bool hit = traceScene<false, S, BLOCK>(sc, tlasRoot, instCount, make_float3(o.x, o.y, o.z),
-                                                     make_float3(d.x, d.y, d.z), kEpsRay, 1e30f, lds, h, overflow);
+        dir = v3(d);
+        hit = traceScene<false, S, BLOCK>(sc, gp->tlasRoot, gp->instanceCount, make_float3(o.x, o.y, o.z),
+                                          make_float3(d.x, d.y, d.z), kEpsRay, 1e30f, lds, h, overflow);
         pv.hit[k] = hit ? make_float4(h.t, h.b1, h.b2, __uint_as_float(h.tri))
                         : make_float4(1e30f, 0.0f, 0.0f, __uint_as_float(kNoHit));
         pv.hitInst[k] = hit ? h.inst : kNoHit;
     }
     if (overflow) atomicOr(pv.counters + kStatusWord, 1u);
-}
-
-// ------------------------------------------------------------- guide prep --
-// For every hit of ray queue `bounce` on a surface that may be guided
-// (opaque, roughness >= guideMinRoughness): a query record (pos01, n, wo) at
-// a compacted row, and the row number in guideIdx (~0u elsewhere).
-__global__ __launch_bounds__(256) void kGuidePrep(SceneView sc, PathView pv, const FrameConsts* __restrict__ gp,
-                                                  uint32_t bounce, uint32_t tailPaths) {
-    uint32_t* cnt = pv.counters + bounce * kCntPerBounce;
-    const uint32_t count = cnt[kCntRays];
-    if (count <= tailPaths) return;  // the tail kernel shades this level
-    const float4* rayD = pv.rayD[bounce & 1];
-    const float minRough = gp->guideMinRoughness;
-    const V3 boxMin = v3(gp->guideBoxMin), boxScale = v3(gp->guideBoxScale);
-    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
-        const uint32_t k = base + threadIdx.x;
+    if (PREP && count > prepAbove) {
         bool guide = false;
-        V3 pos01, n, wo;
-        if (k < count) {
-            const float4 h = pv.hit[k];
-            const uint32_t tri = __float_as_uint(h.w);
-            if (tri != kNoHit) {
-                const V3 dir = v3(rayD[k]);
-                const Surface s = makeSurface(sc, dir, h.y, h.z, tri, pv.hitInst[k]);
-                const Bsdf b = makeBsdf(sc, s);
-                guide = b.transmission <= 0.0f && b.roughness >= minRough;
-                pos01 = (s.pos - boxMin) * boxScale;
-                n = s.nShade;
-                wo = -dir;
-            }
+        V3 pos01, n;
+        if (hit) {
+            const Surface s = makeSurface(sc, dir, h.b1, h.b2, h.tri, h.inst);
+            const Bsdf b = makeBsdf(sc, s);
+            guide = b.transmission <= 0.0f && b.roughness >= gp->guideMinRoughness;
+            pos01 = (s.pos - v3(gp->guideBoxMin)) * v3(gp->guideBoxScale);
+            n = s.nShade;
         }
-        const uint32_t row = waveAppend(cnt + kCntGuideRecords, guide);
+        const uint32_t row = waveAppend(pv.counters + bounce * kCntPerBounce + kCntGuideRecords, guide);
         if (k < count) pv.guideIdx[k] = guide ? row : kNoHit;
         if (guide) {
             float4* r = pv.guideRec + size_t(row) * 3;
             r[0] = f4(pos01, 0.0f);
             r[1] = f4(n, 0.0f);
-            r[2] = f4(wo, 0.0f);
+            r[2] = f4(-dir, 0.0f);
         }
     }
 }
@@ -565,8 +553,9 @@ __device__ __forceinline__ void shadowChunk(const SceneView& sc, const PathView&
 // hits only write hit records -- so the slowest rays of one overlap with the
 // other's work instead of each ending a launch of its own.  Chunks
 // [0, extend chunks) of the shared cursor are closest hit, the rest any hit.
-template <int BLOCK, int S>
-__global__ __launch_bounds__(BLOCK) void kTraceBatch(BatchView bv, uint32_t extendLevel, uint32_t shadowLevel) {
+template <int BLOCK, int S, bool PREP>
+__global__ __launch_bounds__(BLOCK) void kTraceBatch(BatchView bv, uint32_t extendLevel, uint32_t shadowLevel,
+                                                     uint32_t prepAbove) {
     __shared__ uint2 ldsStack[S * BLOCK];
     const BatchChunks be = extendLevel != kNoLevel ? batchChunks(bv, extendLevel, kCntRays) : BatchChunks{ 0, 0, 0 };
     const BatchChunks bs = shadowLevel != kNoLevel ? batchChunks(bv, shadowLevel, kCntShadows) : BatchChunks{ 0, 0, 0 };
@@ -577,8 +566,8 @@ __global__ __launch_bounds__(BLOCK) void kTraceBatch(BatchView bv, uint32_t exte
         if (chunk < be.total) {
             batchLocate(be, chunk, r, count, base);
             const BatchEntry& e = bv.entries[r];
-            extendChunk<BLOCK, S>(e.sc, e.pv, e.g->tlasRoot, e.g->instanceCount, extendLevel, count, base,
-                                  ldsStack + threadIdx.x);
+            extendChunk<BLOCK, S, PREP>(e.sc, e.pv, e.g, extendLevel, count, base, ldsStack + threadIdx.x,
+                                        prepAbove);
         } else if (chunk - be.total < bs.total) {
             batchLocate(bs, chunk - be.total, r, count, base);
             const BatchEntry& e = bv.entries[r];
@@ -741,12 +730,6 @@ hipError_t launchShadeGuided(const SceneView& sc, const PathView& pv, const Fram
     return hipGetLastError();
 }
 
-hipError_t launchGuidePrep(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t bounce,
-                           uint32_t tailPaths, uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL(kGuidePrep, dim3(grid), dim3(256), 0, s, sc, pv, g, bounce, tailPaths);
-    return hipGetLastError();
-}
-
 hipError_t launchResolve32(const PathView& pv, uint32_t pixels, uint32_t spp, hipStream_t s) {
     hipLaunchKernelGGL(kResolve32, dim3(gridFor(pixels, 256, 8192)), dim3(256), 0, s, pv, pixels, spp);
     return hipGetLastError();
@@ -755,8 +738,11 @@ hipError_t launchResolve32(const PathView& pv, uint32_t pixels, uint32_t spp, hi
 // ------------------------------------------------------------ batched frame --
 template <int B, int S>
 static hipError_t traceBatchT(const BatchView& bv, uint32_t extendLevel, uint32_t shadowLevel, uint32_t grid,
-                              hipStream_t s) {
-    hipLaunchKernelGGL((kTraceBatch<B, S>), dim3(grid), dim3(B), 0, s, bv, extendLevel, shadowLevel);
+                              bool prep, uint32_t prepAbove, hipStream_t s) {
+    if (prep)
+        hipLaunchKernelGGL((kTraceBatch<B, S, true>), dim3(grid), dim3(B), 0, s, bv, extendLevel, shadowLevel, prepAbove);
+    else
+        hipLaunchKernelGGL((kTraceBatch<B, S, false>), dim3(grid), dim3(B), 0, s, bv, extendLevel, shadowLevel, 0u);
     return hipGetLastError();
 }
 
@@ -772,13 +758,14 @@ hipError_t launchGenerateBatch(const BatchView& bv, hipStream_t s) {
 }
 
 hipError_t launchTraceBatch(const BatchView& bv, uint32_t extendLevel, uint32_t shadowLevel, const TraceShape& shape,
-                            hipStream_t s) {
+                            hipStream_t s, bool guidePrep, uint32_t prepAbove) {
     if (extendLevel == kNoLevel && shadowLevel == kNoLevel) return hipSuccess;
     if ((extendLevel != kNoLevel && extendLevel >= kMaxBounceLevels) ||
         (shadowLevel != kNoLevel && shadowLevel >= kMaxBounceLevels))
         return hipErrorInvalidValue;
 #define X(B, S)                                                                                  \
-    if (shape.block == B && shape.stack == S) return traceBatchT<B, S>(bv, extendLevel, shadowLevel, shape.grid, s);
+    if (shape.block == B && shape.stack == S)                                                     \
+        return traceBatchT<B, S>(bv, extendLevel, shadowLevel, shape.grid, guidePrep, prepAbove, s);
     MRT_PT_SHAPES(X)
 #undef X
     return hipErrorInvalidValue;
@@ -822,9 +809,11 @@ hipError_t batchShadeBlocksPerCU(int* blocksPerCU) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kShadeBatch, 256, 0);
 }
 
-hipError_t batchTraceBlocksPerCU(uint32_t block, uint32_t stack, int* blocksPerCU) {
-#define X(B, S) \
-    if (block == B && stack == S) return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kTraceBatch<B, S>, B, 0);
+hipError_t batchTraceBlocksPerCU(uint32_t block, uint32_t stack, bool guidePrep, int* blocksPerCU) {
+#define X(B, S)                                                                                              \
+    if (block == B && stack == S)                                                                            \
+        return guidePrep ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kTraceBatch<B, S, true>, B, 0)  \
+                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kTraceBatch<B, S, false>, B, 0);
     MRT_PT_SHAPES(X)
 #undef X
     return hipErrorInvalidValue;

@@ -24,14 +24,11 @@ struct TraceShape {
 
 hipError_t launchClearWords(uint32_t* words, uint32_t n, hipStream_t s);
 hipError_t launchResolve32(const PathView& pv, uint32_t pixels, uint32_t spp, hipStream_t s);
-// Guided shading of one engine's frame (needs the guide prep pass of this
-// bounce); both skip a level that holds at most tailPaths rays (the tail
-// kernel shades it, unguided).
+// Guided shading of one engine's frame (needs the query records of this
+// bounce's trace launch and the network's mixtures); skips a level that
+// holds at most tailPaths rays (the tail kernel shades it, unguided).
 hipError_t launchShadeGuided(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t bounce,
                              uint32_t tailPaths, uint32_t grid, hipStream_t s);
-// Query records of guidable hits for the guiding network (grid-stride).
-hipError_t launchGuidePrep(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t bounce,
-                           uint32_t tailPaths, uint32_t grid, hipStream_t s);
 hipError_t guidedShadeBlocksPerCU(int* blocksPerCU);
 
 // A frame over one or more requests (BatchView; an engine's own frame is a
@@ -43,8 +40,11 @@ hipError_t guidedShadeBlocksPerCU(int* blocksPerCU);
 hipError_t launchClearBatch(const BatchView& bv, hipStream_t s);
 hipError_t launchGenerateBatch(const BatchView& bv, hipStream_t s);
 // extendLevel / shadowLevel: queue levels, either may be kNoLevel.
+// guidePrep (a guided engine's batch of one): also write the guiding
+// network's query records of the closest hits when the level holds more than
+// prepAbove rays.
 hipError_t launchTraceBatch(const BatchView& bv, uint32_t extendLevel, uint32_t shadowLevel, const TraceShape& shape,
-                            hipStream_t s);
+                            hipStream_t s, bool guidePrep = false, uint32_t prepAbove = 0);
 // Plain shading of level `bounce`; skipped when all requests together hold at
 // most tailPaths rays there, in which case launchTailBatch finishes the
 // frame for them (both are launched for every level; 0 = never the tail).
@@ -54,7 +54,7 @@ hipError_t launchTailBatch(const BatchView& bv, uint32_t bounce, uint32_t tailPa
 hipError_t batchTailBlocksPerCU(uint32_t block, uint32_t stack, int* blocksPerCU);
 hipError_t launchFinishBatch(const BatchView& bv, hipStream_t s);
 hipError_t batchShadeBlocksPerCU(int* blocksPerCU);
-hipError_t batchTraceBlocksPerCU(uint32_t block, uint32_t stack, int* blocksPerCU);
+hipError_t batchTraceBlocksPerCU(uint32_t block, uint32_t stack, bool guidePrep, int* blocksPerCU);
 
 bool ptShapeSupported(uint32_t block, uint32_t stack);
 

@@ -64,14 +64,52 @@ def bf16(a: np.ndarray) -> np.ndarray:
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+# Taylor coefficients of sin(pi a) (degree 11 .. 1) and cos(pi a) (12 .. 2),
+# rounded to float32: pi^k / k! with alternating signs.
+_SIN_PI = [np.float32(float.fromhex(h)) for h in
+           ("-0x1.e3075p-8", "0x1.507834p-4", "-0x1.32d2ccp-1", "0x1.466bc6p+1", "-0x1.4abbcep+2", "0x1.921fb6p+1")]
+_COS_PI = [np.float32(float.fromhex(h)) for h in
+           ("0x1.f9d38ap-10", "-0x1.a6d1f2p-6", "0x1.e1f506p-3", "-0x1.55d3c8p+0", "0x1.03c1fp+2", "-0x1.3bd3ccp+2")]
+
+
+def _sincospi_series(t: np.ndarray):
+    """sin(pi t), cos(pi t) for float32 |t| <= 1, operation for operation as
+    the device evaluates them (guide.hip sinCosPiSeries): reflection to
+    a <= 1/2, Horner on a^2, every product and sum rounded to float32."""
+    f = np.float32
+    a0 = np.abs(t)
+    refl = a0 > f(0.5)
+    a = np.where(refl, f(1.0) - a0, a0).astype(np.float32)
+    z = a * a
+    ps = np.full_like(z, _SIN_PI[0])
+    for k in _SIN_PI[1:]:
+        ps = ps * z + k
+    pc = np.full_like(z, _COS_PI[0])
+    for k in _COS_PI[1:]:
+        pc = pc * z + k
+    sn = ps * a
+    cs = pc * z + f(1.0)
+    cs = np.where(refl, -cs, cs)
+    sn = np.where(t < f(0.0), -sn, sn)
+    return sn.astype(np.float32), cs.astype(np.float32)
+
+
 def encode_f32(pos01, nrm, wo) -> np.ndarray:
-    """encode() of float32 positions the way the device forms it: x = p * 2^b
-    (exact in float32), then sin / cos of pi x (sinpi / cospi)."""
+    """encode() as the device forms it in float32: per axis, p reduced exactly
+    to t = p - 2 rint(p / 2), (sin, cos)(pi t) from a float32 Taylor series,
+    then band b + 1 from band b by the double-angle step
+    (s, c) -> (2 s c, (c - s)(c + s)).  Pinned against encode() (float64
+    sin/cos of pi 2^b p) to the recurrence's error growth (< 1e-3)."""
+    f = np.float32
     p = np.asarray(pos01, np.float32)
-    freq = (np.float32(2.0) ** np.arange(BANDS, dtype=np.float32))
-    x = (p[:, :, None] * freq[None, None, :]).astype(np.float64)
-    ang = np.pi * np.fmod(x, 2.0)
-    sc = np.stack([np.sin(ang), np.cos(ang)], axis=-1).reshape(p.shape[0], 3 * BANDS * 2)
+    t = (p - f(2.0) * np.rint(f(0.5) * p)).astype(np.float32)
+    sn, cs = _sincospi_series(t)
+    bands = []
+    for b in range(BANDS):
+        if b > 0:
+            sn, cs = (f(2.0) * sn) * cs, (cs - sn) * (cs + sn)
+        bands.append(np.stack([sn, cs], axis=-1))
+    sc = np.stack(bands, axis=2).astype(np.float64).reshape(p.shape[0], 3 * BANDS * 2)
     return np.concatenate([sc, np.asarray(nrm, np.float64), np.asarray(wo, np.float64)], axis=1)
 
 

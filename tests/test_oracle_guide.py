@@ -66,9 +66,25 @@ def test_encoding_special_values():
     sz = e[48:72:2]
     np.testing.assert_allclose(sz[:2], [np.sin(np.pi / 4), 1.0], atol=1e-12)
     np.testing.assert_array_equal(e[72:], [0, 0, 1, 1, 0, 0])
-    # float32-angle variant agrees to float32 argument precision
+    # float32 device form: the double-angle steps double the error per band,
+    # to below 1e-3 at band 11 (under the bf16 rounding of the features)
     pos, nrm, wo, _, _ = _records(256, 3)
-    assert np.abs(G.encode(pos, nrm, wo) - G.encode_f32(pos, nrm, wo)).max() < 2e-3
+    assert np.abs(G.encode(pos, nrm, wo) - G.encode_f32(pos, nrm, wo)).max() < 1e-3
+
+
+def test_sincospi_series():
+    """The device's series for sin / cos(pi t) against float64 on a dense grid
+    of [-1, 1] (a wrong coefficient or reflection shows up as > 1e-5), and the
+    exact values at the quarter points."""
+    t = np.linspace(-1.0, 1.0, 400001).astype(np.float32)
+    sn, cs = G._sincospi_series(t)
+    ref = np.pi * t.astype(np.float64)
+    assert np.abs(sn - np.sin(ref)).max() < 3e-7
+    assert np.abs(cs - np.cos(ref)).max() < 3e-7
+    q = np.array([-1.0, -0.5, 0.0, 0.5, 1.0], np.float32)
+    sq, cq = G._sincospi_series(q)
+    np.testing.assert_allclose(sq, [0, -1, 0, 1, 0], atol=1e-7)
+    np.testing.assert_allclose(cq, [-1, 0, 1, 0, -1], atol=1e-7)
 
 
 def test_bf16_rounding():
