@@ -24,7 +24,7 @@ struct mrtxQueryScene {
     // Default launch shape: the fastest measured on a 10k-triangle soup.
     uint32_t block = 128;
     uint32_t stack = 8;
-    uint32_t loop = 1;
+    uint32_t loop = 0;
 };
 
 namespace {

@@ -559,6 +559,7 @@ __global__ __launch_bounds__(BLOCK) void kTraceBatch(BatchView bv, uint32_t exte
     __shared__ uint2 ldsStack[S * BLOCK];
     const BatchChunks be = extendLevel != kNoLevel ? batchChunks(bv, extendLevel, kCntRays) : BatchChunks{ 0, 0, 0 };
     const BatchChunks bs = shadowLevel != kNoLevel ? batchChunks(bv, shadowLevel, kCntShadows) : BatchChunks{ 0, 0, 0 };
+    if (be.total == 0 && bs.total == 0) return;  // levels the tail kernel finished: no atomics
     uint32_t* cursor = bv.cursors + (extendLevel != kNoLevel ? extendLevel * 4 + 0 : shadowLevel * 4 + 2);
     while (true) {
         const uint32_t chunk = batchFetch(cursor);
