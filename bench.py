@@ -13,6 +13,9 @@ Workloads:
              lit through a window (1280x720, 499k triangles).
   batch      64 uneven requests rendered by one batched graph launch vs one
              frame call at a time (images must be identical).
+  scale      guided and batched together: 16 requests at 4K / 1080p over a
+             10.4M-triangle instanced field and the room, depth 12, one
+             guiding network per request (not part of the default run).
 
 Prints ONE JSON line.  Inputs are resident in device memory before the timed
 region; the timed region is bracketed by device synchronisation.  The
@@ -442,12 +445,136 @@ def copy_workload(args, torch) -> dict:
             "bytes_read": 4 * n, "bytes_written": 4 * n, "gbs": 8 * n / dt / 1e9}
 
 
+# ------------------------------------------------------- guided + batched at scale --
+def _scale_engine(k, sky, seed):
+    """Request k of the scale workload: scene by parity (sphere field / room),
+    resolution by (k // 2) % 2 (3840x2160 / 1920x1080), max depth 12."""
+    from miniraytrace_amd import scenes
+    from miniraytrace_amd.engine import Engine
+    cfg = scenes.CONFIG5
+    W, H = cfg["resolutions"][(k // 2) % 2]
+    e = Engine(width=W, height=H)
+    e.set_render_settings(width=W, height=H, max_bounces=cfg["max_depth"], spp_limit=1 << 30,
+                          firefly_clamp=0.0, denoise=False)
+    if k % 2 == 0:
+        info = scenes.load_sphere_field(e, cfg, seed=1 + k // 2, sky_img=sky)
+    else:
+        info = scenes.load_room_scene(e, scenes.CONFIG4, seed=1, sky_img=sky)
+        e.set_camera(**scenes.CONFIG4["camera"])
+    e.set_path_options(seed=seed)
+    e.commit()
+    return e, W, H, info
+
+
+def _mse(a, b) -> float:
+    return float(((a - b) ** 2).mean())
+
+
+def scale_workload(args, torch) -> dict:
+    """Guided and batched together at scale: 16 requests, alternating between
+    the field of 256 instanced 40,500-triangle displaced spheres (10.4M
+    triangles) and the window-lit room, at 3840x2160 and 1920x1080, max depth
+    12.  The same engines render `frames` frames per request through one
+    mrtxRenderBatch call, first plain (requests share one set of kernels per
+    frame), then each with its own online-trained guiding network (after
+    `warm` training frames whose samples are discarded).
+
+    Equal-RMSE speedup of guided over plain batched rendering: per-sample MSE
+    ratio (guided / plain at equal samples per pixel, against a guided render
+    of 4x the samples) measured on the 1920x1080 field and room requests,
+    times the ratio of batched throughputs."""
+    from miniraytrace_amd import scenes
+    from miniraytrace_amd.batch import render_batch
+    cfg = scenes.CONFIG5
+    frames, warm = 8, 8
+    sky = scenes.sky()
+    t0 = time.perf_counter()
+    engines = [_scale_engine(k, sky, 100 + k) for k in range(cfg["requests"])]
+    t_setup = time.perf_counter() - t0
+    samples = sum(W * H for _, W, H, _ in engines) * frames
+    _log(f"scale: {len(engines)} requests, setup {t_setup:.1f} s, {samples / frames / 1e6:.1f} Mpixels per frame")
+    es = [e for e, _, _, _ in engines]
+    render_batch(es, [1] * len(es))  # first use: graph capture and caches
+    for e in es:
+        e.reset_accumulation()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st_plain = render_batch(es, [frames] * len(es))
+    t_plain = time.perf_counter() - t0
+    _log(f"scale: plain batched {samples / t_plain / 1e6:.1f} Msamples/s (mode {st_plain['mode']})")
+    for e in es:
+        e.reset_accumulation()
+        e.set_guiding(sampling=True, training=True)
+    render_batch(es, [warm] * len(es))
+    for e in es:
+        e.reset_accumulation()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st_guided = render_batch(es, [frames] * len(es))
+    t_guided = time.perf_counter() - t0
+    _log(f"scale: guided batched {samples / t_guided / 1e6:.1f} Msamples/s (mode {st_guided['mode']})")
+    for e in es:
+        e.close()
+    engines = es = None
+
+    # Per-sample MSE ratio on the two 1920x1080 requests (k = 2: field, 3: room).
+    spp = 32
+    per_request = {}
+    for k in (2, 3):
+        ref, W, H, _ = _scale_engine(k, sky, 900 + k)
+        ref.set_guiding(sampling=True, training=True)
+        for _ in range(warm):
+            ref.render_frame()
+        ref.reset_accumulation()
+        for _ in range(4 * spp):
+            ref.render_frame()
+        img_r = ref.read_rgba32f(W, H)[..., :3].astype(np.float64)
+        ref.close()
+        plain, _, _, _ = _scale_engine(k, sky, 700 + k)
+        for _ in range(spp):
+            plain.render_frame()
+        mse_p = _mse(plain.read_rgba32f(W, H)[..., :3].astype(np.float64), img_r)
+        plain.close()
+        g, _, _, _ = _scale_engine(k, sky, 800 + k)
+        g.set_guiding(sampling=True, training=True)
+        for _ in range(warm):
+            g.render_frame()
+        g.reset_accumulation()
+        for _ in range(spp):
+            g.render_frame()
+        mse_g = _mse(g.read_rgba32f(W, H)[..., :3].astype(np.float64), img_r)
+        g.close()
+        name = "field" if k % 2 == 0 else "room"
+        per_request[name] = {"spp": spp, "reference_spp": 4 * spp, "mse_plain": mse_p, "mse_guided": mse_g,
+                             "mse_ratio": mse_g / mse_p}
+        _log(f"scale: {name} 1080p per-sample MSE ratio guided / plain {mse_g / mse_p:.3f}")
+    thr_ratio = t_plain / t_guided  # guided samples/s over plain samples/s
+    speedups = {n: thr_ratio / r["mse_ratio"] for n, r in per_request.items()}
+    return {
+        "workload": "16 requests: 256 x 40,500-tri displaced spheres (10.4M tris) / 495k-tri room, "
+                    "3840x2160 and 1920x1080, depth 12, per-request guiding network",
+        "requests": cfg["requests"],
+        "frames": frames,
+        "samples": samples,
+        "setup_s": t_setup,
+        "plain_batched_s": t_plain,
+        "plain_msamples_s": samples / t_plain / 1e6,
+        "plain_mode": st_plain["mode"],
+        "guided_batched_s": t_guided,
+        "guided_msamples_s": samples / t_guided / 1e6,
+        "guided_mode": st_guided["mode"],
+        "guided_warmup_frames": warm,
+        "per_sample_mse": per_request,
+        "equal_rmse_speedup": speedups,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="pt", choices=["pt", "traversal", "guided", "batch", "copy"])
+    ap.add_argument("--workload", default="pt", choices=["pt", "traversal", "guided", "batch", "scale", "copy"])
     ap.add_argument("--guided-spp", type=int, default=1024,
                     help="plain sample count whose RMSE the guided run must reach")
     ap.add_argument("--no-guided", dest="with_guided", action="store_false", default=True)
@@ -479,6 +606,13 @@ def main() -> int:
         print(json.dumps({"metric": "batched Msamples/s", "value": res["msamples_s_batch"], "unit": "Msamples/s",
                           "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
                           "config": {"workload": res["workload"]}, "batch": res}))
+        return 0
+    if args.workload == "scale":
+        res = scale_workload(args, torch)
+        print(json.dumps({"metric": "guided + batched Msamples/s, 16 requests at 4K / 1080p", "value":
+                          res["guided_msamples_s"], "unit": "Msamples/s", "n_gpus": 1, "higher_is_better": True,
+                          "dtype": "f32", "data": "synthetic", "config": {"workload": res["workload"]},
+                          "scale": res}))
         return 0
     if args.workload == "guided":
         res = guided_workload(args, torch)

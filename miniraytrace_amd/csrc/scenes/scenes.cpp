@@ -6,6 +6,7 @@
 // standard), converted to floats as u = (x >> 40) * 2^-24, which is exact in
 // binary32 and lies in [0, 1).
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -324,6 +325,100 @@ void mrts_materials(uint32_t count, uint64_t seed, float* out) {
         o[3] = float(0.05 + 0.75 * r.ud());
         o[4] = r.ud() < 0.5 ? 0.0f : 1.0f;
         for (int c = 0; c < 3; ++c) o[c] = float(0.2 + 0.7 * r.ud());
+    }
+}
+
+// ---- displaced geodesic sphere ----------------------------------------------
+// Each icosahedron face split into n x n triangles (20 n^2 triangles; n = 45
+// gives 40,500), vertices shared along face edges, projected to the unit
+// sphere and moved radially: r = 1 + amplitude * sum_k w_k cos(f_k (d . a_k) +
+// phi_k) / sum_k w_k over 8 seeded waves (a_k uniform directions, f_k in
+// [3, 11], phi_k uniform, w_k = 1 / f_k).  Returns the vertex count;
+// positions: at most 10 n^2 + 2 vertices x 3, indices: 20 n^2 x 3.
+uint32_t mrts_displaced_sphere(uint32_t n, float amplitude, uint64_t seed, float* positions, uint32_t* indices) {
+    const double t = (1.0 + std::sqrt(5.0)) / 2.0;
+    const double ico[12][3] = { { -1, t, 0 }, { 1, t, 0 }, { -1, -t, 0 }, { 1, -t, 0 }, { 0, -1, t }, { 0, 1, t },
+                                { 0, -1, -t }, { 0, 1, -t }, { t, 0, -1 }, { t, 0, 1 }, { -t, 0, -1 }, { -t, 0, 1 } };
+    const uint32_t faces[20][3] = { { 0, 11, 5 }, { 0, 5, 1 }, { 0, 1, 7 }, { 0, 7, 10 }, { 0, 10, 11 },
+                                    { 1, 5, 9 }, { 5, 11, 4 }, { 11, 10, 2 }, { 10, 7, 6 }, { 7, 1, 8 },
+                                    { 3, 9, 4 }, { 3, 4, 2 }, { 3, 2, 6 }, { 3, 6, 8 }, { 3, 8, 9 },
+                                    { 4, 9, 5 }, { 2, 4, 11 }, { 6, 2, 10 }, { 8, 6, 7 }, { 9, 8, 1 } };
+    Rng r(seed);
+    double wa[8][3], wf[8], wp[8], ww[8], wsum = 0.0;
+    for (int k = 0; k < 8; ++k) {
+        const double z = 2.0 * r.ud() - 1.0, ph = 2.0 * M_PI * r.ud(), s = std::sqrt(std::max(0.0, 1.0 - z * z));
+        wa[k][0] = s * std::cos(ph); wa[k][1] = s * std::sin(ph); wa[k][2] = z;
+        wf[k] = 3.0 + 8.0 * r.ud();
+        wp[k] = 2.0 * M_PI * r.ud();
+        ww[k] = 1.0 / wf[k];
+        wsum += ww[k];
+    }
+    // Vertex key: the grid point's barycentric coordinates (a, b, c; a + b +
+    // c = n) on the global icosahedron corners, canonical per shared point.
+    std::map<std::array<uint64_t, 3>, uint32_t> ids;
+    uint32_t nv = 0, nt = 0;
+    auto vert = [&](uint32_t f, uint32_t i, uint32_t j) {
+        // point = (i * A + j * B + (n - i - j) * C) / n on face f = (A, B, C)
+        const uint32_t w[3] = { i, j, n - i - j };
+        std::array<std::pair<uint32_t, uint32_t>, 3> parts;
+        for (int c = 0; c < 3; ++c) parts[c] = { faces[f][c], w[c] };
+        std::sort(parts.begin(), parts.end());
+        std::array<uint64_t, 3> key;
+        for (int c = 0; c < 3; ++c) key[c] = (uint64_t(parts[c].second == 0 ? 0xFFFF : parts[c].first) << 32) | parts[c].second;
+        std::sort(key.begin(), key.end());
+        auto it = ids.find(key);
+        if (it != ids.end()) return it->second;
+        double p[3] = { 0, 0, 0 };
+        for (int c = 0; c < 3; ++c)
+            for (int a = 0; a < 3; ++a) p[a] += double(w[c]) * ico[faces[f][c]][a];
+        const double l = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+        for (double& x : p) x /= l;
+        double h = 0.0;
+        for (int k = 0; k < 8; ++k)
+            h += ww[k] * std::cos(wf[k] * (p[0] * wa[k][0] + p[1] * wa[k][1] + p[2] * wa[k][2]) + wp[k]);
+        const double rad = 1.0 + amplitude * h / wsum;
+        for (int a = 0; a < 3; ++a) positions[size_t(nv) * 3 + a] = float(p[a] * rad);
+        ids.emplace(key, nv);
+        return nv++;
+    };
+    for (uint32_t f = 0; f < 20; ++f)
+        for (uint32_t i = 0; i < n; ++i)
+            for (uint32_t j = 0; i + j < n; ++j) {
+                const uint32_t a = vert(f, i, j), b = vert(f, i + 1, j), c = vert(f, i, j + 1);
+                uint32_t* o = indices + size_t(nt++) * 3;
+                o[0] = a; o[1] = b; o[2] = c;
+                if (i + j + 1 < n) {
+                    const uint32_t d = vert(f, i + 1, j + 1);
+                    o = indices + size_t(nt++) * 3;
+                    o[0] = b; o[1] = d; o[2] = c;
+                }
+            }
+    return nv;
+}
+
+// Instance transforms (3 x 4 row-major) uniformly random in a box of edge
+// `box` centred at the origin: uniform rotation (random unit quaternion),
+// uniform scale in [smin, smax], uniform translation; material index
+// uniform in [0, nmat).
+void mrts_random_instances(uint32_t count, float box, float smin, float smax, uint32_t nmat, uint64_t seed,
+                           float* xforms, uint32_t* mats) {
+    Rng r(seed);
+    for (uint32_t i = 0; i < count; ++i) {
+        // Shoemake's uniform quaternion from three uniforms
+        const double u1 = r.ud(), u2 = r.ud(), u3 = r.ud();
+        const double a = std::sqrt(1.0 - u1), b = std::sqrt(u1);
+        const double qx = a * std::sin(2 * M_PI * u2), qy = a * std::cos(2 * M_PI * u2);
+        const double qz = b * std::sin(2 * M_PI * u3), qw = b * std::cos(2 * M_PI * u3);
+        const double s = smin + (smax - smin) * r.ud();
+        const double R[3][3] = { { 1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qz * qw), 2 * (qx * qz + qy * qw) },
+                                 { 2 * (qx * qy + qz * qw), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qx * qw) },
+                                 { 2 * (qx * qz - qy * qw), 2 * (qy * qz + qx * qw), 1 - 2 * (qx * qx + qy * qy) } };
+        float* m = xforms + size_t(i) * 12;
+        for (int row = 0; row < 3; ++row) {
+            for (int c = 0; c < 3; ++c) m[row * 4 + c] = float(R[row][c] * s);
+            m[row * 4 + 3] = float((r.ud() - 0.5) * box);
+        }
+        mats[i] = uint32_t(r.g() % nmat);
     }
 }
 

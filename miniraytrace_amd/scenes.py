@@ -38,6 +38,9 @@ def _bind(h):
     h.mrts_materials.argtypes = [U32, U64, P]
     h.mrts_room.argtypes = [U32, F, F, F, F, F, U64, P, P, P]
     h.mrts_room.restype = U32
+    h.mrts_displaced_sphere.argtypes = [U32, F, U64, P, P]
+    h.mrts_displaced_sphere.restype = U32
+    h.mrts_random_instances.argtypes = [U32, F, F, F, U32, U64, P, P]
 
 
 def terrain(nx: int = 1000, nz: int = 500, sx: float = 20.0, sz: float = 10.0, amplitude: float = 0.2,
@@ -166,6 +169,52 @@ def load_room_scene(engine, cfg: dict = CONFIG4, seed: int = 1, sky_img: np.ndar
     engine.set_environment(sky_img, 0.0, 1.0)
     engine.set_camera(**cfg["camera"])
     return {"triangles": idx.shape[0], "instances": 1}
+
+
+def displaced_sphere(n: int = 45, amplitude: float = 0.15, seed: int = 1) -> tuple[np.ndarray, np.ndarray]:
+    """Geodesic sphere of 20 n^2 triangles (n = 45: 40,500), radially
+    displaced by seeded waves: (10 n^2 + 2, 3) positions, (20 n^2, 3) indices."""
+    pos = np.empty((10 * n * n + 2, 3), np.float32)
+    idx = np.empty((20 * n * n, 3), np.uint32)
+    nv = lib().mrts_displaced_sphere(n, amplitude, seed, pos.ctypes.data, idx.ctypes.data)
+    return pos[:nv], idx
+
+
+def random_instances(count: int, box: float, smin: float, smax: float, nmat: int,
+                     seed: int) -> tuple[np.ndarray, np.ndarray]:
+    """(count, 12) row-major 3x4 transforms uniformly random in a box of edge
+    `box` (rotation, scale in [smin, smax], translation) and material ids."""
+    xf = np.empty((count, 12), np.float32)
+    mats = np.empty(count, np.uint32)
+    lib().mrts_random_instances(count, box, smin, smax, nmat, seed, xf.ctypes.data, mats.ctypes.data)
+    return xf, mats
+
+
+# Guided and batched together at scale: 256 instances of a 40,500-triangle
+# displaced sphere (10.4M triangles instanced) in a 20^3 box, 3 GGX materials
+# as in the terrain scene; requests alternate between this field and the
+# window-lit room, half at 3840x2160 and half at 1920x1080, max depth 12.
+CONFIG5 = dict(instances=256, sphere_n=45, box=20.0, smin=0.5, smax=1.5, requests=16, max_depth=12,
+               resolutions=((3840, 2160), (1920, 1080)),
+               camera=dict(position=(0.0, 4.0, 26.0), target=(0.0, 0.0, 0.0), fov_y_deg=50.0))
+
+
+def load_sphere_field(engine, cfg: dict = CONFIG5, seed: int = 1, sky_img: np.ndarray | None = None) -> dict:
+    """The instanced displaced-sphere field of CONFIG5 into an Engine."""
+    pos, idx = displaced_sphere(cfg["sphere_n"], 0.15, seed)
+    mats = materials(3, seed + 100)
+    mat_ids = [engine.add_material(base_color=tuple(m[:3]), roughness=float(m[3]), metallic=float(m[4]))
+               for m in mats]
+    mesh = engine.add_mesh(pos, idx)
+    xf, which = random_instances(cfg["instances"], cfg["box"], cfg["smin"], cfg["smax"], 3, seed + 7)
+    for k in range(cfg["instances"]):
+        engine.add_instance(mesh, xf[k], mat_ids[int(which[k])])
+    if sky_img is None:
+        sky_img = sky()
+    engine.set_environment(sky_img, 0.0, 1.0)
+    engine.set_camera(**cfg["camera"])
+    return {"triangles": cfg["instances"] * idx.shape[0], "unique_triangles": idx.shape[0],
+            "instances": cfg["instances"]}
 
 
 def soup(count: int = 10_000, seed: int = 1) -> np.ndarray:
