@@ -24,9 +24,10 @@ namespace mrt {
 constexpr uint32_t kDefaultTraceBlock = 256;
 constexpr uint32_t kDefaultTraceStack = 8;
 // Tail kernel shape: its registers (shading and traversal in one kernel)
-// allow 2 waves per SIMD, so a deep LDS stack costs no occupancy.
+// allow 3 waves per SIMD, which 128-thread blocks with 16 LDS stack entries
+// per lane (16 KB per block) keep.
 constexpr uint32_t kTailBlock = 128;
-constexpr uint32_t kTailStack = 32;
+constexpr uint32_t kTailStack = 16;
 // Default tail threshold: the tail kernel takes over a bounce level once all
 // requests together hold at most this many rays per compute unit there
 // (measured on the 1.256M-triangle terrain at 1080p: the tail then covers the

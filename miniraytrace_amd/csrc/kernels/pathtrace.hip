@@ -628,9 +628,12 @@ __device__ __forceinline__ void tailChunk(const SceneView& sc, const PathView& p
 }
 
 // Runs when all requests together hold 1..tailPaths rays at level `bounce`
-// (kShadeBatch runs otherwise).
+// (kShadeBatch runs otherwise).  Held to at most 168 VGPRs (3 waves per SIMD,
+// a few spills): measured faster than the compiler's 183 at 2 waves.
 template <int BLOCK, int S>
-__global__ __launch_bounds__(BLOCK) void kTailBatch(BatchView bv, uint32_t bounce, uint32_t tailPaths) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void kTailBatch(BatchView bv,
+                                                                                            uint32_t bounce,
+                                                                                            uint32_t tailPaths) {
     __shared__ uint2 ldsStack[S * BLOCK];
     const BatchChunks bc = batchChunks(bv, bounce, kCntRays);
     const uint32_t items = batchItems(bc);
