@@ -34,9 +34,10 @@ struct BatchGraphs {
 };
 
 // The launch shape that most of the batch's work was tuned for (weighted by
-// pixels x frames of each request).
+// pixels x frames of each request); guidePrep: the grid of the trace variant
+// that also writes guided requests' query records.
 mrt::dev::TraceShape voteShape(const std::vector<mrt::Engine*>& engines, const std::vector<uint32_t>& frames,
-                               bool shadow) {
+                               bool shadow, bool guidePrep) {
     std::map<std::pair<uint32_t, uint32_t>, double> weight;
     for (size_t i = 0; i < engines.size(); ++i) {
         const mrt::dev::TraceShape& s = shadow ? engines[i]->shadowShape() : engines[i]->extendShape();
@@ -46,13 +47,15 @@ mrt::dev::TraceShape voteShape(const std::vector<mrt::Engine*>& engines, const s
                                  [](const auto& a, const auto& b) { return a.second < b.second; });
     mrt::dev::TraceShape t{ best->first.first, best->first.second, 0 };
     int perCU = 0;
-    MRT_HIP(mrt::dev::batchTraceBlocksPerCU(t.block, t.stack, false, &perCU));
+    MRT_HIP(mrt::dev::batchTraceBlocksPerCU(t.block, t.stack, guidePrep, &perCU));
     t.grid = uint32_t(std::max(1, perCU) * engines[0]->cuCount());
     return t;
 }
 
 // Mode 0: all requests' queues served by one set of persistent kernels per
-// batched frame.
+// batched frame.  Guided requests add their own passes to the frame: the
+// network inference of their query records after each trace launch and the
+// training after the frame (each request has its own network).
 void renderMerged(const std::vector<mrt::Engine*>& engines, std::vector<uint32_t>& frames, mrtxBatchStats& st) {
     const size_t n = engines.size();
     std::vector<const mrt::FrameConsts*> consts(n, nullptr);
@@ -76,17 +79,17 @@ void renderMerged(const std::vector<mrt::Engine*>& engines, std::vector<uint32_t
             e.pixels = engines[i]->pixels();
             e.pixelBase = base[i];
             e.active = f < frames[i] ? 1u : 0u;
-            e.pad = 0;
+            e.tailAt = engines[i]->tailAt();
         }
+    bool guided = false;
+    for (mrt::Engine* e : engines) guided = guided || e->guideFrame();
     mrt::DeviceBuffer dTable, dCursors;
     dTable.allocate(table.size() * sizeof(mrt::dev::BatchEntry));
     dTable.upload(table.data(), table.size() * sizeof(mrt::dev::BatchEntry));
     dCursors.allocate(mrt::dev::kMaxBounceLevels * 4 * sizeof(uint32_t));
-    const mrt::dev::TraceShape ext = voteShape(engines, frames, false);
-    const mrt::dev::TraceShape sha = voteShape(engines, frames, true);
-    int perCU = 0;
-    MRT_HIP(mrt::dev::batchShadeBlocksPerCU(&perCU));
-    const uint32_t shadeGrid = uint32_t(std::max(1, perCU) * engines[0]->cuCount());
+    const mrt::dev::TraceShape ext = voteShape(engines, frames, false, guided);
+    const mrt::dev::TraceShape sha = voteShape(engines, frames, true, false);
+    const uint32_t shadeGrid = engines[0]->shadeGrid(guided);
     const mrt::dev::TraceShape tail = engines[0]->tailShape();
     // The tail threshold of the first request (the batch shares one tail).
     const uint32_t tailPaths = engines[0]->tailPaths();
@@ -102,13 +105,20 @@ void renderMerged(const std::vector<mrt::Engine*>& engines, std::vector<uint32_t
             bv.pixelSlots = slots;
             bv.cursors = dCursors.as<uint32_t>();
             MRT_HIP(mrt::dev::launchClearBatch(bv, s));
+            for (size_t i = 0; i < n; ++i)
+                if (f < frames[i]) engines[i]->enqueueGuideFrameBegin(s, consts[i] + f);
             MRT_HIP(mrt::dev::launchGenerateBatch(bv, s));
             for (uint32_t b = 0; b < levels; ++b) {
-                MRT_HIP(mrt::dev::launchTraceBatch(bv, b, b == 0 ? mrt::dev::kNoLevel : b - 1, ext, s));
-                MRT_HIP(mrt::dev::launchShadeBatch(bv, b, tailPaths, shadeGrid, s));
+                MRT_HIP(mrt::dev::launchTraceBatch(bv, b, b == 0 ? mrt::dev::kNoLevel : b - 1, ext, tailPaths, guided,
+                                                   s));
+                for (size_t i = 0; i < n; ++i)
+                    if (f < frames[i]) engines[i]->enqueueGuideInfer(s, b);
+                MRT_HIP(mrt::dev::launchShadeBatch(bv, b, tailPaths, guided, shadeGrid, s));
                 MRT_HIP(mrt::dev::launchTailBatch(bv, b, tailPaths, tail, s));
             }
-            MRT_HIP(mrt::dev::launchTraceBatch(bv, mrt::dev::kNoLevel, levels - 1, sha, s));
+            MRT_HIP(mrt::dev::launchTraceBatch(bv, mrt::dev::kNoLevel, levels - 1, sha, tailPaths, false, s));
+            for (size_t i = 0; i < n; ++i)
+                if (f < frames[i]) engines[i]->enqueueGuideTraining(s, consts[i] + f);
             MRT_HIP(mrt::dev::launchFinishBatch(bv, s));
         }
     } catch (...) {
@@ -185,13 +195,11 @@ MRT_API mrtResult mrtxRenderBatch(mrtxBatchRequest* reqs, uint32_t count, mrtxBa
             const auto t0 = Clock::now();
             std::vector<mrt::Engine*> engines(count);
             std::vector<uint32_t> frames(count);
-            bool guided = false;
             for (uint32_t i = 0; i < count; ++i) {
                 engines[i] = mrt::unwrap(reqs[i].engine);
                 frames[i] = reqs[i].frames;
-                guided = guided || engines[i]->guideActive();
             }
-            if (!guided && count <= mrt::dev::kMaxBatchRequests) renderMerged(engines, frames, st);
+            if (count <= mrt::dev::kMaxBatchRequests) renderMerged(engines, frames, st);
             else renderChains(engines, frames, st);
             st.captureMs = msSince(t0) - st.launchMs;
             for (uint32_t i = 0; i < count; ++i) {

@@ -184,14 +184,14 @@ void Engine::chooseShapes() {
     prepTraceGrid_ = uint32_t(std::max(1, perCU) * cuCount_);
     MRT_HIP(dev::batchTraceBlocksPerCU(shadowShape_.block, shadowShape_.stack, false, &perCU));
     shadowShape_.grid = uint32_t(std::max(1, perCU) * cuCount_);
-    MRT_HIP(dev::batchShadeBlocksPerCU(&perCU));
+    MRT_HIP(dev::batchShadeBlocksPerCU(false, &perCU));
     shadeGrid_ = uint32_t(std::max(1, perCU) * cuCount_);
     MRT_HIP(dev::batchTailBlocksPerCU(tailBlock_, tailStack_, &perCU));
     tailGrid_ = uint32_t(std::max(1, perCU) * cuCount_);
     tailPaths_ = tailOption_ == MRTX_TAIL_DISABLED ? 0u
                  : tailOption_ != 0                ? tailOption_
                                                    : kTailPathsPerCU * uint32_t(cuCount_);
-    MRT_HIP(dev::guidedShadeBlocksPerCU(&perCU));
+    MRT_HIP(dev::batchShadeBlocksPerCU(true, &perCU));
     shadeGuideGrid_ = uint32_t(std::max(1, perCU) * cuCount_);
     invalidateGraph();
 }
@@ -208,6 +208,7 @@ void Engine::fillEntries(dev::BatchEntry* out, const FrameConsts* g0, uint32_t f
         e.pixels = frameW_ * frameH_;
         e.pixelBase = 0;
         e.active = 1;
+        e.tailAt = tailAt();
     }
 }
 
@@ -808,18 +809,9 @@ void Engine::fillConsts(FrameConsts& g) const {
 }
 
 void Engine::enqueueFrame(hipStream_t s, bool timed, const FrameConsts* g, const dev::BatchEntry* entry) {
-    const dev::SceneView sc = sceneView();
-    const dev::PathView pv = pathView();
     const uint32_t pixels = frameW_ * frameH_;
     const uint32_t maxB = std::min(settings_.maxBounces, 16u);
-    const bool guide = pv.guideIdx != nullptr;
-    const bool train = pv.pathRec != nullptr;
-    const bool infer = guide && guide_.enableSampling;
-    // Vertices the tail kernel shades are not guided, so a guided frame hands
-    // over only its last few paths (at most 1/8 of its pixels).
-    const uint32_t tail = guide ? std::min(tailPaths_, pixels / 8) : tailPaths_;
-    dev::TraceShape traceShape = extendShape_;
-    if (guide) traceShape.grid = prepTraceGrid_;  // the query-record variant holds more registers
+    const bool guide = guideFrame();
     dev::BatchView bv;
     bv.entries = entry;
     bv.count = 1;
@@ -832,30 +824,21 @@ void Engine::enqueueFrame(hipStream_t s, bool timed, const FrameConsts* g, const
     };
     if (timed) MRT_HIP(hipEventRecord(ev_[0], s));
     MRT_HIP(dev::launchClearBatch(bv, s));
-    if (train) MRT_HIP(dev::launchGuideFrameBegin(guideNet_->deviceWords(), g, s));
+    enqueueGuideFrameBegin(s, g);
     MRT_HIP(dev::launchGenerateBatch(bv, s));
     if (timed) MRT_HIP(hipEventRecord(ev_[1], s));
     // Per bounce b: one trace launch (closest hits of level b with the shadow
-    // rays of level b - 1), the guide passes, shade; the deepest level's
-    // shadow rays get a trace launch of their own.
+    // rays of level b - 1), the network inference, shade and tail; the
+    // deepest level's shadow rays get a trace launch of their own.
     for (uint32_t b = 0; b <= maxB; ++b) {
         if (timed) MRT_HIP(hipEventRecord(ev_[2], s));
-        MRT_HIP(dev::launchTraceBatch(bv, b, b == 0 ? dev::kNoLevel : b - 1, traceShape, s, guide, tail));
+        MRT_HIP(dev::launchTraceBatch(bv, b, b == 0 ? dev::kNoLevel : b - 1, traceShape(guide), tailPaths_, guide, s));
         if (timed) MRT_HIP(hipEventRecord(ev_[3], s));
-        if (guide) {
-            if (infer) {
-                dev::RecView rv;
-                rv.base = pv.guideRec;
-                rv.stride = 3;
-                MRT_HIP(dev::launchGuideInfer(rv, pv.counters + b * dev::kCntPerBounce + dev::kCntGuideRecords,
-                                              pixels, guideNet_->deviceImage(), pv.guideMix, true, s));
-            }
-        }
+        enqueueGuideInfer(s, b);
         if (timed) MRT_HIP(hipEventRecord(ev_[9], s));
-        if (guide) MRT_HIP(dev::launchShadeGuided(sc, pv, g, b, tail, shadeGuideGrid_, s));
-        else MRT_HIP(dev::launchShadeBatch(bv, b, tail, shadeGrid_, s));
+        MRT_HIP(dev::launchShadeBatch(bv, b, tailPaths_, guide, shadeGrid(guide), s));
         if (timed) MRT_HIP(hipEventRecord(ev_[4], s));
-        MRT_HIP(dev::launchTailBatch(bv, b, tail, tailShape(), s));
+        MRT_HIP(dev::launchTailBatch(bv, b, tailPaths_, tailShape(), s));
         if (timed) {
             MRT_HIP(hipEventRecord(ev_[8], s));
             MRT_HIP(hipEventSynchronize(ev_[8]));
@@ -866,14 +849,14 @@ void Engine::enqueueFrame(hipStream_t s, bool timed, const FrameConsts* g, const
         }
     }
     if (timed) MRT_HIP(hipEventRecord(ev_[4], s));
-    MRT_HIP(dev::launchTraceBatch(bv, dev::kNoLevel, maxB, shadowShape_, s));
+    MRT_HIP(dev::launchTraceBatch(bv, dev::kNoLevel, maxB, shadowShape_, tailPaths_, false, s));
     if (timed) {
         MRT_HIP(hipEventRecord(ev_[5], s));
         MRT_HIP(hipEventSynchronize(ev_[5]));
         stats_.stageMs[3] += elapsed(ev_[4], ev_[5]);
     }
     if (timed) MRT_HIP(hipEventRecord(ev_[10], s));
-    if (train) enqueueGuideTraining(s, g);
+    enqueueGuideTraining(s, g);
     if (timed) MRT_HIP(hipEventRecord(ev_[6], s));
     MRT_HIP(dev::launchFinishBatch(bv, s));  // accumulate + resolve
     if (timed) {
@@ -885,9 +868,40 @@ void Engine::enqueueFrame(hipStream_t s, bool timed, const FrameConsts* g, const
     }
 }
 
+// Guiding passes of this engine's frame (no-ops for unguided frames), also
+// enqueued per request by the merged batch: the per-frame stamp of the
+// training records, the network inference of level `bounce`'s query
+// records, and the training after the frame.
+bool Engine::guideFrame() const { return pathView().guideIdx != nullptr; }
+
+void Engine::enqueueGuideFrameBegin(hipStream_t s, const FrameConsts* g) {
+    if (pathView().pathRec) MRT_HIP(dev::launchGuideFrameBegin(guideNet_->deviceWords(), g, s));
+}
+
+void Engine::enqueueGuideInfer(hipStream_t s, uint32_t bounce) {
+    const dev::PathView pv = pathView();
+    if (!pv.guideIdx || !guide_.enableSampling) return;
+    dev::RecView rv;
+    rv.base = pv.guideRec;
+    rv.stride = 3;
+    MRT_HIP(dev::launchGuideInfer(rv, pv.counters + bounce * dev::kCntPerBounce + dev::kCntGuideRecords,
+                                  frameW_ * frameH_, guideNet_->deviceImage(), pv.guideMix, true, s));
+}
+
+// Vertices the tail kernel shades are not guided, so a guided request hands
+// over only its last few paths (at most 1/8 of its pixels per level).
+uint32_t Engine::tailAt() const { return guideFrame() ? frameW_ * frameH_ / 8 : ~0u; }
+
+dev::TraceShape Engine::traceShape(bool guidePrep) const {
+    dev::TraceShape t = extendShape_;
+    if (guidePrep) t.grid = prepTraceGrid_;  // the query-record variant holds more registers
+    return t;
+}
+
 // Training records of this frame -> pool -> stepsPerFrame Adam steps, batch t
 // taking pool rows t, t + steps, t + 2 steps, ...
 void Engine::enqueueGuideTraining(hipStream_t s, const FrameConsts* g) {
+    if (!pathView().pathRec) return;
     const uint32_t steps = guide_.stepsPerFrame;
     const uint32_t poolCap = guide_.batchSize * steps;
     dev::GuideCollectArgs ca;
@@ -1056,8 +1070,8 @@ void Engine::framesDone(uint32_t frames) {
 }
 
 // Launch-shape autotuner for the path tracer: one profiled frame per
-// candidate (extend and shadow share the candidate), best stage time wins for
-// each stage.  Traversal results do not depend on the shape, so neither does
+// candidate shape; the one with the least time in the trace and tail
+// kernels wins.  Traversal results do not depend on the shape, so neither does
 // the image; the trial frames are discarded (accumulation restarts).  Results
 // are cached per (triangle count, pixel count, depth) class.
 void Engine::autotuneShapes(mrtxPathOptions& chosen) {

@@ -88,6 +88,17 @@ public:
     const dev::TraceShape& extendShape() const { return extendShape_; }
     const dev::TraceShape& shadowShape() const { return shadowShape_; }
     uint32_t tailPaths() const { return tailPaths_; }
+    // Guiding passes of a frame (no-ops for an unguided frame), also used by
+    // the merged batch for each of its requests.
+    bool guideFrame() const;
+    void enqueueGuideFrameBegin(hipStream_t s, const FrameConsts* g);
+    void enqueueGuideInfer(hipStream_t s, uint32_t bounce);
+    void enqueueGuideTraining(hipStream_t s, const FrameConsts* g);
+    // BatchEntry::tailAt of this engine's requests.
+    uint32_t tailAt() const;
+    // Trace-launch shape (guidePrep: the variant that writes query records).
+    dev::TraceShape traceShape(bool guidePrep) const;
+    uint32_t shadeGrid(bool guided) const { return guided ? shadeGuideGrid_ : shadeGrid_; }
     dev::TraceShape tailShape() const { return dev::TraceShape{ tailBlock_, tailStack_, tailGrid_ }; }
     uint32_t pixels() const { return frameW_ * frameH_; }
     uint32_t bounceLevels() const { return std::min(settings_.maxBounces, 16u) + 1; }
@@ -122,7 +133,7 @@ private:
     uint32_t guideLevels() const { return std::min(settings_.maxBounces, 16u) + 1; }
     void ensureGuideNet();
     void allocGuide();
-    void enqueueGuideTraining(hipStream_t s, const FrameConsts* g);
+
 
     int device_ = 0;
     int cuCount_ = 0;

@@ -148,17 +148,34 @@ __device__ __forceinline__ uint32_t batchRequestOfSlot(const BatchView& bv, uint
 // Work split of one queue stage over all requests: lane j holds request j's
 // queue length and the inclusive prefix of 64-item chunks.
 struct BatchChunks {
-    uint32_t n;      // lane j: items in request j's queue
+    uint32_t n;      // lane j: items of request j in this split
     uint32_t incl;   // lane j: chunks of requests 0..j
     uint32_t total;
+    bool tail;       // lane j: the tail kernel serves request j at this level
 };
 
-__device__ __forceinline__ BatchChunks batchChunks(const BatchView& bv, uint32_t bounce, uint32_t slot) {
+// Which part of a level's queues a kernel works on.  The tail kernel serves
+// request j at a level when all requests together hold at most tailPaths
+// rays there and request j holds at most its own limit (BatchEntry::tailAt;
+// guided requests hand over only their last few paths); the per-level
+// kernels serve the rest.
+enum BatchPart : int { kPartAll = 0, kPartLevel = 1, kPartTail = 2 };
+
+__device__ __forceinline__ BatchChunks batchChunks(const BatchView& bv, uint32_t bounce, uint32_t slot,
+                                                   uint32_t tailPaths = 0, int part = kPartAll) {
     const uint32_t j = laneId();
     BatchChunks bc;
-    bc.n = 0;
-    if (j < bv.count && bv.entries[j].active) bc.n = bv.entries[j].pv.counters[bounce * kCntPerBounce + slot];
-    uint32_t c = (bc.n + 63u) / 64u;
+    uint32_t n = 0, tailAt = 0;
+    if (j < bv.count && bv.entries[j].active) {
+        n = bv.entries[j].pv.counters[bounce * kCntPerBounce + slot];
+        tailAt = bv.entries[j].tailAt;
+    }
+    uint32_t all = n;
+    for (int off = 32; off >= 1; off >>= 1) all += __shfl_xor(all, off, 64);
+    bc.tail = n > 0 && all <= tailPaths && n <= tailAt;
+    if ((part == kPartLevel && bc.tail) || (part == kPartTail && !bc.tail)) n = 0;
+    bc.n = n;
+    uint32_t c = (n + 63u) / 64u;
     for (int off = 1; off < 64; off <<= 1) {
         const uint32_t t = __shfl_up(c, off, 64);
         if (j >= uint32_t(off)) c += t;
@@ -218,16 +235,15 @@ __global__ __launch_bounds__(256) void kGenerateBatch(BatchView bv) {
 
 // ------------------------------------------------------------------ extend --
 // Closest hits for queue slots [base, base + 64) of one request's ray queue.
-// PREP (guided engines, when count > prepAbove): the same wave then forms the
-// guiding network's query record of every hit on a surface that may be
-// guided (opaque, roughness >= guideMinRoughness) -- (pos01, n, wo) at a
-// compacted row -- and the row number in guideIdx (~0u elsewhere), while
-// the hit is still in registers.  A level at or below prepAbove rays is
-// shaded by the tail kernel, which needs no records.
+// PREP && prep (a guided request at a level the tail kernel does not take):
+// the same wave then forms the guiding network's query record of every hit
+// on a surface that may be guided (opaque, roughness >= guideMinRoughness)
+// -- (pos01, n, wo) at a compacted row -- and the row number in guideIdx
+// (~0u elsewhere), while the hit is still in registers.
 template <int BLOCK, int S, bool PREP>
 __device__ __forceinline__ void extendChunk(const SceneView& sc, const PathView& pv, const FrameConsts* __restrict__ gp,
                                             uint32_t bounce, uint32_t count, uint32_t base, uint2* lds,
-                                            uint32_t prepAbove) {
+                                            bool prep) {
     const uint32_t k = base + laneId();
     bool overflow = false;
     bool hit = false;
@@ -243,7 +259,7 @@ __device__ __forceinline__ void extendChunk(const SceneView& sc, const PathView&
         pv.hitInst[k] = hit ? h.inst : kNoHit;
     }
     if (overflow) atomicOr(pv.counters + kStatusWord, 1u);
-    if (PREP && count > prepAbove) {
+    if (PREP && prep) {  // wave-uniform
         bool guide = false;
         V3 pos01, n;
         if (hit) {
@@ -456,7 +472,7 @@ __device__ __forceinline__ void shadeChunk(const SceneView& sc, const PathView& 
         const float4 ro = pv.rayO[bounce & 1][k], rd = pv.rayD[bounce & 1][k];
         p = __float_as_uint(ro.w);
         o = shadePath<GUIDE>(sc, pv, g, bounce, p, v3(rd), pv.hit[k], pv.hitInst[k],
-                             GUIDE ? pv.guideIdx[k] : kNoHit);
+                             GUIDE && pv.guideIdx ? pv.guideIdx[k] : kNoHit);
     }
     const uint32_t rs = waveAppend(next + kCntRays, o.emitRay);
     if (o.emitRay) {
@@ -471,42 +487,21 @@ __device__ __forceinline__ void shadeChunk(const SceneView& sc, const PathView& 
     }
 }
 
-// Guided shading of one engine's frame (guided requests are never merged
-// into a shared batch: each has its own network).
-// Skipped when the level holds at most tailPaths rays: the tail kernel then
-// finishes the frame, shading those vertices from the BSDF alone (unguided
-// one-sample MIS with selection probability 0, still unbiased).
-__global__ __launch_bounds__(256) void kShadeGuided(SceneView sc, PathView pv, const FrameConsts* __restrict__ gp,
-                                                    uint32_t bounce, uint32_t tailPaths) {
-    const FrameConsts g = *gp;
-    uint32_t* cnt = pv.counters + bounce * kCntPerBounce;
-    const uint32_t count = cnt[kCntRays];
-    if (count <= tailPaths) return;
-    while (true) {
-        const uint32_t base = waveFetch(cnt + kCntShadeFetch);
-        if (base >= count) break;
-        shadeChunk<true>(sc, pv, g, bounce, count, base);
-    }
-}
-
-// Items of a stage over all requests of a batch (wave-uniform).
-__device__ __forceinline__ uint32_t batchItems(const BatchChunks& bc) {
-    uint32_t n = bc.n;
-    for (int off = 32; off >= 1; off >>= 1) n += __shfl_xor(n, off, 64);
-    return __builtin_amdgcn_readfirstlane(n);
-}
-
-// Plain shading of every active request of a batched frame, unless the
-// tail kernel takes this level over (all requests together hold at most
-// tailPaths rays here).
+// Shading of every active request of a batched frame at level `bounce`,
+// except the requests the tail kernel serves there.  GUIDE: requests with
+// guiding buffers shade with their learned mixtures (the query records and
+// mixtures of this level are ready); plain requests in the same batch shade
+// exactly as the plain kernel does (no mixture: the same operations and
+// random numbers).  Vertices the tail kernel shades are unguided (one-sample
+// MIS with selection probability 0, still unbiased).
+template <bool GUIDE>
 __global__ __launch_bounds__(256) void kShadeBatch(BatchView bv, uint32_t bounce, uint32_t tailPaths) {
-    const BatchChunks bc = batchChunks(bv, bounce, kCntRays);
-    if (batchItems(bc) <= tailPaths) return;
+    const BatchChunks bc = batchChunks(bv, bounce, kCntRays, tailPaths, kPartLevel);
     uint32_t r, count, base;
     while (batchNext(bc, bv.cursors + bounce * 4 + 1, r, count, base)) {
         const BatchEntry& e = bv.entries[r];
         const FrameConsts g = *e.g;
-        shadeChunk<false>(e.sc, e.pv, g, bounce, count, base);
+        shadeChunk<GUIDE>(e.sc, e.pv, g, bounce, count, base);
     }
 }
 
@@ -555,10 +550,12 @@ __device__ __forceinline__ void shadowChunk(const SceneView& sc, const PathView&
 // [0, extend chunks) of the shared cursor are closest hit, the rest any hit.
 template <int BLOCK, int S, bool PREP>
 __global__ __launch_bounds__(BLOCK) void kTraceBatch(BatchView bv, uint32_t extendLevel, uint32_t shadowLevel,
-                                                     uint32_t prepAbove) {
+                                                     uint32_t tailPaths) {
     __shared__ uint2 ldsStack[S * BLOCK];
-    const BatchChunks be = extendLevel != kNoLevel ? batchChunks(bv, extendLevel, kCntRays) : BatchChunks{ 0, 0, 0 };
-    const BatchChunks bs = shadowLevel != kNoLevel ? batchChunks(bv, shadowLevel, kCntShadows) : BatchChunks{ 0, 0, 0 };
+    const BatchChunks be = extendLevel != kNoLevel ? batchChunks(bv, extendLevel, kCntRays, tailPaths)
+                                                   : BatchChunks{ 0, 0, 0, false };
+    const BatchChunks bs = shadowLevel != kNoLevel ? batchChunks(bv, shadowLevel, kCntShadows)
+                                                   : BatchChunks{ 0, 0, 0, false };
     if (be.total == 0 && bs.total == 0) return;  // levels the tail kernel finished: no atomics
     uint32_t* cursor = bv.cursors + (extendLevel != kNoLevel ? extendLevel * 4 + 0 : shadowLevel * 4 + 2);
     while (true) {
@@ -567,8 +564,9 @@ __global__ __launch_bounds__(BLOCK) void kTraceBatch(BatchView bv, uint32_t exte
         if (chunk < be.total) {
             batchLocate(be, chunk, r, count, base);
             const BatchEntry& e = bv.entries[r];
-            extendChunk<BLOCK, S, PREP>(e.sc, e.pv, e.g, extendLevel, count, base, ldsStack + threadIdx.x,
-                                        prepAbove);
+            // query records for guided requests whose level the tail kernel does not take
+            const bool prep = PREP && e.pv.guideIdx != nullptr && !__shfl(int(be.tail), int(r), 64);
+            extendChunk<BLOCK, S, PREP>(e.sc, e.pv, e.g, extendLevel, count, base, ldsStack + threadIdx.x, prep);
         } else if (chunk - be.total < bs.total) {
             batchLocate(bs, chunk - be.total, r, count, base);
             const BatchEntry& e = bv.entries[r];
@@ -627,17 +625,16 @@ __device__ __forceinline__ void tailChunk(const SceneView& sc, const PathView& p
     if (overflow) atomicOr(pv.counters + kStatusWord, 1u);
 }
 
-// Runs when all requests together hold 1..tailPaths rays at level `bounce`
-// (kShadeBatch runs otherwise).  Held to at most 168 VGPRs (3 waves per SIMD,
+// Serves the requests batchChunks assigns to the tail at level `bounce`
+// (kShadeBatch serves the others).  Held to at most 168 VGPRs (3 waves per SIMD,
 // a few spills): measured faster than the compiler's 183 at 2 waves.
 template <int BLOCK, int S>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void kTailBatch(BatchView bv,
                                                                                             uint32_t bounce,
                                                                                             uint32_t tailPaths) {
     __shared__ uint2 ldsStack[S * BLOCK];
-    const BatchChunks bc = batchChunks(bv, bounce, kCntRays);
-    const uint32_t items = batchItems(bc);
-    if (items == 0 || items > tailPaths) return;
+    const BatchChunks bc = batchChunks(bv, bounce, kCntRays, tailPaths, kPartTail);
+    if (bc.total == 0) return;
     uint32_t r, count, base;
     while (batchNext(bc, bv.cursors + bounce * 4 + 3, r, count, base)) {
         const BatchEntry& e = bv.entries[r];
@@ -728,12 +725,6 @@ hipError_t launchClearWords(uint32_t* words, uint32_t n, hipStream_t s) {
 
 #define MRT_PT_SHAPES(X) X(64, 16) X(128, 8) X(128, 16) X(128, 32) X(256, 8) X(256, 16)
 
-hipError_t launchShadeGuided(const SceneView& sc, const PathView& pv, const FrameConsts* g, uint32_t bounce,
-                             uint32_t tailPaths, uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL(kShadeGuided, dim3(grid), dim3(256), 0, s, sc, pv, g, bounce, tailPaths);
-    return hipGetLastError();
-}
-
 hipError_t launchResolve32(const PathView& pv, uint32_t pixels, uint32_t spp, hipStream_t s) {
     hipLaunchKernelGGL(kResolve32, dim3(gridFor(pixels, 256, 8192)), dim3(256), 0, s, pv, pixels, spp);
     return hipGetLastError();
@@ -742,11 +733,11 @@ hipError_t launchResolve32(const PathView& pv, uint32_t pixels, uint32_t spp, hi
 // ------------------------------------------------------------ batched frame --
 template <int B, int S>
 static hipError_t traceBatchT(const BatchView& bv, uint32_t extendLevel, uint32_t shadowLevel, uint32_t grid,
-                              bool prep, uint32_t prepAbove, hipStream_t s) {
+                              bool prep, uint32_t tailPaths, hipStream_t s) {
     if (prep)
-        hipLaunchKernelGGL((kTraceBatch<B, S, true>), dim3(grid), dim3(B), 0, s, bv, extendLevel, shadowLevel, prepAbove);
+        hipLaunchKernelGGL((kTraceBatch<B, S, true>), dim3(grid), dim3(B), 0, s, bv, extendLevel, shadowLevel, tailPaths);
     else
-        hipLaunchKernelGGL((kTraceBatch<B, S, false>), dim3(grid), dim3(B), 0, s, bv, extendLevel, shadowLevel, 0u);
+        hipLaunchKernelGGL((kTraceBatch<B, S, false>), dim3(grid), dim3(B), 0, s, bv, extendLevel, shadowLevel, tailPaths);
     return hipGetLastError();
 }
 
@@ -762,21 +753,25 @@ hipError_t launchGenerateBatch(const BatchView& bv, hipStream_t s) {
 }
 
 hipError_t launchTraceBatch(const BatchView& bv, uint32_t extendLevel, uint32_t shadowLevel, const TraceShape& shape,
-                            hipStream_t s, bool guidePrep, uint32_t prepAbove) {
+                            uint32_t tailPaths, bool guidePrep, hipStream_t s) {
     if (extendLevel == kNoLevel && shadowLevel == kNoLevel) return hipSuccess;
     if ((extendLevel != kNoLevel && extendLevel >= kMaxBounceLevels) ||
         (shadowLevel != kNoLevel && shadowLevel >= kMaxBounceLevels))
         return hipErrorInvalidValue;
 #define X(B, S)                                                                                  \
     if (shape.block == B && shape.stack == S)                                                     \
-        return traceBatchT<B, S>(bv, extendLevel, shadowLevel, shape.grid, guidePrep, prepAbove, s);
+        return traceBatchT<B, S>(bv, extendLevel, shadowLevel, shape.grid, guidePrep, tailPaths, s);
     MRT_PT_SHAPES(X)
 #undef X
     return hipErrorInvalidValue;
 }
 
-hipError_t launchShadeBatch(const BatchView& bv, uint32_t bounce, uint32_t tailPaths, uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL(kShadeBatch, dim3(grid), dim3(256), 0, s, bv, bounce, tailPaths);
+hipError_t launchShadeBatch(const BatchView& bv, uint32_t bounce, uint32_t tailPaths, bool guided, uint32_t grid,
+                            hipStream_t s) {
+    if (guided)
+        hipLaunchKernelGGL(kShadeBatch<true>, dim3(grid), dim3(256), 0, s, bv, bounce, tailPaths);
+    else
+        hipLaunchKernelGGL(kShadeBatch<false>, dim3(grid), dim3(256), 0, s, bv, bounce, tailPaths);
     return hipGetLastError();
 }
 
@@ -809,8 +804,9 @@ hipError_t launchFinishBatch(const BatchView& bv, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t batchShadeBlocksPerCU(int* blocksPerCU) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kShadeBatch, 256, 0);
+hipError_t batchShadeBlocksPerCU(bool guided, int* blocksPerCU) {
+    return guided ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kShadeBatch<true>, 256, 0)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kShadeBatch<false>, 256, 0);
 }
 
 hipError_t batchTraceBlocksPerCU(uint32_t block, uint32_t stack, bool guidePrep, int* blocksPerCU) {
@@ -830,9 +826,6 @@ bool ptShapeSupported(uint32_t block, uint32_t stack) {
     return false;
 }
 
-hipError_t guidedShadeBlocksPerCU(int* blocksPerCU) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, kShadeGuided, 256, 0);
-}
 
 } // namespace dev
 } // namespace mrt

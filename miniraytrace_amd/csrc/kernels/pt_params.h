@@ -74,7 +74,8 @@ struct BatchEntry {
     uint32_t pixels;                 // width * height
     uint32_t pixelBase;              // first slot in the batch pixel space (multiple of 64)
     uint32_t active;                 // 1 when the request renders this frame
-    uint32_t pad;
+    uint32_t tailAt;                 // the tail kernel serves this request at a level holding at
+                                     // most this many of its rays (~0u: no limit of its own)
 };
 
 constexpr uint32_t kMaxBatchRequests = 64;  // one wave lane per request in the work split

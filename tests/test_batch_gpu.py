@@ -69,13 +69,14 @@ def test_batch_equals_one_at_a_time(gpu):
         assert np.array_equal(a.read_rgba32f(r["width"], r["height"]), b.read_rgba32f(r["width"], r["height"]))
 
 
-def test_batch_chain_mode_equals_one_at_a_time(gpu):
-    """A request with guiding training on takes the per-request-chain mode;
+def test_batch_with_guided_request_equals_one_at_a_time(gpu):
+    """A request with guiding training on joins the merged batch (its query
+    records, and training after each frame, run inside the shared graph);
     training without guided sampling leaves images as plain, so they still
     match one-at-a-time rendering."""
     from miniraytrace_amd import scenes
     from miniraytrace_amd.batch import render_batch
-    _print("case: batch chain mode")
+    _print("case: batch with a guided (training-only) request")
     sky = scenes.sky(256, 128)
     reqs = REQS[:3]
     batch = [_engine(r, sky) for r in reqs]
@@ -83,12 +84,64 @@ def test_batch_chain_mode_equals_one_at_a_time(gpu):
     for e in (batch[1], single[1]):
         e.set_guiding(sampling=False, training=True, batch_size=1024, steps_per_frame=1)
     st = render_batch(batch, [r["spp"] for r in reqs])
-    assert st["mode"] == 1 and st["framesDone"] == [r["spp"] for r in reqs]
+    assert st["mode"] == 0 and st["framesDone"] == [r["spp"] for r in reqs]
     for e, r in zip(single, reqs):
         for _ in range(r["spp"]):
             e.render_frame()
     for a, b, r in zip(batch, single, reqs):
         assert np.array_equal(a.read_rgba32f(r["width"], r["height"]), b.read_rgba32f(r["width"], r["height"]))
+    assert batch[1].guiding_stats()["steps"] == single[1].guiding_stats()["steps"] > 0
+
+
+def test_batch_guided_sampling_runs_merged(gpu):
+    """Guided sampling in the merged batch: every request finishes its frames
+    with finite images whose mean matches plain rendering (unbiased) within
+    the noise of a few frames."""
+    from miniraytrace_amd import scenes
+    from miniraytrace_amd.batch import render_batch
+    _print("case: batch with guided sampling")
+    sky = scenes.sky(256, 128)
+    r = REQS[2]
+    g1, g2, plain = _engine(r, sky), _engine(r, sky), _engine(r, sky)
+    for e in (g1, g2):
+        e.set_guiding(sampling=True, training=True, batch_size=4096, steps_per_frame=2)
+    st = render_batch([g1, g2, plain], [32, 32, 32])
+    assert st["mode"] == 0 and st["framesDone"] == [32, 32, 32]
+    imgs = [e.read_rgba32f(r["width"], r["height"])[..., :3].astype(np.float64) for e in (g1, g2, plain)]
+    for im in imgs:
+        assert np.isfinite(im).all() and im.mean() > 0
+    assert abs(imgs[0].mean() / imgs[2].mean() - 1.0) < 0.05
+    assert abs(imgs[1].mean() / imgs[2].mean() - 1.0) < 0.05
+
+
+def test_batch_chain_mode_equals_one_at_a_time(gpu):
+    """More requests than one merged batch holds (64) run as per-request graph
+    chains in one launch; images equal one-at-a-time rendering."""
+    from miniraytrace_amd import scenes
+    from miniraytrace_amd.batch import render_batch
+    from miniraytrace_amd.engine import Engine
+    _print("case: batch chain mode (65 requests)")
+    sky = scenes.sky(64, 32)
+
+    def small(k):
+        e = Engine(width=16, height=16)
+        e.set_render_settings(width=16, height=16, max_bounces=4, spp_limit=1 << 20, firefly_clamp=0.0,
+                              denoise=False)
+        scenes.load_terrain_scene(e, scenes.terrain_config(2_000), seed=2000 + k, sky_img=sky)
+        e.commit()
+        return e
+
+    batch = [small(k) for k in range(65)]
+    single = [small(k) for k in range(65)]
+    st = render_batch(batch, [2] * 65)
+    assert st["mode"] == 1 and st["framesDone"] == [2] * 65
+    for e in single:
+        for _ in range(2):
+            e.render_frame()
+    for a, b in zip(batch, single):
+        assert np.array_equal(a.read_rgba32f(16, 16), b.read_rgba32f(16, 16))
+    for e in batch + single:
+        e.close()
 
 
 def test_batch_argument_checks(gpu):
