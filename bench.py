@@ -260,6 +260,28 @@ def _room_engine(W, H, seed, sky):
     return e, info
 
 
+# Guiding network FLOPs (78 -> 64 -> 64 -> 64 -> 64 -> 40): forward 2 * sum of
+# in x out = 39,680 per query; backward 39,680 for the weight gradients plus
+# 2 * (3 * 64 * 64 + 64 * 40) = 29,696 for the deltas (no input gradient).
+MLP_FWD_FLOP = 2 * (78 * 64 + 3 * 64 * 64 + 64 * 40)
+MLP_TRAIN_FLOP = MLP_FWD_FLOP + MLP_FWD_FLOP + 2 * (3 * 64 * 64 + 64 * 40)
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def _mlp_roofline(rows, infer_ms, train_samples, train_ms) -> dict:
+    """MFMA view of one guided frame: the network inference (all levels'
+    query rows) and the training steps, against the dense BF16 peak.  The
+    stage times include the launches' non-MFMA work (encoding, decoding,
+    record collection, Adam), so these are lower bounds on MFMA use."""
+    infer = rows * MLP_FWD_FLOP / (infer_ms * 1e-3) / 1e12 if infer_ms > 0 else None
+    train = train_samples * MLP_TRAIN_FLOP / (train_ms * 1e-3) / 1e12 if train_ms > 0 else None
+    return {"bound": "mfma", "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "infer_rows_per_frame": rows, "infer_flop_per_row": MLP_FWD_FLOP, "infer_ms": infer_ms,
+            "infer_achieved": infer, "infer_frac": infer / MFMA_BF16_PEAK_TFLOPS if infer else None,
+            "train_samples_per_frame": train_samples, "train_flop_per_sample": MLP_TRAIN_FLOP, "train_ms": train_ms,
+            "train_achieved": train, "train_frac": train / MFMA_BF16_PEAK_TFLOPS if train else None}
+
+
 def _timed_frames(e, n, torch) -> float:
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -315,10 +337,11 @@ def guided_workload(args, torch) -> dict:
         curve.append((done, t_g, float(((img_g - img_r) ** 2).mean())))
         if curve[-1][2] <= mse_p and t_g >= t_plain:
             break
-    st = g.guiding_stats()
     g.set_profiling(True)
     g.render_frame()
     prof = g.frame_stats()
+    st = g.guiding_stats()
+    opts = g.guiding()
     g.close()
 
     def crossing(key_idx, target, out_idx):
@@ -349,6 +372,8 @@ def guided_workload(args, torch) -> dict:
         "guided_stage_ms": prof["gpuMs"],
         "train_steps": st["steps"],
         "last_loss": st["lastLoss"],
+        "mlp": _mlp_roofline(st["lastFrameRows"], prof["gpuMs"]["guide"], st["lastBatch"] * opts["stepsPerFrame"],
+                             prof["gpuMs"]["train"]),
     }
     _log(f"guided: plain {spp} spp in {t_plain:.2f} s (RMSE {out['plain_rmse']:.4g}); guided reaches it in "
          f"{t_equal_rmse} s -> ratio {out['time_ratio']}; equal-time RMSE ratio {out['equal_time_rmse_ratio']}")

@@ -251,6 +251,7 @@ typedef struct mrtxGuidingStats {
     float    lastLoss;        /* weighted negative log-likelihood of that batch */
     uint32_t poolRecords;     /* training records collected in the last frame */
     uint32_t candidates;      /* positive-weight path records found in the last frame */
+    uint32_t lastFrameRows;   /* network queries (inferred rows) of the last profiled frame */
 } mrtxGuidingStats;
 
 /* Changing the options re-captures the frame graph; it does not reset the

@@ -163,7 +163,8 @@ class mrtxGuidingOptions(_Sized):
 
 class mrtxGuidingStats(_Sized):
     _fields_ = [("structSize", C.c_size_t), ("steps", C.c_uint32), ("lastBatch", C.c_uint32),
-                ("lastLoss", C.c_float), ("poolRecords", C.c_uint32), ("candidates", C.c_uint32)]
+                ("lastLoss", C.c_float), ("poolRecords", C.c_uint32), ("candidates", C.c_uint32),
+                ("lastFrameRows", C.c_uint32)]
 
 
 class mrtxSceneExport(_Sized):

@@ -951,7 +951,9 @@ void Engine::renderFrame(mrtFrameInfo* out) {
                                    hipMemcpyDeviceToHost, stream_));
             MRT_HIP(hipStreamSynchronize(stream_));
             const uint32_t* c = static_cast<const uint32_t*>(hCounters_.get());
+            stats_.guideRows = 0;
             for (uint32_t b = 0; b < dev::kMaxBounceLevels; ++b) {
+                stats_.guideRows += c[b * dev::kCntPerBounce + dev::kCntGuideRecords];
                 stats_.rays[b] = c[b * dev::kCntPerBounce + dev::kCntRays] +
                                  c[b * dev::kCntPerBounce + dev::kCntTailRays];
                 stats_.shadowRays[b] = c[b * dev::kCntPerBounce + dev::kCntShadows] +
@@ -1178,6 +1180,7 @@ void Engine::guidingStats(mrtxGuidingStats& o) {
     std::memcpy(&o.lastLoss, &w[dev::kGwLastLoss], sizeof(float));
     o.poolRecords = std::min(w[dev::kGwPool], guide_.batchSize * guide_.stepsPerFrame);
     o.candidates = w[dev::kGwCand0 + (guideStamp_ & 1u)];
+    o.lastFrameRows = stats_.guideRows;
 }
 
 void Engine::getGuideParams(float* dst) {

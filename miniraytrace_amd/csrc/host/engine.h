@@ -46,10 +46,11 @@ struct Settings {
 
 struct FrameStats {
     float frameMs = 0.0f;
-    float stageMs[8] = {};  // generate, extend, shade, shadow, accumulate + resolve, (0), guide, train
+    float stageMs[8] = {};  // generate, extend, shade, shadow, accumulate + resolve, tail, guide, train
     uint32_t rays[dev::kMaxBounceLevels] = {};
     uint32_t shadowRays[dev::kMaxBounceLevels] = {};
     uint32_t status = 0;
+    uint32_t guideRows = 0;  // guiding-network query records of the frame (all levels)
 };
 
 class Engine {
