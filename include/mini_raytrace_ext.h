@@ -237,7 +237,7 @@ typedef struct mrtxGuidingOptions {
     float    selectProb;      /* probability of a mixture draw at a guided vertex (0.5) */
     float    learningRate;    /* Adam step size (1e-3) */
     uint32_t batchSize;       /* training records per Adam step (65,536) */
-    uint32_t stepsPerFrame;   /* Adam steps per rendered frame (4) */
+    uint32_t stepsPerFrame;   /* Adam steps per rendered frame (2) */
     float    minRoughness;    /* less rough or transmissive vertices use the BSDF only (0.2) */
     uint32_t seed;            /* network initialisation seed (used when the network is created) */
     float    positionScale;   /* network position input = (p - scene box min) / largest box extent

@@ -145,7 +145,7 @@ Engine::Engine(const mrtEngineDesc& desc) {
     guide_.selectProb = 0.5f;
     guide_.learningRate = 1e-3f;
     guide_.batchSize = 65536;
-    guide_.stepsPerFrame = 4;
+    guide_.stepsPerFrame = 2;
     guide_.minRoughness = 0.2f;
     guide_.seed = 1;
     guide_.positionScale = 1.0f;
@@ -671,10 +671,16 @@ void Engine::allocGuide() {
     dGuideIdx_.allocate(size_t(n) * 4);
     dGuideRec_.allocate(size_t(n) * 3 * 16);
     dGuideMix_.allocate(size_t(n) * dev::kGuideMixFloat4 * 16);
-    // Training paths: about four candidate records per path, so every
-    // stride-th path yields roughly the pool's worth of records per frame.
+    // Training paths: every path of the frame keeps its per-level records
+    // (64 B each) while the path-record buffer stays within kPathRecordBytes;
+    // larger frames record every stride-th path.  A path yields well under
+    // one positive-weight record on average (most paths end early or carry
+    // no radiance), so sparser recording would leave the Adam batches short of
+    // batchSize.  The collection pass samples the pool down to
+    // batchSize x stepsPerFrame records.
     const uint64_t poolCap = uint64_t(guide_.batchSize) * guide_.stepsPerFrame;
-    guideTrainStride_ = uint32_t(std::max<uint64_t>(1, uint64_t(n) * 4 / std::max<uint64_t>(poolCap, 1)));
+    const uint64_t recBytes = uint64_t(n) * levels * 64;
+    guideTrainStride_ = uint32_t(std::max<uint64_t>(1, (recBytes + kPathRecordBytes - 1) / kPathRecordBytes));
     const uint32_t trainPaths = (n + guideTrainStride_ - 1) / guideTrainStride_;
     dPathRec_.allocate(size_t(levels) * trainPaths * 4 * 16);
     dPathRec_.zero(stream_);  // stamp 0 never matches a frame
@@ -1143,7 +1149,7 @@ void Engine::setGuiding(const mrtxGuidingOptions& in) {
     if (o.selectProb == 0.0f) o.selectProb = 0.5f;
     if (o.learningRate == 0.0f) o.learningRate = 1e-3f;
     if (o.batchSize == 0) o.batchSize = 65536;
-    if (o.stepsPerFrame == 0) o.stepsPerFrame = 4;
+    if (o.stepsPerFrame == 0) o.stepsPerFrame = 2;
     if (o.positionScale == 0.0f) o.positionScale = 1.0f;
     if (!(o.positionScale > 0.0f) || !std::isfinite(o.positionScale))
         throw Error(MRT_ERROR_INVALID_ARGUMENT, "guiding positionScale must be > 0");

@@ -33,6 +33,9 @@ constexpr uint32_t kTailStack = 16;
 // (measured on the 1.256M-triangle terrain at 1080p: the tail then covers the
 // levels whose launches are dominated by their slowest rays).
 constexpr uint32_t kTailPathsPerCU = 1280;
+// Upper bound of the guiding path-record buffer (levels x recorded paths x
+// 64 B); 1280x720 at 9 levels records every path within it.
+constexpr uint64_t kPathRecordBytes = 1ull << 30;
 
 struct Settings {
     uint32_t width = 1280, height = 720;

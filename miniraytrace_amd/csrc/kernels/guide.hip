@@ -603,9 +603,13 @@ __device__ __forceinline__ float hashUnit(uint32_t a, uint32_t b, uint32_t c) {
     return float(h >> 8) * (1.0f / 16777216.0f);
 }
 
-// One thread per training path, levels visited in order; every lane of a
-// wave runs the same number of iterations (the wave-wide appends need all
-// lanes).
+// One thread per training path: a lane gathers the levels of its path that
+// hold a record of this frame with positive weight (the candidates) and those
+// kept for the pool, then the workgroup reserves its pool slots with one
+// atomic (prefix sums over lanes and waves give each record its slot).  Every
+// lane of the workgroup runs the same number of iterations (the barriers need
+// all of them); the per-wave totals are double buffered by iteration parity,
+// so one iteration's writes never meet the previous one's reads.
 __global__ __launch_bounds__(256) void kGuideCollect(GuideCollectArgs a) {
     const uint32_t stamp = a.g->guideStamp;
     const uint32_t kStride = a.g->guideTrainStride, phase = a.g->guideTrainPhase;
@@ -613,33 +617,69 @@ __global__ __launch_bounds__(256) void kGuideCollect(GuideCollectArgs a) {
     const uint32_t prev = a.words[kGwCand0 + ((stamp + 1u) & 1u)];
     const float keep = prev > a.poolCap ? float(a.poolCap) / float(prev) : 1.0f;
     const uint32_t stride = gridDim.x * blockDim.x;
+    constexpr uint32_t kWaves = 256 / 64;
+    __shared__ uint32_t sKeep[2][kWaves], sTake[2][kWaves], sBase[2];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t buf = 0;
     for (uint32_t base = blockIdx.x * blockDim.x; base < a.trainPaths; base += stride) {
         const uint32_t q = base + threadIdx.x;
         const uint32_t p = q * kStride + phase;
         const bool valid = q < a.trainPaths && p < a.paths;
         const float lf = valid ? luminance(v3(a.radiance[p])) : 0.0f;
-        for (uint32_t b = 0; b < a.levels; ++b) {
-            const float4* rec = a.pathRec + (size_t(b) * a.trainPaths + (valid ? q : 0u)) * 4;
-            bool take = false;
-            float w = 0.0f;
-            if (valid) {
-                const float4 r2 = rec[2];
-                if (__float_as_uint(r2.w) == stamp) {
-                    w = (lf - rec[1].w) / fmaxf(rec[0].w, 1e-20f);
-                    take = w > 0.0f && isfinite(w);
-                }
+        // Levels of this path with a record of this frame and positive
+        // weight (candidates), and those kept for the pool.
+        uint32_t takeMask = 0, keepMask = 0;
+        if (valid) {
+            for (uint32_t lv = 0; lv < a.levels; ++lv) {
+                const float4* rec = a.pathRec + (size_t(lv) * a.trainPaths + q) * 4;
+                if (__float_as_uint(rec[2].w) != stamp) continue;
+                const float w = (lf - rec[1].w) / fmaxf(rec[0].w, 1e-20f);
+                if (!(w > 0.0f && isfinite(w))) continue;
+                takeMask |= 1u << lv;
+                if (keep >= 1.0f || hashUnit(p, lv, stamp) < keep) keepMask |= 1u << lv;
             }
-            (void)waveAppend(cand, take);
-            const bool kept = take && (keep >= 1.0f || hashUnit(p, b, stamp) < keep);
-            const uint32_t slot = waveAppend(a.words + kGwPool, kept);
-            if (kept && slot < a.poolCap) {
+        }
+        // Inclusive scan of the lanes' kept counts within the wave.
+        const uint32_t nKeep = __builtin_popcount(keepMask);
+        uint32_t incl = nKeep;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t t = __shfl_up(incl, off, 64);
+            if (lane >= uint32_t(off)) incl += t;
+        }
+        uint32_t nTake = __builtin_popcount(takeMask);
+        for (int off = 32; off >= 1; off >>= 1) nTake += __shfl_xor(nTake, off, 64);
+        if (lane == 63) {
+            sKeep[buf][wave] = incl;
+            sTake[buf][wave] = nTake;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t take = 0, kept = 0;
+            for (uint32_t w = 0; w < kWaves; ++w) {
+                take += sTake[buf][w];
+                kept += sKeep[buf][w];
+            }
+            if (take) atomicAdd(cand, take);
+            sBase[buf] = kept ? atomicAdd(a.words + kGwPool, kept) : 0u;
+        }
+        __syncthreads();
+        uint32_t slot = sBase[buf] + incl - nKeep;
+        for (uint32_t w = 0; w < wave; ++w) slot += sKeep[buf][w];
+        buf ^= 1u;
+        while (keepMask) {
+            const uint32_t lv = uint32_t(__builtin_ctz(keepMask));
+            keepMask &= keepMask - 1u;
+            if (slot < a.poolCap) {
+                const float4* rec = a.pathRec + (size_t(lv) * a.trainPaths + q) * 4;
                 const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
+                const float w = (lf - r1.w) / fmaxf(r0.w, 1e-20f);
                 float4* dst = a.pool + size_t(slot) * 4;
                 dst[0] = make_float4(r0.x, r0.y, r0.z, w);
                 dst[1] = make_float4(r1.x, r1.y, r1.z, 0.0f);
                 dst[2] = make_float4(r2.x, r2.y, r2.z, 0.0f);
                 dst[3] = make_float4(r3.x, r3.y, r3.z, 0.0f);
             }
+            ++slot;
         }
     }
 }
