@@ -45,7 +45,8 @@ void GuideNetwork::allocWorkspace(uint32_t cap) {
     raw_.allocate(size_t(dev::kGuideOutputs) * stride * sizeof(float));
     delta_[0].allocate(size_t(64) * stride * sizeof(uint16_t));
     delta_[1].allocate(size_t(64) * stride * sizeof(uint16_t));
-    partial_.allocate(size_t(dev::guideGradChunks(cap)) * dev::kGuideParamCount * sizeof(float));
+    // weight-gradient partials, then the per-chunk weight sums
+    partial_.allocate(size_t(dev::guideGradChunks(cap)) * (dev::kGuideParamCount + 1) * sizeof(float));
     MRT_HIP(hipStreamSynchronize(nullptr));
     cap_ = cap;
     stride_ = stride;
@@ -107,6 +108,7 @@ dev::GuideNet GuideNetwork::view() const {
     n.delta[0] = delta_[0].as<uint16_t>();
     n.delta[1] = delta_[1].as<uint16_t>();
     n.partial = partial_.as<float>();
+    n.weightPart = n.partial + size_t(dev::guideGradChunks(cap_)) * dev::kGuideParamCount;
     n.words = words_.as<uint32_t>();
     n.cap = cap_;
     n.stride = stride_;

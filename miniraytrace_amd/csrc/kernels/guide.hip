@@ -20,8 +20,7 @@
 // Kernels:
 //   kGuidePack        FP32 params -> packed bf16 forward image
 //   kGuideInfer       fused forward for rendering (features -> mixture)
-//   kGuideStepBegin   batch size and counters of one training step
-//   kGuideWeightSum   sum of the batch's training weights (fixed order)
+//   kGuideStepBegin   batch size, counters and per-chunk weight sums of one step
 //   kGuideFwdTrain    forward over the batch, keeping the activations
 //   kGuideOutGrad     d(weighted NLL)/d(raw outputs) per row
 //   kGuideBackLayer   delta_{l-1} = (W_l^T delta_l) * relu'(a_{l-1})
@@ -299,32 +298,55 @@ __global__ __launch_bounds__(kFwdWaves * 64) void kGuideInfer(RecView rv, const 
 }
 
 // ------------------------------------------------------------------- training --
-__global__ void kGuideStepBegin(GuideBatch b, uint32_t* __restrict__ words, int countStep) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const uint32_t n = batchCount(b);
-    words[kGwBatch] = n;
-    *wordF(words, kGwWeightSum) = 0.0f;
-    *wordF(words, kGwLoss) = 0.0f;
-    if (countStep && n > 0) words[kGwStep] += 1u;
-}
-
-// Sum of the batch's weights in a fixed order (one block, LDS tree): the
-// sum scales every delta before its bf16 rounding, so an order-dependent sum
-// would make the gradient differ from run to run.
-constexpr int kWeightSumBlock = 1024;
-__global__ __launch_bounds__(kWeightSumBlock) void kGuideWeightSum(GuideBatch b, uint32_t* __restrict__ words) {
+// Start of a training step.  Block 0 publishes the batch size, clears the
+// loss and advances the Adam step counter (countStep, non-empty batch); the
+// later kernels of the step read them.  Every block sums the weights of its
+// row chunk in a fixed order: block k sums chunk k (kGuideGradChunk rows, four
+// per thread, then an LDS tree) into weightPart[k], and kGuideOutGrad adds
+// the chunk sums in index order.  The sum scales every delta before its bf16
+// rounding, so an order-dependent sum would make the gradient differ from run
+// to run.
+constexpr int kWeightSumBlock = 256;
+__global__ __launch_bounds__(kWeightSumBlock) void kGuideStepBegin(GuideBatch b, uint32_t* __restrict__ words,
+                                                                   float* __restrict__ weightPart, int countStep) {
     __shared__ float part[kWeightSumBlock];
-    const uint32_t n = words[kGwBatch];
+    const uint32_t n = batchCount(b);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        words[kGwBatch] = n;
+        *wordF(words, kGwLoss) = 0.0f;
+        if (countStep && n > 0) words[kGwStep] += 1u;
+    }
     const RecView rv = batchView(b);
+    const uint32_t row0 = blockIdx.x * kGuideGradChunk + threadIdx.x;
     float s = 0.0f;
-    for (uint32_t i = threadIdx.x; i < n; i += kWeightSumBlock) s += recRow(rv, i)[0].w;
+    for (uint32_t i = 0; i < kGuideGradChunk / kWeightSumBlock; ++i) {
+        const uint32_t row = row0 + i * kWeightSumBlock;
+        if (row < n) s += recRow(rv, row)[0].w;
+    }
     part[threadIdx.x] = s;
     __syncthreads();
     for (int half = kWeightSumBlock / 2; half > 0; half >>= 1) {
         if (threadIdx.x < half) part[threadIdx.x] += part[threadIdx.x + half];
         __syncthreads();
     }
-    if (threadIdx.x == 0) *wordF(words, kGwWeightSum) = part[0];
+    if (threadIdx.x == 0) weightPart[blockIdx.x] = part[0];
+}
+
+// Sum of the chunk weight sums in index order, the same in every block
+// (threads add strided slices in order, then an LDS tree).
+__device__ float batchWeightSum(const float* __restrict__ weightPart, uint32_t chunks) {
+    __shared__ float part[256];
+    float s = 0.0f;
+    for (uint32_t i = threadIdx.x; i < chunks; i += 256) s += weightPart[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int half = 128; half > 0; half >>= 1) {
+        if (threadIdx.x < half) part[threadIdx.x] += part[threadIdx.x + half];
+        __syncthreads();
+    }
+    const float total = part[0];
+    __syncthreads();
+    return total;
 }
 
 // Forward over the batch's padded rows [0, pad32(count)): keeps the encoded
@@ -368,15 +390,18 @@ __global__ __launch_bounds__(kFwdWaves * 64) void kGuideFwdTrain(GuideBatch b, c
 
 // delta (feature-major bf16, 64 features, 40..63 zero) = d L / d raw with
 // L = sum_j w_j (-log q(wi_j)) / sum_j w_j; padding rows get zero deltas.
-__global__ __launch_bounds__(256) void kGuideOutGrad(GuideBatch b, uint32_t* __restrict__ words, GuideNet net) {
+__global__ __launch_bounds__(256) void kGuideOutGrad(GuideBatch b, uint32_t* __restrict__ words, GuideNet net,
+                                                     uint32_t chunks) {
     const uint32_t count = words[kGwBatch];
     const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t stride = net.stride;
+    const float wsum = batchWeightSum(net.weightPart, chunks);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *wordF(words, kGwWeightSum) = wsum;
     float loss = 0.0f;
     if (row < pad32(count)) {
         uint16_t* dcol = net.delta[0] + row;
         if (row < count) {
-            const float scale = 1.0f / fmaxf(*wordF(words, kGwWeightSum), 1e-30f);
+            const float scale = 1.0f / fmaxf(wsum, 1e-30f);
             const float4* q = recRow(batchView(b), row);
             const float wgt = q[0].w * scale;
             const V3 wi = v3(q[3]);
@@ -506,7 +531,25 @@ __global__ __launch_bounds__(256) void kGuideWeightGrad(int l, const uint32_t* _
     const uint32_t one2 = 0x3F803F80u;  // two bf16 1.0
     const bf16x8 ones = __builtin_bit_cast(bf16x8, u32x4{ one2, one2, one2, one2 });
     f32x16 acc = zero16(), accB = zero16();
-    for (uint32_t r = s0; r < s1; r += 16) {
+    // Eight row steps' operands are loaded before their products, so a wave
+    // waits once per eight steps instead of once per step; the products are
+    // accumulated in row order either way.
+    constexpr int kAhead = 8;
+    uint32_t r = s0;
+    for (; r + 16 * kAhead <= s1; r += 16 * kAhead) {
+        bf16x8 a[kAhead], b[kAhead];
+#pragma unroll
+        for (int u = 0; u < kAhead; ++u) {
+            a[u] = ld8(aRow + r + 16 * u);
+            b[u] = ld8(bRow + r + 16 * u);
+        }
+#pragma unroll
+        for (int u = 0; u < kAhead; ++u) {
+            acc = mfma(a[u], b[u], acc);
+            if (nt == 0) accB = mfma(a[u], ones, accB);
+        }
+    }
+    for (; r < s1; r += 16) {
         const bf16x8 a = ld8(aRow + r);
         acc = mfma(a, ld8(bRow + r), acc);
         if (nt == 0) accB = mfma(a, ones, accB);
@@ -530,14 +573,21 @@ __global__ __launch_bounds__(256) void kGuideWeightGrad(int l, const uint32_t* _
         }
 }
 
+// 64 parameters per block; thread (q, j) sums chunks q, q + 4, ... of
+// parameter j in order, then the four slices are added in q order.
+constexpr int kReduceParams = 64;
 __global__ __launch_bounds__(256) void kGuideGradReduce(const uint32_t* __restrict__ words,
                                                         const float* __restrict__ partial, float* __restrict__ grad) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= kGuideParams) return;
+    __shared__ float slice[4][kReduceParams];
+    const int j = threadIdx.x & (kReduceParams - 1), q = threadIdx.x / kReduceParams;
+    const int i = blockIdx.x * kReduceParams + j;
     const uint32_t chunks = (pad32(words[kGwBatch]) + kGuideGradChunk - 1) / kGuideGradChunk;
     float s = 0.0f;
-    for (uint32_t c = 0; c < chunks; ++c) s += partial[size_t(c) * kGuideParams + i];
-    grad[i] = s;
+    if (i < kGuideParams)
+        for (uint32_t c = q; c < chunks; c += 4) s += partial[size_t(c) * kGuideParams + i];
+    slice[q][j] = s;
+    __syncthreads();
+    if (q == 0 && i < kGuideParams) grad[i] = ((slice[0][j] + slice[1][j]) + slice[2][j]) + slice[3][j];
 }
 
 __global__ __launch_bounds__(256) void kGuideAdam(float* __restrict__ params, float* __restrict__ m,
@@ -709,16 +759,16 @@ hipError_t launchGuideGradient(const GuideBatch& b, const GuideNet& net, bool co
     if (b.cap > net.cap || b.cap == 0 || net.stride != guideRowStride(net.cap)) return hipErrorInvalidValue;
     const uint32_t rows = guideRowStride(b.cap);
     hipError_t e;
-    hipLaunchKernelGGL(kGuideStepBegin, dim3(1), dim3(64), 0, s, b, net.words, countStep ? 1 : 0);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(kGuideWeightSum, dim3(1), dim3(kWeightSumBlock), 0, s, b, net.words);
+    const uint32_t chunks = guideGradChunks(b.cap);
+    hipLaunchKernelGGL(kGuideStepBegin, dim3(chunks), dim3(kWeightSumBlock), 0, s, b, net.words, net.weightPart,
+                       countStep ? 1 : 0);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(kGuideFwdTrain, dim3(cappedGrid(rows, kFwdWaves * 32, 1024)), dim3(kFwdWaves * 64), 0, s, b,
                        net.words, static_cast<const u32x4*>(net.image), net);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(kGuideOutGrad, dim3(cappedGrid(rows, 256, 1u << 20)), dim3(256), 0, s, b, net.words, net);
+    hipLaunchKernelGGL(kGuideOutGrad, dim3(cappedGrid(rows, 256, 1u << 20)), dim3(256), 0, s, b, net.words, net,
+                       chunks);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    const uint32_t chunks = guideGradChunks(b.cap);
     int cur = 0;
     for (int l = 4; l >= 0; --l) {
         const uint16_t* prev = l == 0 ? net.actIn : net.act + size_t(l - 1) * 64 * net.stride;
@@ -733,8 +783,8 @@ hipError_t launchGuideGradient(const GuideBatch& b, const GuideNet& net, bool co
             cur ^= 1;
         }
     }
-    hipLaunchKernelGGL(kGuideGradReduce, dim3((kGuideParams + 255) / 256), dim3(256), 0, s, net.words, net.partial,
-                       net.grad);
+    hipLaunchKernelGGL(kGuideGradReduce, dim3((kGuideParams + kReduceParams - 1) / kReduceParams), dim3(256), 0, s,
+                       net.words, net.partial, net.grad);
     return hipGetLastError();
 }
 

@@ -77,6 +77,7 @@ struct GuideNet {
     float* raw = nullptr;        // FP32 [40][stride] raw outputs
     uint16_t* delta[2] = { nullptr, nullptr };  // bf16 [64][stride] back-propagated error
     float* partial = nullptr;    // FP32 [guideGradChunks(cap)][kGuideParamCount]
+    float* weightPart = nullptr; // FP32 [guideGradChunks(cap)]: batch weight sum per row chunk
     uint32_t* words = nullptr;   // kGwWords
     uint32_t cap = 0;
     uint32_t stride = 0;

@@ -548,8 +548,11 @@ __device__ __forceinline__ void shadowChunk(const SceneView& sc, const PathView&
 // hits only write hit records -- so the slowest rays of one overlap with the
 // other's work instead of each ending a launch of its own.  Chunks
 // [0, extend chunks) of the shared cursor are closest hit, the rest any hit.
+// Register budget: 6 waves per SIMD (80 VGPRs).  Traversal waits on
+// dependent node fetches, and the sixth wave hides more of that latency than
+// the registers it costs; at 8 waves (64 VGPRs) the spills outweigh it.
 template <int BLOCK, int S, bool PREP>
-__global__ __launch_bounds__(BLOCK) void kTraceBatch(BatchView bv, uint32_t extendLevel, uint32_t shadowLevel,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void kTraceBatch(BatchView bv, uint32_t extendLevel, uint32_t shadowLevel,
                                                      uint32_t tailPaths) {
     __shared__ uint2 ldsStack[S * BLOCK];
     const BatchChunks be = extendLevel != kNoLevel ? batchChunks(bv, extendLevel, kCntRays, tailPaths)
