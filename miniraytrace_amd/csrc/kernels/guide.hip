@@ -90,20 +90,34 @@ __device__ __forceinline__ RecView batchView(const GuideBatch& b) {
 }
 
 // ------------------------------------------------------------ forward image --
-// W_l output-major, layer 0 rows padded to 80 inputs, the output layer padded
-// to 64 rows: 21,504 bf16, then 5 x 64 FP32 biases.
-constexpr int kWOff1 = 64 * kGuideInPad;
-constexpr int kWTotal = kWOff1 + 4 * 64 * 64;
-constexpr int kImageU4 = (kWTotal * 2 + 320 * 4) / 16;  // 2,768
-constexpr int kImageWU4 = kWTotal * 2 / 16;              // 2,688
+// W_l output-major, 64 rows per layer (the output layer's 40 padded with
+// zeros), then 5 x 64 FP32 biases.  The image is copied to LDS as it is, and
+// its row strides (88 bf16 for layer 0's 80 inputs, 72 for the 64 of the
+// others; 176 and 144 bytes) spread the 32 rows one MFMA operand load reads
+// over 16 bank offsets: two lanes share a bank, where 64-element rows (128
+// bytes) would put 16 on one.  Pad columns are zero.
+constexpr int kWRow0 = 88;
+constexpr int kWRow = 72;
+constexpr int kWOff1 = 64 * kWRow0;
+constexpr int kWTotal = kWOff1 + 4 * 64 * kWRow;         // 24,064 bf16
+constexpr int kImageU4 = (kWTotal * 2 + 320 * 4) / 16;  // 3,088
+constexpr int kImageWU4 = kWTotal * 2 / 16;              // 3,008
 static_assert(kWTotal * 2 + 320 * 4 == int(kGuideImageBytes), "image size");
-__host__ __device__ constexpr int wOff(int l) { return l == 0 ? 0 : kWOff1 + (l - 1) * 64 * 64; }
-__host__ __device__ constexpr int wStride(int l) { return l == 0 ? kGuideInPad : 64; }
+static_assert(kWRow0 >= kGuideInPad && kWRow >= 64, "image rows");
+__host__ __device__ constexpr int wOff(int l) { return l == 0 ? 0 : kWOff1 + (l - 1) * 64 * kWRow; }
+__host__ __device__ constexpr int wStride(int l) { return l == 0 ? kWRow0 : kWRow; }
+// Per-wave row buffers in LDS: bf16 activations at a row stride of kActStride
+// (176 bytes; 80 encoded inputs or 64 hidden outputs), reused after the last
+// layer as FP32 raw outputs at a row stride of kRawStride (also 176 bytes).
+// Both strides put the 32 rows of a tile on 16 bank offsets.
+constexpr int kActStride = 88;
+constexpr int kRawStride = 44;
+static_assert(kActStride >= kGuideInPad && kRawStride >= kGuideOut && kRawStride * 4 <= kActStride * 2, "row buffers");
 
 __global__ __launch_bounds__(256) void kGuidePack(const float* __restrict__ params, uint16_t* __restrict__ image) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < kWTotal) {
-        const int l = i < kWOff1 ? 0 : 1 + (i - kWOff1) / (64 * 64);
+        const int l = i < kWOff1 ? 0 : 1 + (i - kWOff1) / (64 * kWRow);
         const int r = (i - wOff(l)) / wStride(l), c = (i - wOff(l)) % wStride(l);
         float v = 0.0f;
         if (r < guideLayerOut(l) && c < guideLayerIn(l)) v = params[guideWeightOffset(l) + r * guideLayerIn(l) + c];
@@ -124,9 +138,12 @@ __device__ __forceinline__ void loadImage(const u32x4* __restrict__ image, uint1
     }
 }
 
-// One layer over the wave's 32 rows in act (row-major, stride kGuideInPad).
+// One layer over the wave's 32 rows in act (row-major, stride kActStride).
+// The weights are the A operand and the rows the B operand, so lane (r, h)
+// ends with outputs 8 j + 4 h + i (c0) and 32 + 8 j + 4 h + i (c1) of row r:
+// four consecutive features of one row, stored back with one 8-byte write.
 // Hidden layers write ReLU(W a + b) back into act as bf16; the output layer
-// leaves W a + b in the two accumulator tiles (columns 0..31, 32..63).
+// leaves W a + b in the two accumulator tiles.
 __device__ __forceinline__ void layerForward(int l, uint16_t* act, const uint16_t* W, const float* bias,
                                              f32x16& c0, f32x16& c1) {
     const int lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
@@ -137,24 +154,31 @@ __device__ __forceinline__ void layerForward(int l, uint16_t* act, const uint16_
     c1 = zero16();
     waveHandoff();
     for (int kb = 0; kb < K; kb += 16) {
-        const bf16x8 a = ld8(act + r * kGuideInPad + kb + 8 * h);
-        const bf16x8 b0 = ld8(Wl + r * ws + kb + 8 * h);
-        const bf16x8 b1 = ld8(Wl + (32 + r) * ws + kb + 8 * h);
-        c0 = mfma(a, b0, c0);
-        c1 = mfma(a, b1, c1);
+        const bf16x8 x = ld8(act + r * kActStride + kb + 8 * h);
+        c0 = mfma(ld8(Wl + r * ws + kb + 8 * h), x, c0);
+        c1 = mfma(ld8(Wl + (32 + r) * ws + kb + 8 * h), x, c1);
     }
-    const float bb0 = bias[l * 64 + r], bb1 = bias[l * 64 + 32 + r];
-    for (int i = 0; i < 16; ++i) {
-        c0[i] += bb0;
-        c1[i] += bb1;
+    for (int j = 0; j < 4; ++j) {
+        const float4 b0 = *reinterpret_cast<const float4*>(bias + l * 64 + 8 * j + 4 * h);
+        const float4 b1 = *reinterpret_cast<const float4*>(bias + l * 64 + 32 + 8 * j + 4 * h);
+        c0[4 * j] += b0.x;
+        c0[4 * j + 1] += b0.y;
+        c0[4 * j + 2] += b0.z;
+        c0[4 * j + 3] += b0.w;
+        c1[4 * j] += b1.x;
+        c1[4 * j + 1] += b1.y;
+        c1[4 * j + 2] += b1.z;
+        c1[4 * j + 3] += b1.w;
     }
     if (l < 4) {
-        for (int j = 0; j < 4; ++j)
-            for (int i = 0; i < 4; ++i) {
-                const int m = 8 * j + 4 * h + i;
-                act[m * kGuideInPad + r] = toBf16(fmaxf(c0[4 * j + i], 0.0f));
-                act[m * kGuideInPad + 32 + r] = toBf16(fmaxf(c1[4 * j + i], 0.0f));
-            }
+        auto relu4 = [](const f32x16& c, int j) {
+            return make_uint2(pack2(toBf16(fmaxf(c[4 * j], 0.0f)), toBf16(fmaxf(c[4 * j + 1], 0.0f))),
+                              pack2(toBf16(fmaxf(c[4 * j + 2], 0.0f)), toBf16(fmaxf(c[4 * j + 3], 0.0f))));
+        };
+        for (int j = 0; j < 4; ++j) {
+            *reinterpret_cast<uint2*>(act + r * kActStride + 8 * j + 4 * h) = relu4(c0, j);
+            *reinterpret_cast<uint2*>(act + r * kActStride + 32 + 8 * j + 4 * h) = relu4(c1, j);
+        }
     }
     waveHandoff();
 }
@@ -187,26 +211,33 @@ __device__ __forceinline__ void sinCosPiSeries(float t, float& s, float& c) {
     if (t < 0.0f) s = -s;
 }
 
-// Encodes rows [row0, row0 + 32) into act.  Feature pair j = 12 a + b is
-// (sin, cos)(pi 2^b p_a): band 0 from the series after exact reduction of
-// p_a to [-1, 1], each further band by the double-angle step
-// (s, c) -> (2 s c, (c - s)(c + s)).  Lanes 0..31 write pairs 0..17 of row
-// lane, lanes 32..63 pairs 18..35, the normal, wo and two zero pads.  Rows
-// past count are zero.
-__device__ __forceinline__ void encodeRows(const RecView& rv, uint32_t row0, uint32_t count, uint16_t* act) {
+// The three query float4 of one row (pos01, normal, wo); zero past count.
+struct RowIn {
+    float4 a, b, c;
+};
+__device__ __forceinline__ RowIn loadRow(const RecView& rv, uint32_t row, uint32_t count) {
+    RowIn q{ make_float4(0, 0, 0, 0), make_float4(0, 0, 0, 0), make_float4(0, 0, 0, 0) };
+    if (row < count) {
+        const float4* p = recRow(rv, row);
+        q.a = p[0];
+        q.b = p[1];
+        q.c = p[2];
+    }
+    return q;
+}
+
+// Encodes the wave's 32 rows into act; lanes r and r + 32 both hold row r's
+// query (in).  Feature pair j = 12 a + b is (sin, cos)(pi 2^b p_a): band 0
+// from the series after exact reduction of p_a to [-1, 1], each further band
+// by the double-angle step (s, c) -> (2 s c, (c - s)(c + s)).  Lanes 0..31
+// write pairs 0..17 of row lane, lanes 32..63 pairs 18..35, the normal, wo
+// and two zero pads.  Invalid rows (past count) are zero.
+__device__ __forceinline__ void encodeRows(const RowIn& in, bool valid, uint16_t* act) {
 #pragma clang fp contract(off)
     const int lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
-    const uint32_t row = row0 + r;
-    const bool valid = row < count;
-    float4 a = make_float4(0, 0, 0, 0), b = a, c = a;
-    if (valid) {
-        const float4* q = recRow(rv, row);
-        a = q[0];
-        b = q[1];
-        c = q[2];
-    }
+    const float4 a = in.a, b = in.b, c = in.c;
     const float p[3] = { a.x, a.y, a.z };
-    uint32_t* dst = reinterpret_cast<uint32_t*>(act + r * kGuideInPad);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(act + r * kActStride);
     waveHandoff();  // the previous tile's reads of act come first
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
@@ -236,13 +267,12 @@ __device__ __forceinline__ void encodeRows(const RecView& rv, uint32_t row0, uin
 // float[32][40]).
 __device__ __forceinline__ void stashOutputs(const f32x16& c0, const f32x16& c1, uint16_t* act) {
     const int lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
-    float* fo = reinterpret_cast<float*>(act);
+    float* fo = reinterpret_cast<float*>(act) + r * kRawStride;
     for (int j = 0; j < 4; ++j)
-        for (int i = 0; i < 4; ++i) {
-            const int m = 8 * j + 4 * h + i;
-            fo[m * kGuideOut + r] = c0[4 * j + i];
-            if (r < kGuideOut - 32) fo[m * kGuideOut + 32 + r] = c1[4 * j + i];
-        }
+        *reinterpret_cast<float4*>(fo + 8 * j + 4 * h) = make_float4(c0[4 * j], c0[4 * j + 1], c0[4 * j + 2],
+                                                                     c0[4 * j + 3]);
+    // outputs 32..39: j = 0 of the second tile
+    *reinterpret_cast<float4*>(fo + 32 + 4 * h) = make_float4(c1[0], c1[1], c1[2], c1[3]);
     waveHandoff();
 }
 
@@ -253,20 +283,20 @@ __device__ __forceinline__ void storeFeatureMajor(const uint16_t* act, int nfeat
     const int lane = threadIdx.x & 63;
     for (int f = lane; f < nfeat; f += 64) {
         uint32_t w[16];
-        for (int m = 0; m < 16; ++m) w[m] = pack2(act[(2 * m) * kGuideInPad + f], act[(2 * m + 1) * kGuideInPad + f]);
+        for (int m = 0; m < 16; ++m) w[m] = pack2(act[(2 * m) * kActStride + f], act[(2 * m + 1) * kActStride + f]);
         u32x4* d = reinterpret_cast<u32x4*>(dst + size_t(f) * stride + row0);
         for (int q = 0; q < 4; ++q) d[q] = u32x4{ w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3] };
     }
 }
 
 // ------------------------------------------------------------------ inference --
-constexpr int kFwdWaves = 16;  // one 1,024-thread block per CU: 43 KB weights + 16 x 5 KB rows
+constexpr int kFwdWaves = 16;  // one 1,024-thread block per CU: 48 KB weights + 16 x 5.5 KB rows
 __global__ __launch_bounds__(kFwdWaves * 64) void kGuideInfer(RecView rv, const uint32_t* __restrict__ countPtr,
                                                               const u32x4* __restrict__ image, void* __restrict__ out,
                                                               int decoded) {
     __shared__ __attribute__((aligned(16))) uint16_t W[kWTotal];
     __shared__ __attribute__((aligned(16))) float bias[5 * 64];
-    __shared__ __attribute__((aligned(16))) uint16_t actAll[kFwdWaves][32 * kGuideInPad];
+    __shared__ __attribute__((aligned(16))) uint16_t actAll[kFwdWaves][32 * kActStride];
     const uint32_t count = *countPtr;
     if (blockIdx.x * kFwdWaves * 32 >= count) return;  // uniform per block
     loadImage(image, W, bias);
@@ -275,9 +305,14 @@ __global__ __launch_bounds__(kFwdWaves * 64) void kGuideInfer(RecView rv, const 
     uint16_t* act = actAll[wave];
     const float* fo = reinterpret_cast<const float*>(act);
     const uint32_t totalWaves = gridDim.x * kFwdWaves;
-    for (uint32_t blk = blockIdx.x * kFwdWaves + wave; blk * 32 < count; blk += totalWaves) {
+    // The next tile's queries are loaded while the current tile computes.
+    uint32_t blk = blockIdx.x * kFwdWaves + wave;
+    RowIn next = loadRow(rv, blk * 32 + (lane & 31), count);
+    for (; blk * 32 < count; blk += totalWaves) {
         const uint32_t row0 = blk * 32;
-        encodeRows(rv, row0, count, act);
+        const RowIn cur = next;
+        next = loadRow(rv, (blk + totalWaves) * 32 + (lane & 31), count);
+        encodeRows(cur, row0 + (lane & 31) < count, act);
         f32x16 c0, c1;
         for (int l = 0; l < 5; ++l) layerForward(l, act, W, bias, c0, c1);
         stashOutputs(c0, c1, act);
@@ -285,13 +320,13 @@ __global__ __launch_bounds__(kFwdWaves * 64) void kGuideInfer(RecView rv, const 
             // lanes r and r + 32 decode lobes 0..3 and 4..7 of row r
             const int r = lane & 31;
             const uint32_t row = row0 + r;
-            storeMixHalf(fo + r * kGuideOut, lane >> 5, row < count,
+            storeMixHalf(fo + r * kRawStride, lane >> 5, row < count,
                          static_cast<float4*>(out) + size_t(row) * kGuideMixF4);
         } else {
             float* o = static_cast<float*>(out);
             for (int i = lane; i < 32 * kGuideOut; i += 64) {
                 const uint32_t row = row0 + i / kGuideOut;
-                if (row < count) o[size_t(row) * kGuideOut + i % kGuideOut] = fo[i];
+                if (row < count) o[size_t(row) * kGuideOut + i % kGuideOut] = fo[(i / kGuideOut) * kRawStride + i % kGuideOut];
             }
         }
     }
@@ -356,7 +391,7 @@ __global__ __launch_bounds__(kFwdWaves * 64) void kGuideFwdTrain(GuideBatch b, c
                                                                  const u32x4* __restrict__ image, GuideNet net) {
     __shared__ __attribute__((aligned(16))) uint16_t W[kWTotal];
     __shared__ __attribute__((aligned(16))) float bias[5 * 64];
-    __shared__ __attribute__((aligned(16))) uint16_t actAll[kFwdWaves][32 * kGuideInPad];
+    __shared__ __attribute__((aligned(16))) uint16_t actAll[kFwdWaves][32 * kActStride];
     const uint32_t count = words[kGwBatch];
     const uint32_t rows = pad32(count);
     if (blockIdx.x * kFwdWaves * 32 >= rows) return;  // uniform per block
@@ -368,9 +403,13 @@ __global__ __launch_bounds__(kFwdWaves * 64) void kGuideFwdTrain(GuideBatch b, c
     const float* fo = reinterpret_cast<const float*>(act);
     const uint32_t stride = net.stride;
     const uint32_t totalWaves = gridDim.x * kFwdWaves;
-    for (uint32_t blk = blockIdx.x * kFwdWaves + wave; blk * 32 < rows; blk += totalWaves) {
+    uint32_t blk = blockIdx.x * kFwdWaves + wave;
+    RowIn next = loadRow(rv, blk * 32 + (lane & 31), count);
+    for (; blk * 32 < rows; blk += totalWaves) {
         const uint32_t row0 = blk * 32;
-        encodeRows(rv, row0, count, act);
+        const RowIn cur = next;
+        next = loadRow(rv, (blk + totalWaves) * 32 + (lane & 31), count);
+        encodeRows(cur, row0 + (lane & 31) < count, act);
         storeFeatureMajor(act, kGuideInPad, net.actIn, stride, row0);
         f32x16 c0, c1;
         for (int l = 0; l < 5; ++l) {
@@ -381,9 +420,9 @@ __global__ __launch_bounds__(kFwdWaves * 64) void kGuideFwdTrain(GuideBatch b, c
         if (lane < kGuideOut) {
             float* d = net.raw + size_t(lane) * stride + row0;
             for (int m = 0; m < 32; m += 4)
-                *reinterpret_cast<float4*>(d + m) = make_float4(fo[m * kGuideOut + lane], fo[(m + 1) * kGuideOut + lane],
-                                                                fo[(m + 2) * kGuideOut + lane],
-                                                                fo[(m + 3) * kGuideOut + lane]);
+                *reinterpret_cast<float4*>(d + m) = make_float4(fo[m * kRawStride + lane], fo[(m + 1) * kRawStride + lane],
+                                                                fo[(m + 2) * kRawStride + lane],
+                                                                fo[(m + 3) * kRawStride + lane]);
         }
     }
 }
