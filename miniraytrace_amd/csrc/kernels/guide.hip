@@ -485,23 +485,24 @@ __global__ __launch_bounds__(256) void kGuideOutGrad(GuideBatch b, uint32_t* __r
 }
 
 // deltaPrev[in][row] = (sum_o W_l[o][in] delta[o][row]) * (actPrev[in][row] > 0),
-// l >= 1, one wave per 32 rows.  W_l^T is staged in LDS as WT[in][o]; the
-// wave's delta tile is staged as sD[o][row] (row stride 34 against bank
-// conflicts between the two lane halves).
-constexpr int kDeltaPad = 34;
+// l >= 1, one wave per 32 rows.  W_l^T is staged in LDS as WT[in][o] and the
+// wave's delta tile transposed to sD[row][o], so both MFMA operands are
+// 16-byte LDS loads; both use the 144-byte row stride.
 __global__ __launch_bounds__(256) void kGuideBackLayer(int l, const uint32_t* __restrict__ words,
                                                        const float* __restrict__ params,
                                                        const uint16_t* __restrict__ delta,
                                                        const uint16_t* __restrict__ actPrev,
                                                        uint16_t* __restrict__ deltaPrev, uint32_t stride) {
-    __shared__ __attribute__((aligned(16))) uint16_t WT[64 * 64];
-    __shared__ __attribute__((aligned(16))) uint16_t sDAll[4][64 * kDeltaPad];
+    // W^T rows at the forward image's hidden row stride (144 bytes): the 32
+    // rows of one operand load fall on 16 LDS bank offsets.
+    __shared__ __attribute__((aligned(16))) uint16_t WT[64 * kWRow];
+    __shared__ __attribute__((aligned(16))) uint16_t sDAll[4][32 * kWRow];
     const uint32_t rows = pad32(words[kGwBatch]);
     if (blockIdx.x * 128 >= rows) return;  // uniform per block
     const int nin = guideLayerIn(l), nout = guideLayerOut(l);
     for (int i = threadIdx.x; i < 64 * 64; i += blockDim.x) {
         const int in = i / 64, o = i % 64;
-        WT[i] = (o < nout && in < nin) ? toBf16(params[guideWeightOffset(l) + o * nin + in]) : uint16_t(0);
+        WT[in * kWRow + o] = (o < nout && in < nin) ? toBf16(params[guideWeightOffset(l) + o * nin + in]) : uint16_t(0);
     }
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
@@ -509,33 +510,32 @@ __global__ __launch_bounds__(256) void kGuideBackLayer(int l, const uint32_t* __
     const uint32_t totalWaves = gridDim.x * 4;
     for (uint32_t blk = blockIdx.x * 4 + wave; blk * 32 < rows; blk += totalWaves) {
         const uint32_t row0 = blk * 32;
-        // stage delta[0..63][row0 .. row0 + 31]: 256 16-byte pieces, 4 per lane
+        // stage delta[0..63][row0 .. row0 + 31] transposed: 256 16-byte
+        // pieces (8 rows of one feature), 4 per lane
         for (int q = lane; q < 256; q += 64) {
             const int o = q >> 2, part = q & 3;
             const u32x4 v = *reinterpret_cast<const u32x4*>(delta + size_t(o) * stride + row0 + 8 * part);
-            uint32_t* d = reinterpret_cast<uint32_t*>(sD + o * kDeltaPad + 8 * part);
-            d[0] = v.x;
-            d[1] = v.y;
-            d[2] = v.z;
-            d[3] = v.w;
+            const uint32_t w[4] = { v.x, v.y, v.z, v.w };
+            for (int t = 0; t < 8; ++t) sD[(8 * part + t) * kWRow + o] = uint16_t(w[t >> 1] >> (16 * (t & 1)));
         }
         waveHandoff();
+        bf16x8 b[4];
+        for (int k = 0; k < 4; ++k) b[k] = ld8(sD + r * kWRow + 16 * k + 8 * h);
+        // The delta rows are the A operand and W^T the B operand, so lane
+        // (r, h) ends with rows 8 j + 4 h + i of input feature 32 mt + r: four
+        // consecutive rows of deltaPrev and actPrev per 8-byte access.
         for (int mt = 0; mt < 2; ++mt) {
             f32x16 acc = zero16();
-            for (int kb = 0; kb < 64; kb += 16) {
-                const bf16x8 a = ld8(WT + (32 * mt + r) * 64 + kb + 8 * h);
-                const uint16_t* col = sD + (kb + 8 * h) * kDeltaPad + r;
-                const u32x4 bv = u32x4{ pack2(col[0], col[kDeltaPad]), pack2(col[2 * kDeltaPad], col[3 * kDeltaPad]),
-                                        pack2(col[4 * kDeltaPad], col[5 * kDeltaPad]),
-                                        pack2(col[6 * kDeltaPad], col[7 * kDeltaPad]) };
-                acc = mfma(a, __builtin_bit_cast(bf16x8, bv), acc);
+            for (int k = 0; k < 4; ++k) acc = mfma(b[k], ld8(WT + (32 * mt + r) * kWRow + 16 * k + 8 * h), acc);
+            for (int j = 0; j < 4; ++j) {
+                const size_t at = size_t(32 * mt + r) * stride + row0 + 8 * j + 4 * h;
+                const uint2 ap = *reinterpret_cast<const uint2*>(actPrev + at);
+                const uint32_t a4[4] = { ap.x & 0xFFFFu, ap.x >> 16, ap.y & 0xFFFFu, ap.y >> 16 };
+                uint16_t d4[4];
+                for (int i = 0; i < 4; ++i)
+                    d4[i] = fromBf16(uint16_t(a4[i])) > 0.0f ? toBf16(acc[4 * j + i]) : uint16_t(0);
+                *reinterpret_cast<uint2*>(deltaPrev + at) = make_uint2(pack2(d4[0], d4[1]), pack2(d4[2], d4[3]));
             }
-            for (int j = 0; j < 4; ++j)
-                for (int i = 0; i < 4; ++i) {
-                    const int in = 32 * mt + 8 * j + 4 * h + i;
-                    const size_t at = size_t(in) * stride + row0 + r;
-                    deltaPrev[at] = fromBf16(actPrev[at]) > 0.0f ? toBf16(acc[4 * j + i]) : uint16_t(0);
-                }
         }
         waveHandoff();  // the next tile's staging overwrites sD
     }
@@ -548,13 +548,15 @@ __global__ __launch_bounds__(256) void kGuideBackLayer(int l, const uint32_t* __
 // wave 0 adds their accumulators in wave order (fixed summation order, no
 // atomics).  Tiles with nt == 0 also form the bias sums with an all-ones B
 // operand.
-constexpr uint32_t kGradSlice = kGuideGradChunk / 4;  // rows per wave
-__global__ __launch_bounds__(256) void kGuideWeightGrad(int l, const uint32_t* __restrict__ words,
-                                                        const uint16_t* __restrict__ delta,
-                                                        const uint16_t* __restrict__ actPrev, uint32_t stride,
-                                                        float* __restrict__ partial) {
-    __shared__ float sAcc[4][16][64];
-    __shared__ float sBias[4][16][64];
+constexpr uint32_t kGradWaves = 4;
+constexpr uint32_t kGradSlice = kGuideGradChunk / kGradWaves;  // rows per wave
+__global__ __launch_bounds__(kGradWaves * 64) void kGuideWeightGrad(int l, const uint32_t* __restrict__ words,
+                                                                    const uint16_t* __restrict__ delta,
+                                                                    const uint16_t* __restrict__ actPrev,
+                                                                    uint32_t stride, float* __restrict__ partial) {
+    // per wave and lane, the 16 accumulator values contiguous (16-byte LDS accesses)
+    __shared__ __attribute__((aligned(16))) float sAcc[kGradWaves][64][16];
+    __shared__ __attribute__((aligned(16))) float sBias[kGradWaves][64][16];
     const uint32_t rows = pad32(words[kGwBatch]);
     const int nin = guideLayerIn(l), nout = guideLayerOut(l);
     const int ntIn = l == 0 ? 3 : 2;
@@ -593,16 +595,30 @@ __global__ __launch_bounds__(256) void kGuideWeightGrad(int l, const uint32_t* _
         acc = mfma(a, ld8(bRow + r), acc);
         if (nt == 0) accB = mfma(a, ones, accB);
     }
-    for (int i = 0; i < 16; ++i) {
-        sAcc[w][i][lane] = acc[i];
-        sBias[w][i][lane] = accB[i];
+    for (int q = 0; q < 4; ++q) {
+        reinterpret_cast<float4*>(sAcc[w][lane])[q] = make_float4(acc[4 * q], acc[4 * q + 1], acc[4 * q + 2],
+                                                                  acc[4 * q + 3]);
+        if (nt == 0)
+            reinterpret_cast<float4*>(sBias[w][lane])[q] = make_float4(accB[4 * q], accB[4 * q + 1],
+                                                                       accB[4 * q + 2], accB[4 * q + 3]);
     }
     __syncthreads();
     if (w != 0) return;
-    for (int i = 0; i < 16; ++i) {
-        acc[i] = ((sAcc[0][i][lane] + sAcc[1][i][lane]) + sAcc[2][i][lane]) + sAcc[3][i][lane];
-        accB[i] = ((sBias[0][i][lane] + sBias[1][i][lane]) + sBias[2][i][lane]) + sBias[3][i][lane];
-    }
+    for (uint32_t v = 1; v < kGradWaves; ++v)
+        for (int q = 0; q < 4; ++q) {
+            const float4 x = reinterpret_cast<const float4*>(sAcc[v][lane])[q];
+            acc[4 * q] += x.x;
+            acc[4 * q + 1] += x.y;
+            acc[4 * q + 2] += x.z;
+            acc[4 * q + 3] += x.w;
+            if (nt == 0) {
+                const float4 y = reinterpret_cast<const float4*>(sBias[v][lane])[q];
+                accB[4 * q] += y.x;
+                accB[4 * q + 1] += y.y;
+                accB[4 * q + 2] += y.z;
+                accB[4 * q + 3] += y.w;
+            }
+        }
     float* dst = partial + size_t(chunk) * kGuideParams;
     for (int j = 0; j < 4; ++j)
         for (int i = 0; i < 4; ++i) {
@@ -812,8 +828,8 @@ hipError_t launchGuideGradient(const GuideBatch& b, const GuideNet& net, bool co
     for (int l = 4; l >= 0; --l) {
         const uint16_t* prev = l == 0 ? net.actIn : net.act + size_t(l - 1) * 64 * net.stride;
         const uint32_t jobs = chunks * (l == 0 ? 6u : 4u);
-        hipLaunchKernelGGL(kGuideWeightGrad, dim3(jobs), dim3(256), 0, s, l, net.words, net.delta[cur], prev,
-                           net.stride, net.partial);
+        hipLaunchKernelGGL(kGuideWeightGrad, dim3(jobs), dim3(kGradWaves * 64), 0, s, l, net.words, net.delta[cur],
+                           prev, net.stride, net.partial);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if (l > 0) {
             hipLaunchKernelGGL(kGuideBackLayer, dim3(cappedGrid(rows, 128, 1024)), dim3(256), 0, s, l, net.words,

@@ -22,7 +22,7 @@ constexpr int kGuideInputsPadded = 80;   // bf16 features per row (78 used)
 constexpr int kGuideMixFloat4 = 12;      // decoded mixture per query (see guide_device.h)
 // Packed forward image: bf16 weights (24,064 with row padding) then FP32 biases (5 x 64), in bytes.
 constexpr size_t kGuideImageBytes = 24064 * 2 + 320 * 4;
-constexpr uint32_t kGuideGradChunk = 1024;  // training rows per weight-gradient partial (4 x 16 x 16)
+constexpr uint32_t kGuideGradChunk = 1024;  // training rows per weight-gradient partial (4 waves x 256 rows)
 
 __host__ __device__ constexpr uint32_t guideRowStride(uint32_t cap) { return (cap + 31u) & ~31u; }
 __host__ __device__ constexpr uint32_t guideGradChunks(uint32_t cap) {
