@@ -205,8 +205,11 @@ __device__ __forceinline__ uint32_t batchFetch(uint32_t* cursor) {
 
 // Next chunk of the stage for this wave: request r (wave-uniform), the
 // request's queue length and the chunk's first item.  False when done.
+// An empty stage returns before touching the cursor: every wave of the
+// persistent grid would otherwise queue one atomic on the same word.
 __device__ __forceinline__ bool batchNext(const BatchChunks& bc, uint32_t* cursor, uint32_t& r, uint32_t& count,
                                           uint32_t& base) {
+    if (bc.total == 0) return false;
     const uint32_t chunk = batchFetch(cursor);
     if (chunk >= bc.total) return false;
     batchLocate(bc, chunk, r, count, base);
