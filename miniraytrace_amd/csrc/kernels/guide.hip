@@ -40,14 +40,17 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-// Round to nearest even (inputs are finite).
-__device__ __forceinline__ uint16_t toBf16(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return uint16_t(u >> 16);
-}
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// Round to nearest even: one v_cvt_pk_bf16_f32 per one or two values.
+__device__ __forceinline__ uint16_t toBf16(float f) { return __builtin_bit_cast(uint16_t, __bf16(f)); }
 __device__ __forceinline__ float fromBf16(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
 __device__ __forceinline__ uint32_t pack2(uint16_t lo, uint16_t hi) { return uint32_t(lo) | (uint32_t(hi) << 16); }
+// bf16(lo) | bf16(hi) << 16
+__device__ __forceinline__ uint32_t pack2f(float lo, float hi) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2{ lo, hi }), bf16x2));
+}
 
 // Eight consecutive bf16 at a 16-byte aligned address (LDS or global).
 __device__ __forceinline__ bf16x8 ld8(const uint16_t* p) {
@@ -172,8 +175,8 @@ __device__ __forceinline__ void layerForward(int l, uint16_t* act, const uint16_
     }
     if (l < 4) {
         auto relu4 = [](const f32x16& c, int j) {
-            return make_uint2(pack2(toBf16(fmaxf(c[4 * j], 0.0f)), toBf16(fmaxf(c[4 * j + 1], 0.0f))),
-                              pack2(toBf16(fmaxf(c[4 * j + 2], 0.0f)), toBf16(fmaxf(c[4 * j + 3], 0.0f))));
+            return make_uint2(pack2f(fmaxf(c[4 * j], 0.0f), fmaxf(c[4 * j + 1], 0.0f)),
+                              pack2f(fmaxf(c[4 * j + 2], 0.0f), fmaxf(c[4 * j + 3], 0.0f)));
         };
         for (int j = 0; j < 4; ++j) {
             *reinterpret_cast<uint2*>(act + r * kActStride + 8 * j + 4 * h) = relu4(c0, j);
@@ -251,13 +254,13 @@ __device__ __forceinline__ void encodeRows(const RowIn& in, bool valid, uint16_t
                 sn = s2;
             }
             const int j = 12 * ax + band;  // feature pair; this lane half writes 18 h .. 18 h + 17
-            if (j / 18 == h) dst[j] = valid ? pack2(toBf16(sn), toBf16(cs)) : 0u;
+            if (j / 18 == h) dst[j] = valid ? pack2f(sn, cs) : 0u;
         }
     }
     if (h == 1) {
-        dst[36] = valid ? pack2(toBf16(b.x), toBf16(b.y)) : 0u;
-        dst[37] = valid ? pack2(toBf16(b.z), toBf16(c.x)) : 0u;
-        dst[38] = valid ? pack2(toBf16(c.y), toBf16(c.z)) : 0u;
+        dst[36] = valid ? pack2f(b.x, b.y) : 0u;
+        dst[37] = valid ? pack2f(b.z, c.x) : 0u;
+        dst[38] = valid ? pack2f(c.y, c.z) : 0u;
         dst[39] = 0u;
     }
     waveHandoff();
@@ -289,6 +292,119 @@ __device__ __forceinline__ void storeFeatureMajor(const uint16_t* act, int nfeat
     }
 }
 
+// ------------------------------------------- register-resident forward pass --
+// The inference kernel keeps a tile's activations in registers.  With the
+// weights as the A operand, lane (r, h) ends a layer with features
+// 8 j + 4 h .. 8 j + 4 h + 3 of row r (j = 0..3 in c0, the same plus 32 in
+// c1), and the next layer's B operand of k step k needs features
+// 16 k + 8 h .. 16 k + 8 h + 7 of row r: group j = 2 k' + h (k' = k within
+// the 32-feature tile) of both lane halves.  A lane keeps its own group and
+// trades the other with its partner lane r + 32 (or r - 32).
+
+// v_permlane32_swap: lanes 32..63 of a trade places with lanes 0..31 of b.
+// With a = group 2 k' and b = group 2 k' + 1 of every lane, a ends holding
+// features +0..3 of the operand (lane half 0: its own group 2 k'; half 1:
+// group 2 k' + 1 of its partner, i.e. features +8..11) and b the rest, so
+// the operand is (a, b) in every lane.
+__device__ __forceinline__ void swapHalves(uint32_t& a, uint32_t& b) {
+    const auto s = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+    a = s[0];
+    b = s[1];
+}
+
+// ReLU of two packed bf16: as int16 a negative bf16 is negative, so max
+// with 0 zeroes exactly the negative values (rounding then ReLU equals ReLU
+// then rounding).
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t reluBf16x2(uint32_t v) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2, v), s16x2{ 0, 0 }));
+}
+
+// Layer-0 B operands of the wave's 32 rows, encoded in registers exactly as
+// encodeRows writes them: lane (r, h) gets feature pairs 8 k + 4 h ..
+// 8 k + 4 h + 3 of row r for k steps k = 0..4 (pairs 0..35 the sin/cos bands,
+// 36..38 normal and wo, 39 zero).  Invalid rows are zero.
+__device__ __forceinline__ void encodeOperands(const RowIn& in, bool valid, u32x4 (&b)[5]) {
+#pragma clang fp contract(off)
+    const bool hi = (threadIdx.x & 32) != 0;
+    uint32_t pr[40];
+    const float p[3] = { in.a.x, in.a.y, in.a.z };
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        float sn, cs;
+        sinCosPiSeries(p[ax] - 2.0f * rintf(0.5f * p[ax]), sn, cs);
+#pragma unroll
+        for (int band = 0; band < kGuideBands; ++band) {
+            if (band > 0) {
+                const float s2 = (2.0f * sn) * cs;
+                cs = (cs - sn) * (cs + sn);
+                sn = s2;
+            }
+            pr[12 * ax + band] = pack2f(sn, cs);
+        }
+    }
+    pr[36] = pack2f(in.b.x, in.b.y);
+    pr[37] = pack2f(in.b.z, in.c.x);
+    pr[38] = pack2f(in.c.y, in.c.z);
+    pr[39] = 0u;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        uint32_t q[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) q[i] = valid ? (hi ? pr[8 * k + 4 + i] : pr[8 * k + i]) : 0u;
+        b[k] = u32x4{ q[0], q[1], q[2], q[3] };
+    }
+}
+
+// One layer from register B operands (KS k steps); outputs as in
+// layerForward, with the accumulators starting from the bias.
+template <int KS>
+__device__ __forceinline__ void layerOperands(int l, const u32x4 (&x)[KS], const uint16_t* W, const float* bias,
+                                              f32x16& c0, f32x16& c1) {
+    const int lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
+    const uint16_t* Wl = W + wOff(l);
+    const int ws = wStride(l);
+    for (int j = 0; j < 4; ++j) {
+        const float4 b0 = *reinterpret_cast<const float4*>(bias + l * 64 + 8 * j + 4 * h);
+        const float4 b1 = *reinterpret_cast<const float4*>(bias + l * 64 + 32 + 8 * j + 4 * h);
+        c0[4 * j] = b0.x;
+        c0[4 * j + 1] = b0.y;
+        c0[4 * j + 2] = b0.z;
+        c0[4 * j + 3] = b0.w;
+        c1[4 * j] = b1.x;
+        c1[4 * j + 1] = b1.y;
+        c1[4 * j + 2] = b1.z;
+        c1[4 * j + 3] = b1.w;
+    }
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+        const bf16x8 xb = __builtin_bit_cast(bf16x8, x[k]);
+        c0 = mfma(ld8(Wl + r * ws + 16 * k + 8 * h), xb, c0);
+        c1 = mfma(ld8(Wl + (32 + r) * ws + 16 * k + 8 * h), xb, c1);
+    }
+}
+
+// Hidden outputs -> bf16 -> ReLU -> the next layer's four B operands.
+__device__ __forceinline__ void nextOperands(const f32x16& c0, const f32x16& c1, u32x4 (&x)[4]) {
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const f32x16& c = t ? c1 : c0;
+        uint32_t g[4][2];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            g[j][0] = reluBf16x2(pack2f(c[4 * j], c[4 * j + 1]));
+            g[j][1] = reluBf16x2(pack2f(c[4 * j + 2], c[4 * j + 3]));
+        }
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk) {
+            uint32_t a0 = g[2 * kk][0], a1 = g[2 * kk][1], b0 = g[2 * kk + 1][0], b1 = g[2 * kk + 1][1];
+            swapHalves(a0, b0);
+            swapHalves(a1, b1);
+            x[2 * t + kk] = u32x4{ a0, a1, b0, b1 };
+        }
+    }
+}
+
 // ------------------------------------------------------------------ inference --
 constexpr int kFwdWaves = 16;  // one 1,024-thread block per CU: 48 KB weights + 16 x 5.5 KB rows
 __global__ __launch_bounds__(kFwdWaves * 64) void kGuideInfer(RecView rv, const uint32_t* __restrict__ countPtr,
@@ -312,9 +428,16 @@ __global__ __launch_bounds__(kFwdWaves * 64) void kGuideInfer(RecView rv, const 
         const uint32_t row0 = blk * 32;
         const RowIn cur = next;
         next = loadRow(rv, (blk + totalWaves) * 32 + (lane & 31), count);
-        encodeRows(cur, row0 + (lane & 31) < count, act);
+        u32x4 x0[5];
+        encodeOperands(cur, row0 + (lane & 31) < count, x0);
         f32x16 c0, c1;
-        for (int l = 0; l < 5; ++l) layerForward(l, act, W, bias, c0, c1);
+        layerOperands<5>(0, x0, W, bias, c0, c1);
+        for (int l = 1; l < 5; ++l) {
+            u32x4 x[4];
+            nextOperands(c0, c1, x);
+            layerOperands<4>(l, x, W, bias, c0, c1);
+        }
+        waveHandoff();  // the previous tile's reads of the stash come first
         stashOutputs(c0, c1, act);
         if (decoded) {
             // lanes r and r + 32 decode lobes 0..3 and 4..7 of row r
