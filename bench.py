@@ -296,7 +296,7 @@ def guided_workload(args, torch) -> dict:
     204 x 204 quads, 499,392 triangles, albedo 0.7) lit only by the sky and
     its 0.5 degree sun through a window of 5 % of one wall, 1280x720, max
     depth 8, no firefly clamp.  Plain renders `spp` frames; guided starts from
-    an untrained network, trains online (batch 65,536, 4 Adam steps per frame)
+    an untrained network, trains online (batch 65,536, 2 Adam steps per frame)
     and samples the learned mixture with probability 0.5, and is timed until
     its RMSE reaches plain's.  RMSE is taken against an independent guided
     render of `2 spp` samples; its own error adds equally to both sides."""
@@ -324,8 +324,10 @@ def guided_workload(args, torch) -> dict:
 
     g, _ = _room_engine(W, H, 12, sky)
     g.set_guiding(sampling=True, training=True)
-    warm = 32
-    t_g = _timed_frames(g, warm, torch)  # training from an untrained network, samples discarded
+    warm = args.guided_warmup
+    # frames that train from the untrained network with their samples
+    # discarded (0 = every guided sample counts); their time is counted
+    t_g = _timed_frames(g, warm, torch) if warm else 0.0
     g.reset_accumulation()
     # Render until guided has both reached plain's error and run plain's wall
     # time (at most 2 x spp frames); checkpoints every 16 frames (readbacks untimed).
@@ -602,6 +604,8 @@ def main() -> int:
     ap.add_argument("--workload", default="pt", choices=["pt", "traversal", "guided", "batch", "scale", "copy"])
     ap.add_argument("--guided-spp", type=int, default=1024,
                     help="plain sample count whose RMSE the guided run must reach")
+    ap.add_argument("--guided-warmup", type=int, default=0,
+                    help="guided frames rendered (and timed) before its samples count")
     ap.add_argument("--no-guided", dest="with_guided", action="store_false", default=True)
     ap.add_argument("--batch-requests", type=int, default=64)
     ap.add_argument("--no-batch", dest="with_batch", action="store_false", default=True)
