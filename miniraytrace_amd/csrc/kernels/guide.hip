@@ -12,8 +12,10 @@
 // A and B fill their k slots from the same k, so the product does not depend
 // on how the instruction orders k internally; only the C layout matters.
 //
-// Inference keeps one wave's 32 rows in LDS between layers (row-major).
-// Training stores activations and deltas feature-major ([feature][row],
+// Inference keeps one wave's 32 rows in registers between layers (lane-half
+// swaps turn a layer's outputs into the next layer's operands); the training
+// forward passes them through LDS (row-major), from where it also stores
+// them.  Training stores activations and deltas feature-major ([feature][row],
 // rows padded to 32) so every operand of the backward and weight-gradient
 // products is a contiguous 16-byte fragment in HBM.
 //

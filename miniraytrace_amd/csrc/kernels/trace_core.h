@@ -36,8 +36,21 @@ struct Node {
     float4 b;  // hi.xyz, packed
 };
 
+// 16-byte load through a pointer marked as global memory.  Pointers the
+// compiler cannot place (members of structures read from memory, such as a
+// batch entry's scene view) otherwise compile to FLAT instructions, which
+// also count against the LDS wait counter: every wait on the LDS traversal
+// stack would then wait on the node and triangle fetches in flight as well.
+// (The load goes through the native vector type, which has no constructor
+// to bind to an address-space-qualified object.)
+typedef float nativeF32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ldGlobal4(const float4* p) {
+    const nativeF32x4 v = *(const __attribute__((address_space(1))) nativeF32x4*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 __device__ __forceinline__ Node loadNode(const Node* __restrict__ nodes, uint32_t i) {
-    return nodes[i];
+    return Node{ ldGlobal4(&nodes[i].a), ldGlobal4(&nodes[i].b) };
 }
 __device__ __forceinline__ uint32_t nodeLeft(const Node& n) { return __float_as_uint(n.a.w); }
 __device__ __forceinline__ uint32_t nodeCount(const Node& n) { return __float_as_uint(n.b.w) >> 2; }
@@ -75,7 +88,7 @@ __device__ __forceinline__ uint32_t entryCode(uint2 e) { return e.x >> kEntryInd
 // Cursor of a node entry (code != kEntryInstance).
 __device__ __forceinline__ Cursor entryCursor(const Node* __restrict__ nodes, uint2 e) {
     const uint32_t code = entryCode(e), idx = e.x & kEntryIndexMask;
-    if (code == kEntryRefetch) return cursorOf(nodes[idx]);
+    if (code == kEntryRefetch) return cursorOf(loadNode(nodes, idx));
     return Cursor{ idx, code };
 }
 

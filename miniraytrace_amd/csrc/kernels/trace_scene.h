@@ -64,7 +64,12 @@ __device__ bool traceScene(const SceneView& sc, uint32_t tlasRoot, uint32_t inst
         if (pendingInst) {
             // Enter an instance: object-space ray, BLAS root.
             pendingInst = false;
-            const TraceInstance ti = sc.traceInst[pendingIdx];
+            const TraceInstance* tp = sc.traceInst + pendingIdx;
+            TraceInstance ti;
+            ti.row0 = ldGlobal4(&tp->row0);
+            ti.row1 = ldGlobal4(&tp->row1);
+            ti.row2 = ldGlobal4(&tp->row2);
+            ti.blasRoot = __float_as_uint(ldGlobal4(reinterpret_cast<const float4*>(&tp->blasRoot)).x);
             float3 oo, dd;
             toObject(ti, o.x, o.y, o.z, d.x, d.y, d.z, oo, dd);
             r = setupRay(oo.x, oo.y, oo.z, dd.x, dd.y, dd.z, tmin);
@@ -97,9 +102,10 @@ __device__ bool traceScene(const SceneView& sc, uint32_t tlasRoot, uint32_t inst
                 }
             } else if (inBlas) {
                 for (uint32_t k = lp; k < lp + cnt; ++k) {
-                    const float4 v0 = sc.triVerts[3 * size_t(k)];
-                    const float4 v1 = sc.triVerts[3 * size_t(k) + 1];
-                    const float4 v2 = sc.triVerts[3 * size_t(k) + 2];
+                    const float4* tv = sc.triVerts + 3 * size_t(k);
+                    const float4 v0 = ldGlobal4(tv);
+                    const float4 v1 = ldGlobal4(tv + 1);
+                    const float4 v2 = ldGlobal4(tv + 2);
                     float t, b1, b2;
                     if (!isectTri(r, v0, v1, v2, best, t, b1, b2)) continue;
                     if (ANY) {
