@@ -16,9 +16,12 @@
 // The box test uses IEEE minNum/maxNum (a NaN from 0*inf drops out) and widens
 // far distances by 2^-20 and the hit limit by 2^-16, so rounding never culls a
 // box that holds a hit at t <= limit; together with a (t, id) tie-break this
-// makes traversal results equal brute force exactly.  All floating-point
-// expressions in this file are evaluated without FMA contraction so that the
-// CPU oracle (g++ -ffp-contract=off) reproduces them bit for bit.
+// makes traversal results equal brute force exactly.  The box test's
+// reciprocal direction comes from v_rcp_f32 (1 ulp): with the subtraction
+// and product roundings a slab distance is off by at most ~2^-22 relative,
+// well inside the widening.  The triangle test's constants use IEEE
+// division, and all its expressions are evaluated without FMA contraction,
+// so that the CPU oracle (g++ -ffp-contract=off) reproduces t bit for bit.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -111,7 +114,9 @@ __device__ __forceinline__ RayK setupRay(float ox, float oy, float oz, float dx,
 #pragma clang fp contract(off)
     RayK r;
     r.ox = ox; r.oy = oy; r.oz = oz;
-    r.ix = 1.0f / dx; r.iy = 1.0f / dy; r.iz = 1.0f / dz;
+    r.ix = __builtin_amdgcn_rcpf(dx);
+    r.iy = __builtin_amdgcn_rcpf(dy);
+    r.iz = __builtin_amdgcn_rcpf(dz);
     const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
     int kz = 0;
     if (ay > ax) kz = 1;
