@@ -109,14 +109,26 @@ __device__ __forceinline__ float pick3(float x, float y, float z, int k) {
     return k == 0 ? x : (k == 1 ? y : z);
 }
 
-__device__ __forceinline__ RayK setupRay(float ox, float oy, float oz, float dx, float dy,
-                                         float dz, float tmin) {
-#pragma clang fp contract(off)
+// Box-test constants only (origin, reciprocal direction, tmin): all that the
+// top level of a two-level traversal needs, whose leaves are instances; the
+// shear of the triangle test is left at a dummy identity.
+__device__ __forceinline__ RayK setupRayBox(float ox, float oy, float oz, float dx, float dy,
+                                            float dz, float tmin) {
     RayK r;
     r.ox = ox; r.oy = oy; r.oz = oz;
     r.ix = __builtin_amdgcn_rcpf(dx);
     r.iy = __builtin_amdgcn_rcpf(dy);
     r.iz = __builtin_amdgcn_rcpf(dz);
+    r.sx = 0.0f; r.sy = 0.0f; r.sz = 1.0f;
+    r.kx = 0; r.ky = 1; r.kz = 2;
+    r.tmin = tmin;
+    return r;
+}
+
+__device__ __forceinline__ RayK setupRay(float ox, float oy, float oz, float dx, float dy,
+                                         float dz, float tmin) {
+#pragma clang fp contract(off)
+    RayK r = setupRayBox(ox, oy, oz, dx, dy, dz, tmin);
     const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
     int kz = 0;
     if (ay > ax) kz = 1;
@@ -129,7 +141,6 @@ __device__ __forceinline__ RayK setupRay(float ox, float oy, float oz, float dx,
     r.sx = pick3(dx, dy, dz, kx) / dkz;
     r.sy = pick3(dx, dy, dz, ky) / dkz;
     r.sz = 1.0f / dkz;
-    r.tmin = tmin;
     return r;
 }
 

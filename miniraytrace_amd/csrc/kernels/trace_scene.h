@@ -41,8 +41,10 @@ __device__ bool traceScene(const SceneView& sc, uint32_t tlasRoot, uint32_t inst
                            SceneHit& out, bool& overflow) {
     if (instanceCount == 0) return false;
     // The world ray's constants are rebuilt when traversal returns from an
-    // instance rather than kept, which keeps register pressure down.
-    RayK r = setupRay(o.x, o.y, o.z, d.x, d.y, d.z, tmin);
+    // instance rather than kept, which keeps register pressure down.  The top
+    // level only tests boxes (its leaves are instances), so the world ray
+    // needs no triangle-test shear.
+    RayK r = setupRayBox(o.x, o.y, o.z, d.x, d.y, d.z, tmin);
     uint2 spill[kSpill];
     ShortStack<S, BLOCK> stack(ldsLane, spill);
     bool inBlas = false;
@@ -136,7 +138,7 @@ __device__ bool traceScene(const SceneView& sc, uint32_t tlasRoot, uint32_t inst
         while (true) {
             if (inBlas && stack.sp == blasBase) {
                 inBlas = false;
-                r = setupRay(o.x, o.y, o.z, d.x, d.y, d.z, tmin);
+                r = setupRayBox(o.x, o.y, o.z, d.x, d.y, d.z, tmin);
             }
             if (stack.empty()) break;
             const uint2 e = stack.pop();
