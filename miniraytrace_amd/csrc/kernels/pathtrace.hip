@@ -554,8 +554,15 @@ __device__ __forceinline__ void shadowChunk(const SceneView& sc, const PathView&
 // Register budget: 6 waves per SIMD (80 VGPRs).  Traversal waits on
 // dependent node fetches, and the sixth wave hides more of that latency than
 // the registers it costs; at 8 waves (64 VGPRs) the spills outweigh it.
+// Shapes whose LDS stacks allow fewer waves per SIMD (160 KB of LDS per CU,
+// S x 8 bytes per lane) get the registers of the waves they can hold.
+constexpr int traceWavesPerSimd(int block, int stack) {
+    const int blocksPerCU = (160 * 1024) / (stack * 8 * block);
+    const int waves = blocksPerCU * (block / 64) / 4;
+    return waves < 1 ? 1 : (waves > 6 ? 6 : waves);
+}
 template <int BLOCK, int S, bool PREP>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void kTraceBatch(BatchView bv, uint32_t extendLevel, uint32_t shadowLevel,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(traceWavesPerSimd(BLOCK, S)))) void kTraceBatch(BatchView bv, uint32_t extendLevel, uint32_t shadowLevel,
                                                      uint32_t tailPaths) {
     __shared__ uint2 ldsStack[S * BLOCK];
     const BatchChunks be = extendLevel != kNoLevel ? batchChunks(bv, extendLevel, kCntRays, tailPaths)
