@@ -62,15 +62,21 @@ __device__ Surface makeSurface(const SceneView& sc, V3 dir, float b1, float b2, 
     V3 nGeom = normalize(colDot3(inst.worldToObject, cross(p1 - p0, p2 - p0)));
     const V3 no = v3(sc.normals[tri.i0]) * b0 + v3(sc.normals[tri.i1]) * b1 + v3(sc.normals[tri.i2]) * b2;
     V3 nShade = normalize(colDot3(inst.worldToObject, no));
-    const float2 uv0 = sc.uvs[tri.i0], uv1 = sc.uvs[tri.i1], uv2 = sc.uvs[tri.i2];
-    s.u = uv0.x * b0 + uv1.x * b1 + uv2.x * b2;
-    s.v = uv0.y * b0 + uv1.y * b1 + uv2.y * b2;
     s.front = dot(dir, nGeom) < 0.0f;
     if (!s.front) {
         nGeom = -nGeom;
         nShade = -nShade;
     }
     s.mat = sc.materials[inst.materialIndex];
+    // Texture coordinates feed only texture lookups: fetched when the
+    // material has a texture (a texture slot >= 0 clears the AND's sign bit).
+    s.u = 0.0f;
+    s.v = 0.0f;
+    if ((s.mat.baseColorTex & s.mat.roughnessTex & s.mat.metallicTex & s.mat.normalTex & s.mat.emissionTex) >= 0) {
+        const float2 uv0 = sc.uvs[tri.i0], uv1 = sc.uvs[tri.i1], uv2 = sc.uvs[tri.i2];
+        s.u = uv0.x * b0 + uv1.x * b1 + uv2.x * b2;
+        s.v = uv0.y * b0 + uv1.y * b1 + uv2.y * b2;
+    }
     if (s.mat.normalTex >= 0) {
         const float4 t0 = sc.tangents[tri.i0], t1 = sc.tangents[tri.i1], t2 = sc.tangents[tri.i2];
         const float4 tanH = make_float4(t0.x * b0 + t1.x * b1 + t2.x * b2, t0.y * b0 + t1.y * b1 + t2.y * b2,
