@@ -103,28 +103,36 @@ __device__ bool traceScene(const SceneView& sc, uint32_t tlasRoot, uint32_t inst
                     continue;
                 }
             } else if (inBlas) {
-                for (uint32_t k = lp; k < lp + cnt; ++k) {
-                    const float4* tv = sc.triVerts + 3 * size_t(k);
-                    const float4 v0 = ldGlobal4(tv);
-                    const float4 v1 = ldGlobal4(tv + 1);
-                    const float4 v2 = ldGlobal4(tv + 2);
-                    float t, b1, b2;
-                    if (!isectTri(r, v0, v1, v2, best, t, b1, b2)) continue;
+                // One triangle per step: in almost every step some lane of the
+                // wave is at a leaf, so the wave's leaf branch costs one test
+                // instead of the longest leaf's count; the lane goes on with
+                // the rest of its leaf in the following steps.
+                const uint32_t k = lp;
+                const float4* tv = sc.triVerts + 3 * size_t(k);
+                const float4 v0 = ldGlobal4(tv);
+                const float4 v1 = ldGlobal4(tv + 1);
+                const float4 v2 = ldGlobal4(tv + 2);
+                float t, b1, b2;
+                if (isectTri(r, v0, v1, v2, best, t, b1, b2)) {
                     if (ANY) {
                         if (t < tmax) return true;
-                        continue;
+                    } else {
+                        const bool better = t < best ||
+                                            (found && t == best &&
+                                             (curInst < bestInst || (curInst == bestInst && k < bestTri)));
+                        if (better) {
+                            best = t;
+                            found = true;
+                            bestTri = k;
+                            bestInst = curInst;
+                            bestB1 = b1;
+                            bestB2 = b2;
+                        }
                     }
-                    const bool better = t < best ||
-                                        (found && t == best &&
-                                         (curInst < bestInst || (curInst == bestInst && k < bestTri)));
-                    if (better) {
-                        best = t;
-                        found = true;
-                        bestTri = k;
-                        bestInst = curInst;
-                        bestB1 = b1;
-                        bestB2 = b2;
-                    }
+                }
+                if (cnt > 1) {
+                    cur = Cursor{ lp + 1, cnt - 1 };
+                    continue;
                 }
             } else {
                 // TLAS leaf: one or more instances [lp, lp + cnt).
