@@ -622,19 +622,19 @@ __device__ __forceinline__ void tailChunk(const SceneView& sc, const PathView& p
             waveAppend(pv.counters + b * kCntPerBounce + kCntTailShadows, o.emitShadow);
             if (o.emitShadow) {
                 SceneHit sh;
-                const bool occluded = traceScene<true, S, BLOCK>(sc, g.tlasRoot, g.instanceCount,
-                                                                 make_float3(o.shO.x, o.shO.y, o.shO.z),
-                                                                 make_float3(o.shD.x, o.shD.y, o.shD.z), kEpsRay,
-                                                                 o.shMax, lds, sh, overflow);
+                const bool occluded = traceScene<true, S, BLOCK, false>(sc, g.tlasRoot, g.instanceCount,
+                                                                        make_float3(o.shO.x, o.shO.y, o.shO.z),
+                                                                        make_float3(o.shD.x, o.shD.y, o.shD.z), kEpsRay,
+                                                                        o.shMax, lds, sh, overflow);
                 if (!occluded) applyShadow(pv, g, b, p, o.shC);
             }
             if (!o.emitRay) break;
             waveAppend(pv.counters + (b + 1) * kCntPerBounce + kCntTailRays, true);
             SceneHit hh;
-            const bool hit = traceScene<false, S, BLOCK>(sc, g.tlasRoot, g.instanceCount,
-                                                         make_float3(o.newO.x, o.newO.y, o.newO.z),
-                                                         make_float3(o.newD.x, o.newD.y, o.newD.z), kEpsRay, 1e30f,
-                                                         lds, hh, overflow);
+            const bool hit = traceScene<false, S, BLOCK, false>(sc, g.tlasRoot, g.instanceCount,
+                                                                make_float3(o.newO.x, o.newO.y, o.newO.z),
+                                                                make_float3(o.newD.x, o.newD.y, o.newD.z), kEpsRay, 1e30f,
+                                                                lds, hh, overflow);
             h = hit ? make_float4(hh.t, hh.b1, hh.b2, __uint_as_float(hh.tri))
                     : make_float4(1e30f, 0.0f, 0.0f, __uint_as_float(kNoHit));
             inst = hit ? hh.inst : kNoHit;

@@ -33,9 +33,42 @@ __device__ __forceinline__ void toObject(const TraceInstance& ti, float ox, floa
     d.z = (ti.row2.x * dx + ti.row2.y * dy) + ti.row2.z * dz;
 }
 
+// Tests triangle k of the current instance.  ANY: returns true when it
+// occludes (t < tmax).  Closest hit: keeps it if it is nearer than `best`, ties
+// going to the lowest (instance, triangle) pair; returns false.
+template <bool ANY>
+__device__ __forceinline__ bool blasTriangle(const SceneView& sc, const RayK& r, uint32_t k, uint32_t curInst,
+                                             float tmax, float& best, bool& found, uint32_t& bestTri,
+                                             uint32_t& bestInst, float& bestB1, float& bestB2) {
+    const float4* tv = sc.triVerts + 3 * size_t(k);
+    const float4 v0 = ldGlobal4(tv);
+    const float4 v1 = ldGlobal4(tv + 1);
+    const float4 v2 = ldGlobal4(tv + 2);
+    float t, b1, b2;
+    if (!isectTri(r, v0, v1, v2, best, t, b1, b2)) return false;
+    if (ANY) return t < tmax;
+    const bool better = t < best ||
+                        (found && t == best && (curInst < bestInst || (curInst == bestInst && k < bestTri)));
+    if (better) {
+        best = t;
+        found = true;
+        bestTri = k;
+        bestInst = curInst;
+        bestB1 = b1;
+        bestB2 = b2;
+    }
+    return false;
+}
+
 // ANY = false: closest hit in (tmin, tmax) -> returns true and fills `out`.
 // ANY = true: returns true as soon as some triangle has tmin < t < tmax.
-template <bool ANY, int S, int BLOCK>
+// TRI_STEP = true: a traversal step tests one triangle of a leaf and the lane
+// goes on with the rest of its leaf in the following steps.  In the batched
+// trace kernel almost every step has some lane of the wave at a leaf, so the
+// wave's leaf branch then costs one test instead of the longest leaf's count.
+// TRI_STEP = false tests the whole leaf in one step (the tail kernel, whose
+// waves hold few live lanes, is faster that way).
+template <bool ANY, int S, int BLOCK, bool TRI_STEP = true>
 __device__ bool traceScene(const SceneView& sc, uint32_t tlasRoot, uint32_t instanceCount,
                            float3 o, float3 d, float tmin, float tmax, uint2* ldsLane,
                            SceneHit& out, bool& overflow) {
@@ -103,36 +136,17 @@ __device__ bool traceScene(const SceneView& sc, uint32_t tlasRoot, uint32_t inst
                     continue;
                 }
             } else if (inBlas) {
-                // One triangle per step: in almost every step some lane of the
-                // wave is at a leaf, so the wave's leaf branch costs one test
-                // instead of the longest leaf's count; the lane goes on with
-                // the rest of its leaf in the following steps.
-                const uint32_t k = lp;
-                const float4* tv = sc.triVerts + 3 * size_t(k);
-                const float4 v0 = ldGlobal4(tv);
-                const float4 v1 = ldGlobal4(tv + 1);
-                const float4 v2 = ldGlobal4(tv + 2);
-                float t, b1, b2;
-                if (isectTri(r, v0, v1, v2, best, t, b1, b2)) {
-                    if (ANY) {
-                        if (t < tmax) return true;
-                    } else {
-                        const bool better = t < best ||
-                                            (found && t == best &&
-                                             (curInst < bestInst || (curInst == bestInst && k < bestTri)));
-                        if (better) {
-                            best = t;
-                            found = true;
-                            bestTri = k;
-                            bestInst = curInst;
-                            bestB1 = b1;
-                            bestB2 = b2;
-                        }
+                if constexpr (TRI_STEP) {
+                    if (blasTriangle<ANY>(sc, r, lp, curInst, tmax, best, found, bestTri, bestInst, bestB1, bestB2))
+                        return true;
+                    if (cnt > 1) {
+                        cur = Cursor{ lp + 1, cnt - 1 };
+                        continue;
                     }
-                }
-                if (cnt > 1) {
-                    cur = Cursor{ lp + 1, cnt - 1 };
-                    continue;
+                } else {
+                    for (uint32_t k = lp; k < lp + cnt; ++k)
+                        if (blasTriangle<ANY>(sc, r, k, curInst, tmax, best, found, bestTri, bestInst, bestB1, bestB2))
+                            return true;
                 }
             } else {
                 // TLAS leaf: one or more instances [lp, lp + cnt).
