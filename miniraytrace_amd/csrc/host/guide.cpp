@@ -43,8 +43,9 @@ void GuideNetwork::allocWorkspace(uint32_t cap) {
     actIn_.zero();
     act_.allocate(size_t(4) * 64 * stride * sizeof(uint16_t));
     raw_.allocate(size_t(dev::kGuideOutputs) * stride * sizeof(float));
-    delta_[0].allocate(size_t(64) * stride * sizeof(uint16_t));
-    delta_[1].allocate(size_t(64) * stride * sizeof(uint16_t));
+    // one delta block per layer, so that one launch forms every layer's
+    // weight gradient after the backward pass
+    for (DeviceBuffer& d : delta_) d.allocate(size_t(64) * stride * sizeof(uint16_t));
     // weight-gradient partials, then the per-chunk weight sums
     partial_.allocate(size_t(dev::guideGradChunks(cap)) * (dev::kGuideParamCount + 1) * sizeof(float));
     MRT_HIP(hipStreamSynchronize(nullptr));
@@ -105,8 +106,7 @@ dev::GuideNet GuideNetwork::view() const {
     n.actIn = actIn_.as<uint16_t>();
     n.act = act_.as<uint16_t>();
     n.raw = raw_.as<float>();
-    n.delta[0] = delta_[0].as<uint16_t>();
-    n.delta[1] = delta_[1].as<uint16_t>();
+    for (int l = 0; l < 5; ++l) n.delta[l] = delta_[l].as<uint16_t>();
     n.partial = partial_.as<float>();
     n.weightPart = n.partial + size_t(dev::guideGradChunks(cap_)) * dev::kGuideParamCount;
     n.words = words_.as<uint32_t>();

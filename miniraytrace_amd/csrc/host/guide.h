@@ -38,7 +38,7 @@ private:
 
     uint32_t cap_ = 0, stride_ = 0;
     DeviceBuffer params_, m_, v_, grad_, image_, words_;
-    DeviceBuffer actIn_, act_, raw_, delta_[2], partial_;
+    DeviceBuffer actIn_, act_, raw_, delta_[5], partial_;
 };
 
 } // namespace mrt

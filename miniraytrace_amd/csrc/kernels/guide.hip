@@ -26,7 +26,8 @@
 //   kGuideFwdTrain    forward over the batch, keeping the activations
 //   kGuideOutGrad     d(weighted NLL)/d(raw outputs) per row
 //   kGuideBackLayer   delta_{l-1} = (W_l^T delta_l) * relu'(a_{l-1})
-//   kGuideWeightGrad  per row chunk: grad W_l = delta_l a_{l-1}^T, grad b_l
+//   kGuideWeightGrad  per row chunk, all layers in one launch after the
+//                     backward pass: grad W_l = delta_l a_{l-1}^T, grad b_l
 //   kGuideGradReduce  sum of the chunk partials (fixed order, no atomics)
 //   kGuideAdam        Adam update of the FP32 master weights
 //   kGuideCollect     path records -> weighted training records
@@ -563,7 +564,7 @@ __global__ __launch_bounds__(256) void kGuideOutGrad(GuideBatch b, uint32_t* __r
     if (blockIdx.x == 0 && threadIdx.x == 0) *wordF(words, kGwWeightSum) = wsum;
     float loss = 0.0f;
     if (row < pad32(count)) {
-        uint16_t* dcol = net.delta[0] + row;
+        uint16_t* dcol = net.delta[4] + row;
         if (row < count) {
             const float scale = 1.0f / fmaxf(wsum, 1e-30f);
             const float4* q = recRow(batchView(b), row);
@@ -667,7 +668,8 @@ __global__ __launch_bounds__(256) void kGuideBackLayer(int l, const uint32_t* __
 }
 
 // Partial weight and bias gradients of layer l over one chunk of rows.
-// Block job = (tile, chunk); tile (mt, nt) covers outputs 32 mt.. and inputs
+// One launch forms the weight gradients of all five layers.  Block job =
+// (chunk, layer, tile); tile (mt, nt) covers outputs 32 mt.. and inputs
 // 32 nt.. (layer 0: 3 input tiles over the 96 zero-padded input features).
 // The block's 4 waves take consecutive quarters of the chunk's rows, and
 // wave 0 adds their accumulators in wave order (fixed summation order, no
@@ -675,19 +677,22 @@ __global__ __launch_bounds__(256) void kGuideBackLayer(int l, const uint32_t* __
 // operand.
 constexpr uint32_t kGradWaves = 4;
 constexpr uint32_t kGradSlice = kGuideGradChunk / kGradWaves;  // rows per wave
-__global__ __launch_bounds__(kGradWaves * 64) void kGuideWeightGrad(int l, const uint32_t* __restrict__ words,
-                                                                    const uint16_t* __restrict__ delta,
-                                                                    const uint16_t* __restrict__ actPrev,
-                                                                    uint32_t stride, float* __restrict__ partial) {
+constexpr uint32_t kGradJobsPerChunk = 2 * 3 + 4 * (2 * 2);    // layer 0: 6 tiles, layers 1..4: 4 each
+__global__ __launch_bounds__(kGradWaves * 64) void kGuideWeightGrad(GuideNet net) {
     // per wave and lane, the 16 accumulator values contiguous (16-byte LDS accesses)
     __shared__ __attribute__((aligned(16))) float sAcc[kGradWaves][64][16];
     __shared__ __attribute__((aligned(16))) float sBias[kGradWaves][64][16];
-    const uint32_t rows = pad32(words[kGwBatch]);
-    const int nin = guideLayerIn(l), nout = guideLayerOut(l);
-    const int ntIn = l == 0 ? 3 : 2;
-    const uint32_t tile = blockIdx.x % uint32_t(2 * ntIn), chunk = blockIdx.x / uint32_t(2 * ntIn);
+    const uint32_t rows = pad32(net.words[kGwBatch]);
+    const uint32_t chunk = blockIdx.x / kGradJobsPerChunk, job = blockIdx.x % kGradJobsPerChunk;
     const uint32_t r0 = chunk * kGuideGradChunk;
     if (r0 >= rows) return;  // whole block
+    const int l = job < 6 ? 0 : 1 + int(job - 6) / 4;
+    const uint32_t tile = job < 6 ? job : (job - 6) % 4;
+    const uint32_t stride = net.stride;
+    const uint16_t* delta = net.delta[l];
+    const uint16_t* actPrev = l == 0 ? net.actIn : net.act + size_t(l - 1) * 64 * stride;
+    const int nin = guideLayerIn(l), nout = guideLayerOut(l);
+    const int ntIn = l == 0 ? 3 : 2;
     const uint32_t w = threadIdx.x >> 6;
     const uint32_t s0 = r0 + w * kGradSlice, s1 = min(s0 + kGradSlice, rows);
     const int mt = int(tile) / ntIn, nt = int(tile) % ntIn;
@@ -744,7 +749,7 @@ __global__ __launch_bounds__(kGradWaves * 64) void kGuideWeightGrad(int l, const
                 accB[4 * q + 3] += y.w;
             }
         }
-    float* dst = partial + size_t(chunk) * kGuideParams;
+    float* dst = net.partial + size_t(chunk) * kGuideParams;
     for (int j = 0; j < 4; ++j)
         for (int i = 0; i < 4; ++i) {
             const int o = 32 * mt + 8 * j + 4 * h + i, in = 32 * nt + lr;
@@ -949,20 +954,14 @@ hipError_t launchGuideGradient(const GuideBatch& b, const GuideNet& net, bool co
     hipLaunchKernelGGL(kGuideOutGrad, dim3(cappedGrid(rows, 256, 1u << 20)), dim3(256), 0, s, b, net.words, net,
                        chunks);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    int cur = 0;
-    for (int l = 4; l >= 0; --l) {
-        const uint16_t* prev = l == 0 ? net.actIn : net.act + size_t(l - 1) * 64 * net.stride;
-        const uint32_t jobs = chunks * (l == 0 ? 6u : 4u);
-        hipLaunchKernelGGL(kGuideWeightGrad, dim3(jobs), dim3(kGradWaves * 64), 0, s, l, net.words, net.delta[cur],
-                           prev, net.stride, net.partial);
+    for (int l = 4; l >= 1; --l) {
+        const uint16_t* prev = net.act + size_t(l - 1) * 64 * net.stride;
+        hipLaunchKernelGGL(kGuideBackLayer, dim3(cappedGrid(rows, 128, 1024)), dim3(256), 0, s, l, net.words,
+                           net.params, net.delta[l], prev, net.delta[l - 1], net.stride);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        if (l > 0) {
-            hipLaunchKernelGGL(kGuideBackLayer, dim3(cappedGrid(rows, 128, 1024)), dim3(256), 0, s, l, net.words,
-                               net.params, net.delta[cur], prev, net.delta[cur ^ 1], net.stride);
-            if ((e = hipGetLastError()) != hipSuccess) return e;
-            cur ^= 1;
-        }
     }
+    hipLaunchKernelGGL(kGuideWeightGrad, dim3(chunks * kGradJobsPerChunk), dim3(kGradWaves * 64), 0, s, net);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(kGuideGradReduce, dim3((kGuideParams + kReduceParams - 1) / kReduceParams), dim3(256), 0, s,
                        net.words, net.partial, net.grad);
     return hipGetLastError();

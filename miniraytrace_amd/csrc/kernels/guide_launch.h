@@ -75,7 +75,7 @@ struct GuideNet {
     uint16_t* actIn = nullptr;   // bf16 [96][stride] encoded inputs (rows 78..95 zero)
     uint16_t* act = nullptr;     // bf16 [4][64][stride] hidden-layer outputs
     float* raw = nullptr;        // FP32 [40][stride] raw outputs
-    uint16_t* delta[2] = { nullptr, nullptr };  // bf16 [64][stride] back-propagated error
+    uint16_t* delta[5] = {};     // bf16 [64][stride] per layer: error at layer l's outputs
     float* partial = nullptr;    // FP32 [guideGradChunks(cap)][kGuideParamCount]
     float* weightPart = nullptr; // FP32 [guideGradChunks(cap)]: batch weight sum per row chunk
     uint32_t* words = nullptr;   // kGwWords
