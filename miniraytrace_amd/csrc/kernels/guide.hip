@@ -25,7 +25,7 @@
 //   kGuideStepBegin   batch size, counters and per-chunk weight sums of one step
 //   kGuideFwdTrain    forward over the batch, keeping the activations
 //   kGuideOutGrad     d(weighted NLL)/d(raw outputs) per row
-//   kGuideBackLayer   delta_{l-1} = (W_l^T delta_l) * relu'(a_{l-1})
+//   kGuideBackward    delta_{l-1} = (W_l^T delta_l) * relu'(a_{l-1}), l = 4 .. 1
 //   kGuideWeightGrad  per row chunk, all layers in one launch after the
 //                     backward pass: grad W_l = delta_l a_{l-1}^T, grad b_l
 //   kGuideGradReduce  sum of the chunk partials (fixed order, no atomics)
@@ -610,25 +610,29 @@ __global__ __launch_bounds__(256) void kGuideOutGrad(GuideBatch b, uint32_t* __r
     if ((threadIdx.x & 63) == 0 && loss != 0.0f) atomicAdd(wordF(words, kGwLoss), loss);
 }
 
-// deltaPrev[in][row] = (sum_o W_l[o][in] delta[o][row]) * (actPrev[in][row] > 0),
-// l >= 1, one wave per 32 rows.  W_l^T is staged in LDS as WT[in][o] and the
-// wave's delta tile transposed to sD[row][o], so both MFMA operands are
-// 16-byte LDS loads; both use the 144-byte row stride.
-__global__ __launch_bounds__(256) void kGuideBackLayer(int l, const uint32_t* __restrict__ words,
-                                                       const float* __restrict__ params,
-                                                       const uint16_t* __restrict__ delta,
-                                                       const uint16_t* __restrict__ actPrev,
-                                                       uint16_t* __restrict__ deltaPrev, uint32_t stride) {
+// Backward pass through the hidden layers in one launch: for l = 4 .. 1,
+// delta_{l-1}[in][row] = (sum_o W_l[o][in] delta_l[o][row]) * (a_{l-1}[in][row] > 0),
+// one wave per 32-row tile.  The four W_l^T are staged in LDS once per block
+// (WT[l-1][in][o]); the wave's tile of delta_4 is read from HBM and staged
+// transposed (sD[row][o]), and each following layer's tile is written
+// straight into sD from the accumulators, so both MFMA operands are 16-byte
+// LDS loads (144-byte row stride) and only delta_4 is read back.  Every
+// delta_{l-1} is also written out for the weight gradient.
+__global__ __launch_bounds__(256) void kGuideBackward(GuideNet net) {
     // W^T rows at the forward image's hidden row stride (144 bytes): the 32
     // rows of one operand load fall on 16 LDS bank offsets.
-    __shared__ __attribute__((aligned(16))) uint16_t WT[64 * kWRow];
+    __shared__ __attribute__((aligned(16))) uint16_t WT[4][64 * kWRow];
     __shared__ __attribute__((aligned(16))) uint16_t sDAll[4][32 * kWRow];
-    const uint32_t rows = pad32(words[kGwBatch]);
+    const uint32_t rows = pad32(net.words[kGwBatch]);
     if (blockIdx.x * 128 >= rows) return;  // uniform per block
-    const int nin = guideLayerIn(l), nout = guideLayerOut(l);
-    for (int i = threadIdx.x; i < 64 * 64; i += blockDim.x) {
-        const int in = i / 64, o = i % 64;
-        WT[in * kWRow + o] = (o < nout && in < nin) ? toBf16(params[guideWeightOffset(l) + o * nin + in]) : uint16_t(0);
+    const uint32_t stride = net.stride;
+    for (int l = 1; l <= 4; ++l) {
+        const int nin = guideLayerIn(l), nout = guideLayerOut(l);
+        for (int i = threadIdx.x; i < 64 * 64; i += blockDim.x) {
+            const int in = i / 64, o = i % 64;
+            WT[l - 1][in * kWRow + o] =
+                (o < nout && in < nin) ? toBf16(net.params[guideWeightOffset(l) + o * nin + in]) : uint16_t(0);
+        }
     }
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
@@ -636,38 +640,45 @@ __global__ __launch_bounds__(256) void kGuideBackLayer(int l, const uint32_t* __
     const uint32_t totalWaves = gridDim.x * 4;
     for (uint32_t blk = blockIdx.x * 4 + wave; blk * 32 < rows; blk += totalWaves) {
         const uint32_t row0 = blk * 32;
-        // stage delta[0..63][row0 .. row0 + 31] transposed: 256 16-byte
+        // stage delta_4[0..63][row0 .. row0 + 31] transposed: 256 16-byte
         // pieces (8 rows of one feature), 4 per lane
         for (int q = lane; q < 256; q += 64) {
             const int o = q >> 2, part = q & 3;
-            const u32x4 v = *reinterpret_cast<const u32x4*>(delta + size_t(o) * stride + row0 + 8 * part);
+            const u32x4 v = *reinterpret_cast<const u32x4*>(net.delta[4] + size_t(o) * stride + row0 + 8 * part);
             const uint32_t w[4] = { v.x, v.y, v.z, v.w };
             for (int t = 0; t < 8; ++t) sD[(8 * part + t) * kWRow + o] = uint16_t(w[t >> 1] >> (16 * (t & 1)));
         }
         waveHandoff();
-        bf16x8 b[4];
-        for (int k = 0; k < 4; ++k) b[k] = ld8(sD + r * kWRow + 16 * k + 8 * h);
-        // The delta rows are the A operand and W^T the B operand, so lane
-        // (r, h) ends with rows 8 j + 4 h + i of input feature 32 mt + r: four
-        // consecutive rows of deltaPrev and actPrev per 8-byte access.
-        for (int mt = 0; mt < 2; ++mt) {
-            f32x16 acc = zero16();
-            for (int k = 0; k < 4; ++k) acc = mfma(b[k], ld8(WT + (32 * mt + r) * kWRow + 16 * k + 8 * h), acc);
-            for (int j = 0; j < 4; ++j) {
-                const size_t at = size_t(32 * mt + r) * stride + row0 + 8 * j + 4 * h;
-                const uint2 ap = *reinterpret_cast<const uint2*>(actPrev + at);
-                const uint32_t a4[4] = { ap.x & 0xFFFFu, ap.x >> 16, ap.y & 0xFFFFu, ap.y >> 16 };
-                uint16_t d4[4];
-                for (int i = 0; i < 4; ++i)
-                    d4[i] = fromBf16(uint16_t(a4[i])) > 0.0f ? toBf16(acc[4 * j + i]) : uint16_t(0);
-                *reinterpret_cast<uint2*>(deltaPrev + at) = make_uint2(pack2(d4[0], d4[1]), pack2(d4[2], d4[3]));
+        for (int l = 4; l >= 1; --l) {
+            bf16x8 b[4];
+            for (int k = 0; k < 4; ++k) b[k] = ld8(sD + r * kWRow + 16 * k + 8 * h);
+            waveHandoff();  // every lane has its operand before sD is overwritten
+            const uint16_t* actPrev = net.act + size_t(l - 1) * 64 * stride;
+            uint16_t* deltaPrev = net.delta[l - 1];
+            const uint16_t* W = WT[l - 1];
+            // The delta rows are the A operand and W^T the B operand, so lane
+            // (r, h) ends with rows 8 j + 4 h + i of input feature 32 mt + r:
+            // four consecutive rows of deltaPrev and actPrev per 8-byte access.
+            for (int mt = 0; mt < 2; ++mt) {
+                f32x16 acc = zero16();
+                for (int k = 0; k < 4; ++k) acc = mfma(b[k], ld8(W + (32 * mt + r) * kWRow + 16 * k + 8 * h), acc);
+                for (int j = 0; j < 4; ++j) {
+                    const size_t at = size_t(32 * mt + r) * stride + row0 + 8 * j + 4 * h;
+                    const uint2 ap = *reinterpret_cast<const uint2*>(actPrev + at);
+                    const uint32_t a4[4] = { ap.x & 0xFFFFu, ap.x >> 16, ap.y & 0xFFFFu, ap.y >> 16 };
+                    uint16_t d4[4];
+                    for (int i = 0; i < 4; ++i)
+                        d4[i] = fromBf16(uint16_t(a4[i])) > 0.0f ? toBf16(acc[4 * j + i]) : uint16_t(0);
+                    *reinterpret_cast<uint2*>(deltaPrev + at) = make_uint2(pack2(d4[0], d4[1]), pack2(d4[2], d4[3]));
+                    if (l > 1)
+                        for (int i = 0; i < 4; ++i) sD[(8 * j + 4 * h + i) * kWRow + 32 * mt + r] = d4[i];
+                }
             }
+            waveHandoff();  // the next layer (or tile) reads sD
         }
-        waveHandoff();  // the next tile's staging overwrites sD
     }
 }
 
-// Partial weight and bias gradients of layer l over one chunk of rows.
 // One launch forms the weight gradients of all five layers.  Block job =
 // (chunk, layer, tile); tile (mt, nt) covers outputs 32 mt.. and inputs
 // 32 nt.. (layer 0: 3 input tiles over the 96 zero-padded input features).
@@ -954,12 +965,8 @@ hipError_t launchGuideGradient(const GuideBatch& b, const GuideNet& net, bool co
     hipLaunchKernelGGL(kGuideOutGrad, dim3(cappedGrid(rows, 256, 1u << 20)), dim3(256), 0, s, b, net.words, net,
                        chunks);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    for (int l = 4; l >= 1; --l) {
-        const uint16_t* prev = net.act + size_t(l - 1) * 64 * net.stride;
-        hipLaunchKernelGGL(kGuideBackLayer, dim3(cappedGrid(rows, 128, 1024)), dim3(256), 0, s, l, net.words,
-                           net.params, net.delta[l], prev, net.delta[l - 1], net.stride);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-    }
+    hipLaunchKernelGGL(kGuideBackward, dim3(cappedGrid(rows, 128, 1024)), dim3(256), 0, s, net);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(kGuideWeightGrad, dim3(chunks * kGradJobsPerChunk), dim3(kGradWaves * 64), 0, s, net);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(kGuideGradReduce, dim3((kGuideParams + kReduceParams - 1) / kReduceParams), dim3(256), 0, s,
