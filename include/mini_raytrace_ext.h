@@ -126,6 +126,9 @@ typedef struct mrtxPathOptions {
                              * requests together), one kernel finishes those paths instead of a
                              * launch per level; 0 = default (1280 per compute unit),
                              * MRTX_TAIL_DISABLED = never.  Images do not depend on it. */
+    uint32_t cameraPerLane; /* 0 (default) = each wave traces its 8x8 tile of camera rays together
+                             * (shared node, triangle and instance reads); 1 = one traversal per
+                             * lane.  Hits, and so images, do not depend on it. */
 } mrtxPathOptions;
 
 #define MRTX_TAIL_DISABLED 0xFFFFFFFFu

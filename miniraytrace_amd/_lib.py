@@ -125,7 +125,8 @@ class mrtxTraversalConfig(_Sized):
 class mrtxPathOptions(_Sized):
     _fields_ = [("structSize", C.c_size_t), ("seed", C.c_uint32), ("useGraph", C.c_uint32),
                 ("extendBlock", C.c_uint32), ("extendStack", C.c_uint32),
-                ("shadowBlock", C.c_uint32), ("shadowStack", C.c_uint32), ("tailPaths", C.c_uint32)]
+                ("shadowBlock", C.c_uint32), ("shadowStack", C.c_uint32), ("tailPaths", C.c_uint32),
+                ("cameraPerLane", C.c_uint32)]
 
 
 TAIL_DISABLED = 0xFFFFFFFF

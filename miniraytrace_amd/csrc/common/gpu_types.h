@@ -16,6 +16,7 @@ namespace mrt {
 enum FrameFlags : uint32_t {
     kFlagHasEnv = 1u << 0,
     kFlagOrthographic = 1u << 2,
+    kFlagPacketCamera = 1u << 3,  // camera rays traced by wave packets
 };
 
 enum LightKind : int32_t { kLightEnv = 0, kLightSun = 1, kLightPoint = 2, kLightRect = 3 };

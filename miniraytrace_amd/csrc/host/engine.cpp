@@ -235,6 +235,7 @@ void Engine::setSettings(const Settings& s) {
 void Engine::setPathOptions(const mrtxPathOptions& o) {
     seed_ = o.seed;
     useGraph_ = o.useGraph != 0;
+    packetCamera_ = o.cameraPerLane == 0;
     const uint32_t eb = o.extendBlock ? o.extendBlock : kDefaultTraceBlock;
     const uint32_t es = o.extendStack ? o.extendStack : kDefaultTraceStack;
     const uint32_t sb = o.shadowBlock ? o.shadowBlock : kDefaultTraceBlock;
@@ -258,6 +259,7 @@ void Engine::getPathOptions(mrtxPathOptions& o) const {
     o.shadowBlock = shadowShape_.block;
     o.shadowStack = shadowShape_.stack;
     o.tailPaths = tailOption_;
+    o.cameraPerLane = packetCamera_ ? 0u : 1u;
 }
 
 // --------------------------------------------------------------------- commit --
@@ -792,7 +794,8 @@ void Engine::fillConsts(FrameConsts& g) const {
     g.lightCount = lightCount_;
     g.envWidth = hasEnv_ ? envW_ : 0;
     g.envHeight = hasEnv_ ? envH_ : 0;
-    g.flags = (hasEnv_ ? kFlagHasEnv : 0u) | (ortho ? kFlagOrthographic : 0u);
+    g.flags = (hasEnv_ ? kFlagHasEnv : 0u) | (ortho ? kFlagOrthographic : 0u) |
+              (packetCamera_ ? kFlagPacketCamera : 0u);
     g.envRotation = scene_.envRotation();
     g.envIntensity = scene_.envIntensity();
     g.envIntegral = envIntegral_;

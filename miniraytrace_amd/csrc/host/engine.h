@@ -147,6 +147,7 @@ private:
     uint32_t frameIndex_ = 0;
     uint32_t seed_ = 0;
     bool useGraph_ = true;
+    bool packetCamera_ = true;  // camera rays by wave packets (mrtxPathOptions::cameraPerLane == 0)
     bool profiling_ = false;
     FrameStats stats_;
 

@@ -252,7 +252,7 @@ __global__ __launch_bounds__(256) void kGenerateBatch(BatchView bv) {
 template <int BLOCK, int S, bool PREP>
 __device__ __forceinline__ void extendChunk(const SceneView& sc, const PathView& pv, const FrameConsts* __restrict__ gp,
                                             uint32_t bounce, uint32_t count, uint32_t base, uint2* lds,
-                                            bool prep) {
+                                            bool prep, uint32_t* waveStack) {
     const uint32_t k = base + laneId();
     bool overflow = false;
     bool hit = false;
@@ -261,8 +261,13 @@ __device__ __forceinline__ void extendChunk(const SceneView& sc, const PathView&
     if (k < count) {
         const float4 o = pv.rayO[bounce & 1][k], d = pv.rayD[bounce & 1][k];
         dir = v3(d);
-        hit = traceScene<false, S, BLOCK>(sc, gp->tlasRoot, gp->instanceCount, make_float3(o.x, o.y, o.z),
-                                          make_float3(d.x, d.y, d.z), kEpsRay, 1e30f, lds, h, overflow);
+        // Camera rays (level 0) come in 8 x 8 pixel tiles per wave: the wave
+        // traverses together (same hits as the per-lane traversal).
+        hit = bounce == 0 && (gp->flags & kFlagPacketCamera) ? traceScenePacket(sc, gp->tlasRoot, gp->instanceCount, make_float3(o.x, o.y, o.z),
+                                             make_float3(d.x, d.y, d.z), kEpsRay, 1e30f, waveStack, h, overflow)
+                          : traceScene<false, S, BLOCK>(sc, gp->tlasRoot, gp->instanceCount,
+                                                        make_float3(o.x, o.y, o.z), make_float3(d.x, d.y, d.z),
+                                                        kEpsRay, 1e30f, lds, h, overflow);
         pv.hit[k] = hit ? make_float4(h.t, h.b1, h.b2, __uint_as_float(h.tri))
                         : make_float4(1e30f, 0.0f, 0.0f, __uint_as_float(kNoHit));
         pv.hitInst[k] = hit ? h.inst : kNoHit;
@@ -563,7 +568,7 @@ __device__ __forceinline__ void shadowChunk(const SceneView& sc, const PathView&
 // Shapes whose LDS stacks allow fewer waves per SIMD (160 KB of LDS per CU,
 // S x 8 bytes per lane) get the registers of the waves they can hold.
 constexpr int traceWavesPerSimd(int block, int stack) {
-    const int blocksPerCU = (160 * 1024) / (stack * 8 * block);
+    const int blocksPerCU = (160 * 1024) / (stack * 8 * block + (block / 64) * kPacketStack * 4);
     const int waves = blocksPerCU * (block / 64) / 4;
     return waves < 1 ? 1 : (waves > 6 ? 6 : waves);
 }
@@ -571,6 +576,7 @@ template <int BLOCK, int S, bool PREP>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(traceWavesPerSimd(BLOCK, S)))) void kTraceBatch(BatchView bv, uint32_t extendLevel, uint32_t shadowLevel,
                                                      uint32_t tailPaths) {
     __shared__ uint2 ldsStack[S * BLOCK];
+    __shared__ uint32_t packetStack[BLOCK / 64][kPacketStack];  // per wave, for camera rays
     const BatchChunks be = extendLevel != kNoLevel ? batchChunks(bv, extendLevel, kCntRays, tailPaths)
                                                    : BatchChunks{ 0, 0, 0, false };
     const BatchChunks bs = shadowLevel != kNoLevel ? batchChunks(bv, shadowLevel, kCntShadows)
@@ -585,7 +591,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(traceWave
             const BatchEntry& e = bv.entries[r];
             // query records for guided requests whose level the tail kernel does not take
             const bool prep = PREP && e.pv.guideIdx != nullptr && !__shfl(int(be.tail), int(r), 64);
-            extendChunk<BLOCK, S, PREP>(e.sc, e.pv, e.g, extendLevel, count, base, ldsStack + threadIdx.x, prep);
+            extendChunk<BLOCK, S, PREP>(e.sc, e.pv, e.g, extendLevel, count, base, ldsStack + threadIdx.x, prep,
+                                        packetStack[threadIdx.x >> 6]);
         } else if (chunk - be.total < bs.total) {
             batchLocate(bs, chunk - be.total, r, count, base);
             const BatchEntry& e = bv.entries[r];

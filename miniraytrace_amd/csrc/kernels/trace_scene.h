@@ -33,17 +33,14 @@ __device__ __forceinline__ void toObject(const TraceInstance& ti, float ox, floa
     d.z = (ti.row2.x * dx + ti.row2.y * dy) + ti.row2.z * dz;
 }
 
-// Tests triangle k of the current instance.  ANY: returns true when it
-// occludes (t < tmax).  Closest hit: keeps it if it is nearer than `best`, ties
-// going to the lowest (instance, triangle) pair; returns false.
+// Tests triangle k (vertices v0..v2) of the current instance.  ANY: returns
+// true when it occludes (t < tmax).  Closest hit: keeps it if it is nearer
+// than `best`, ties going to the lowest (instance, triangle) pair; returns
+// false.
 template <bool ANY>
-__device__ __forceinline__ bool blasTriangle(const SceneView& sc, const RayK& r, uint32_t k, uint32_t curInst,
-                                             float tmax, float& best, bool& found, uint32_t& bestTri,
-                                             uint32_t& bestInst, float& bestB1, float& bestB2) {
-    const float4* tv = sc.triVerts + 3 * size_t(k);
-    const float4 v0 = ldGlobal4(tv);
-    const float4 v1 = ldGlobal4(tv + 1);
-    const float4 v2 = ldGlobal4(tv + 2);
+__device__ __forceinline__ bool testTriangle(const RayK& r, const float4& v0, const float4& v1, const float4& v2,
+                                             uint32_t k, uint32_t curInst, float tmax, float& best, bool& found,
+                                             uint32_t& bestTri, uint32_t& bestInst, float& bestB1, float& bestB2) {
     float t, b1, b2;
     if (!isectTri(r, v0, v1, v2, best, t, b1, b2)) return false;
     if (ANY) return t < tmax;
@@ -58,6 +55,15 @@ __device__ __forceinline__ bool blasTriangle(const SceneView& sc, const RayK& r,
         bestB2 = b2;
     }
     return false;
+}
+
+template <bool ANY>
+__device__ __forceinline__ bool blasTriangle(const SceneView& sc, const RayK& r, uint32_t k, uint32_t curInst,
+                                             float tmax, float& best, bool& found, uint32_t& bestTri,
+                                             uint32_t& bestInst, float& bestB1, float& bestB2) {
+    const float4* tv = sc.triVerts + 3 * size_t(k);
+    return testTriangle<ANY>(r, ldGlobal4(tv), ldGlobal4(tv + 1), ldGlobal4(tv + 2), k, curInst, tmax, best, found,
+                             bestTri, bestInst, bestB1, bestB2);
 }
 
 // ANY = false: closest hit in (tmin, tmax) -> returns true and fills `out`.
@@ -179,6 +185,148 @@ __device__ bool traceScene(const SceneView& sc, uint32_t tlasRoot, uint32_t inst
     }
     if (stack.overflow) overflow = true;
     if (ANY || !found) return false;
+    out.t = best;
+    out.b1 = bestB1;
+    out.b2 = bestB2;
+    out.tri = bestTri;
+    out.inst = bestInst;
+    return true;
+}
+
+// ------------------------------------------------------------ wave packets --
+// Closest hit for rays that arrive in coherent groups (camera rays: a wave
+// holds an 8 x 8 pixel tile).  The wave walks the trees together: it enters a
+// node when the box test passes for some lane, so every node that a lane's own
+// traversal would enter is entered; culling stays per lane (each lane tests
+// every box against its own limit).  Node, triangle and instance reads are
+// wave-uniform and go through the scalar data cache; each lane keeps its own
+// closest hit with the same tie rule (lowest instance, then triangle), so the
+// hits equal those of traceScene whatever order the nodes are visited in.
+// The stack is one wave-uniform list of node indices (and instances to
+// enter) in LDS; a popped node is loaded again and tested against every
+// lane's current limit.  Must be called by every lane of the wave that is
+// still active in the caller (the control flow inside is wave-uniform).
+constexpr int kPacketStack = kSpill + 8;  // the scene's stack depth is bounded by this on the host
+constexpr uint32_t kPacketInstance = 0x80000000u;
+
+__device__ __forceinline__ uint32_t waveUniform(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// 16-byte read of scene data that no launch writes, through the scalar data
+// cache when the address is wave-uniform.
+__device__ __forceinline__ float4 ldScene4(const float4* p) {
+    const nativeF32x4 v = *(const __attribute__((address_space(4))) nativeF32x4*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ Node loadNodeUniform(const Node* __restrict__ nodes, uint32_t i) {
+    return Node{ ldScene4(&nodes[i].a), ldScene4(&nodes[i].b) };
+}
+
+__device__ bool traceScenePacket(const SceneView& sc, uint32_t tlasRoot, uint32_t instanceCount, float3 o,
+                                 float3 d, float tmin, float tmax, uint32_t* waveStack, SceneHit& out,
+                                 bool& overflow) {
+    if (instanceCount == 0) return false;
+    RayK r = setupRayBox(o.x, o.y, o.z, d.x, d.y, d.z, tmin);
+    float best = tmax;
+    bool found = false;
+    uint32_t bestTri = kNoHit, bestInst = kNoHit;
+    float bestB1 = 0.0f, bestB2 = 0.0f;
+    uint32_t sp = 0, blasBase = 0, curInst = 0;
+    bool inBlas = false;
+    float te;
+    Node cn = loadNodeUniform(sc.nodes, waveUniform(tlasRoot));  // node the wave stands at
+    bool have = __ballot(slab(cn, r, best * kLimitWiden, te)) != 0;
+    uint32_t pendingInst = kNoHit;
+
+    while (true) {
+        if (pendingInst != kNoHit) {
+            // Enter an instance: object-space rays, BLAS root.
+            const TraceInstance* tp = sc.traceInst + pendingInst;
+            TraceInstance ti;
+            ti.row0 = ldScene4(&tp->row0);
+            ti.row1 = ldScene4(&tp->row1);
+            ti.row2 = ldScene4(&tp->row2);
+            ti.blasRoot = waveUniform(__float_as_uint(ldScene4(reinterpret_cast<const float4*>(&tp->blasRoot)).x));
+            float3 oo, dd;
+            toObject(ti, o.x, o.y, o.z, d.x, d.y, d.z, oo, dd);
+            r = setupRay(oo.x, oo.y, oo.z, dd.x, dd.y, dd.z, tmin);
+            inBlas = true;
+            blasBase = sp;
+            curInst = pendingInst;
+            pendingInst = kNoHit;
+            cn = loadNodeUniform(sc.nodes, ti.blasRoot);
+            have = __ballot(slab(cn, r, best * kLimitWiden, te)) != 0;
+        }
+        if (have) {
+            const uint32_t lp = waveUniform(nodeLeft(cn));
+            const uint32_t cnt = waveUniform(nodeCount(cn));
+            if (cnt == 0) {
+                const Node c0 = loadNodeUniform(sc.nodes, lp);
+                const Node c1 = loadNodeUniform(sc.nodes, lp + 1);
+                const float lim = best * kLimitWiden;
+                float t0, t1;
+                const bool h0 = slab(c0, r, lim, t0);
+                const bool h1 = slab(c1, r, lim, t1);
+                const uint64_t m0 = __ballot(h0), m1 = __ballot(h1);
+                if (m0 != 0 && m1 != 0) {
+                    // Both children: the one nearer for more lanes first.
+                    const uint64_t near0 = __ballot(h0 && (!h1 || t0 <= t1));
+                    const uint64_t near1 = __ballot(h1 && (!h0 || t1 < t0));
+                    const bool first0 = __popcll(near0) >= __popcll(near1);
+                    if (sp < uint32_t(kPacketStack)) {
+                        waveStack[sp] = first0 ? lp + 1 : lp;
+                        ++sp;
+                    } else {
+                        overflow = true;
+                    }
+                    cn = first0 ? c0 : c1;
+                    continue;
+                }
+                if (m0 != 0 || m1 != 0) {
+                    cn = m0 != 0 ? c0 : c1;
+                    continue;
+                }
+            } else if (inBlas) {
+                for (uint32_t k = lp; k < lp + cnt; ++k) {
+                    const float4* tv = sc.triVerts + 3 * size_t(k);
+                    testTriangle<false>(r, ldScene4(tv), ldScene4(tv + 1), ldScene4(tv + 2), k, curInst, tmax, best,
+                                        found, bestTri, bestInst, bestB1, bestB2);
+                }
+            } else {
+                // TLAS leaf: instances [lp, lp + cnt), the first entered now.
+                for (uint32_t j = cnt - 1; j >= 1; --j) {
+                    if (sp < uint32_t(kPacketStack)) {
+                        waveStack[sp] = kPacketInstance | (lp + j);
+                        ++sp;
+                    } else {
+                        overflow = true;
+                    }
+                }
+                pendingInst = lp;
+                continue;
+            }
+        }
+        have = false;
+        while (true) {
+            if (inBlas && sp == blasBase) {
+                inBlas = false;
+                r = setupRayBox(o.x, o.y, o.z, d.x, d.y, d.z, tmin);
+            }
+            if (sp == 0) break;
+            --sp;
+            const uint32_t e = waveUniform(waveStack[sp]);
+            if (e & kPacketInstance) {
+                pendingInst = e & ~kPacketInstance;
+                break;
+            }
+            cn = loadNodeUniform(sc.nodes, e);
+            if (__ballot(slab(cn, r, best * kLimitWiden, te)) != 0) {
+                have = true;
+                break;
+            }
+        }
+        if (!have && pendingInst == kNoHit) break;
+    }
+    if (!found) return false;
     out.t = best;
     out.b1 = bestB1;
     out.b2 = bestB2;

@@ -205,18 +205,21 @@ class Engine:
         L.lib().mrtResetAccumulation(self._h)
 
     # -- extensions --------------------------------------------------------------------
-    def set_path_options(self, seed=0, use_graph=True, extend=(0, 0), shadow=(0, 0), tail_paths=0) -> None:
-        """tail_paths: 0 = default, L.TAIL_DISABLED = never use the tail kernel."""
+    def set_path_options(self, seed=0, use_graph=True, extend=(0, 0), shadow=(0, 0), tail_paths=0,
+                         packet_camera=True) -> None:
+        """tail_paths: 0 = default, L.TAIL_DISABLED = never use the tail kernel.
+        packet_camera: trace camera rays by wave packets (same hits either way)."""
         o = L.mrtxPathOptions(seed=seed, useGraph=int(use_graph), extendBlock=extend[0],
                               extendStack=extend[1], shadowBlock=shadow[0], shadowStack=shadow[1],
-                              tailPaths=tail_paths)
+                              tailPaths=tail_paths, cameraPerLane=int(not packet_camera))
         L.check(L.lib().mrtxSetPathOptions(self._h, C.byref(o)), "mrtxSetPathOptions")
 
     def path_options(self) -> dict:
         o = L.mrtxPathOptions()
         L.check(L.lib().mrtxGetPathOptions(self._h, C.byref(o)), "mrtxGetPathOptions")
         return {"seed": o.seed, "use_graph": bool(o.useGraph), "extend": (o.extendBlock, o.extendStack),
-                "shadow": (o.shadowBlock, o.shadowStack), "tail_paths": o.tailPaths}
+                "shadow": (o.shadowBlock, o.shadowStack), "tail_paths": o.tailPaths,
+                "packet_camera": not o.cameraPerLane}
 
     # -- path guiding (mini_raytrace_ext.h section 5) --
     def set_guiding(self, sampling: bool = True, training: bool = True, select_prob: float = 0.5,
