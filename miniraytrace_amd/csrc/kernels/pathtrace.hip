@@ -562,11 +562,12 @@ __device__ __forceinline__ void shadowChunk(const SceneView& sc, const PathView&
 // hits only write hit records -- so the slowest rays of one overlap with the
 // other's work instead of each ending a launch of its own.  Chunks
 // [0, extend chunks) of the shared cursor are closest hit, the rest any hit.
-// Register budget: 6 waves per SIMD (80 VGPRs).  Traversal waits on
+// Register budget: 6 waves per SIMD (up to 80 VGPRs).  Traversal waits on
 // dependent node fetches, and the sixth wave hides more of that latency than
 // the registers it costs; at 8 waves (64 VGPRs) the spills outweigh it.
 // Shapes whose LDS stacks allow fewer waves per SIMD (160 KB of LDS per CU,
-// S x 8 bytes per lane) get the registers of the waves they can hold.
+// S x 8 bytes per lane and a camera-ray packet stack per wave) get the
+// registers of the waves they can hold.
 constexpr int traceWavesPerSimd(int block, int stack) {
     const int blocksPerCU = (160 * 1024) / (stack * 8 * block + (block / 64) * kPacketStack * 4);
     const int waves = blocksPerCU * (block / 64) / 4;
