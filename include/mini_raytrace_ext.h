@@ -291,6 +291,23 @@ MRT_API mrtResult mrtxGuideNetworkGradient(mrtxGuideNetwork* net, const float* r
 /* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the records' gradient. */
 MRT_API mrtResult mrtxGuideNetworkTrain(mrtxGuideNetwork* net, const float* records, uint32_t count,
                                         float learningRate, float* loss);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (work under HIP graph capture, tight serving or online-training
+ * loops): the step of mrtxGuideNetworkTrain on count device-resident records
+ * (same 16-float layout), enqueued on hipStream (NULL = the network's own
+ * stream; another stream is ordered against the network's other calls by the
+ * caller).  It calls no hipMalloc, hipFree or other allocating routine: the
+ * training workspace lives in one fixed device array per GPU, sized for
+ * MRTX_GUIDE_NOALLOC_MAX_RECORDS records (about 13 MB, held until the process
+ * ends); more records return MRT_ERROR_INVALID_ARGUMENT.  Every network on a
+ * GPU shares that GPU's array, so calls that overlap on one GPU (other
+ * streams or threads) race.  loss may be NULL; when given, the call waits for
+ * the stream and stores the step's loss (leave it NULL under graph capture).
+ * Same parameters, Adam state and loss, bit for bit, as mrtxGuideNetworkTrain
+ * on the same records. */
+#define MRTX_GUIDE_NOALLOC_MAX_RECORDS 8192u
+MRT_API mrtResult mrtxGuideNetworkTrainNoAlloc(mrtxGuideNetwork* net, const float* dRecords, uint32_t count,
+                                               float learningRate, void* hipStream, float* loss);
 /* Mixture sampling from raw outputs: u holds 3 uniforms per row; out holds
  * (direction.xyz, mixture pdf) per row. */
 MRT_API mrtResult mrtxGuideNetworkSample(mrtxGuideNetwork* net, const float* raw, const float* u,

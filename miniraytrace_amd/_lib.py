@@ -153,6 +153,7 @@ class mrtxBatchStats(_Sized):
 
 MRTX_GUIDE_PARAM_COUNT = 20136
 MRTX_GUIDE_OUTPUTS = 40
+MRTX_GUIDE_NOALLOC_MAX_RECORDS = 8192
 
 
 class mrtxGuidingOptions(_Sized):
@@ -249,6 +250,7 @@ _SIGS = {
     "mrtxGuideNetworkInfer": (_R, [_P, _P, _U32, _P]),
     "mrtxGuideNetworkGradient": (_R, [_P, _P, _U32, _P, C.c_size_t, C.POINTER(C.c_float)]),
     "mrtxGuideNetworkTrain": (_R, [_P, _P, _U32, C.c_float, C.POINTER(C.c_float)]),
+    "mrtxGuideNetworkTrainNoAlloc": (_R, [_P, _P, _U32, C.c_float, _P, C.POINTER(C.c_float)]),
     "mrtxGuideNetworkSample": (_R, [_P, _P, _P, _U32, _P]),
     "mrtxGuideNetworkPdf": (_R, [_P, _P, _P, _U32, _P]),
 }

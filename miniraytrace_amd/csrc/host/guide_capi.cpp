@@ -9,6 +9,7 @@
 
 #include <cstring>
 #include <memory>
+#include <string>
 
 struct mrtxGuideNetwork {
     int device = 0;
@@ -225,6 +226,34 @@ MRT_API mrtResult mrtxGuideNetworkTrain(mrtxGuideNetwork* n, const float* record
         uint32_t w[mrt::dev::kGwWords];
         n->net->readWords(w, n->stream);
         if (loss) *loss = wordAsFloat(w, mrt::dev::kGwLastLoss);
+        return MRT_SUCCESS;
+    });
+}
+
+MRT_API mrtResult mrtxGuideNetworkTrainNoAlloc(mrtxGuideNetwork* n, const float* dRecords, uint32_t count,
+                                               float learningRate, void* hipStream, float* loss) {
+    if (!n) return MRT_ERROR_INVALID_HANDLE;
+    if (!dRecords || count == 0 || !(learningRate > 0.0f)) return MRT_ERROR_INVALID_ARGUMENT;
+    if (count > MRTX_GUIDE_NOALLOC_MAX_RECORDS) {
+        mrt::setLastErrorMessage("mrtxGuideNetworkTrainNoAlloc: " + std::to_string(count) +
+                                 " records exceed the fixed workspace of " +
+                                 std::to_string(MRTX_GUIDE_NOALLOC_MAX_RECORDS));
+        return MRT_ERROR_INVALID_ARGUMENT;
+    }
+    return mrt::guarded([&] {
+        mrt::DeviceScope scope(n->device);
+        hipStream_t s = hipStream ? static_cast<hipStream_t>(hipStream) : n->stream;
+        mrt::dev::GuideAdam h;
+        h.lr = learningRate;
+        MRT_HIP(mrt::dev::launchGuideTrainNoAlloc(n->net->view(), reinterpret_cast<const float4*>(dRecords), count,
+                                                  h, n->device, s));
+        if (loss) {
+            uint32_t w = 0;
+            MRT_HIP(hipMemcpyAsync(&w, n->net->deviceWords() + mrt::dev::kGwLastLoss, sizeof(w),
+                                   hipMemcpyDeviceToHost, s));
+            MRT_HIP(hipStreamSynchronize(s));
+            *loss = wordAsFloat(&w, 0);
+        }
         return MRT_SUCCESS;
     });
 }

@@ -104,6 +104,16 @@ hipError_t launchGuideGradient(const GuideBatch& b, const GuideNet& net, bool co
 // the forward image.
 hipError_t launchGuideAdam(const GuideNet& net, const GuideAdam& h, hipStream_t s);
 
+// One training step (gradient + Adam) on `count` device-resident training
+// records (4 float4 each) with the workspace in the library's fixed device
+// array (guide_noalloc.hip) instead of state's own: only the parameters,
+// Adam moments, gradient, image and state words of `state` are used.
+// count must be 1..guideNoAllocMaxRows(); `device` is the current device.
+// Allocates nothing.  Calls that overlap on one device race on the array.
+hipError_t launchGuideTrainNoAlloc(const GuideNet& state, const float4* records, uint32_t count, const GuideAdam& h,
+                                   int device, hipStream_t s);
+uint32_t guideNoAllocMaxRows();
+
 // Decodes raw outputs and samples the mixture with the given uniforms:
 // out row i = (direction.xyz, mixture pdf of that direction).
 hipError_t launchGuideSample(const float* raw, const float* u, uint32_t count, float4* out, hipStream_t s);

@@ -100,6 +100,19 @@ class GuideNetwork:
                 "mrtxGuideNetworkTrain")
         return float(loss.value)
 
+    def train_step_no_alloc(self, records_ptr: int, count: int, lr: float = 1e-3, stream: int = 0,
+                            want_loss: bool = True):
+        """mrtxGuideNetworkTrainNoAlloc: one step on `count` device-resident
+        records at address records_ptr (16 floats each), enqueued on the HIP
+        stream handle `stream` (0 = the network's own stream).  Returns the
+        loss (the call then waits for the stream) or None."""
+        loss = C.c_float()
+        L.check(L.lib().mrtxGuideNetworkTrainNoAlloc(self._h, C.c_void_p(records_ptr), count, lr,
+                                                     C.c_void_p(stream) if stream else None,
+                                                     C.byref(loss) if want_loss else None),
+                "mrtxGuideNetworkTrainNoAlloc")
+        return float(loss.value) if want_loss else None
+
     def sample(self, raw: np.ndarray, u: np.ndarray) -> np.ndarray:
         """(N, 4): sampled unit direction xyz and the mixture pdf of it."""
         raw = np.ascontiguousarray(raw, np.float32)
