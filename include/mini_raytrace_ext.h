@@ -180,7 +180,17 @@ typedef struct mrtxSceneExport {
     void*    materials;    /* materialCount x 80 B (index 0 = default) */
     void*    lights;       /* lightCount x 80 B (environment first when present) */
     void*    envAlias;     /* envWidth*envHeight x (float prob, uint32 alias) */
+    /* Appended fields (a structSize that ends at envAlias is still accepted
+     * and leaves them untouched): */
+    float*   tangents;     /* vertexCount x 4 floats, w = handedness */
+    void*    textureTable; /* MRTX_TEXTURE_SLOTS x 16 B: texel offset into its pool, width, height,
+                              format (width 0 = empty slot) */
+    uint32_t tex8Count;    /* texels in the RGBA8 pool (one uint32 each) */
+    uint32_t tex32Count;   /* texels in the RGBA32F pool (4 floats each) */
+    void*    tex8;
+    float*   tex32;
 } mrtxSceneExport;
+#define MRTX_TEXTURE_SLOTS 64
 
 MRT_API mrtResult mrtxExportScene(mrtEngine* engine, mrtxSceneExport* inout);
 

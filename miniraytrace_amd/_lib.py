@@ -154,6 +154,7 @@ class mrtxBatchStats(_Sized):
 MRTX_GUIDE_PARAM_COUNT = 20136
 MRTX_GUIDE_OUTPUTS = 40
 MRTX_GUIDE_NOALLOC_MAX_RECORDS = 8192
+MRTX_TEXTURE_SLOTS = 64
 
 
 class mrtxGuidingOptions(_Sized):
@@ -177,7 +178,9 @@ class mrtxSceneExport(_Sized):
                 ("envIntegral", C.c_float), ("nodes", C.c_void_p), ("triangles", C.c_void_p),
                 ("positions", C.c_void_p), ("normals", C.c_void_p), ("uvs", C.c_void_p),
                 ("instances", C.c_void_p), ("materials", C.c_void_p), ("lights", C.c_void_p),
-                ("envAlias", C.c_void_p)]
+                ("envAlias", C.c_void_p), ("tangents", C.c_void_p), ("textureTable", C.c_void_p),
+                ("tex8Count", C.c_uint32), ("tex32Count", C.c_uint32), ("tex8", C.c_void_p),
+                ("tex32", C.c_void_p)]
 
 
 LOG_FN = C.CFUNCTYPE(None, C.c_int, C.c_char_p, C.c_void_p)

@@ -4,11 +4,14 @@
 // library was built with; Add* return MRT_INVALID_ID on failure; void
 // mutators ignore bad handles; exceptions never cross the boundary.
 #include "capi_handles.h"
+
+#include <cstddef>
 #include "common.h"
 #include "engine.h"
 #include "mini_raytrace.h"
 #include "mini_raytrace_ext.h"
 
+#include <cstddef>
 #include <memory>
 
 namespace {
@@ -255,7 +258,8 @@ MRT_API mrtResult mrtxGetFrameStats(mrtEngine* e, mrtxFrameStats* out) {
 MRT_API mrtResult mrtxExportScene(mrtEngine* e, mrtxSceneExport* io) {
     mrt::Engine* eng = unwrap(e);
     if (!eng) return MRT_ERROR_INVALID_HANDLE;
-    if (!validStruct(io)) return MRT_ERROR_INVALID_ARGUMENT;
+    // the fields from `tangents` on were appended: the shorter struct is accepted
+    if (!io || io->structSize < offsetof(mrtxSceneExport, tangents)) return MRT_ERROR_INVALID_ARGUMENT;
     return mrt::guarded([&] {
         eng->exportScene(*io);
         return MRT_SUCCESS;

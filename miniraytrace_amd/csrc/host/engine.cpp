@@ -625,6 +625,8 @@ void Engine::uploadTextures() {
     }
     if (tex.size() > kMaxTextureSlots)
         logMessage(LogLevel::Warn, "texture count exceeds 64 slots; extra textures ignored");
+    tex8Texels_ = pool8.size();
+    tex32Texels_ = pool32.size() / 4;
     uploadVec(dTexTable_, table.data(), table.size() * sizeof(GpuTexture));
     uploadVec(dTex8_, pool8.data(), pool8.size() * 4);
     uploadVec(dTex32_, pool32.data(), pool32.size() * 4);
@@ -1233,6 +1235,13 @@ void Engine::exportScene(mrtxSceneExport& o) {
     if (o.materials) dMats_.download(o.materials, size_t(o.materialCount) * sizeof(GpuMaterial));
     if (o.lights) dLights_.download(o.lights, size_t(lightCount_) * sizeof(GpuLight));
     if (o.envAlias && hasEnv_) dAlias_.download(o.envAlias, size_t(envW_) * envH_ * sizeof(AliasEntry));
+    if (o.structSize < sizeof(mrtxSceneExport)) return;
+    o.tex8Count = uint32_t(tex8Texels_);
+    o.tex32Count = uint32_t(tex32Texels_);
+    if (o.tangents) dTan_.download(o.tangents, totalVerts_ * 16);
+    if (o.textureTable) dTexTable_.download(o.textureTable, size_t(kMaxTextureSlots) * sizeof(GpuTexture));
+    if (o.tex8 && o.tex8Count) dTex8_.download(o.tex8, size_t(o.tex8Count) * 4);
+    if (o.tex32 && o.tex32Count) dTex32_.download(o.tex32, size_t(o.tex32Count) * 16);
 }
 
 } // namespace mrt

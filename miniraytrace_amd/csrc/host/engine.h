@@ -170,6 +170,7 @@ private:
     DeviceBuffer dNodes_, dTriVerts_, dTris_, dPos_, dNrm_, dUv_, dTan_;
     DeviceBuffer dInst_, dTraceInst_, dMats_, dLights_, dEnv_, dAlias_;
     DeviceBuffer dTexTable_, dTex8_, dTex32_;
+    size_t tex8Texels_ = 0, tex32Texels_ = 0;  // texels held by the two pools
     // frame buffers
     DeviceBuffer dConsts_, dBeta_, dRad_, dRng_, dRayO_[2], dRayD_[2], dHit_, dHitInst_;
     DeviceBuffer dShO_, dShD_, dShC_, dCounters_, dAccum_, dOut8_, dOut32_;

@@ -282,6 +282,10 @@ class Engine:
             "lights": np.zeros((e.lightCount, 20), dtype=np.float32),
             "envAlias": np.zeros((e.envWidth * e.envHeight, 2), dtype=np.uint32),
         }
+        out["tangents"] = np.zeros((e.vertexCount, 4), dtype=np.float32)
+        out["textureTable"] = np.zeros((L.MRTX_TEXTURE_SLOTS, 4), dtype=np.uint32)
+        out["tex8"] = np.zeros((e.tex8Count, 4), dtype=np.uint8)
+        out["tex32"] = np.zeros((e.tex32Count, 4), dtype=np.float32)
         for k, a in out.items():
             setattr(e, k, a.ctypes.data if a.size else None)
         L.check(L.lib().mrtxExportScene(self._h, C.byref(e)), "mrtxExportScene")

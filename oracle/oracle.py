@@ -180,7 +180,18 @@ class _PtScene(C.Structure):
                 ("triangles", C.c_void_p), ("positions", C.c_void_p), ("normals", C.c_void_p),
                 ("instances", C.c_void_p), ("materials", C.c_void_p), ("lights", C.c_void_p),
                 ("lightCount", C.c_uint32), ("env", C.c_void_p), ("envW", C.c_uint32), ("envH", C.c_uint32),
-                ("envRotation", C.c_float), ("envIntensity", C.c_float), ("envAlias", C.c_void_p)]
+                ("envRotation", C.c_float), ("envIntensity", C.c_float), ("envAlias", C.c_void_p),
+                ("uvs", C.c_void_p), ("tangents", C.c_void_p), ("textures", C.c_void_p),
+                ("textureCount", C.c_uint32)]
+
+
+class _Texture(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("format", C.c_uint32), ("pixels", C.c_void_p)]
+
+
+class _BsdfParams(C.Structure):
+    _fields_ = [("base", C.c_double * 3), ("roughness", C.c_double), ("metallic", C.c_double),
+                ("transmission", C.c_double), ("ior", C.c_double), ("entering", C.c_int32)]
 
 
 class _PtCamera(C.Structure):
@@ -203,6 +214,92 @@ def _bind_pt(h):
                                    C.c_uint32, C.POINTER(_PtCamera)]
     h.or_env_cdf_probs.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
     h.or_env_cdf_probs.restype = C.c_double
+    h.or_bsdf_eval.argtypes = [C.POINTER(_BsdfParams), C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p]
+    h.or_bsdf_eval.restype = None
+    h.or_bsdf_sample.argtypes = [C.POINTER(_BsdfParams), C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
+                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    h.or_bsdf_sample.restype = None
+    h.or_light_sample.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
+    h.or_light_sample.restype = None
+    h.or_texture_sample.argtypes = [C.POINTER(_Texture), C.c_double, C.c_double, C.c_void_p]
+    h.or_texture_sample.restype = None
+
+
+def _bsdf(base, roughness, metallic=0.0, transmission=0.0, ior=1.5, entering=True) -> _BsdfParams:
+    m = _BsdfParams()
+    m.base[:] = [float(x) for x in base]
+    m.roughness, m.metallic, m.transmission, m.ior = roughness, metallic, transmission, ior
+    m.entering = int(bool(entering))
+    return m
+
+
+def bsdf_eval(material: dict, wo, wi):
+    """(f (N, 3), combined lobe pdf (N,)) of bsdf.glsl:100-144 in the local
+    frame for one wo and wi (N, 3).  material: base, roughness, metallic,
+    transmission, ior, entering."""
+    _bind_pt(lib())
+    m = _bsdf(**material)
+    wo = np.ascontiguousarray(wo, np.float64)
+    wi = np.ascontiguousarray(np.atleast_2d(wi), np.float64)
+    n = wi.shape[0]
+    f = np.zeros((n, 3))
+    pdf = np.zeros(n)
+    lib().or_bsdf_eval(C.byref(m), wo.ctypes.data, n, wi.ctypes.data, f.ctypes.data, pdf.ctypes.data)
+    return f, pdf
+
+
+def bsdf_sample(material: dict, wo, u: np.ndarray):
+    """bsdfSample (bsdf.glsl:147-192) for each row of u (pick, u0, u1,
+    Fresnel choice).  Returns (wi (N, 3), f (N, 3), pdf (N,), branch (N,),
+    valid (N,) bool)."""
+    _bind_pt(lib())
+    m = _bsdf(**material)
+    wo = np.ascontiguousarray(wo, np.float64)
+    u = np.ascontiguousarray(u, np.float64)
+    n = u.shape[0]
+    wi = np.zeros((n, 3))
+    f = np.zeros((n, 3))
+    pdf = np.zeros(n)
+    branch = np.zeros(n, np.int32)
+    valid = np.zeros(n, np.uint8)
+    lib().or_bsdf_sample(C.byref(m), wo.ctypes.data, n, u.ctypes.data, wi.ctypes.data, f.ctypes.data,
+                         pdf.ctypes.data, branch.ctypes.data, valid.ctypes.data)
+    return wi, f, pdf, branch, valid.astype(bool)
+
+
+def light_sample(light20, pick_count: int, p, u):
+    """Analytic light sample (lights.glsl:70-134 after the pick): dict with
+    wi, radiance, pdf, dist, delta, valid."""
+    _bind_pt(lib())
+    l = _f32(light20)
+    p = np.ascontiguousarray(p, np.float64)
+    u = np.ascontiguousarray(u, np.float64)
+    out = np.zeros(10)
+    lib().or_light_sample(l.ctypes.data, pick_count, p.ctypes.data, u.ctypes.data, out.ctypes.data)
+    return {"wi": out[0:3], "radiance": out[3:6], "pdf": out[6], "dist": out[7], "delta": bool(out[8]),
+            "valid": bool(out[9])}
+
+
+def _texture_struct(tex: dict | None):
+    t = _Texture()
+    if tex is None:
+        return t, None
+    px = np.ascontiguousarray(tex["pixels"], np.float32 if tex["format"] == 2 else np.uint8)
+    t.height, t.width = px.shape[:2]
+    t.format = tex["format"]
+    t.pixels = px.ctypes.data
+    return t, px
+
+
+def texture_sample(tex: dict, u: float, v: float) -> np.ndarray:
+    """Level-0 bilinear lookup (REPEAT) decoded to linear RGBA.  tex: pixels
+    (H, W, 4) uint8 or float32, format 0 sRGB8 / 1 UNORM8 / 2 RGBA32F."""
+    _bind_pt(lib())
+    t, keep = _texture_struct(tex)
+    out = np.zeros(4)
+    lib().or_texture_sample(C.byref(t), float(u), float(v), out.ctypes.data)
+    del keep
+    return out
 
 
 def camera_lookat(position, target, up, fov_y_deg, width, height) -> _PtCamera:
@@ -229,9 +326,11 @@ def pt_render(exported: dict, env: np.ndarray | None, camera: _PtCamera, width: 
               crop=(0, 0, None, None), frame0: int = 0, spp: int = 1, max_bounces: int = 8,
               firefly_clamp: float = 0.0, env_rotation: float = 0.0, env_intensity: float = 1.0,
               use_alias: bool = False, seed_mix: int = 0, threads: int = 0, count_fetches: bool = False):
-    """Renders a crop with the FP64 restatement.  `exported` is the packed
-    scene from Engine.export_scene().  Returns (sum of radiance over the spp
-    frames, shape (ch, cw, 3), counts[6] or None)."""
+    """Renders a crop with the FP64 restatement.  `exported` is a packed
+    scene (tests/scene_pack.py, or Engine.export_scene()); optional keys
+    "uvs" (V, 2), "tangents" (V, 4) and "textures" (list of slot dicts for
+    texture_sample, None = empty slot).  Returns (sum of radiance over the
+    spp frames, shape (ch, cw, 3), counts[6] or None)."""
     _bind_pt(lib())
     x0, y0, cw, ch = crop
     cw = width - x0 if cw is None else cw
@@ -257,6 +356,19 @@ def pt_render(exported: dict, env: np.ndarray | None, camera: _PtCamera, width: 
         sc.envIntensity = env_intensity
         if use_alias:
             sc.envAlias = keep["envAlias"].ctypes.data
+    for k in ("uvs", "tangents"):
+        if k in keep:
+            keep[k] = np.ascontiguousarray(keep[k], np.float32)
+            setattr(sc, k, keep[k].ctypes.data)
+    texs = exported.get("textures") or []
+    tex_arr = (_Texture * max(len(texs), 1))()
+    tex_keep = []
+    for i, t in enumerate(texs):
+        tex_arr[i], px = _texture_struct(t)
+        tex_keep.append(px)
+    if texs:
+        sc.textures = C.cast(tex_arr, C.c_void_p)
+        sc.textureCount = len(texs)
     p = _PtParams(width=width, height=height, x0=x0, y0=y0, cw=cw, ch=ch, frame0=frame0, spp=spp,
                   maxBounces=max_bounces, fireflyClamp=firefly_clamp, seedMix=seed_mix, threads=threads,
                   countFetches=int(count_fetches))

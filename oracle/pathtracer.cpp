@@ -142,6 +142,46 @@ double buildCdf(const float* px, uint32_t w, uint32_t h, std::vector<float>& cdf
     return integral;
 }
 
+// ----------------------------------------------------------------- textures --
+// Level-0 bilinear lookup with REPEAT addressing (the reference's sampler,
+// GpuContext.cpp:190-198; a compute shader's implicit LOD is the base level),
+// texels decoded to linear first (sRGB formats decode before filtering).
+using D4 = std::array<double, 4>;
+double srgbDecode(double c) { return c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4); }
+D4 texelOf(const OrTexture& t, uint32_t x, uint32_t y) {
+    const size_t i = size_t(y) * t.width + x;
+    if (t.format == 2) {
+        const float* p = static_cast<const float*>(t.pixels) + i * 4;
+        return { p[0], p[1], p[2], p[3] };
+    }
+    const uint8_t* p = static_cast<const uint8_t*>(t.pixels) + i * 4;
+    D4 c{ p[0] / 255.0, p[1] / 255.0, p[2] / 255.0, p[3] / 255.0 };
+    if (t.format == 0)
+        for (int k = 0; k < 3; ++k) c[k] = srgbDecode(c[k]);
+    return c;
+}
+D4 texSample(const OrTexture* t, double u, double v) {
+    if (!t || t->width == 0 || t->height == 0 || !t->pixels) return { 1, 1, 1, 1 };  // dummy: white
+    // non-finite or huge coordinates read texel (0, 0)'s neighbourhood
+    if (!(u == u) || !(v == v) || std::fabs(u) > 1e7 || std::fabs(v) > 1e7) u = v = 0;
+    const double fx = u * t->width - 0.5, fy = v * t->height - 0.5;
+    const double x0f = std::floor(fx), y0f = std::floor(fy);
+    auto wrap = [](double i, uint32_t n) {
+        long m = long(i) % long(n);
+        return uint32_t(m < 0 ? m + long(n) : m);
+    };
+    const uint32_t x0 = wrap(x0f, t->width), y0 = wrap(y0f, t->height);
+    const uint32_t x1 = (x0 + 1) % t->width, y1 = (y0 + 1) % t->height;
+    const double ax = fx - x0f, ay = fy - y0f;
+    const D4 c00 = texelOf(*t, x0, y0), c10 = texelOf(*t, x1, y0), c01 = texelOf(*t, x0, y1), c11 = texelOf(*t, x1, y1);
+    D4 r;
+    for (int k = 0; k < 4; ++k) {
+        const double a = c00[k] * (1 - ax) + c10[k] * ax, b = c01[k] * (1 - ax) + c11[k] * ax;
+        r[k] = a * (1 - ay) + b * ay;
+    }
+    return r;
+}
+
 // -------------------------------------------------------------------- scene --
 struct Inst {
     double o2w[16], w2o[16];
@@ -498,28 +538,33 @@ D3 refractD(D3 i, D3 n, double eta) {
     if (k < 0) return d3(0, 0, 0);
     return i * eta - n * (eta * ni + std::sqrt(k));
 }
-bool bsdfSample(const Bsdf& m, D3 wo, Rng& rng, D3& wi, D3& f, double& pdf) {
+// bsdf.glsl:147-192 with the draws given: pick, (u0, u1), and fres() for the
+// transmission lobe's Fresnel choice (drawn only on that lobe, after u0/u1).
+template <class Fres>
+bool bsdfSampleU(const Bsdf& m, D3 wo, double pick, double u0, double u1, Fres fres, D3& wi, D3& f, double& pdf,
+                 int& branch) {
     const double a = std::max(m.rough * m.rough, 1e-3);
     double wd, ws, wt;
     lobes(m, wo, wd, ws, wt);
     const double eta = m.entering ? m.ior : 1 / m.ior;
-    const double pick = rng.r1();
-    double u0, u1;
-    rng.r2(u0, u1);
     if (pick < wd) {
+        branch = 0;
         const double r = std::sqrt(u0), phi = kTwoPi * u1;
         wi = d3(r * std::cos(phi), r * std::sin(phi), std::sqrt(std::max(1 - u0, 0.0)));
     } else if (pick < wd + ws) {
+        branch = 1;
         const D3 h = vndf(wo, a, u0, u1);
         wi = reflectD(-wo, h);
         if (wi.z <= 0) return false;
     } else {
         const D3 h = vndf(wo, a, u0, u1);
         const double F = fresnelD(std::fabs(dot(wo, h)), eta);
-        if (rng.r1() < F) {
+        if (fres() < F) {
+            branch = 2;
             wi = reflectD(-wo, h);
             if (wi.z <= 0) return false;
         } else {
+            branch = 3;
             wi = refractD(-wo, h, 1 / eta);
             if (wi.z >= 0 || len(wi) < 1e-6) return false;
             wi = norm(wi);
@@ -527,6 +572,13 @@ bool bsdfSample(const Bsdf& m, D3 wo, Rng& rng, D3& wi, D3& f, double& pdf) {
     }
     bsdfEval(m, wo, wi, f, pdf);
     return pdf >= 1e-9;
+}
+bool bsdfSample(const Bsdf& m, D3 wo, Rng& rng, D3& wi, D3& f, double& pdf) {
+    const double pick = rng.r1();
+    double u0, u1;
+    rng.r2(u0, u1);
+    int branch;
+    return bsdfSampleU(m, wo, pick, u0, u1, [&] { return rng.r1(); }, wi, f, pdf, branch);
 }
 
 // ------------------------------------------------------------------- lights --
@@ -562,25 +614,14 @@ bool sampleEnvD(const Env& env, Rng& rng, D3& dir, D3& rad, double& pdf) {
     pdf = env.pdf(dir);
     return pdf > 1e-9;
 }
-LS sampleLight(const Scene& sc, D3 p, Rng& rng) {
+// Analytic light record (sun, point, rect) at p; u0, u1 = the light's rng2
+// (unused for a point light).
+LS sampleAnalytic(const float* l, double pick, D3 p, double u0, double u1) {
     LS ls;
-    const OrPtScene& s = *sc.s;
-    if (s.lightCount == 0) return ls;
-    uint32_t idx = uint32_t(float(rng.r1()) * float(s.lightCount));
-    idx = std::min(idx, s.lightCount - 1);
-    const double pick = 1.0 / s.lightCount;
-    const float* l = s.lights + size_t(idx) * 20;
     int32_t type;
     std::memcpy(&type, l + 16, 4);
-    if (type == 0) {
-        D3 dir, rad;
-        double pdf;
-        if (!sampleEnvD(sc.env, rng, dir, rad, pdf)) return ls;
-        ls.wi = dir; ls.rad = rad; ls.pdf = pdf * pick; ls.dist = 1e30; ls.valid = true;
-    } else if (type == 1) {
+    if (type == 1) {
         const double cosMax = l[3];
-        double u0, u1;
-        rng.r2(u0, u1);
         const double cosT = 1 + (cosMax - 1) * u0, sinT = std::sqrt(std::max(1 - cosT * cosT, 0.0));
         const double phi = kTwoPi * u1;
         D3 t, b;
@@ -597,9 +638,7 @@ LS sampleLight(const Scene& sc, D3 p, Rng& rng) {
         ls.wi = dv / ls.dist;
         ls.rad = f3(l + 4) / d2;
         ls.pdf = pick; ls.delta = true; ls.valid = true;
-    } else {
-        double u0, u1;
-        rng.r2(u0, u1);
+    } else if (type == 3) {
         const D3 q = f3(l) + f3(l + 4) * u0 + f3(l + 8) * u1;
         const D3 dv = q - p;
         const double d2 = std::max(dot(dv, dv), 1e-8);
@@ -614,6 +653,28 @@ LS sampleLight(const Scene& sc, D3 p, Rng& rng) {
         ls.delta = true; ls.valid = true;
     }
     return ls;
+}
+
+LS sampleLight(const Scene& sc, D3 p, Rng& rng) {
+    LS ls;
+    const OrPtScene& s = *sc.s;
+    if (s.lightCount == 0) return ls;
+    uint32_t idx = uint32_t(float(rng.r1()) * float(s.lightCount));
+    idx = std::min(idx, s.lightCount - 1);
+    const double pick = 1.0 / s.lightCount;
+    const float* l = s.lights + size_t(idx) * 20;
+    int32_t type;
+    std::memcpy(&type, l + 16, 4);
+    if (type == 0) {
+        D3 dir, rad;
+        double pdf;
+        if (!sampleEnvD(sc.env, rng, dir, rad, pdf)) return ls;
+        ls.wi = dir; ls.rad = rad; ls.pdf = pdf * pick; ls.dist = 1e30; ls.valid = true;
+        return ls;
+    }
+    double u0 = 0, u1 = 0;
+    if (type == 1 || type == 3) rng.r2(u0, u1);
+    return sampleAnalytic(l, pick, p, u0, u1);
 }
 
 double balance(double a, double b) { return a / std::max(a + b, 1e-12); }
@@ -670,12 +731,55 @@ D3 samplePath(const Scene& sc, const OrPtCamera& cam, const OrPtParams& P, uint3
         const bool front = dot(d, ng) < 0;
         if (!front) { ng = -ng; ns = -ns; }
         const float* mf = s.materials + size_t(in.material) * 20;
-        const Mat m{ f3(mf), f3(mf + 4), mf[3], mf[7], mf[8], mf[9], mf[10] };
+        Mat m{ f3(mf), f3(mf + 4), mf[3], mf[7], mf[8], mf[9], mf[10] };
+        int32_t slot[5];  // base colour, roughness, metallic, normal, emission
+        std::memcpy(slot, mf + 12, sizeof(slot));
+        auto tex = [&](int k) -> const OrTexture* {
+            if (slot[k] < 0) return nullptr;
+            static const OrTexture kWhite{ 0, 0, 0, nullptr };
+            return uint32_t(slot[k]) < s.textureCount ? &s.textures[slot[k]] : &kWhite;
+        };
+        double tu = 0, tv = 0;
+        if (s.uvs) {
+            const float* q0 = s.uvs + size_t(tr[0]) * 2, *q1 = s.uvs + size_t(tr[1]) * 2, *q2 = s.uvs + size_t(tr[2]) * 2;
+            tu = q0[0] * b0 + q1[0] * h.b1 + q2[0] * h.b2;
+            tv = q0[1] * b0 + q1[1] * h.b1 + q2[1] * h.b2;
+        }
+        // normal mapping, pathtrace.comp:65-71 (tangent space, +z up)
+        if (slot[3] >= 0) {
+            double th[4] = { 1, 0, 0, 1 };
+            if (s.tangents) {
+                const float* t0 = s.tangents + size_t(tr[0]) * 4, *t1 = s.tangents + size_t(tr[1]) * 4,
+                            *t2 = s.tangents + size_t(tr[2]) * 4;
+                for (int k = 0; k < 4; ++k) th[k] = t0[k] * b0 + t1[k] * h.b1 + t2[k] * h.b2;
+            } else {
+                th[0] = b0 + h.b1 + h.b2;
+                th[3] = th[0];
+            }
+            const D3 tangent = norm(xfDir(in.o2w, d3(th[0], th[1], th[2])));
+            const D4 c = texSample(tex(3), tu, tv);
+            const D3 tn = d3(c[0] * 2 - 1, c[1] * 2 - 1, c[2] * 2 - 1);
+            const D3 bit = cross(ns, tangent) * th[3];
+            ns = norm(tangent * tn.x + bit * tn.y + ns * tn.z);
+            if (dot(ns, ng) < 0) ns = ng;
+        }
+        // textured parameters, pathtrace.comp:82-94 and :158-159
+        if (slot[0] >= 0) {
+            const D4 c = texSample(tex(0), tu, tv);
+            m.base = m.base * d3(c[0], c[1], c[2]);
+        }
+        if (slot[1] >= 0) m.rough *= texSample(tex(1), tu, tv)[0];
+        if (slot[2] >= 0) m.metal *= texSample(tex(2), tu, tv)[0];
         if (m.opacity < 1 && rng.r1() >= m.opacity) {
             o = pos + d * kEps;
             continue;
         }
-        if (maxc(m.emission) > 0 && front) L = L + beta * m.emission;
+        D3 emission = m.emission;
+        if (slot[4] >= 0) {
+            const D4 c = texSample(tex(4), tu, tv);
+            emission = emission * d3(c[0], c[1], c[2]);
+        }
+        if (maxc(emission) > 0 && front) L = L + beta * emission;
         const Bsdf bs{ m.base, m.rough, m.metal, m.trans, m.ior, front };
         D3 T, B;
         onbD(ns, T, B);
@@ -753,6 +857,50 @@ void or_camera_lookat(const float pos[3], const float target[3], const float up[
     out->frustum[2] = -tanHalf;
     out->frustum[3] = tanHalf;
     out->ortho = 0;
+}
+
+void or_bsdf_eval(const OrBsdfParams* p, const double wo[3], uint64_t n, const double* wi, double* f, double* pdf) {
+    const Bsdf m{ d3(p->base[0], p->base[1], p->base[2]), p->roughness, p->metallic, p->transmission, p->ior,
+                  p->entering != 0 };
+    const D3 o = d3(wo[0], wo[1], wo[2]);
+    for (uint64_t i = 0; i < n; ++i) {
+        D3 ff;
+        bsdfEval(m, o, d3(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]), ff, pdf[i]);
+        f[3 * i] = ff.x;
+        f[3 * i + 1] = ff.y;
+        f[3 * i + 2] = ff.z;
+    }
+}
+
+void or_bsdf_sample(const OrBsdfParams* p, const double wo[3], uint64_t n, const double* u, double* wi, double* f,
+                    double* pdf, int32_t* branch, uint8_t* valid) {
+    const Bsdf m{ d3(p->base[0], p->base[1], p->base[2]), p->roughness, p->metallic, p->transmission, p->ior,
+                  p->entering != 0 };
+    const D3 o = d3(wo[0], wo[1], wo[2]);
+    for (uint64_t i = 0; i < n; ++i) {
+        const double* ui = u + 4 * i;
+        D3 w{ 0, 0, 0 }, ff{ 0, 0, 0 };
+        double pp = 0;
+        int br = -1;
+        const bool ok = bsdfSampleU(m, o, ui[0], ui[1], ui[2], [&] { return ui[3]; }, w, ff, pp, br);
+        wi[3 * i] = w.x; wi[3 * i + 1] = w.y; wi[3 * i + 2] = w.z;
+        f[3 * i] = ff.x; f[3 * i + 1] = ff.y; f[3 * i + 2] = ff.z;
+        pdf[i] = pp;
+        branch[i] = br;
+        valid[i] = ok ? 1 : 0;
+    }
+}
+
+void or_light_sample(const float* l, uint32_t pickCount, const double p[3], const double u[3], double out[10]) {
+    const LS ls = sampleAnalytic(l, 1.0 / std::max(pickCount, 1u), d3(p[0], p[1], p[2]), u[1], u[2]);
+    const double v[10] = { ls.wi.x, ls.wi.y, ls.wi.z, ls.rad.x, ls.rad.y, ls.rad.z, ls.pdf, ls.dist,
+                           ls.delta ? 1.0 : 0.0, ls.valid ? 1.0 : 0.0 };
+    std::memcpy(out, v, sizeof(v));
+}
+
+void or_texture_sample(const OrTexture* t, double u, double v, double out[4]) {
+    const D4 c = texSample(t, u, v);
+    for (int k = 0; k < 4; ++k) out[k] = c[k];
 }
 
 double or_env_cdf_probs(const float* env, uint32_t w, uint32_t h, double* prob) {

@@ -11,8 +11,10 @@
 // layouts the GPU consumes), traversed in double precision with a watertight
 // test.  Environment light sampling uses either the reference's
 // marginal/conditional CDFs (built here, Renderer.cpp:605-662) or a supplied
-// alias table.  Textures are not modelled (parity unpinned for textured
-// materials; the GPU tests pin texture sampling with exact-emission scenes).
+// alias table.  Textures are sampled at level 0 with bilinear filtering and
+// REPEAT addressing (a compute shader's implicit LOD is the base level),
+// after per-texel sRGB decoding for RGBA8_SRGB; normal maps follow
+// pathtrace.comp:65-71.
 #pragma once
 
 #include "oracle.h"
@@ -20,6 +22,14 @@
 #include <cstdint>
 
 extern "C" {
+
+// Texture slot (the engine's 64-entry table): format 0 = RGBA8 sRGB,
+// 1 = RGBA8 UNORM, 2 = RGBA32F (mini_raytrace.h texture formats).  width 0
+// = empty slot (samples as white, the reference's dummy texture).
+struct OrTexture {
+    uint32_t width, height, format;
+    const void* pixels;  // row-major, top row first; 4 bytes or 4 floats per texel
+};
 
 struct OrPtScene {
     const OrNode* nodes;
@@ -36,6 +46,10 @@ struct OrPtScene {
     uint32_t envW, envH;
     float envRotation, envIntensity;
     const uint32_t* envAlias;   // envW*envH x (prob bits, alias) or NULL = reference CDFs
+    const float* uvs;           // 2 per vertex (NULL = all zero)
+    const float* tangents;      // 4 per vertex, w = handedness (NULL = (1,0,0,1))
+    const OrTexture* textures;  // slot table indexed by the materials' texture slots
+    uint32_t textureCount;
 };
 
 struct OrPtCamera {
@@ -64,6 +78,31 @@ void or_pt_render(const OrPtScene* scene, const OrPtCamera* cam, const OrPtParam
 // Camera basis and frustum of the legacy look-at camera (Renderer.cpp:724-745).
 void or_camera_lookat(const float pos[3], const float target[3], const float up[3], float fovYDeg,
                       uint32_t width, uint32_t height, OrPtCamera* out);
+
+// ---- pieces of the path tracer, exported for the statistical and
+// closed-form tests of tests/test_oracle_pt.py ---------------------------------
+// BSDF of bsdf.glsl:29-192 in the local frame (z = shading normal).
+struct OrBsdfParams {
+    double base[3];
+    double roughness, metallic, transmission, ior;
+    int32_t entering;  // front-face hit
+};
+// n evaluations for one wo: f (3 per wi) and the combined lobe pdf.
+void or_bsdf_eval(const OrBsdfParams* m, const double wo[3], uint64_t n, const double* wi, double* f, double* pdf);
+// n samples for one wo; u: 4 per sample = (lobe pick, direction u0,
+// direction u1, Fresnel choice), the draws bsdfSample makes in that order
+// (the Fresnel draw only on the transmission lobe).  valid[i] = 1 for a
+// valid sample; branch[i] = 0 diffuse, 1 specular, 2 transmission-lobe
+// reflection, 3 refraction (also set for a rejected sample).
+void or_bsdf_sample(const OrBsdfParams* m, const double wo[3], uint64_t n, const double* u, double* wi, double* f,
+                    double* pdf, int32_t* branch, uint8_t* valid);
+// sampleOneLight (lights.glsl:70-134) for an analytic light record (GpuLight
+// layout, 20 floats) at point p with uniforms u (u[0] unused: the light pick
+// is the caller's; u[1], u[2] the light's own rng2).  pickCount = lights in
+// the scene.  out: wi xyz, radiance rgb, pdf, dist, delta, valid.
+void or_light_sample(const float* light20, uint32_t pickCount, const double p[3], const double u[3], double out[10]);
+// Texture lookup at (u, v): bilinear, REPEAT, level 0, decoded to linear.
+void or_texture_sample(const OrTexture* tex, double u, double v, double out[4]);
 
 // Environment integral and per-texel sampling probabilities of the
 // reference's CDF construction (Renderer.cpp:620-652), in double:
