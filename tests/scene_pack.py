@@ -160,6 +160,7 @@ class SceneDesc:
         self.slots = []       # slot -> dict(pixels, format) or None (removed)
         self.free_slots = []
         self.env = None       # (pixels, rotation, intensity)
+        self.camera = None    # set_camera keyword arguments
         self.calls = []       # replay log
 
     def _id(self):
@@ -257,6 +258,10 @@ class SceneDesc:
     def set_environment(self, pixels, rotation=0.0, intensity=1.0):
         self.env = None if pixels is None else (_f(pixels), float(rotation), float(intensity))
         self.calls.append(("set_environment", None, (pixels,), dict(rotation=rotation, intensity=intensity)))
+
+    def set_camera(self, **kw):
+        self.camera = dict(kw)
+        self.calls.append(("set_camera", None, (), dict(kw)))
 
     # -- replay ------------------------------------------------------------------------------
     def load(self, engine):

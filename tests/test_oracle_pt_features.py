@@ -609,3 +609,27 @@ def test_camera_lookat_basis(oracle):
     assert np.cross(R, U) @ Fw < 0                       # right-handed: right x up = -forward
     assert fr[3] == pytest.approx(np.tan(np.radians(fov / 2)), rel=1e-6) and fr[2] == -fr[3]
     assert fr[1] / fr[3] == pytest.approx(W / H, rel=1e-6) and fr[0] == -fr[1]
+
+
+# -------------------------------------------------------------------------- resolve --
+def test_resolve_closed_forms():
+    from oracle import resolve as R
+    # Narkowicz ACES fit: 0 -> 0, 1 -> (a + b) / (c + d + e), saturates at 1, increasing
+    assert R.aces(0.0) == 0.0
+    assert R.aces(1.0) == pytest.approx((2.51 + 0.03) / (2.43 + 0.59 + 0.14), rel=1e-15)
+    assert R.aces(1e3) == 1.0
+    x = np.linspace(0, 8, 4001)
+    assert np.all(np.diff(R.aces(x)) >= 0)
+    # sRGB OETF is the inverse of the IEC decoding, continuous at the knee
+    c = np.linspace(0, 1, 10001)
+    np.testing.assert_allclose(R.linear_to_srgb(srgb_eotf(c)), c, atol=1e-12)
+    knee = 0.0031308
+    assert R.linear_to_srgb(knee) == pytest.approx(1.055 * knee ** (1 / 2.4) - 0.055, abs=2e-6)
+    assert R.linear_to_srgb(0.5) == pytest.approx(1.055 * 0.5 ** (1 / 2.4) - 0.055, rel=1e-15)
+    # one EV doubles the linear value; above 1 (after the curve) the store saturates
+    v = np.array([[[0.01, 0.1, 0.3]]])
+    assert np.array_equal(R.resolve_rgba8(v, 1.0, R.TONEMAP_LINEAR), R.resolve_rgba8(2 * v, 0.0, R.TONEMAP_LINEAR))
+    out = R.resolve_rgba8(np.array([[[1.5, 0.0, 2e4]]]), 0.0, R.TONEMAP_LINEAR)
+    assert list(out[0, 0]) == [255, 0, 255, 255]
+    assert list(R.resolve_rgba8(np.array([[[1.0, 0.5, 0.2]]]), 0.0, R.TONEMAP_LINEAR)[0, 0, :3]) == [
+        255, round(255 * (1.055 * 0.5 ** (1 / 2.4) - 0.055)), round(255 * (1.055 * 0.2 ** (1 / 2.4) - 0.055))]
