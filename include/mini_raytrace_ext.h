@@ -200,11 +200,12 @@ MRT_API mrtResult mrtxExportScene(mrtEngine* engine, mrtxSceneExport* inout);
  * engine ends in exactly the state that `frames` calls of mrtRenderFrame
  * would leave (same images, bit for bit), clamped to its spp limit.  All
  * engines must be on one device and appear once.
- * Mode 0 (up to 64 requests, none guided): one set of persistent kernels per
- * batched frame works on the queues of all requests at once (each wave takes
- * 64-item chunks of any request's queue).  Mode 1 (otherwise): each request's
- * frames are captured as a chain and the chains are independent nodes of the
- * graph. */
+ * Mode 0 (up to 64 requests, plain or guided): one set of persistent kernels
+ * per batched frame works on the queues of all requests at once (each wave
+ * takes 64-item chunks of any request's queue); a guided request adds its own
+ * network inference and training launches.  Mode 1 (more than 64 requests):
+ * each request's frames are captured as a chain and the chains are
+ * independent nodes of the graph. */
 typedef struct mrtxBatchRequest {
     mrtEngine* engine;
     uint32_t   frames;        /* samples per pixel to add */
@@ -226,8 +227,9 @@ MRT_API mrtResult mrtxRenderBatch(mrtxBatchRequest* requests, uint32_t count, mr
 /* Path-tracer launch-shape autotuner: one profiled trial frame per candidate
  * (block 64/128/256 x LDS stack 8/16/32); the shape with the least trace time
  * over the frame is kept for both trace launch kinds and reported.  Results
- * are cached per (triangle count, pixel count, depth) class.  Images do not
- * depend on the shape; the trial frames are discarded (accumulation restarts). */
+ * are cached per (triangle count, pixel count, depth, compute units) class.
+ * Images do not depend on the shape; the trial frames are discarded
+ * (accumulation restarts) and do not train the engine's guiding network. */
 MRT_API mrtResult mrtxAutotunePath(mrtEngine* engine, mrtxPathOptions* chosen);
 
 /* ---- 6. path guiding -------------------------------------------------------

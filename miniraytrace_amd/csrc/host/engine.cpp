@@ -1088,21 +1088,24 @@ void Engine::framesDone(uint32_t frames) {
 // Launch-shape autotuner for the path tracer: one profiled frame per
 // candidate shape; the one with the least time in the trace and tail
 // kernels wins.  Traversal results do not depend on the shape, so neither does
-// the image; the trial frames are discarded (accumulation restarts).  Results
-// are cached per (triangle count, pixel count, depth) class.
+// the image; the trial frames are discarded (accumulation restarts) and do
+// not train a guiding network.  Results are cached per (triangle count, pixel
+// count, depth, compute units) class.
 void Engine::autotuneShapes(mrtxPathOptions& chosen) {
     struct Key {
-        int tris, pixels, depth;
+        int tris, pixels, depth, cus;
         bool operator<(const Key& o) const {
-            return std::tie(tris, pixels, depth) < std::tie(o.tris, o.pixels, o.depth);
+            return std::tie(tris, pixels, depth, cus) < std::tie(o.tris, o.pixels, o.depth, o.cus);
         }
     };
     static std::mutex cacheMutex;
     static std::map<Key, std::pair<dev::TraceShape, dev::TraceShape>> cache;
     commit(nullptr);
+    // The class includes the device's compute-unit count: the best shape
+    // depends on how many waves the chip holds.
     const Key key{ int(std::lround(std::log2(double(std::max<size_t>(totalTris_, 1))) * 2.0)),
                    int(std::lround(std::log2(double(frameW_) * frameH_) * 2.0)),
-                   int(std::min(settings_.maxBounces, 16u)) };
+                   int(std::min(settings_.maxBounces, 16u)), cuCount_ };
     bool hit = false;
     {
         std::lock_guard<std::mutex> lock(cacheMutex);
@@ -1117,10 +1120,23 @@ void Engine::autotuneShapes(mrtxPathOptions& chosen) {
     }
     if (!hit) {
         static const uint32_t kCand[][2] = { { 64, 16 }, { 128, 8 }, { 128, 16 }, { 128, 32 }, { 256, 8 }, { 256, 16 } };
-        const bool prof = profiling_;
-        const uint32_t sppLimit = settings_.sppLimit;
-        settings_.sppLimit = std::max(sppLimit, 1u << 30);
+        // Trial frames run profiled, without an spp limit and without
+        // training (so the network, its Adam state and the training-record
+        // stamp are as before); the guard restores all of it on any exit.
+        struct TrialState {
+            Engine& e;
+            bool profiling;
+            uint32_t sppLimit, training, stamp;
+            ~TrialState() {
+                e.profiling_ = profiling;
+                e.settings_.sppLimit = sppLimit;
+                e.guide_.enableTraining = training;
+                e.guideStamp_ = stamp;
+            }
+        } saved{ *this, profiling_, settings_.sppLimit, guide_.enableTraining, guideStamp_ };
+        settings_.sppLimit = std::max(saved.sppLimit, 1u << 30);
         profiling_ = true;
+        guide_.enableTraining = 0;
         // One shape for both trace launch kinds and the tail kernel: the one
         // with the least time in those kernels over a whole frame.
         float best = 1e30f;
@@ -1138,8 +1154,6 @@ void Engine::autotuneShapes(mrtxPathOptions& chosen) {
                 bs = shadowShape_;
             }
         }
-        profiling_ = prof;
-        settings_.sppLimit = sppLimit;
         extendShape_ = be;
         shadowShape_ = bs;
         std::lock_guard<std::mutex> lock(cacheMutex);

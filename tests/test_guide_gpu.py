@@ -17,7 +17,7 @@ Rendering:
 * training without guided sampling renders bit-identically to plain;
 * guided sampling is unbiased (block means agree with plain within noise);
 * guided sampling lowers the error at equal sample count on a room lit
-  through a window;
+  through a window (against a plain reference render);
 * at 1280x720 both Adam steps of a frame train on full 65,536-record
   batches.
 """
@@ -280,18 +280,25 @@ def test_guided_is_unbiased(gpu, depth):
     gz = (gq.mean() - gp.mean()) / np.sqrt(gq.var(ddof=1) / B + gp.var(ddof=1) / B)
     _print(f"global mean plain {gp.mean():.6f} guided {gq.mean():.6f} (z {gz:.2f}); "
            f"block |z| > 4: {(np.abs(z) > 4).sum()} of {z.size}; max {np.abs(z).max():.2f}")
+    # Each of the B batch means is an independent unbiased estimate whatever
+    # the network (which keeps training, and whose training records arrive
+    # in run-dependent order) looked like, so the t statistics hold for any
+    # run: |t| > 4.5 at 14 degrees of freedom has probability about 5e-4 per
+    # block, about 0.05 of the 108 blocks.
     assert abs(gz) < 4.5
     assert (np.abs(z) > 4.5).sum() <= 2
 
 
 def test_guided_lowers_error(gpu):
-    _print("case: guided vs plain error at equal samples")
-    W, H, N = 64, 36, 64
+    """Error against a 4096-spp plain render (the unbiased estimator the
+    oracle pins, so a bias of the guided estimator is not shared by the
+    reference): at 64 spp the guided image's MSE is below plain's (measured
+    ratio about 0.83; the reference's own error adds the same amount to
+    both)."""
+    _print("case: guided vs plain error at equal samples, plain reference")
+    W, H, N = 96, 54, 64
     ref_e = _room_engine(W, H, 10**6)
-    ref_e.set_guiding(sampling=True, training=True, batch_size=4096, steps_per_frame=2)
-    _render(ref_e, 64, W, H)
     ref_e.set_path_options(seed=777)
-    ref_e.reset_accumulation()
     ref = _render(ref_e, 4096, W, H)
     plain = _room_engine(W, H, 10**6)
     plain.set_path_options(seed=5)
@@ -306,7 +313,7 @@ def test_guided_lowers_error(gpu):
     mse_g = ((ig - ref) ** 2).mean()
     _print(f"MSE at {N} spp: plain {mse_p:.4e}, guided {mse_g:.4e}, ratio {mse_g / mse_p:.3f}; "
            f"stats {guided.guiding_stats()}")
-    assert mse_g < mse_p
+    assert mse_g < 0.95 * mse_p
 
 
 def test_training_batches_are_full_at_720p(gpu):
@@ -336,3 +343,22 @@ def test_training_batches_are_full_at_720p(gpu):
     assert st["candidates"] >= 2 * 65536
     assert st["poolRecords"] >= 0.95 * 2 * 65536
     assert st["lastBatch"] >= 0.95 * 65536
+
+
+def test_autotune_does_not_train_or_change_settings(gpu):
+    """mrtxAutotunePath's trial frames leave a training engine's network,
+    Adam step count and spp limit as they were (a size class no other test
+    tunes, so the trials run rather than hit the shape cache)."""
+    _print("case: autotune on a guided, training engine")
+    W, H = 24, 16
+    e = _room_engine(W, H, 3)
+    e.set_guiding(sampling=True, training=True, batch_size=1024, steps_per_frame=2)
+    _render(e, 2, W, H)
+    p0, st0 = e.guide_params(), e.guiding_stats()["steps"]
+    assert st0 > 0
+    e.autotune_path()
+    assert np.array_equal(e.guide_params().view(np.uint32), p0.view(np.uint32))
+    assert e.guiding_stats()["steps"] == st0
+    infos = [e.render_frame() for _ in range(4)]
+    assert [i["spp"] for i in infos] == [1, 2, 3, 3] and infos[-1]["converged"] == 1
+    assert e.guiding_stats()["steps"] > st0  # training resumes with the frames
