@@ -6,7 +6,8 @@
 Workloads:
   pt         (default) plain progressive path tracing at 1920x1080 on a
              1.256M-triangle terrain scene; one step = one frame (1 spp).
-             The default run appends the traversal and guided lines below.
+             The default run appends the traversal, batch, guided,
+             convergence and (shortened) scale objects below.
   traversal  closest-hit queries: 10,000-triangle seeded soup, 1,000,000
              rays (one step = one mrtxTraceClosest launch over all rays).
   guided     equal-RMSE wall time of guided vs plain path tracing on a room
@@ -15,7 +16,10 @@ Workloads:
              frame call at a time (images must be identical).
   scale      guided and batched together: 16 requests at 4K / 1080p over a
              10.4M-triangle instanced field and the room, depth 12, one
-             guiding network per request (not part of the default run).
+             guiding network per request (8 frames per request; the default
+             run appends a shortened run: 4 frames, MSE at 16 spp).
+  converge   RMSE of the plain 1080p terrain frame's centre crop against
+             the FP64 oracle at every power-of-two spp up to 1024.
 
 Prints ONE JSON line.  Inputs are resident in device memory before the timed
 region; the timed region is bracketed by device synchronisation.  The
@@ -53,6 +57,74 @@ def _measured_traffic(kernel: str, key: str):
     except (OSError, ValueError, KeyError):
         return None
     return None if rec is None else rec.get(key)
+
+
+# -------------------------------------------------------------------- CPU leg --
+# Every use of the oracle (oracle/, the CPU restatement) in this file is in
+# the functions of this section, and runs only with the CPU baseline on (not
+# --no-cpu-baseline): the timed CPU baselines, the reference-order node and
+# triangle counts that price the rooflines' algorithmic bytes, and the FP64
+# reference images that RMSE figures are measured against.  No GPU result is
+# computed by it.
+def _cpu_camera(camera, W, H):
+    from oracle import oracle as orc
+    return orc.camera_lookat(camera["position"], camera["target"], camera.get("up", (0, 1, 0)),
+                             camera["fov_y_deg"], W, H)
+
+
+def _centre_crop(W, H, cw, ch):
+    cw, ch = min(cw, W), min(ch, H)
+    return ((W - cw) // 2, (H - ch) // 2, cw, ch)
+
+
+def _cpu_fetch_counts(exported, sky, camera, W, H, crop, max_bounces) -> np.ndarray:
+    """Reference-order fetch counts of one frame's paths on a crop (the
+    traversal order of intersect.glsl:48-178): closest queries, nodes
+    popped, triangles tested, then the same three for shadow queries."""
+    from oracle import oracle as orc
+    _, counts = orc.pt_render(exported, sky, _cpu_camera(camera, W, H), W, H, crop=crop, frame0=0, spp=1,
+                              max_bounces=max_bounces, threads=orc.hardware_threads(), count_fetches=True)
+    return counts
+
+
+def _cpu_fp64_reference(exported, sky, camera, W, H, crop, spp, max_bounces) -> np.ndarray:
+    """FP64 crop image of the reference estimator (reference CDF environment
+    sampling) over spp frames numbered from 2^20, so its samples are
+    independent of every GPU frame."""
+    from oracle import oracle as orc
+    img, _ = orc.pt_render(exported, sky, _cpu_camera(camera, W, H), W, H, crop=crop, frame0=1 << 20, spp=spp,
+                           max_bounces=max_bounces, threads=orc.hardware_threads())
+    return img / spp
+
+
+def _trace_bytes(counts, rays, shadows) -> dict:
+    """Algorithmic bytes of one frame's trace launches: per closest-hit query
+    a 32-byte ray read and a 20-byte hit record write, per shadow query a
+    48-byte shadow-ray read, and for both 32 B per node popped and 64 B per
+    triangle tested in the reference order; per-query figures from a crop,
+    scaled to the frame's queue sizes (rays, shadows: per bounce level)."""
+    q, nn, nt, qs, ns, ts = (int(x) for x in counts)
+    per_query = 32 + 20 + 32 * nn / max(q, 1) + 64 * nt / max(q, 1)
+    per_shadow = 48 + 32 * ns / max(qs, 1) + 64 * ts / max(qs, 1)
+    return {"bytes_per_frame": per_query * sum(rays) + per_shadow * sum(shadows),
+            "bytes_per_query": per_query, "bytes_per_shadow_query": per_shadow,
+            "nodes_per_query": nn / max(q, 1), "tris_per_query": nt / max(q, 1),
+            "nodes_per_shadow_query": ns / max(qs, 1), "tris_per_shadow_query": ts / max(qs, 1)}
+
+
+def _profiled_rays(e):
+    """Per-level ray and shadow-ray counts of one profiled frame; the
+    engine's accumulation restarts afterwards."""
+    e.set_profiling(True)
+    e.render_frame()
+    st = e.frame_stats()
+    e.set_profiling(False)
+    e.reset_accumulation()
+    return [int(x) for x in st["rays"]], [int(x) for x in st["shadowRays"]]
+
+
+def _rmse(a, b) -> float:
+    return float(np.sqrt(((a - b) ** 2).mean()))
 
 
 # ----------------------------------------------------------------- traversal --
@@ -195,29 +267,17 @@ def pt_workload(args, torch) -> dict:
         from oracle import oracle as orc
         threads = orc.hardware_threads()
         exp = e.export_scene()
-        cam = orc.camera_lookat(scenes.CONFIG2["camera"]["position"], scenes.CONFIG2["camera"]["target"],
-                                (0, 1, 0), scenes.CONFIG2["camera"]["fov_y_deg"], W, H)
+        cam = _cpu_camera(scenes.CONFIG2["camera"], W, H)
         # Bounded CPU sample: a 96x54 crop of the frame centre, 1 spp,
         # FP64 restatement with reference-order fetch counting.
         cw, ch = 96, 54
-        crop = ((W - cw) // 2, (H - ch) // 2, cw, ch)
+        crop = _centre_crop(W, H, cw, ch)
         t1 = time.perf_counter()
-        _, counts = orc.pt_render(exp, sky, cam, W, H, crop=crop, frame0=0, spp=1, max_bounces=8,
-                                  threads=threads, count_fetches=True)
+        counts = _cpu_fetch_counts(exp, sky, scenes.CONFIG2["camera"], W, H, crop, 8)
         t_count = time.perf_counter() - t1
-        q, nn, nt = int(counts[0]), int(counts[1]), int(counts[2])
-        qs, ns, ts = int(counts[3]), int(counts[4]), int(counts[5])
-        # Trace-launch algorithmic bytes per frame (closest hits of every
-        # bounce and the shadow rays, which share the launches): per
-        # closest-hit query a 32-byte ray read and a 20-byte hit record
-        # write, per shadow query a 48-byte shadow-ray read, and for both
-        # 32 B per node popped and 64 B per triangle tested in the reference
-        # order; per-query figures from the crop, scaled to the frame's queue
-        # sizes.
-        per_query = 32 + 20 + 32 * nn / max(q, 1) + 64 * nt / max(q, 1)
-        per_shadow = 48 + 32 * ns / max(qs, 1) + 64 * ts / max(qs, 1)
+        tb = _trace_bytes(counts, rays, shadows)
         frame_queries, frame_shadows = sum(rays), sum(shadows)
-        bytes_alg = per_query * frame_queries + per_shadow * frame_shadows
+        bytes_alg = tb["bytes_per_frame"]
         # the tail kernel traces the last few paths' rays (and shades them)
         trace_ms = stage["extend"] + stage["shadow"] + stage["tail"]
         achieved = bytes_alg / (trace_ms * 1e-3) / 1e9
@@ -229,9 +289,10 @@ def pt_workload(args, torch) -> dict:
                                      "with the tail kernel)",
                            "bytes_per_frame": bytes_alg, "trace_ms_per_frame": trace_ms,
                            "queries_per_frame": frame_queries, "shadow_queries_per_frame": frame_shadows,
-                           "nodes_per_query": nn / max(q, 1), "tris_per_query": nt / max(q, 1),
-                           "nodes_per_shadow_query": ns / max(qs, 1), "tris_per_shadow_query": ts / max(qs, 1),
-                           "crop_queries_per_path": q / (cw * ch)}
+                           "nodes_per_query": tb["nodes_per_query"], "tris_per_query": tb["tris_per_query"],
+                           "nodes_per_shadow_query": tb["nodes_per_shadow_query"],
+                           "tris_per_shadow_query": tb["tris_per_shadow_query"],
+                           "crop_queries_per_path": int(counts[0]) / (cw * ch)}
         # CPU baseline: the FP64 path tracer timed on the same crop without counting.
         reps, t_cpu = 0, 0.0
         while t_cpu < 10.0 and reps < 64:
@@ -245,6 +306,61 @@ def pt_workload(args, torch) -> dict:
                                          "pathtrace.comp (no CPU path tracer exists in the reference)",
                                "count_pass_s": t_count}
     e.close()
+    return out
+
+
+# ------------------------------------------------------------------ convergence --
+def convergence_workload(args, torch, sky) -> dict:
+    """Convergence of the plain tracer on the config-2 frame (1920x1080, the
+    1.256M-triangle terrain scene, depth 8, no firefly clamp): RMSE of the
+    192x108 centre crop at spp = 1, 2, 4, ..., args.converge_spp against the
+    FP64 oracle's crop at 4096 samples (reference CDF environment sampling,
+    independent frames), with the fit rmse^2 = sigma^2 / spp + sigma^2 / 4096
+    (the reference's own noise) and each checkpoint's ratio to it.  The sun
+    (5e4 x the zenith radiance) makes single samples heavy-tailed, so at
+    high spp a checkpoint's RMSE moves with the few sun paths it holds."""
+    from miniraytrace_amd import scenes
+    from miniraytrace_amd.engine import Engine
+    W, H = 1920, 1080
+    top = args.converge_spp
+    e = Engine(width=W, height=H)
+    e.set_render_settings(width=W, height=H, max_bounces=8, spp_limit=top, firefly_clamp=0.0, denoise=False)
+    info = scenes.load_terrain_scene(e, scenes.CONFIG2, seed=1, sky_img=sky)
+    e.commit()
+    x0, y0, cw, ch = crop = _centre_crop(W, H, 192, 108)
+    checkpoints = [1 << k for k in range(top.bit_length()) if (1 << k) <= top]
+    crops, done = {}, 0
+    t_gpu = 0.0
+    for s_ in checkpoints:
+        t_gpu += _timed_frames(e, s_ - done, torch)
+        done = s_
+        crops[s_] = e.read_rgba32f(W, H)[y0:y0 + ch, x0:x0 + cw, :3].astype(np.float64)
+    exp = e.export_scene()
+    e.close()
+    ref_spp = 4096
+    t0 = time.perf_counter()
+    ref = _cpu_fp64_reference(exp, sky, info["camera"], W, H, crop, ref_spp, 8)
+    t_ref = time.perf_counter() - t0
+    rmse = np.array([_rmse(crops[s_], ref) for s_ in checkpoints])
+    x = 1.0 / np.array(checkpoints, np.float64) + 1.0 / ref_spp
+    sigma2 = float((rmse ** 2 * x).sum() / (x * x).sum())
+    model = np.sqrt(sigma2 * x)
+    out = {
+        "workload": "plain PT convergence, 1920x1080 terrain 1.256M tris, 192x108 centre crop vs FP64 at 4096 spp",
+        "spp": checkpoints,
+        "rmse": [float(v) for v in rmse],
+        "model_rmse": [float(v) for v in model],
+        "rmse_over_model": [float(v) for v in rmse / model],
+        "rmse_sqrt_spp": [float(v * np.sqrt(s_)) for v, s_ in zip(rmse, checkpoints)],
+        "sigma": float(np.sqrt(sigma2)),
+        "reference_spp": ref_spp,
+        "reference_rmse_floor": float(np.sqrt(sigma2 / ref_spp)),
+        "gpu_s": t_gpu,
+        "reference_cpu_s": t_ref,
+    }
+    _log("convergence: RMSE " + ", ".join(f"{s_}:{v:.4g}" for s_, v in zip(checkpoints, rmse)) +
+         f"; sigma {out['sigma']:.4f}; worst ratio to the 1/sqrt(spp) model "
+         f"{float(np.abs(rmse / model - 1).max()):.3f}; FP64 reference {t_ref:.1f} s")
     return out
 
 
@@ -316,8 +432,12 @@ def guided_workload(args, torch) -> dict:
     threshold for the scene (chosen over 8-frame trials); guided starts from
     an untrained network, trains online (batch 65,536, 2 Adam steps per frame)
     and samples the learned mixture with probability 0.5, and is timed until
-    its RMSE reaches plain's.  RMSE is taken against an independent guided
-    render of `2 spp` samples; its own error adds equally to both sides."""
+    its RMSE reaches plain's.  RMSE is taken against an independent plain
+    render of 2 x spp samples (the unbiased estimator the oracle pins, so a
+    bias of the guided estimator would show, not cancel); its own error adds
+    equally to both sides.  With the CPU leg on, plain's spp-sample image and
+    the last guided checkpoint are also compared with the FP64 oracle on a
+    64x36 centre crop (4096 samples)."""
     from miniraytrace_amd import scenes
     W, H = scenes.CONFIG4["width"], scenes.CONFIG4["height"]
     spp = args.guided_spp
@@ -336,12 +456,11 @@ def guided_workload(args, torch) -> dict:
     plain.close()
 
     ref, _ = _room_engine(W, H, 99, sky)
-    ref.set_guiding(sampling=True, training=True)
-    _timed_frames(ref, 64, torch)
-    ref.reset_accumulation()
+    ref.set_path_options(seed=99, tail_paths=plain_tail * torch.cuda.get_device_properties(0).multi_processor_count)
     n_ref = 2 * spp
     t_ref = _timed_frames(ref, n_ref, torch)
     img_r = ref.read_rgba32f(W, H)[..., :3].astype(np.float64)
+    exp = ref.export_scene() if not args.no_cpu_baseline else None
     ref.close()
     mse_p = float(((img_p - img_r) ** 2).mean())
 
@@ -362,6 +481,7 @@ def guided_workload(args, torch) -> dict:
         curve.append((done, t_g, float(((img_g - img_r) ** 2).mean())))
         if curve[-1][2] <= mse_p and t_g >= t_plain:
             break
+    img_g_last = g.read_rgba32f(W, H)[..., :3].astype(np.float64)
     g.set_profiling(True)
     g.render_frame()
     prof = g.frame_stats()
@@ -401,6 +521,21 @@ def guided_workload(args, torch) -> dict:
         "mlp": _mlp_roofline(st["lastFrameRows"], prof["gpuMs"]["guide"], st["lastBatch"] * opts["stepsPerFrame"],
                              prof["gpuMs"]["train"]),
     }
+    if exp is not None:
+        crop = _centre_crop(W, H, 64, 36)
+        x0, y0, cw, ch = crop
+        t0 = time.perf_counter()
+        fp64 = _cpu_fp64_reference(exp, sky, info["camera"], W, H, crop, 4096, 8)
+        out["fp64_crop"] = {
+            "crop": list(crop), "reference_spp": 4096, "cpu_s": time.perf_counter() - t0,
+            "plain_rmse": _rmse(img_p[y0:y0 + ch, x0:x0 + cw], fp64),
+            "plain_reference_rmse": _rmse(img_r[y0:y0 + ch, x0:x0 + cw], fp64),
+            "guided_rmse_last_checkpoint": _rmse(img_g_last[y0:y0 + ch, x0:x0 + cw], fp64),
+            "guided_frames_last_checkpoint": curve[-1][0],
+        }
+        _log(f"guided: FP64 crop RMSE plain {spp} spp {out['fp64_crop']['plain_rmse']:.4g}, plain reference "
+             f"{out['fp64_crop']['plain_reference_rmse']:.4g}, guided {curve[-1][0]} spp "
+             f"{out['fp64_crop']['guided_rmse_last_checkpoint']:.4g}")
     _log(f"guided: plain {spp} spp in {t_plain:.2f} s (RMSE {out['plain_rmse']:.4g}); guided reaches it in "
          f"{t_equal_rmse} s -> ratio {out['time_ratio']}; equal-time RMSE ratio {out['equal_time_rmse_ratio']}")
     return out
@@ -425,7 +560,8 @@ def batch_workload(args, torch) -> dict:
         e = Engine(width=r["width"], height=r["height"])
         e.set_render_settings(width=r["width"], height=r["height"], max_bounces=8, spp_limit=1 << 20,
                               firefly_clamp=0.0, denoise=False)
-        scenes.load_terrain_scene(e, scenes.terrain_config(r["triangles"]), seed=r["seed"], sky_img=sky)
+        r["camera"] = scenes.load_terrain_scene(e, scenes.terrain_config(r["triangles"]), seed=r["seed"],
+                                                sky_img=sky)["camera"]
         e.commit()
         engines.append(e)
     t_setup = time.perf_counter() - t0
@@ -433,6 +569,14 @@ def batch_workload(args, torch) -> dict:
     shapes = [e.autotune_path() for e in engines]
     t_tune = time.perf_counter() - t0
     _log(f"batch: {len(reqs)} requests, setup {t_setup:.1f} s, autotune {t_tune:.1f} s")
+    ray_counts = [_profiled_rays(e) for e in engines]
+    counts = None
+    if not args.no_cpu_baseline:
+        t0 = time.perf_counter()
+        counts = [_cpu_fetch_counts(e.export_scene(), sky, r["camera"], r["width"], r["height"],
+                                    _centre_crop(r["width"], r["height"], 32, 18), 8)
+                  for e, r in zip(engines, reqs)]
+        _log(f"batch: reference-order fetch counts of {len(reqs)} crops in {time.perf_counter() - t0:.1f} s")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     st = render_batch(engines, [r["spp"] for r in reqs])
@@ -473,8 +617,19 @@ def batch_workload(args, torch) -> dict:
         "setup_s": t_setup,
         "shapes": shape_counts,
     }
+    if counts is not None:
+        per_req = [_trace_bytes(c, rr, ss)["bytes_per_frame"] for c, (rr, ss) in zip(counts, ray_counts)]
+        total = sum(b * r["spp"] for b, r in zip(per_req, reqs))
+        achieved = total / (st["launchMs"] * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                           "kernel": "the whole batched graph (all requests' trace, shade, tail, generate and "
+                                     "resolve launches): a lower bound for the trace kernels",
+                           "bytes": total, "launch_ms": st["launchMs"],
+                           "bytes_per_request_frame_min": min(per_req), "bytes_per_request_frame_max": max(per_req)}
     _log(f"batch: {samples / 1e6:.1f} Msamples in {t_batch:.2f} s (capture {st['captureMs']:.0f} ms) vs "
-         f"sequential {t_seq:.2f} s; identical {identical}")
+         f"sequential {t_seq:.2f} s; identical {identical}"
+         + (f"; {out['roofline']['achieved']:.0f} GB/s algorithmic over the graph" if "roofline" in out else ""))
     return out
 
 
@@ -511,7 +666,6 @@ def _scale_engine(k, sky, seed):
         info = scenes.load_sphere_field(e, cfg, seed=1 + k // 2, sky_img=sky)
     else:
         info = scenes.load_room_scene(e, scenes.CONFIG4, seed=1, sky_img=sky)
-        e.set_camera(**scenes.CONFIG4["camera"])
     e.set_path_options(seed=seed)
     e.commit()
     return e, W, H, info
@@ -521,7 +675,7 @@ def _mse(a, b) -> float:
     return float(((a - b) ** 2).mean())
 
 
-def scale_workload(args, torch) -> dict:
+def scale_workload(args, torch, frames: int = 8, warm: int = 8, spp: int = 32) -> dict:
     """Guided and batched together at scale: 16 requests, alternating between
     the field of 256 instanced 40,500-triangle displaced spheres (10.4M
     triangles) and the window-lit room, at 3840x2160 and 1920x1080, max depth
@@ -531,13 +685,12 @@ def scale_workload(args, torch) -> dict:
     `warm` training frames whose samples are discarded).
 
     Equal-RMSE speedup of guided over plain batched rendering: per-sample MSE
-    ratio (guided / plain at equal samples per pixel, against a guided render
-    of 4x the samples) measured on the 1920x1080 field and room requests,
-    times the ratio of batched throughputs."""
+    ratio (guided / plain at `spp` samples per pixel, against a plain render
+    of 4 x spp samples, the unbiased estimator) measured on the 1920x1080
+    field and room requests, times the ratio of batched throughputs."""
     from miniraytrace_amd import scenes
     from miniraytrace_amd.batch import render_batch
     cfg = scenes.CONFIG5
-    frames, warm = 8, 8
     sky = scenes.sky()
     t0 = time.perf_counter()
     engines = [_scale_engine(k, sky, 100 + k) for k in range(cfg["requests"])]
@@ -545,6 +698,13 @@ def scale_workload(args, torch) -> dict:
     samples = sum(W * H for _, W, H, _ in engines) * frames
     _log(f"scale: {len(engines)} requests, setup {t_setup:.1f} s, {samples / frames / 1e6:.1f} Mpixels per frame")
     es = [e for e, _, _, _ in engines]
+    ray_counts = [_profiled_rays(e) for e in es]
+    counts = None
+    if not args.no_cpu_baseline:
+        counts = []
+        for k, (e, W, H, info) in enumerate(engines):
+            counts.append(_cpu_fetch_counts(e.export_scene(), sky, info["camera"], W, H,
+                                            _centre_crop(W, H, 48, 27), cfg["max_depth"]))
     render_batch(es, [1] * len(es))  # first use: graph capture and caches
     for e in es:
         e.reset_accumulation()
@@ -569,14 +729,9 @@ def scale_workload(args, torch) -> dict:
     engines = es = None
 
     # Per-sample MSE ratio on the two 1920x1080 requests (k = 2: field, 3: room).
-    spp = 32
     per_request = {}
     for k in (2, 3):
         ref, W, H, _ = _scale_engine(k, sky, 900 + k)
-        ref.set_guiding(sampling=True, training=True)
-        for _ in range(warm):
-            ref.render_frame()
-        ref.reset_accumulation()
         for _ in range(4 * spp):
             ref.render_frame()
         img_r = ref.read_rgba32f(W, H)[..., :3].astype(np.float64)
@@ -596,12 +751,12 @@ def scale_workload(args, torch) -> dict:
         mse_g = _mse(g.read_rgba32f(W, H)[..., :3].astype(np.float64), img_r)
         g.close()
         name = "field" if k % 2 == 0 else "room"
-        per_request[name] = {"spp": spp, "reference_spp": 4 * spp, "mse_plain": mse_p, "mse_guided": mse_g,
-                             "mse_ratio": mse_g / mse_p}
+        per_request[name] = {"spp": spp, "reference_spp": 4 * spp, "reference": "plain",
+                             "mse_plain": mse_p, "mse_guided": mse_g, "mse_ratio": mse_g / mse_p}
         _log(f"scale: {name} 1080p per-sample MSE ratio guided / plain {mse_g / mse_p:.3f}")
     thr_ratio = t_plain / t_guided  # guided samples/s over plain samples/s
     speedups = {n: thr_ratio / r["mse_ratio"] for n, r in per_request.items()}
-    return {
+    out = {
         "workload": "16 requests: 256 x 40,500-tri displaced spheres (10.4M tris) / 495k-tri room, "
                     "3840x2160 and 1920x1080, depth 12, per-request guiding network",
         "requests": cfg["requests"],
@@ -618,6 +773,15 @@ def scale_workload(args, torch) -> dict:
         "per_sample_mse": per_request,
         "equal_rmse_speedup": speedups,
     }
+    if counts is not None:
+        total = frames * sum(_trace_bytes(c, rr, ss)["bytes_per_frame"] for c, (rr, ss) in zip(counts, ray_counts))
+        achieved = total / (st_plain["launchMs"] * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                           "kernel": "the whole plain batched graph (trace, shade, tail, generate and resolve "
+                                     "launches of all 16 requests): a lower bound for the trace kernels",
+                           "bytes": total, "launch_ms": st_plain["launchMs"]}
+    return out
 
 
 def main() -> int:
@@ -625,13 +789,17 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="pt", choices=["pt", "traversal", "guided", "batch", "scale", "copy"])
+    ap.add_argument("--workload", default="pt",
+                    choices=["pt", "traversal", "guided", "batch", "scale", "converge", "copy"])
     ap.add_argument("--guided-spp", type=int, default=1024,
                     help="plain sample count whose RMSE the guided run must reach")
     ap.add_argument("--guided-warmup", type=int, default=0,
                     help="guided frames rendered (and timed) before its samples count")
     ap.add_argument("--no-guided", dest="with_guided", action="store_false", default=True)
     ap.add_argument("--batch-requests", type=int, default=64)
+    ap.add_argument("--converge-spp", type=int, default=1024)
+    ap.add_argument("--no-converge", dest="with_converge", action="store_false", default=True)
+    ap.add_argument("--no-scale", dest="with_scale", action="store_false", default=True)
     ap.add_argument("--no-batch", dest="with_batch", action="store_false", default=True)
     ap.add_argument("--autotune", action="store_true", default=True)
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
@@ -659,6 +827,13 @@ def main() -> int:
         print(json.dumps({"metric": "batched Msamples/s", "value": res["msamples_s_batch"], "unit": "Msamples/s",
                           "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
                           "config": {"workload": res["workload"]}, "batch": res}))
+        return 0
+    if args.workload == "converge":
+        from miniraytrace_amd import scenes
+        res = convergence_workload(args, torch, scenes.sky())
+        print(json.dumps({"metric": "RMSE vs FP64 at 1024 spp, 1080p terrain crop", "value": res["rmse"][-1],
+                          "unit": "rmse", "n_gpus": 1, "higher_is_better": False, "dtype": "f32",
+                          "data": "synthetic", "config": {"workload": res["workload"]}, "convergence": res}))
         return 0
     if args.workload == "scale":
         res = scale_workload(args, torch)
@@ -716,6 +891,11 @@ def main() -> int:
         gd = guided_workload(args, torch)
         line["guided"] = gd
         line["equal_rmse_time_ratio"] = gd["time_ratio"]
+    if args.with_converge and not args.no_cpu_baseline:
+        from miniraytrace_amd import scenes
+        line["convergence"] = convergence_workload(args, torch, scenes.sky())
+    if args.with_scale:
+        line["scale"] = scale_workload(args, torch, frames=4, warm=8, spp=16)
     print(json.dumps(line))
     return 0
 

@@ -110,7 +110,7 @@ def load_terrain_scene(engine, cfg: dict = CONFIG2, seed: int = 1, sky_img: np.n
     engine.set_environment(sky_img, 0.0, 1.0)
     engine.set_camera(**cfg["camera"])
     return {"triangles": tidx.shape[0] + cfg["spheres"] * sidx.shape[0], "instances": cfg["spheres"] + 1,
-            "materials": mats}
+            "materials": mats, "camera": dict(cfg["camera"])}
 
 
 def terrain_config(triangles: int) -> dict:
@@ -168,7 +168,7 @@ def load_room_scene(engine, cfg: dict = CONFIG4, seed: int = 1, sky_img: np.ndar
         sky_img = sky()
     engine.set_environment(sky_img, 0.0, 1.0)
     engine.set_camera(**cfg["camera"])
-    return {"triangles": idx.shape[0], "instances": 1}
+    return {"triangles": idx.shape[0], "instances": 1, "camera": dict(cfg["camera"])}
 
 
 def displaced_sphere(n: int = 45, amplitude: float = 0.15, seed: int = 1) -> tuple[np.ndarray, np.ndarray]:
@@ -214,7 +214,7 @@ def load_sphere_field(engine, cfg: dict = CONFIG5, seed: int = 1, sky_img: np.nd
     engine.set_environment(sky_img, 0.0, 1.0)
     engine.set_camera(**cfg["camera"])
     return {"triangles": cfg["instances"] * idx.shape[0], "unique_triangles": idx.shape[0],
-            "instances": cfg["instances"]}
+            "instances": cfg["instances"], "camera": dict(cfg["camera"])}
 
 
 def soup(count: int = 10_000, seed: int = 1) -> np.ndarray:
