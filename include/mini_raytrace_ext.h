@@ -37,6 +37,19 @@ MRT_API mrtResult mrtxBuildBvh(const float* bounds, uint32_t count, uint32_t max
                                void* nodes, uint64_t nodeCap, uint32_t* primOrder,
                                uint64_t* outNodeCount);
 
+/* The same build on the GPU: identical nodes and primitive order (the
+ * reference builder's, node for node), inputs and outputs in device memory of
+ * the current HIP device.  dBounds: 6 floats per primitive; nodeCap must be
+ * at least 2 * count - 1 (the largest possible tree).  Work is enqueued on
+ * hipStream (NULL = the null stream); the call waits for it between tree
+ * levels and returns once the outputs are written.  A node that needs the
+ * reference's median split (SAH preferring a leaf of more than 2 * maxLeaf
+ * primitives, core/src/accel/Bvh.cpp:104-111) is built on the CPU instead,
+ * with the same result. */
+MRT_API mrtResult mrtxBuildBvhDevice(const float* dBounds, uint32_t count, uint32_t maxLeaf,
+                                     void* dNodes, uint64_t nodeCap, uint32_t* dPrimOrder,
+                                     uint64_t* outNodeCount, void* hipStream);
+
 typedef struct mrtxQueryScene mrtxQueryScene;
 
 /* Ray record on the device: 8 floats, 32 bytes:

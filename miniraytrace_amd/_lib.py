@@ -220,6 +220,7 @@ _SIGS = {
     "mrtGetLastErrorMessage": (C.c_char_p, []),
     # ext: ray queries
     "mrtxBuildBvh": (_R, [_P, _U32, _U32, _P, C.c_uint64, _P, C.POINTER(C.c_uint64)]),
+    "mrtxBuildBvhDevice": (_R, [_P, _U32, _U32, _P, C.c_uint64, _P, C.POINTER(C.c_uint64), _P]),
     "mrtxCreateQueryScene": (_R, [_P, _U32, _P, _U32, _U32, C.POINTER(_P)]),
     "mrtxDestroyQueryScene": (None, [_P]),
     "mrtxGetQuerySceneInfo": (_R, [_P, C.POINTER(mrtxQuerySceneInfo)]),

@@ -1,6 +1,8 @@
 // Engine implementation: scene packing, device buffers, frame loop, readback.
 #include "engine.h"
 
+#include "bvh_gpu_build.h"
+
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -27,6 +29,8 @@ float msSince(Clock::time_point t0) {
 constexpr uint32_t kMaxTraversalStack = 64 + 8;
 // Traversal stack entries hold node, triangle and instance indices in 27 bits.
 constexpr size_t kMaxTraversalIndex = (size_t(1) << 27) - 1;
+// Meshes with at least this many triangles build their BVH on the GPU.
+constexpr uint32_t kGpuBuildMinTriangles = 1u << 17;
 
 // A node whose inverted box fails every slab test (reference Renderer.cpp:51-58).
 BvhNode sentinelNode() {
@@ -320,9 +324,27 @@ uint32_t Engine::packGeometry() {
     for (auto& [id, m] : meshes)
         if (m.blasDirty || !blas_.count(id)) jobs.emplace_back(id, &m);
     std::vector<Blas> built(jobs.size());
+    // Large meshes build on the GPU (same nodes, kernels/bvh_gpu.hip), one
+    // after the other on the engine's stream; the rest on host threads.
+    std::vector<char> onGpu(jobs.size(), 0);
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        const MeshData& m = *jobs[j].second;
+        const uint32_t nt = m.triCount();
+        if (nt < kGpuBuildMinTriangles) continue;
+        std::vector<Box3> bounds(nt);
+        for (uint32_t t = 0; t < nt; ++t)
+            for (int k = 0; k < 3; ++k) bounds[t].grow(&m.positions[size_t(m.indices[t * 3 + k]) * 4]);
+        BvhBuild b = buildBvhOnDevice(bounds, 4, stream_);
+        built[j].nodes = std::move(b.nodes);
+        built[j].triOrder = std::move(b.primOrder);
+        built[j].bounds = b.rootBounds;
+        built[j].depth = b.maxDepth;
+        onGpu[j] = 1;
+    }
     std::atomic<size_t> next{ 0 };
     auto worker = [&] {
         for (size_t j; (j = next.fetch_add(1)) < jobs.size();) {
+            if (onGpu[j]) continue;
             const MeshData& m = *jobs[j].second;
             const uint32_t nt = m.triCount();
             std::vector<Box3> bounds(nt);

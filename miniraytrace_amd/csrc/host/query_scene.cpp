@@ -2,6 +2,7 @@
 // batches.  The BVH is the reference's binned-SAH BVH2 (Bvh.cpp:137-156,
 // maxLeaf 4) over triangle bounds accumulated as in Renderer.cpp:348-356.
 #include "bvh_build.h"
+#include "bvh_gpu_build.h"
 #include "common.h"
 #include "mini_raytrace_ext.h"
 #include "../kernels/launch.h"
@@ -148,6 +149,37 @@ MRT_API mrtResult mrtxBuildBvh(const float* bounds, uint32_t count, uint32_t max
         if (r.nodes.size() > nodeCap) throw Error(MRT_ERROR_INVALID_ARGUMENT, "node capacity too small");
         if (!r.nodes.empty()) std::memcpy(nodes, r.nodes.data(), r.nodes.size() * sizeof(mrt::BvhNode));
         if (count) std::memcpy(primOrder, r.primOrder.data(), size_t(count) * sizeof(uint32_t));
+        return MRT_SUCCESS;
+    });
+}
+
+MRT_API mrtResult mrtxBuildBvhDevice(const float* dBounds, uint32_t count, uint32_t maxLeaf, void* dNodes,
+                                     uint64_t nodeCap, uint32_t* dPrimOrder, uint64_t* outNodeCount,
+                                     void* hipStream) {
+    if (count > 0 && (!dBounds || !dNodes || !dPrimOrder)) return MRT_ERROR_INVALID_ARGUMENT;
+    if (count > 0 && nodeCap < uint64_t(2) * count - 1) {
+        mrt::setLastErrorMessage("mrtxBuildBvhDevice: nodeCap must be at least 2 * count - 1");
+        return MRT_ERROR_INVALID_ARGUMENT;
+    }
+    return mrt::guarded([&] {
+        hipStream_t s = static_cast<hipStream_t>(hipStream);
+        mrt::GpuBvhBuilder builder;
+        mrt::GpuBvhInfo info;
+        uint64_t nodes = 0;
+        if (builder.build(dBounds, count, maxLeaf, dNodes, nodeCap, dPrimOrder, info, s)) {
+            nodes = info.nodeCount;
+        } else {
+            // the median split: build on the CPU from the same bounds
+            std::vector<mrt::Box3> b(count);
+            MRT_HIP(hipMemcpyAsync(b.data(), dBounds, size_t(count) * sizeof(mrt::Box3), hipMemcpyDeviceToHost, s));
+            MRT_HIP(hipStreamSynchronize(s));
+            const mrt::BvhBuild r = mrt::buildBvh(b, maxLeaf);
+            nodes = r.nodes.size();
+            MRT_HIP(hipMemcpyAsync(dNodes, r.nodes.data(), nodes * sizeof(mrt::BvhNode), hipMemcpyHostToDevice, s));
+            MRT_HIP(hipMemcpyAsync(dPrimOrder, r.primOrder.data(), size_t(count) * 4, hipMemcpyHostToDevice, s));
+            MRT_HIP(hipStreamSynchronize(s));
+        }
+        if (outNodeCount) *outNodeCount = nodes;
         return MRT_SUCCESS;
     });
 }
