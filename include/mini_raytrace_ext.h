@@ -45,7 +45,9 @@ MRT_API mrtResult mrtxBuildBvh(const float* bounds, uint32_t count, uint32_t max
  * levels and returns once the outputs are written.  A node that needs the
  * reference's median split (SAH preferring a leaf of more than 2 * maxLeaf
  * primitives, core/src/accel/Bvh.cpp:104-111) is built on the CPU instead,
- * with the same result. */
+ * with the same result (an Info log message says so).  count must be below
+ * 2^31.  The library keeps one builder workspace per device until exit;
+ * concurrent calls on one device take turns. */
 MRT_API mrtResult mrtxBuildBvhDevice(const float* dBounds, uint32_t count, uint32_t maxLeaf,
                                      void* dNodes, uint64_t nodeCap, uint32_t* dPrimOrder,
                                      uint64_t* outNodeCount, void* hipStream);

@@ -7,6 +7,8 @@
 #include "kernels/bvh_gpu.h"
 
 #include <algorithm>
+#include <array>
+#include <chrono>
 #include <cstring>
 #include <functional>
 
@@ -38,7 +40,27 @@ struct HostLarge {
 } // namespace
 
 struct GpuBvhBuilder::Work {
-    DeviceBuffer idx, pos, acc, counts, totals, nodes, chunks, tasks, results, tmp, place, large, where, flags;
+    DeviceBuffer idx, pos, acc, counts, totals, level, tasks, results, tmp, placed, flags;
+    PinnedBuffer up, down;  // staging of the per-level copies
+
+    // Host arrays -> one pinned block -> one copy into dst; returns the
+    // offsets of the parts (16-byte aligned).
+    template <size_t N>
+    std::array<size_t, N> upload(DeviceBuffer& dst, const std::array<std::pair<const void*, size_t>, N>& parts,
+                                 hipStream_t s) {
+        std::array<size_t, N> off{};
+        size_t total = 0;
+        for (size_t i = 0; i < N; ++i) {
+            off[i] = total;
+            total += (parts[i].second + 15) & ~size_t(15);
+        }
+        up.ensure(total);
+        for (size_t i = 0; i < N; ++i)
+            if (parts[i].second) std::memcpy(static_cast<char*>(up.get()) + off[i], parts[i].first, parts[i].second);
+        dst.ensure(total);
+        if (total) MRT_HIP(hipMemcpyAsync(dst.get(), up.get(), total, hipMemcpyHostToDevice, s));
+        return off;
+    }
 };
 
 GpuBvhBuilder::GpuBvhBuilder() : w_(new Work) {}
@@ -48,6 +70,11 @@ bool GpuBvhBuilder::build(const float* dBounds, uint32_t n, uint32_t maxLeaf, vo
                           uint32_t* dOrder, GpuBvhInfo& info, hipStream_t s) {
     info = GpuBvhInfo{};
     if (n == 0) return true;
+    if (n >= (1u << 31)) throw Error(MRT_ERROR_INVALID_ARGUMENT, "GPU BVH build: more than 2^31 - 1 primitives");
+    const auto t0 = std::chrono::steady_clock::now();
+    auto msSince = [](std::chrono::steady_clock::time_point t) {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+    };
     maxLeaf = std::max(1u, maxLeaf);
     Work& w = *w_;
     w.idx.ensure(size_t(n) * 4);
@@ -90,19 +117,18 @@ bool GpuBvhBuilder::build(const float* dBounds, uint32_t n, uint32_t maxLeaf, vo
             d.chunks = uint32_t(ch.size()) - d.chunk0;
         }
         const uint32_t nn = uint32_t(ln.size()), nc = uint32_t(ch.size());
-        w.nodes.ensure(nn * sizeof(LargeNode));
-        w.chunks.ensure(nc * sizeof(LargeChunk));
-        w.acc.ensure(size_t(nn) * 236 * sizeof(int));
+        w.acc.ensure(size_t(nn) * dev::kLargeAccWords * sizeof(int));
         w.counts.ensure(size_t(nc) * 2 * 4);
         w.totals.ensure(size_t(nn) * 2 * 4);
-        w.nodes.upload(ln.data(), nn * sizeof(LargeNode), 0, s);
-        w.chunks.upload(ch.data(), nc * sizeof(LargeChunk), 0, s);
+        const auto off = w.upload<2>(w.level, { { { ln.data(), nn * sizeof(LargeNode) },
+                                                  { ch.data(), nc * sizeof(LargeChunk) } } }, s);
+        LargeNode* dLevel = reinterpret_cast<LargeNode*>(static_cast<char*>(w.level.get()) + off[0]);
         dev::LargeLevelArgs a;
         a.bounds = dBounds;
         a.idx = idx;
-        a.nodes = w.nodes.as<LargeNode>();
+        a.nodes = dLevel;
         a.nodeCount = nn;
-        a.chunks = w.chunks.as<const LargeChunk>();
+        a.chunks = reinterpret_cast<const LargeChunk*>(static_cast<char*>(w.level.get()) + off[1]);
         a.chunkCount = nc;
         a.acc = w.acc.as<int>();
         a.counts = w.counts.as<uint32_t>();
@@ -111,9 +137,13 @@ bool GpuBvhBuilder::build(const float* dBounds, uint32_t n, uint32_t maxLeaf, vo
         a.maxLeaf = maxLeaf;
         a.flags = flags;
         MRT_HIP(dev::launchBvhLargeLevel(a, s));
-        w.nodes.download(ln.data(), nn * sizeof(LargeNode), 0, s);
-        w.flags.download(&hostFlags, 4, 0, s);
+        w.down.ensure(nn * sizeof(LargeNode) + 16);
+        MRT_HIP(hipMemcpyAsync(w.down.get(), dLevel, nn * sizeof(LargeNode), hipMemcpyDeviceToHost, s));
+        MRT_HIP(hipMemcpyAsync(static_cast<char*>(w.down.get()) + nn * sizeof(LargeNode), flags, 4,
+                               hipMemcpyDeviceToHost, s));
         MRT_HIP(hipStreamSynchronize(s));
+        std::memcpy(ln.data(), w.down.get(), nn * sizeof(LargeNode));
+        std::memcpy(&hostFlags, static_cast<char*>(w.down.get()) + nn * sizeof(LargeNode), 4);
         if (hostFlags & dev::kBuildNeedsMedian) return false;
         next.clear();
         for (size_t i = 0; i < level.size(); ++i) {
@@ -132,14 +162,16 @@ bool GpuBvhBuilder::build(const float* dBounds, uint32_t n, uint32_t maxLeaf, vo
             large[id].kid[1] = r;
         }
         level.swap(next);
+        ++info.levels;
     }
+    info.msLevels = msSince(t0);
+    const auto t1 = std::chrono::steady_clock::now();
 
     const uint32_t nt = uint32_t(tasks.size());
     std::vector<SmallResult> res(nt);
     if (nt) {
-        w.tasks.ensure(nt * sizeof(SmallTask));
+        w.upload<1>(w.tasks, { { { tasks.data(), nt * sizeof(SmallTask) } } }, s);
         w.results.ensure(nt * sizeof(SmallResult));
-        w.tasks.upload(tasks.data(), nt * sizeof(SmallTask), 0, s);
         dev::SmallArgs a;
         a.bounds = dBounds;
         a.idx = idx;
@@ -150,11 +182,17 @@ bool GpuBvhBuilder::build(const float* dBounds, uint32_t n, uint32_t maxLeaf, vo
         a.maxLeaf = maxLeaf;
         a.flags = flags;
         MRT_HIP(dev::launchBvhSmall(a, s));
-        w.results.download(res.data(), nt * sizeof(SmallResult), 0, s);
-        w.flags.download(&hostFlags, 4, 0, s);
+        w.down.ensure(nt * sizeof(SmallResult) + 16);
+        MRT_HIP(hipMemcpyAsync(w.down.get(), w.results.get(), nt * sizeof(SmallResult), hipMemcpyDeviceToHost, s));
+        MRT_HIP(hipMemcpyAsync(static_cast<char*>(w.down.get()) + nt * sizeof(SmallResult), flags, 4,
+                               hipMemcpyDeviceToHost, s));
         MRT_HIP(hipStreamSynchronize(s));
+        std::memcpy(res.data(), w.down.get(), nt * sizeof(SmallResult));
+        std::memcpy(&hostFlags, static_cast<char*>(w.down.get()) + nt * sizeof(SmallResult), 4);
         if (hostFlags & dev::kBuildNeedsMedian) return false;
     }
+    info.msSubtrees = msSince(t1);
+    const auto t2 = std::chrono::steady_clock::now();
 
     // Depth-first placement: a node's child pair is allocated when the node
     // is reached, then its left subtree, then its right subtree.
@@ -201,28 +239,23 @@ bool GpuBvhBuilder::build(const float* dBounds, uint32_t n, uint32_t maxLeaf, vo
 
     dev::PlaceArgs p{};
     const uint32_t nl = uint32_t(largeOut.size());
-    if (nl) {
-        w.large.ensure(nl * sizeof(OutNode));
-        w.where.ensure(nl * 4);
-        w.large.upload(largeOut.data(), nl * sizeof(OutNode), 0, s);
-        w.where.upload(where.data(), nl * 4, 0, s);
-    }
-    if (nt) {
-        w.place.ensure(nt * sizeof(SmallPlace));
-        w.place.upload(place.data(), nt * sizeof(SmallPlace), 0, s);
-    }
-    p.large = w.large.as<const OutNode>();
-    p.largeWhere = w.where.as<const uint32_t>();
+    const auto po = w.upload<3>(w.placed, { { { largeOut.data(), nl * sizeof(OutNode) },
+                                              { where.data(), nl * 4 },
+                                              { place.data(), nt * sizeof(SmallPlace) } } }, s);
+    const char* placed = static_cast<const char*>(w.placed.get());
+    p.large = reinterpret_cast<const OutNode*>(placed + po[0]);
+    p.largeWhere = reinterpret_cast<const uint32_t*>(placed + po[1]);
     p.largeCount = nl;
     p.tmp = w.tmp.as<const OutNode>();
     p.tasks = w.tasks.as<const SmallTask>();
     p.results = w.results.as<const SmallResult>();
-    p.place = w.place.as<const SmallPlace>();
+    p.place = reinterpret_cast<const SmallPlace*>(placed + po[2]);
     p.taskCount = nt;
     p.dst = static_cast<OutNode*>(dNodes);
     MRT_HIP(dev::launchBvhPlace(p, s));
     MRT_HIP(hipMemcpyAsync(dOrder, idx, size_t(n) * 4, hipMemcpyDeviceToDevice, s));
     MRT_HIP(hipStreamSynchronize(s));
+    info.msPlace = msSince(t2);
     info.nodeCount = total;
     info.maxDepth = maxDepth;
     info.leafCount = leaves;
@@ -231,14 +264,14 @@ bool GpuBvhBuilder::build(const float* dBounds, uint32_t n, uint32_t maxLeaf, vo
     return true;
 }
 
-BvhBuild buildBvhOnDevice(const std::vector<Box3>& bounds, uint32_t maxLeaf, hipStream_t s, bool* onGpu) {
+BvhBuild buildBvhOnDevice(GpuBvhBuilder& builder, const std::vector<Box3>& bounds, uint32_t maxLeaf, hipStream_t s,
+                          bool* onGpu) {
     const uint32_t n = uint32_t(bounds.size());
     if (onGpu) *onGpu = false;
     if (n == 0) return BvhBuild{};
     static_assert(sizeof(Box3) == 24, "Box3 is lo xyz, hi xyz");
     DeviceBuffer dB(size_t(n) * 24), dNodes(size_t(2) * n * sizeof(BvhNode)), dOrder(size_t(n) * 4);
     dB.upload(bounds.data(), size_t(n) * 24, 0, s);
-    GpuBvhBuilder builder;
     GpuBvhInfo info;
     if (!builder.build(dB.as<const float>(), n, maxLeaf, dNodes.get(), uint64_t(2) * n, dOrder.as<uint32_t>(), info,
                        s))

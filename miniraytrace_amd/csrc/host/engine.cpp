@@ -334,7 +334,7 @@ uint32_t Engine::packGeometry() {
         std::vector<Box3> bounds(nt);
         for (uint32_t t = 0; t < nt; ++t)
             for (int k = 0; k < 3; ++k) bounds[t].grow(&m.positions[size_t(m.indices[t * 3 + k]) * 4]);
-        BvhBuild b = buildBvhOnDevice(bounds, 4, stream_);
+        BvhBuild b = buildBvhOnDevice(gpuBvh_, bounds, 4, stream_);
         built[j].nodes = std::move(b.nodes);
         built[j].triOrder = std::move(b.primOrder);
         built[j].bounds = b.rootBounds;

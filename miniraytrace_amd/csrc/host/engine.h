@@ -5,6 +5,7 @@
 #pragma once
 
 #include "bvh_build.h"
+#include "bvh_gpu_build.h"
 #include "common.h"
 #include "common/gpu_types.h"
 #include "guide.h"
@@ -167,6 +168,7 @@ private:
     size_t totalTris_ = 0, totalVerts_ = 0;
 
     // scene buffers
+    GpuBvhBuilder gpuBvh_;  // large meshes' BLAS builds (workspace kept between commits)
     DeviceBuffer dNodes_, dTriVerts_, dTris_, dPos_, dNrm_, dUv_, dTan_;
     DeviceBuffer dInst_, dTraceInst_, dMats_, dLights_, dEnv_, dAlias_;
     DeviceBuffer dTexTable_, dTex8_, dTex32_;

@@ -11,6 +11,8 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <map>
+#include <mutex>
 #include <vector>
 
 struct mrtxQueryScene {
@@ -163,13 +165,24 @@ MRT_API mrtResult mrtxBuildBvhDevice(const float* dBounds, uint32_t count, uint3
     }
     return mrt::guarded([&] {
         hipStream_t s = static_cast<hipStream_t>(hipStream);
-        mrt::GpuBvhBuilder builder;
+        // One builder per device keeps its workspace (about 100 bytes per
+        // primitive of the largest build so far) until the process ends;
+        // calls on one device take turns.
+        static std::mutex lock;
+        static auto* builders = new std::map<int, mrt::GpuBvhBuilder*>;
+        int device = 0;
+        MRT_HIP(hipGetDevice(&device));
+        std::lock_guard<std::mutex> g(lock);
+        mrt::GpuBvhBuilder*& builder = (*builders)[device];
+        if (!builder) builder = new mrt::GpuBvhBuilder;
         mrt::GpuBvhInfo info;
         uint64_t nodes = 0;
-        if (builder.build(dBounds, count, maxLeaf, dNodes, nodeCap, dPrimOrder, info, s)) {
+        if (builder->build(dBounds, count, maxLeaf, dNodes, nodeCap, dPrimOrder, info, s)) {
             nodes = info.nodeCount;
         } else {
             // the median split: build on the CPU from the same bounds
+            mrt::logMessage(mrt::LogLevel::Info,
+                            "mrtxBuildBvhDevice: a node needs the median split; building on the CPU");
             std::vector<mrt::Box3> b(count);
             MRT_HIP(hipMemcpyAsync(b.data(), dBounds, size_t(count) * sizeof(mrt::Box3), hipMemcpyDeviceToHost, s));
             MRT_HIP(hipStreamSynchronize(s));

@@ -7,11 +7,12 @@
 namespace mrt {
 namespace dev {
 
-constexpr uint32_t kSmallMax = 1024;  // nodes with at most this many primitives: one workgroup builds the subtree
+constexpr uint32_t kSmallMax = 512;  // nodes with at most this many primitives: one wavefront builds the subtree
 constexpr uint32_t kChunk = 1024;     // slots per chunk of a large node
 constexpr uint32_t kSplitLeaf = 0xFFFFFFFFu;
 constexpr uint32_t kSplitPending = 0xFFFFFFFEu;
 constexpr uint32_t kBuildNeedsMedian = 1u;  // flag: a node needs the reference's median split
+constexpr uint32_t kLargeAccWords = 18 + 32 * 7;  // accumulator words per large node (boxes, zero codes, 32 bins)
 
 // The 32-byte node record (reference Bvh.hpp:15-26).
 struct OutNode {

@@ -4,9 +4,10 @@ restatement of core/src/accel/Bvh.cpp) and against the oracle builder
 itself at 1M triangles: identical node records and primitive order, byte
 for byte.
 
-Cases: sizes on both sides of the 1024-primitive threshold between the
-level-by-level and the one-workgroup-per-subtree phases, random soups,
-the 1,000,000-triangle config-2 terrain, coincident centroids (leaf by
+Cases: sizes on both sides of the 512-primitive threshold between the
+level-by-level and the one-wavefront-per-subtree phases (and of the
+1024-slot chunks of the first), random soups, the 1,000,000-triangle
+terrain, coincident centroids (leaf by
 degenerate extent), and boxes so large that SAH prefers an oversized leaf,
 which needs the reference's median split and so is built on the CPU by the
 same call.
@@ -40,13 +41,34 @@ def _mesh_bounds(pos, idx):
     return np.concatenate([t.min(1), t.max(1)], 1).astype(np.float32)
 
 
-def _check(torch, bounds, max_leaf=4, label=""):
+@pytest.fixture(scope="module")
+def fallbacks():
+    """Messages of builds that went to the CPU (the median split), through
+    mrtSetLogCallback."""
+    import ctypes as C
+    from miniraytrace_amd import _lib as L
+    seen = []
+
+    def on_log(level, msg, user):
+        text = msg.decode()
+        if "median split" in text:
+            seen.append(text)
+
+    cb = L.LOG_FN(on_log)
+    L.lib().mrtSetLogCallback(cb, None)
+    yield seen
+    L.lib().mrtSetLogCallback(L.LOG_FN(), None)
+
+
+def _check(torch, bounds, max_leaf=4, label="", fallbacks=None, on_gpu=True):
     from miniraytrace_amd.bvh import build_bvh, build_bvh_device
     d = torch.from_numpy(bounds).cuda()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    gn, go = build_bvh_device(d, max_leaf)
-    t_gpu = time.perf_counter() - t0
+    timing = {}
+    before = len(fallbacks) if fallbacks is not None else 0
+    gn, go = build_bvh_device(d, max_leaf, timing=timing)
+    if fallbacks is not None:
+        assert (len(fallbacks) == before) == on_gpu, "built on the GPU" if on_gpu else "built on the CPU"
+    t_gpu = timing["ms"] * 1e-3
     t0 = time.perf_counter()
     cn, co = build_bvh(bounds, max_leaf)
     t_cpu = time.perf_counter() - t0
@@ -57,22 +79,22 @@ def _check(torch, bounds, max_leaf=4, label=""):
     return gn, go
 
 
-@pytest.mark.parametrize("n", [1, 3, 4, 5, 17, 1000, 1024, 1025, 2049, 40_000])
-def test_random_soups(gpu, n):
-    _check(gpu, _soup_bounds(n, 100 + n), label=f"soup {n}")
+@pytest.mark.parametrize("n", [1, 3, 4, 5, 17, 512, 513, 1000, 1024, 1025, 2049, 40_000])
+def test_random_soups(gpu, fallbacks, n):
+    _check(gpu, _soup_bounds(n, 100 + n), label=f"soup {n}", fallbacks=fallbacks)
 
 
-def test_leaf_sizes(gpu):
+def test_leaf_sizes(gpu, fallbacks):
     b = _soup_bounds(5000, 7)
     for leaf in (1, 2, 8):
-        _check(gpu, b, leaf, label=f"soup 5000, max leaf {leaf}")
+        _check(gpu, b, leaf, label=f"soup 5000, max leaf {leaf}", fallbacks=fallbacks)
 
 
-def test_terrain_million_matches_oracle(gpu, oracle):
+def test_terrain_million_matches_oracle(gpu, fallbacks, oracle):
     from miniraytrace_amd import scenes
     pos, idx = scenes.terrain(1000, 500, 20.0, 10.0, 0.2, 1)
     b = _mesh_bounds(pos, idx)
-    gn, go = _check(gpu, b, label="terrain 1M")
+    gn, go = _check(gpu, b, label="terrain 1M", fallbacks=fallbacks)
     t0 = time.perf_counter()
     on, oo = oracle.bvh_build(b, 4)
     _print(f"oracle builder {time.perf_counter() - t0:.1f} s")
@@ -80,21 +102,31 @@ def test_terrain_million_matches_oracle(gpu, oracle):
     assert np.array_equal(go, oo)
 
 
-def test_degenerate_inputs(gpu):
+def test_degenerate_inputs(gpu, fallbacks):
     # every centroid equal: one leaf of all primitives (extent below 1e-8)
     same = np.tile(np.array([[0, 0, 0, 1, 1, 1]], np.float32), (3000, 1))
-    gn, _ = _check(gpu, same, label="coincident centroids")
+    gn, _ = _check(gpu, same, label="coincident centroids", fallbacks=fallbacks)
     assert len(gn) == 1 and (gn[0]["packed"] >> 2) == 3000
     # clusters of coincident centroids among spread ones
     b = _soup_bounds(6000, 3)
     b[:2500] = b[0]
-    _check(gpu, b, label="clustered centroids")
+    _check(gpu, b, label="clustered centroids", fallbacks=fallbacks)
     # boxes much larger than the centroid spread: SAH prefers a leaf, the
     # reference takes the median split (built on the CPU by the same call)
     rng = np.random.default_rng(5)
     c = rng.random((4000, 3), dtype=np.float32)
     huge = np.concatenate([c - 50.0, c + 50.0], 1).astype(np.float32)
-    _check(gpu, huge, label="median-split fallback")
+    _check(gpu, huge, label="median-split fallback", fallbacks=fallbacks, on_gpu=False)
+
+
+@pytest.mark.parametrize("n", [400, 40_000])
+def test_signed_zero_ties(gpu, fallbacks, n):
+    """Bounds that tie between -0.0 and +0.0 keep the first zero in slot
+    order, in the one-wavefront subtrees and in the level phase."""
+    from tests.test_oracle_bvh import signed_zero_bounds, zero_signs
+    gn, _ = _check(gpu, signed_zero_bounds(n, 20 + n), label=f"signed zeros {n}", fallbacks=fallbacks)
+    neg, pos = zero_signs(gn)
+    assert neg > 0 and pos > 0
 
 
 def test_engine_commit_uses_identical_bvh(gpu):

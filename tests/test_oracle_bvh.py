@@ -26,6 +26,30 @@ def random_triangles(count, seed):
     return np.concatenate([a, b, c], axis=1).astype(np.float32)
 
 
+def signed_zero_bounds(count, seed):
+    """Small random boxes in [-1, 1]^3; per axis, boxes within 0.05 of the
+    plane through the origin have the bound nearest it moved onto it, stored
+    as -0.0 or +0.0 at random: node bounds then tie between the two zeros, and
+    the builder keeps the first one in slot order (glm::min / glm::max keep
+    the value they hold on a tie).  Centroids stay distinct and boxes small,
+    so SAH splits down to small leaves."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-1, 1, size=(count, 3)).astype(np.float32)
+    e = rng.uniform(0.001, 0.02, size=(count, 3)).astype(np.float32)
+    lo, hi = c - e, c + e
+    snap = np.abs(c) < 0.05
+    zero = np.where(rng.random((count, 3)) < 0.5, np.float32(-0.0), np.float32(0.0)).astype(np.float32)
+    lo = np.where(snap & (c > 0), zero, lo)
+    hi = np.where(snap & (c <= 0), zero, hi)
+    return np.concatenate([lo, hi], 1).astype(np.float32)
+
+
+def zero_signs(nodes):
+    """(nodes with a -0.0 bound, nodes with a +0.0 bound)."""
+    bits = np.concatenate([nodes["lo"], nodes["hi"]], 1).view(np.uint32)
+    return int((bits == 0x80000000).any(1).sum()), int((bits == 0).any(1).sum())
+
+
 def check_topology(nodes, order, n):
     assert len(order) == n
     seen = np.zeros(n, np.int64)
@@ -193,6 +217,15 @@ def test_product_builder_matches_oracle(oracle):
     assert oracle.bvh_build(same, 4)[0].tobytes() == build_bvh(same, 4)[0].tobytes()
     n_p, o_p = build_bvh(np.zeros((0, 6), np.float32), 4)
     assert len(n_p) == 0
+    # ties between -0.0 and +0.0 in box bounds
+    for n, seed in [(700, 11), (9000, 12)]:
+        b = signed_zero_bounds(n, seed)
+        n_o, o_o = oracle.bvh_build(b, 4)
+        n_p, o_p = build_bvh(b, 4)
+        neg, pos = zero_signs(n_o)
+        assert neg > 0 and pos > 0
+        assert n_o.tobytes() == n_p.tobytes()
+        assert np.array_equal(o_o, o_p)
     # TLAS leaf size 1
     tris = random_triangles(257, 4)
     b = oracle.tri_bounds(tris)
