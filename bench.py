@@ -7,7 +7,7 @@ Workloads:
   pt         (default) plain progressive path tracing at 1920x1080 on a
              1.256M-triangle terrain scene; one step = one frame (1 spp).
              The default run appends the traversal, batch, guided,
-             convergence, (shortened) scale and bvh objects below.
+             convergence, (shortened) scale, bvh and edit objects below.
   traversal  closest-hit queries: 10,000-triangle seeded soup, 1,000,000
              rays (one step = one mrtxTraceClosest launch over all rays).
   guided     equal-RMSE wall time of guided vs plain path tracing on a room
@@ -22,6 +22,8 @@ Workloads:
              the FP64 oracle at every power-of-two spp up to 1024.
   bvh        GPU BLAS build of the 1M-triangle terrain vs the threaded CPU
              builder (identical nodes), and the engine commit of that mesh.
+  edit       edit-to-frame latency on the 1080p terrain scene: an instance
+             moved, the sphere mesh and the terrain deformed, a mesh added.
 
 Prints ONE JSON line.  Inputs are resident in device memory before the timed
 region; the timed region is bracketed by device synchronisation.  The
@@ -834,13 +836,98 @@ def bvh_workload(args, torch, reps: int = 5) -> dict:
                              "sample": "the whole 1M-triangle build, product CPU builder (threaded)"}}
 
 
+def edit_workload(args, torch, reps: int = 5) -> dict:
+    """Edit-to-frame latency on the 1080p terrain scene (1,256,000
+    triangles, depth 8): wall time from the edit call through the commit to
+    the end of the next 1-spp frame, against the 100 ms target of the
+    reference's interactive integration (docs/rhino-integration-design.md:30).
+    Edits: one sphere instance moved (TLAS only); the sphere mesh (1,280
+    triangles, 200 instances) deformed in place; the 1M-triangle terrain
+    deformed in place (GPU BLAS build); a new mesh and instance appended."""
+    from miniraytrace_amd import scenes
+    from miniraytrace_amd.engine import Engine
+    W, H = 1920, 1080
+    e = Engine(width=W, height=H)
+    e.set_render_settings(width=W, height=H, max_bounces=8, spp_limit=1 << 20, firefly_clamp=0.0, denoise=False)
+    info = scenes.load_terrain_scene(e, scenes.CONFIG2, seed=1, sky_img=scenes.sky())
+    e.commit()
+    for _ in range(3):
+        e.render_frame()
+    cfg = scenes.CONFIG2
+    tpos, tidx = scenes.terrain(cfg["nx"], cfg["nz"], cfg["sx"], cfg["sz"], cfg["amplitude"], 1)
+    spos, sidx = scenes.icosphere(cfg["sphere_level"])
+
+    def timed(edit):
+        torch.cuda.synchronize()
+        before = e.frame_stats()
+        t0 = time.perf_counter()
+        edit()
+        tc = time.perf_counter()
+        e.commit()
+        t1 = time.perf_counter()
+        e.render_frame()
+        t2 = time.perf_counter()
+        st = e.frame_stats()
+        return {"edit_to_frame_ms": (t2 - t0) * 1e3, "edit_call_ms": (tc - t0) * 1e3, "commit_ms": (t1 - tc) * 1e3,
+                "frame_ms": (t2 - t1) * 1e3,
+                "upload_bytes": st["commitUploadBytes"],
+                "repacks": st["geometryRepacks"] - before["geometryRepacks"],
+                "meshes_written": st["meshesWritten"] - before["meshesWritten"]}
+
+    def summary(runs):
+        out = {k: round(float(np.median([r[k] for r in runs])), 3)
+               for k in ("edit_to_frame_ms", "edit_call_ms", "commit_ms", "frame_ms")}
+        out.update(upload_bytes=int(runs[-1]["upload_bytes"]), repacks=int(sum(r["repacks"] for r in runs)),
+                   meshes_written=int(sum(r["meshes_written"] for r in runs)), runs=len(runs))
+        return out
+
+    res = {"workload": "edit-to-frame latency, 1080p terrain scene (1,256,000 triangles), depth 8",
+           "target_ms": 100.0}
+    inst = info["instance_ids"][1]
+    xf0 = np.asarray(info["sphere_xforms"][0], np.float32).reshape(3, 4).copy()
+
+    def move(k):
+        xf = xf0.copy()
+        xf[:, 3] += np.float32(0.05 * (k + 1))
+        return lambda: e.set_instance_transform(inst, xf.reshape(12))
+
+    res["instance_transform"] = summary([timed(move(k)) for k in range(reps)])
+
+    def deform_sphere(k):
+        v = (spos * (1 + 0.05 * (k + 1) * np.sin(4 * spos[:, :1]))).astype(np.float32)
+        return lambda: e.update_mesh(info["sphere_mesh"], v, sidx, normals=v)
+
+    res["sphere_mesh_update"] = summary([timed(deform_sphere(k)) for k in range(reps)])
+
+    def deform_terrain(k):
+        p = tpos.copy()
+        p[:, 1] += np.float32(0.02 * (k + 1)) * np.sin(p[:, 0]).astype(np.float32)
+        return lambda: e.update_mesh(info["terrain_mesh"], p, tidx)
+
+    res["terrain_mesh_update"] = summary([timed(deform_terrain(k)) for k in range(3)])
+
+    def add_mesh(k):
+        def go():
+            m = e.add_mesh(spos * np.float32(0.5), sidx, normals=spos)
+            xf = xf0.copy()
+            xf[1, 3] += np.float32(1.0 + 0.3 * k)
+            e.add_instance(m, xf.reshape(12), info["material_ids"][1])
+        return go
+
+    res["mesh_added"] = summary([timed(add_mesh(k)) for k in range(3)])
+    e.close()
+    _log("edit: " + ", ".join(f"{k} {v['edit_to_frame_ms']:.1f} ms (commit {v['commit_ms']:.1f})"
+                              for k, v in res.items() if isinstance(v, dict)))
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="pt",
-                    choices=["pt", "traversal", "guided", "batch", "scale", "converge", "copy", "bvh"])
+                    choices=["pt", "traversal", "guided", "batch", "scale", "converge", "copy", "bvh", "edit"])
     ap.add_argument("--guided-spp", type=int, default=1024,
                     help="plain sample count whose RMSE the guided run must reach")
     ap.add_argument("--guided-warmup", type=int, default=0,
@@ -852,6 +939,7 @@ def main() -> int:
     ap.add_argument("--no-scale", dest="with_scale", action="store_false", default=True)
     ap.add_argument("--no-batch", dest="with_batch", action="store_false", default=True)
     ap.add_argument("--no-bvh", dest="with_bvh", action="store_false", default=True)
+    ap.add_argument("--no-edit", dest="with_edit", action="store_false", default=True)
     ap.add_argument("--autotune", action="store_true", default=True)
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -875,6 +963,13 @@ def main() -> int:
                           "n_gpus": 1, "higher_is_better": False, "dtype": "f32", "data": "synthetic",
                           "config": {"workload": res["workload"]}, "roofline": res["roofline"],
                           "cpu_baseline": res["cpu_baseline"], "bvh": res}))
+        return 0
+    if args.workload == "edit":
+        res = edit_workload(args, torch)
+        print(json.dumps({"metric": "edit-to-frame ms, terrain mesh update", "value":
+                          res["terrain_mesh_update"]["edit_to_frame_ms"], "unit": "ms", "n_gpus": 1,
+                          "higher_is_better": False, "dtype": "f32", "data": "synthetic",
+                          "config": {"workload": res["workload"]}, "edit": res}))
         return 0
     if args.workload == "copy":
         res = copy_workload(args, torch)
@@ -956,6 +1051,8 @@ def main() -> int:
         line["scale"] = scale_workload(args, torch, frames=4, warm=8, spp=16)
     if args.with_bvh:
         line["bvh"] = bvh_workload(args, torch)
+    if args.with_edit:
+        line["edit"] = edit_workload(args, torch)
     print(json.dumps(line))
     return 0
 

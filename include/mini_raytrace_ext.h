@@ -170,6 +170,12 @@ typedef struct mrtxFrameStats {
     uint32_t rays[MRTX_MAX_BOUNCE_LEVELS];           /* rays traced per bounce level */
     uint32_t shadowRays[MRTX_MAX_BOUNCE_LEVELS];     /* shadow rays per bounce level */
     uint32_t status;                                 /* bit 0: traversal stack overflow */
+    /* Appended fields, written when structSize covers them: */
+    float    commitMs;          /* host wall time of the last commit that changed the scene */
+    uint64_t commitUploadBytes; /* bytes that commit copied to the device */
+    uint32_t geometryRepacks;   /* commits that repacked every mesh, since creation */
+    uint32_t meshesWritten;     /* meshes rewritten alone (in place, or appended
+                                   behind the others), since creation */
 } mrtxFrameStats;
 
 /* Profiling mode: each frame runs stage by stage with HIP events between the

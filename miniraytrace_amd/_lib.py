@@ -137,7 +137,9 @@ MRTX_MAX_BOUNCE_LEVELS = 18
 class mrtxFrameStats(_Sized):
     _fields_ = [("structSize", C.c_size_t), ("frameMs", C.c_float), ("gpuMs", C.c_float * 8),
                 ("rays", C.c_uint32 * MRTX_MAX_BOUNCE_LEVELS),
-                ("shadowRays", C.c_uint32 * MRTX_MAX_BOUNCE_LEVELS), ("status", C.c_uint32)]
+                ("shadowRays", C.c_uint32 * MRTX_MAX_BOUNCE_LEVELS), ("status", C.c_uint32),
+                ("commitMs", C.c_float), ("commitUploadBytes", C.c_uint64), ("geometryRepacks", C.c_uint32),
+                ("meshesWritten", C.c_uint32)]
 
 
 class mrtxBatchRequest(C.Structure):

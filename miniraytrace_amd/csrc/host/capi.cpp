@@ -5,7 +5,6 @@
 // mutators ignore bad handles; exceptions never cross the boundary.
 #include "capi_handles.h"
 
-#include <cstddef>
 #include "common.h"
 #include "engine.h"
 #include "mini_raytrace.h"
@@ -243,7 +242,7 @@ MRT_API mrtResult mrtxSetProfiling(mrtEngine* e, int32_t enable) {
 MRT_API mrtResult mrtxGetFrameStats(mrtEngine* e, mrtxFrameStats* out) {
     mrt::Engine* eng = unwrap(e);
     if (!eng) return MRT_ERROR_INVALID_HANDLE;
-    if (!validStruct(out)) return MRT_ERROR_INVALID_ARGUMENT;
+    if (!out || out->structSize < offsetof(mrtxFrameStats, commitMs)) return MRT_ERROR_INVALID_ARGUMENT;
     const mrt::FrameStats& s = eng->stats();
     out->frameMs = s.frameMs;
     for (int i = 0; i < 8; ++i) out->gpuMs[i] = s.stageMs[i];
@@ -252,6 +251,11 @@ MRT_API mrtResult mrtxGetFrameStats(mrtEngine* e, mrtxFrameStats* out) {
         out->shadowRays[b] = s.shadowRays[b];
     }
     out->status = s.status;
+    if (out->structSize < sizeof(mrtxFrameStats)) return MRT_SUCCESS;
+    out->commitMs = s.commitMs;
+    out->commitUploadBytes = s.commitUploadBytes;
+    out->geometryRepacks = s.geometryRepacks;
+    out->meshesWritten = s.meshesWritten;
     return MRT_SUCCESS;
 }
 

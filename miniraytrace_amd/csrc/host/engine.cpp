@@ -296,6 +296,7 @@ void Engine::sync(mrtCommitStats* out) {
     }
     DeviceScope scope(device_);
     MRT_HIP(hipStreamSynchronize(stream_));
+    uploadBytes_ = 0;
     uint32_t rebuilt = 0;
     if (d & kDirtyTextures) uploadTextures();
     if (d & kDirtyMaterials) packMaterials();
@@ -306,10 +307,14 @@ void Engine::sync(mrtCommitStats* out) {
     if (d & kDirtyEnv) packEnvironment();
     scene_.clearDirty();
     resetAccumulation();
+    stats_.commitMs = msSince(t0);
+    stats_.commitUploadBytes = uploadBytes_;
+    stats_.geometryRepacks = geometryRepacks_;
+    stats_.meshesWritten = meshesWritten_;
     if (out) {
         out->blasRebuilt = rebuilt;
         out->tlasRebuilt = tlas ? 1 : 0;
-        out->commitMs = msSince(t0);
+        out->commitMs = stats_.commitMs;
     }
 }
 
@@ -368,73 +373,213 @@ uint32_t Engine::packGeometry() {
         jobs[j].second->blasDirty = false;
     }
 
-    size_t nv = 0, nt = 0, nn = 0;
-    for (auto& [id, m] : meshes) {
-        nv += m.vertexCount();
-        nt += m.triCount();
-        nn += blas_[id].nodes.size();
-    }
-    std::vector<float> pos, nrm, uv, tan, triVerts;
-    std::vector<GpuTriangle> tris;
-    pos.reserve(nv * 4);
-    nrm.reserve(nv * 4);
-    uv.reserve(nv * 2);
-    tan.reserve(nv * 4);
-    tris.reserve(nt);
-    triVerts.reserve(nt * 12);
-    blasNodes_.clear();
-    blasNodes_.reserve(nn + meshes.size() + 1);
+    std::vector<uint32_t> dirty;
+    dirty.reserve(jobs.size());
+    for (auto& j : jobs) dirty.push_back(j.first);
     maxBlasDepth_ = 0;
+    totalTris_ = totalVerts_ = 0;
     for (auto& [id, m] : meshes) {
-        Blas& b = blas_[id];
-        // Every BVH root at an odd index, behind an unused node where needed:
-        // child pairs (root + 1 + 2k) then start on 64-byte boundaries, so the
-        // two children a traversal step reads share one cache line.
-        if (blasNodes_.size() % 2 == 0) blasNodes_.push_back(sentinelNode());
-        const uint32_t vb = uint32_t(pos.size() / 4), tb = uint32_t(tris.size()), nb = uint32_t(blasNodes_.size());
-        b.rootGlobal = nb;
-        maxBlasDepth_ = std::max(maxBlasDepth_, b.depth);
-        pos.insert(pos.end(), m.positions.begin(), m.positions.end());
-        nrm.insert(nrm.end(), m.normals.begin(), m.normals.end());
-        if (m.uvs.empty()) uv.insert(uv.end(), size_t(m.vertexCount()) * 2, 0.0f);
-        else uv.insert(uv.end(), m.uvs.begin(), m.uvs.end());
-        if (m.tangents.empty()) {
-            for (uint32_t i = 0; i < m.vertexCount(); ++i) tan.insert(tan.end(), { 1.0f, 0.0f, 0.0f, 1.0f });
-        } else {
-            tan.insert(tan.end(), m.tangents.begin(), m.tangents.end());
-        }
-        for (uint32_t local : b.triOrder) {
-            GpuTriangle t{ m.indices[local * 3] + vb, m.indices[local * 3 + 1] + vb, m.indices[local * 3 + 2] + vb, 0u };
-            const uint32_t slot = uint32_t(tris.size());
-            tris.push_back(t);
-            for (int k = 0; k < 3; ++k) {
-                const float* p = &m.positions[size_t(m.indices[local * 3 + k]) * 4];
-                triVerts.insert(triVerts.end(), { p[0], p[1], p[2], 0.0f });
-            }
-            std::memcpy(&triVerts[size_t(slot) * 12 + 3], &slot, 4);
-        }
-        for (BvhNode n : b.nodes) {
-            n.leftOrPrim += (n.packed >> 2) == 0 ? nb : tb;
-            blasNodes_.push_back(n);
-        }
+        maxBlasDepth_ = std::max(maxBlasDepth_, blas_[id].depth);
+        totalTris_ += m.triCount();
+        totalVerts_ += m.vertexCount();
     }
-    if (blasNodes_.size() % 2 == 0) blasNodes_.push_back(sentinelNode());
-    tlasRoot_ = uint32_t(blasNodes_.size());
-    totalTris_ = tris.size();
     if (totalTris_ > kMaxTraversalIndex)
         throw Error(MRT_ERROR_OUT_OF_MEMORY, "scene holds more than 2^27 - 1 triangles");
-    totalVerts_ = pos.size() / 4;
-    uploadVec(dPos_, pos.data(), pos.size() * 4);
-    uploadVec(dNrm_, nrm.data(), nrm.size() * 4);
-    uploadVec(dUv_, uv.data(), uv.size() * 4);
-    uploadVec(dTan_, tan.data(), tan.size() * 4);
-    uploadVec(dTris_, tris.data(), tris.size() * sizeof(GpuTriangle));
-    uploadVec(dTriVerts_, triVerts.data(), triVerts.size() * 4);
-    invalidateGraph();
+    if (!layoutValid_ || !placeIncremental(dirty)) repackGeometry();
     return uint32_t(jobs.size());
 }
 
-// Reference Renderer.cpp:448-508.
+// One mesh's vertices, its triangles in BLAS leaf order and its BLAS nodes
+// rebased to the slot's ranges, appended to g.
+void Engine::appendMesh(const MeshData& m, Blas& b, const MeshSlot& sl, GeomArrays& g) {
+    b.rootGlobal = sl.nb;
+    g.pos.insert(g.pos.end(), m.positions.begin(), m.positions.end());
+    g.nrm.insert(g.nrm.end(), m.normals.begin(), m.normals.end());
+    if (m.uvs.empty()) g.uv.insert(g.uv.end(), size_t(m.vertexCount()) * 2, 0.0f);
+    else g.uv.insert(g.uv.end(), m.uvs.begin(), m.uvs.end());
+    if (m.tangents.empty()) {
+        for (uint32_t i = 0; i < m.vertexCount(); ++i) g.tan.insert(g.tan.end(), { 1.0f, 0.0f, 0.0f, 1.0f });
+    } else {
+        g.tan.insert(g.tan.end(), m.tangents.begin(), m.tangents.end());
+    }
+    uint32_t slot = sl.tb;
+    for (uint32_t local : b.triOrder) {
+        g.tris.push_back(GpuTriangle{ m.indices[local * 3] + sl.vb, m.indices[local * 3 + 1] + sl.vb,
+                                      m.indices[local * 3 + 2] + sl.vb, 0u });
+        const size_t at = g.triVerts.size();
+        for (int k = 0; k < 3; ++k) {
+            const float* p = &m.positions[size_t(m.indices[local * 3 + k]) * 4];
+            g.triVerts.insert(g.triVerts.end(), { p[0], p[1], p[2], 0.0f });
+        }
+        std::memcpy(&g.triVerts[at + 3], &slot, 4);
+        ++slot;
+    }
+    for (BvhNode n : b.nodes) {
+        n.leftOrPrim += (n.packed >> 2) == 0 ? sl.nb : sl.tb;
+        g.nodes.push_back(n);
+    }
+}
+
+// Every mesh contiguous in id order (the reference's layout): vertices,
+// triangles, then the BLAS nodes with every root at an odd index (an unused
+// node in front where needed: child pairs root + 1 + 2k then start on 64-byte
+// boundaries, so the two children a traversal step reads share one cache
+// line), then the TLAS region, reserved for half as many instances again as
+// the scene holds.  Buffers keep spare room (a quarter of the used size, and
+// at least the largest mesh plus an eighth) where later commits move edited
+// meshes that outgrow their ranges.
+void Engine::repackGeometry() {
+    auto& meshes = scene_.meshes();
+    GeomArrays g;
+    g.pos.reserve(totalVerts_ * 4);
+    g.nrm.reserve(totalVerts_ * 4);
+    g.uv.reserve(totalVerts_ * 2);
+    g.tan.reserve(totalVerts_ * 4);
+    g.tris.reserve(totalTris_);
+    g.triVerts.reserve(totalTris_ * 12);
+    slots_.clear();
+    holes_ = Holes{};
+    for (auto& [id, m] : meshes) {
+        Blas& b = blas_[id];
+        if (g.nodes.size() % 2 == 0) g.nodes.push_back(sentinelNode());
+        MeshSlot sl;
+        sl.vb = uint32_t(g.pos.size() / 4);
+        sl.vcap = m.vertexCount();
+        sl.tb = uint32_t(g.tris.size());
+        sl.tcap = m.triCount();
+        sl.nb = uint32_t(g.nodes.size());
+        sl.ncap = uint32_t(b.nodes.size());
+        appendMesh(m, b, sl, g);
+        slots_[id] = sl;
+    }
+    if (g.nodes.size() % 2 == 0) g.nodes.push_back(sentinelNode());
+    tlasRoot_ = uint32_t(g.nodes.size());
+    const size_t inst = scene_.instances().size();
+    tlasCap_ = uint32_t(std::max<size_t>(2 * (inst + inst / 2) + 1, 4));
+    vertEnd_ = uint32_t(g.pos.size() / 4);
+    triEnd_ = uint32_t(g.tris.size());
+    nodeEnd_ = tlasRoot_ + tlasCap_;
+    if (size_t(nodeEnd_) > kMaxTraversalIndex)
+        throw Error(MRT_ERROR_OUT_OF_MEMORY, "scene BVH holds more than 2^27 - 1 nodes");
+    size_t bigV = 0, bigT = 0, bigN = 0;
+    for (auto& [id, m] : meshes) {
+        bigV = std::max<size_t>(bigV, m.vertexCount());
+        bigT = std::max<size_t>(bigT, m.triCount());
+        bigN = std::max<size_t>(bigN, blas_[id].nodes.size());
+    }
+    auto roomOf = [](size_t n, size_t big) { return n + std::max(n / 4, big + big / 8 + 8) + 64; };
+    auto put = [&](DeviceBuffer& b, const void* data, size_t bytes, size_t capBytes) {
+        b.ensure(capBytes);
+        b.upload(data, bytes);
+        uploadBytes_ += bytes;
+    };
+    const size_t vRoom = roomOf(vertEnd_, bigV), tRoom = roomOf(triEnd_, bigT), nRoom = roomOf(nodeEnd_, bigN);
+    put(dPos_, g.pos.data(), g.pos.size() * 4, vRoom * 16);
+    put(dNrm_, g.nrm.data(), g.nrm.size() * 4, vRoom * 16);
+    put(dUv_, g.uv.data(), g.uv.size() * 4, vRoom * 8);
+    put(dTan_, g.tan.data(), g.tan.size() * 4, vRoom * 16);
+    put(dTris_, g.tris.data(), g.tris.size() * sizeof(GpuTriangle), tRoom * sizeof(GpuTriangle));
+    put(dTriVerts_, g.triVerts.data(), g.triVerts.size() * 4, tRoom * 48);
+    put(dNodes_, g.nodes.data(), g.nodes.size() * sizeof(BvhNode), nRoom * sizeof(BvhNode));
+    layoutValid_ = true;
+    ++geometryRepacks_;
+    invalidateGraph();
+}
+
+// The rebuilt meshes placed without moving the others: each of a mesh's
+// three ranges (vertices, triangles, nodes) is reused when its new data fits
+// it, else moved behind the used ranges (the old range becomes a hole, as do
+// removed meshes' ranges).  Only those meshes are uploaded.  Returns false,
+// changing nothing, when that needs more than the allocated capacity or
+// would leave holes over half the used space.
+bool Engine::placeIncremental(const std::vector<uint32_t>& dirty) {
+    auto& meshes = scene_.meshes();
+    std::map<uint32_t, MeshSlot> slots = slots_;
+    Holes holes = holes_;
+    for (auto it = slots.begin(); it != slots.end();) {
+        if (meshes.count(it->first)) {
+            ++it;
+            continue;
+        }
+        holes.add(it->second);
+        it = slots.erase(it);
+    }
+    uint64_t ve = vertEnd_, te = triEnd_, ne = nodeEnd_;
+    struct Place {
+        uint32_t id;
+        MeshSlot sl;
+        bool pad;  // an unused node in front of a moved root (at an even index)
+    };
+    std::vector<Place> place;
+    for (uint32_t id : dirty) {
+        const MeshData& m = meshes.at(id);
+        const uint32_t nv = m.vertexCount(), nt = m.triCount(), nn = uint32_t(blas_[id].nodes.size());
+        auto it = slots.find(id);
+        MeshSlot sl;
+        const bool had = it != slots.end();
+        if (had) sl = it->second;
+        // a moved range keeps 1/16 spare: a re-edited mesh whose BLAS has a
+        // few more nodes (or which gains a few triangles) then stays put
+        auto spare = [](uint32_t n) { return n + n / 16 + 4; };
+        if (!had || nv > sl.vcap) {
+            if (had) holes.verts += sl.vcap;
+            sl.vb = uint32_t(ve);
+            sl.vcap = spare(nv);
+            ve += sl.vcap;
+        }
+        if (!had || nt > sl.tcap) {
+            if (had) holes.tris += sl.tcap;
+            sl.tb = uint32_t(te);
+            sl.tcap = spare(nt);
+            te += sl.tcap;
+        }
+        bool pad = false;
+        if (!had || nn > sl.ncap) {
+            if (had) holes.nodes += sl.ncap;
+            sl.nb = uint32_t(ne | 1u);  // roots at odd indices
+            sl.ncap = spare(nn);
+            pad = sl.nb != ne;
+            ne = uint64_t(sl.nb) + sl.ncap;
+        }
+        slots[id] = sl;
+        place.push_back(Place{ id, sl, pad });
+    }
+    if (ne > kMaxTraversalIndex || te > kMaxTraversalIndex) return false;
+    if (ve * 16 > dPos_.size() || ve * 16 > dNrm_.size() || ve * 16 > dTan_.size() || ve * 8 > dUv_.size() ||
+        te * sizeof(GpuTriangle) > dTris_.size() || te * 48 > dTriVerts_.size() ||
+        ne * sizeof(BvhNode) > dNodes_.size())
+        return false;
+    if (2 * holes.verts > ve || 2 * holes.tris > te || 2 * holes.nodes > ne) return false;
+    for (const Place& pl : place) {
+        const MeshSlot& sl = pl.sl;
+        GeomArrays g;
+        appendMesh(meshes.at(pl.id), blas_[pl.id], sl, g);
+        dPos_.upload(g.pos.data(), g.pos.size() * 4, size_t(sl.vb) * 16);
+        dNrm_.upload(g.nrm.data(), g.nrm.size() * 4, size_t(sl.vb) * 16);
+        dUv_.upload(g.uv.data(), g.uv.size() * 4, size_t(sl.vb) * 8);
+        dTan_.upload(g.tan.data(), g.tan.size() * 4, size_t(sl.vb) * 16);
+        dTris_.upload(g.tris.data(), g.tris.size() * sizeof(GpuTriangle), size_t(sl.tb) * sizeof(GpuTriangle));
+        dTriVerts_.upload(g.triVerts.data(), g.triVerts.size() * 4, size_t(sl.tb) * 48);
+        if (pl.pad) {
+            const BvhNode pad = sentinelNode();
+            dNodes_.upload(&pad, sizeof(BvhNode), size_t(sl.nb - 1) * sizeof(BvhNode));
+        }
+        dNodes_.upload(g.nodes.data(), g.nodes.size() * sizeof(BvhNode), size_t(sl.nb) * sizeof(BvhNode));
+        uploadBytes_ += (g.pos.size() + g.nrm.size() + g.uv.size() + g.tan.size() + g.triVerts.size()) * 4 +
+                        g.tris.size() * sizeof(GpuTriangle) + g.nodes.size() * sizeof(BvhNode);
+        ++meshesWritten_;
+    }
+    slots_ = std::move(slots);
+    holes_ = holes;
+    vertEnd_ = uint32_t(ve);
+    triEnd_ = uint32_t(te);
+    nodeEnd_ = uint32_t(ne);
+    return true;
+}
+
+// Reference Renderer.cpp:448-508: the TLAS goes into the region reserved
+// behind the BLAS nodes; instances outgrowing it force a full repack, which
+// reserves room for the new count.
 void Engine::packInstances() {
     std::vector<const InstanceData*> flat;
     std::vector<Box3> bounds;
@@ -446,6 +591,7 @@ void Engine::packInstances() {
     }
     BvhBuild tlas = buildBvh(bounds, 1);
     if (tlas.nodes.empty()) tlas.nodes.push_back(sentinelNode());
+    if (!layoutValid_ || tlas.nodes.size() > tlasCap_) repackGeometry();
     std::vector<GpuInstance> gi;
     std::vector<TraceInstance> ti;
     gi.reserve(flat.size());
@@ -480,8 +626,8 @@ void Engine::packInstances() {
     }
     instanceCount_ = uint32_t(gi.size());
     tlasNodeCount_ = uint32_t(tlas.nodes.size());
-    if (size_t(tlasRoot_) + tlas.nodes.size() > kMaxTraversalIndex || gi.size() > kMaxTraversalIndex)
-        throw Error(MRT_ERROR_OUT_OF_MEMORY, "scene BVH holds more than 2^27 - 1 nodes or instances");
+    if (gi.size() > kMaxTraversalIndex)
+        throw Error(MRT_ERROR_OUT_OF_MEMORY, "scene holds more than 2^27 - 1 instances");
     // Guiding positions: the scene box scaled uniformly into [0, 1].
     if (tlas.rootBounds.valid()) {
         float extent = 0.0f;
@@ -496,14 +642,13 @@ void Engine::packInstances() {
         throw Error(MRT_ERROR_UNKNOWN, "scene BVH too deep for the traversal stack (TLAS depth " +
                                            std::to_string(tlasDepth_) + ", BLAS depth " +
                                            std::to_string(maxBlasDepth_) + ")");
-    std::vector<BvhNode> all;
-    all.reserve(blasNodes_.size() + tlas.nodes.size());
-    all.insert(all.end(), blasNodes_.begin(), blasNodes_.end());
-    all.insert(all.end(), tlas.nodes.begin(), tlas.nodes.end());
-    uploadVec(dNodes_, all.data(), all.size() * sizeof(BvhNode));
+    dNodes_.upload(tlas.nodes.data(), tlas.nodes.size() * sizeof(BvhNode), size_t(tlasRoot_) * sizeof(BvhNode));
+    const void* inst0 = dInst_.get();
+    const void* trace0 = dTraceInst_.get();
     uploadVec(dInst_, gi.data(), gi.size() * sizeof(GpuInstance));
     uploadVec(dTraceInst_, ti.data(), ti.size() * sizeof(TraceInstance));
-    invalidateGraph();
+    uploadBytes_ += tlas.nodes.size() * sizeof(BvhNode) + gi.size() * (sizeof(GpuInstance) + sizeof(TraceInstance));
+    if (dInst_.get() != inst0 || dTraceInst_.get() != trace0) invalidateGraph();
 }
 
 // Reference Renderer.cpp:510-550.
@@ -1251,22 +1396,24 @@ void Engine::exportScene(mrtxSceneExport& o) {
     commit(nullptr);
     DeviceScope scope(device_);
     MRT_HIP(hipStreamSynchronize(stream_));
-    o.blasNodeCount = uint32_t(blasNodes_.size());
+    // the arrays as laid out on the device: after edits they may hold
+    // ranges no mesh uses, and meshes behind the TLAS region
+    o.blasNodeCount = tlasRoot_;
     o.tlasRoot = tlasRoot_;
-    o.triCount = uint32_t(totalTris_);
-    o.vertexCount = uint32_t(totalVerts_);
+    o.triCount = triEnd_;
+    o.vertexCount = vertEnd_;
     o.instanceCount = instanceCount_;
     o.materialCount = uint32_t(scene_.materials().size() + 1);
     o.lightCount = lightCount_;
     o.envWidth = hasEnv_ ? envW_ : 0;
     o.envHeight = hasEnv_ ? envH_ : 0;
     o.envIntegral = envIntegral_;
-    o.nodeCount = uint32_t(blasNodes_.size()) + tlasNodeCount_;
+    o.nodeCount = nodeEnd_ > tlasRoot_ + tlasCap_ ? nodeEnd_ : tlasRoot_ + tlasNodeCount_;
     if (o.nodes) dNodes_.download(o.nodes, size_t(o.nodeCount) * 32);
-    if (o.triangles) dTris_.download(o.triangles, totalTris_ * 16);
-    if (o.positions) dPos_.download(o.positions, totalVerts_ * 16);
-    if (o.normals) dNrm_.download(o.normals, totalVerts_ * 16);
-    if (o.uvs) dUv_.download(o.uvs, totalVerts_ * 8);
+    if (o.triangles) dTris_.download(o.triangles, size_t(triEnd_) * 16);
+    if (o.positions) dPos_.download(o.positions, size_t(vertEnd_) * 16);
+    if (o.normals) dNrm_.download(o.normals, size_t(vertEnd_) * 16);
+    if (o.uvs) dUv_.download(o.uvs, size_t(vertEnd_) * 8);
     if (o.instances) dInst_.download(o.instances, size_t(instanceCount_) * sizeof(GpuInstance));
     if (o.materials) dMats_.download(o.materials, size_t(o.materialCount) * sizeof(GpuMaterial));
     if (o.lights) dLights_.download(o.lights, size_t(lightCount_) * sizeof(GpuLight));
@@ -1274,7 +1421,7 @@ void Engine::exportScene(mrtxSceneExport& o) {
     if (o.structSize < sizeof(mrtxSceneExport)) return;
     o.tex8Count = uint32_t(tex8Texels_);
     o.tex32Count = uint32_t(tex32Texels_);
-    if (o.tangents) dTan_.download(o.tangents, totalVerts_ * 16);
+    if (o.tangents) dTan_.download(o.tangents, size_t(vertEnd_) * 16);
     if (o.textureTable) dTexTable_.download(o.textureTable, size_t(kMaxTextureSlots) * sizeof(GpuTexture));
     if (o.tex8 && o.tex8Count) dTex8_.download(o.tex8, size_t(o.tex8Count) * 4);
     if (o.tex32 && o.tex32Count) dTex32_.download(o.tex32, size_t(o.tex32Count) * 16);

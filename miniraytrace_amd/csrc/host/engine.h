@@ -50,6 +50,10 @@ struct Settings {
 
 struct FrameStats {
     float frameMs = 0.0f;
+    float commitMs = 0.0f;        // last commit that changed something
+    uint64_t commitUploadBytes = 0;
+    uint32_t geometryRepacks = 0;  // commits that repacked every mesh (since creation)
+    uint32_t meshesWritten = 0;    // meshes rewritten alone, in place or appended (since creation)
     float stageMs[8] = {};  // generate, extend, shade, shadow, accumulate + resolve, tail, guide, train
     uint32_t rays[dev::kMaxBounceLevels] = {};
     uint32_t shadowRays[dev::kMaxBounceLevels] = {};
@@ -121,6 +125,27 @@ private:
         uint32_t depth = 0;
         uint32_t rootGlobal = 0;
     };
+    // A mesh's ranges in the vertex, triangle and node arrays: first index
+    // and capacity (its root is node nb).
+    struct MeshSlot {
+        uint32_t vb = 0, vcap = 0, tb = 0, tcap = 0, nb = 0, ncap = 0;
+    };
+    struct Holes {  // entries of the three arrays no mesh uses
+        uint64_t verts = 0, tris = 0, nodes = 0;
+        void add(const MeshSlot& s) {
+            verts += s.vcap;
+            tris += s.tcap;
+            nodes += s.ncap;
+        }
+    };
+    struct GeomArrays {
+        std::vector<float> pos, nrm, uv, tan, triVerts;
+        std::vector<GpuTriangle> tris;
+        std::vector<BvhNode> nodes;
+    };
+    void appendMesh(const MeshData& m, Blas& b, const MeshSlot& sl, GeomArrays& g);
+    void repackGeometry();
+    bool placeIncremental(const std::vector<uint32_t>& dirty);
 
     void sync(mrtCommitStats* out);
     uint32_t packGeometry();
@@ -155,7 +180,14 @@ private:
     // packing state
     std::map<uint32_t, Blas> blas_;
     std::map<uint32_t, uint32_t> materialIndex_;
-    std::vector<BvhNode> blasNodes_;      // concatenated BLAS nodes (global indices)
+    std::map<uint32_t, MeshSlot> slots_;
+    Holes holes_;
+    bool layoutValid_ = false;             // geometry arrays packed (slots_ valid)
+    uint32_t vertEnd_ = 0, triEnd_ = 0;    // first entries behind every slot
+    uint32_t nodeEnd_ = 0;                 // first node behind every slot and the TLAS region
+    uint32_t tlasCap_ = 0;                 // nodes reserved for the TLAS at tlasRoot_
+    uint32_t geometryRepacks_ = 0, meshesWritten_ = 0;  // since creation
+    uint64_t uploadBytes_ = 0;             // by the last commit
     uint32_t tlasRoot_ = 0;
     uint32_t instanceCount_ = 0;
     uint32_t tlasNodeCount_ = 0;

@@ -264,7 +264,9 @@ class Engine:
         L.check(L.lib().mrtxGetFrameStats(self._h, C.byref(s)), "mrtxGetFrameStats")
         names = ["generate", "extend", "shade", "shadow", "accumulate", "tail", "guide", "train"]
         return {"frameMs": s.frameMs, "gpuMs": dict(zip(names, list(s.gpuMs))),
-                "rays": list(s.rays), "shadowRays": list(s.shadowRays), "status": s.status}
+                "rays": list(s.rays), "shadowRays": list(s.shadowRays), "status": s.status,
+                "commitMs": s.commitMs, "commitUploadBytes": s.commitUploadBytes,
+                "geometryRepacks": s.geometryRepacks, "meshesWritten": s.meshesWritten}
 
     def export_scene(self) -> dict:
         """Packed device scene as numpy arrays (two calls: counts, then data)."""

@@ -100,17 +100,18 @@ def load_terrain_scene(engine, cfg: dict = CONFIG2, seed: int = 1, sky_img: np.n
                for m in mats]
     tm = engine.add_mesh(tpos, tidx)
     sm = engine.add_mesh(spos, sidx, normals=spos)
-    engine.add_instance(tm, None, mat_ids[0])
+    inst = [engine.add_instance(tm, None, mat_ids[0])]
     xf, which = scatter_on_terrain(cfg["spheres"], cfg["sx"], cfg["sz"], cfg["amplitude"], seed,
                                    cfg["rmin"], cfg["rmax"], 3, seed + 200)
     for i in range(cfg["spheres"]):
-        engine.add_instance(sm, xf[i], mat_ids[int(which[i])])
+        inst.append(engine.add_instance(sm, xf[i], mat_ids[int(which[i])]))
     if sky_img is None:
         sky_img = sky(*cfg["sky"])
     engine.set_environment(sky_img, 0.0, 1.0)
     engine.set_camera(**cfg["camera"])
     return {"triangles": tidx.shape[0] + cfg["spheres"] * sidx.shape[0], "instances": cfg["spheres"] + 1,
-            "materials": mats, "camera": dict(cfg["camera"])}
+            "materials": mats, "camera": dict(cfg["camera"]), "terrain_mesh": tm, "sphere_mesh": sm,
+            "instance_ids": inst, "sphere_xforms": xf, "material_ids": mat_ids}
 
 
 def terrain_config(triangles: int) -> dict:
