@@ -171,7 +171,8 @@ def traversal_workload(args, torch) -> dict:
         "ms_per_step": wall * 1e3 / args.steps,
         "kernel_ms": kernel_ms,
         "value_mrays_s": n_rays / (kernel_ms * 1e-3) / 1e6,
-        "shape": {"block": blk, "lds_stack": stk, "loop": ["step", "while-while"][loop]},
+        "shape": {"block": blk, "lds_stack": stk,
+                  "loop": ["step", "while-while", "four-wide step", "four-wide while-while"][loop]},
         "bvh": {k: info[k] for k in ("nodeCount", "maxDepth", "leafCount", "buildMs")},
     }
 
@@ -183,10 +184,11 @@ def traversal_workload(args, torch) -> dict:
         href, _, (nn, nt) = orc.glsl_traverse(nodes, order, verts, rays, threads, want_hits=True)
         bytes_alg = 32 * n_rays + 8 * n_rays + 32 * nn + 64 * nt
         achieved = bytes_alg / (kernel_ms * 1e-3) / 1e9
+        kname = f"{'kTraceClosestWide' if loop >= 2 else 'kTraceClosest'}<{blk}, {stk}, {loop}>"
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                           "traffic": _measured_traffic(f"kTraceClosest<{blk}, {stk}, {loop}>", "bytes_per_launch"),
-                           "kernel": f"kTraceClosest<{blk}, {stk}, {loop}>",
+                           "traffic": _measured_traffic(kname, "bytes_per_launch"),
+                           "kernel": kname,
                            "bytes_per_launch": bytes_alg,
                            "nodes_per_ray": nn / n_rays, "tris_per_ray": nt / n_rays}
         # parity spot check against the oracle's exact traversal

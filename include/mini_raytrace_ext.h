@@ -90,7 +90,9 @@ typedef struct mrtxTraversalConfig {
     size_t   structSize;
     uint32_t blockSize;     /* threads per workgroup: 64, 128, 256 */
     uint32_t stackDepth;    /* LDS stack entries per lane: 8, 16, 32 */
-    uint32_t loopMode;      /* 0 = one node or leaf step per iteration, 1 = while-while */
+    uint32_t loopMode;      /* 0 = one node or leaf step per iteration, 1 = while-while;
+                               2, 3 = the same over a four-wide tree collapsed from the
+                               BVH (same hits) */
 } mrtxTraversalConfig;
 
 /* Builds a binned-SAH BVH (reference core/src/accel/Bvh.cpp:137-156

@@ -3,6 +3,7 @@
 // maxLeaf 4) over triangle bounds accumulated as in Renderer.cpp:348-356.
 #include "bvh_build.h"
 #include "bvh_gpu_build.h"
+#include "bvh_wide.h"
 #include "common.h"
 #include "mini_raytrace_ext.h"
 #include "../kernels/launch.h"
@@ -19,15 +20,16 @@ struct mrtxQueryScene {
     int device = 0;
     std::vector<mrt::BvhNode> nodes;
     std::vector<uint32_t> primOrder;
-    mrt::DeviceBuffer dNodes, dTris, dStatus;
+    mrt::DeviceBuffer dNodes, dWide, dTris, dStatus;
     uint32_t triCount = 0;
     uint32_t leafCount = 0;
     uint32_t maxDepth = 0;
     float buildMs = 0.0f;
-    // Default launch shape: the fastest measured on a 10k-triangle soup.
+    // Default launch shape: the fastest measured on a 10k-triangle soup
+    // (the four-wide step walk).
     uint32_t block = 128;
     uint32_t stack = 8;
-    uint32_t loop = 0;
+    uint32_t loop = 2;
 };
 
 namespace {
@@ -108,6 +110,11 @@ mrtxQueryScene* createScene(const float* positions, uint32_t vertexCount, const 
         std::memcpy(dev.data() + 1, sc->nodes.data(), sc->nodes.size() * sizeof(mrt::BvhNode));
         sc->dNodes.allocate(dev.size() * sizeof(mrt::BvhNode));
         sc->dNodes.upload(dev.data(), dev.size() * sizeof(mrt::BvhNode));
+        const std::vector<mrt::WideNode> wide = mrt::collapseWide(sc->nodes);
+        if (wide.size() > kMaxTraversalIndex)
+            throw Error(MRT_ERROR_INVALID_ARGUMENT, "too many triangles for one query scene");
+        sc->dWide.allocate(wide.size() * sizeof(mrt::WideNode));
+        sc->dWide.upload(wide.data(), wide.size() * sizeof(mrt::WideNode));
         sc->dTris.allocate(tris.size() * sizeof(float));
         sc->dTris.upload(tris.data(), tris.size() * sizeof(float));
         sc->dStatus.allocate(sizeof(uint32_t));
@@ -124,6 +131,7 @@ mrt::dev::QueryLaunch makeLaunch(mrtxQueryScene* s, const mrtxRay* rays, uint32_
     mrt::dev::QueryLaunch q;
     // skip the alignment node
     q.nodes = reinterpret_cast<const mrt::dev::Node*>(s->dNodes.as<const char>() + sizeof(mrt::BvhNode));
+    q.wide = s->dWide.as<const float4>();
     q.tris = s->dTris.as<const float4>();
     q.rays = reinterpret_cast<const float4*>(rays);
     q.n = n;
@@ -224,7 +232,7 @@ MRT_API mrtResult mrtxGetQuerySceneInfo(const mrtxQueryScene* s, mrtxQuerySceneI
     out->leafCount = s->leafCount;
     out->maxDepth = s->maxDepth;
     out->buildMs = s->buildMs;
-    out->deviceBytes = s->dNodes.size() + s->dTris.size() + s->dStatus.size();
+    out->deviceBytes = s->dNodes.size() + s->dWide.size() + s->dTris.size() + s->dStatus.size();
     return MRT_SUCCESS;
 }
 
@@ -317,7 +325,7 @@ MRT_API mrtResult mrtxAutotuneTraversal(mrtxQueryScene* s, const mrtxRay* dRays,
         uint32_t bestBlock = s->block, bestStack = s->stack, bestLoop = s->loop;
         const uint32_t blocks[] = { 64, 128, 256 };
         const uint32_t stacks[] = { 8, 16, 32 };
-        for (uint32_t loop = 0; loop < 2; ++loop)
+        for (uint32_t loop = 0; loop < 4; ++loop)
         for (uint32_t blk : blocks)
             for (uint32_t stk : stacks) {
                 const auto q = makeLaunch(s, dRays, n, dHits);

@@ -8,10 +8,20 @@
 // otherwise continues without a node fetch.  Closest-hit ties in t resolve
 // to the lowest triangle id.
 //
-// Two loop structures (chosen per scene by the autotuner):
-//   LOOP 0  one step per iteration: an inner-node step or a leaf step;
+// Loop structures (chosen per scene by the autotuner):
+//   LOOP 0  one step per iteration: an inner-node step or a leaf step (a
+//           whole leaf: one triangle per step measured 30 % slower here);
 //   LOOP 1  "while-while" (Aila & Laine 2009): descend until the lane holds
-//           a leaf, so the lanes of a wave test their leaf triangles together.
+//           a leaf, so the lanes of a wave test their leaf triangles together;
+//   LOOP 2  as 0 over the four-wide tree collapsed from the binary one
+//           (trace_core.h, host/bvh_wide.h): one step tests four child boxes
+//           of one 128-byte node, loading each axis's near and far planes of
+//           all four from the offsets the ray's direction signs pick, visits
+//           the nearest hit child and pushes the other hit children farthest
+//           first;
+//   LOOP 3  as 1 over the four-wide tree.
+// All return the same hits: box tests are conservative and ties in t go to
+// the lowest triangle id, so no result depends on the tree or visiting order.
 #include "launch.h"
 #include "trace_core.h"
 
@@ -78,6 +88,127 @@ __device__ __forceinline__ bool innerStep(const Node* __restrict__ nodes, const 
         return true;
     }
     return false;
+}
+
+// Wide inner step: tests the four children of wide node cur.index, visits
+// the nearest hit child and pushes the other hit children farthest first;
+// false when none is hit.
+template <int S, int BLOCK>
+__device__ __forceinline__ bool wideInnerStep(const float4* __restrict__ wide, const Node* __restrict__ nodes,
+                                              const RayK& r, const WideRows& rows, float lim,
+                                              ShortStack<S, BLOCK>& stack, Cursor& cur) {
+    WideStep ws = wideTest(wide, cur.index, r, rows, lim);
+    const int hits = wideSort(ws);
+    if (hits == 0) return false;
+    if (hits > 3) stack.push(make_uint2(ws.entry[3], ws.key[3]));
+    if (hits > 2) stack.push(make_uint2(ws.entry[2], ws.key[2]));
+    if (hits > 1) stack.push(make_uint2(ws.entry[1], ws.key[1]));
+    cur = wideEntryCursor(nodes, make_uint2(ws.entry[0], ws.key[0]));
+    return true;
+}
+
+template <int S, int BLOCK>
+__device__ __forceinline__ bool widePopNext(ShortStack<S, BLOCK>& stack, const Node* __restrict__ nodes,
+                                            float limit, Cursor& cur) {
+    while (!stack.empty()) {
+        const uint2 e = stack.pop();
+        if (__uint_as_float(e.y) <= limit) {
+            cur = wideEntryCursor(nodes, e);
+            return true;
+        }
+    }
+    return false;
+}
+
+template <int BLOCK, int S, int LOOP>
+__global__ __launch_bounds__(BLOCK) void kTraceClosestWide(const float4* __restrict__ wide,
+                                                           const Node* __restrict__ nodes,
+                                                           const float4* __restrict__ tris,
+                                                           const float4* __restrict__ rays, uint32_t n,
+                                                           uint2* __restrict__ hits, uint32_t* __restrict__ status) {
+    __shared__ uint2 ldsStack[S * BLOCK];
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const float4 r0 = rays[2 * size_t(i)];
+    const float4 r1 = rays[2 * size_t(i) + 1];
+    const RayK r = setupRay(r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r0.w);
+    const WideRows rows = wideRows(r);
+    ClosestState st{ r1.w, kNoHit, false };
+    uint2 spill[kSpill];
+    ShortStack<S, BLOCK> stack(ldsStack + threadIdx.x, spill);
+    Cursor cur{ 0u, 0u };  // the root wide node
+    bool have = true;
+    if (LOOP == 2) {
+        while (have) {
+            if (cur.count == 0) {
+                if (wideInnerStep(wide, nodes, r, rows, st.best * kLimitWiden, stack, cur)) continue;
+            } else {
+                leafClosest(tris, r, cur.index, cur.count, st);
+            }
+            have = widePopNext(stack, nodes, st.best * kLimitWiden, cur);
+        }
+    } else {
+        while (have) {
+            while (have && cur.count == 0) {
+                if (!wideInnerStep(wide, nodes, r, rows, st.best * kLimitWiden, stack, cur))
+                    have = widePopNext(stack, nodes, st.best * kLimitWiden, cur);
+            }
+            if (!have) break;
+            leafClosest(tris, r, cur.index, cur.count, st);
+            have = widePopNext(stack, nodes, st.best * kLimitWiden, cur);
+        }
+    }
+    hits[i] = make_uint2(st.bestId, __float_as_uint(st.best));
+    if (stack.overflow) atomicOr(status, 1u);
+}
+
+template <int BLOCK, int S, int LOOP>
+__global__ __launch_bounds__(BLOCK) void kTraceAnyWide(const float4* __restrict__ wide,
+                                                       const Node* __restrict__ nodes,
+                                                       const float4* __restrict__ tris,
+                                                       const float4* __restrict__ rays, uint32_t n,
+                                                       uint32_t* __restrict__ occluded,
+                                                       uint32_t* __restrict__ status) {
+    __shared__ uint2 ldsStack[S * BLOCK];
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const float4 r0 = rays[2 * size_t(i)];
+    const float4 r1 = rays[2 * size_t(i) + 1];
+    const RayK r = setupRay(r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r0.w);
+    const WideRows rows = wideRows(r);
+    const float tmax = r1.w;
+    const float lim = tmax * kLimitWiden;
+    uint2 spill[kSpill];
+    ShortStack<S, BLOCK> stack(ldsStack + threadIdx.x, spill);
+    uint32_t occ = 0;
+    Cursor cur{ 0u, 0u };
+    bool have = true;
+    while (have) {
+        if (LOOP == 3) {
+            while (have && cur.count == 0) {
+                if (!wideInnerStep(wide, nodes, r, rows, lim, stack, cur)) have = widePopNext(stack, nodes, lim, cur);
+            }
+            if (!have) break;
+        }
+        if (cur.count == 0) {
+            if (wideInnerStep(wide, nodes, r, rows, lim, stack, cur)) continue;
+        } else {
+            const uint32_t lp = cur.index;
+            for (uint32_t k = lp; k < lp + cur.count; ++k) {
+                float t, b1, b2;
+                if (isectTri(r, tris[3 * size_t(k)], tris[3 * size_t(k) + 1], tris[3 * size_t(k) + 2], tmax, t,
+                             b1, b2) &&
+                    t < tmax) {
+                    occ = 1;
+                    break;
+                }
+            }
+            if (occ) break;
+        }
+        have = widePopNext(stack, nodes, lim, cur);
+    }
+    occluded[i] = occ;
+    if (stack.overflow) atomicOr(status, 1u);
 }
 
 template <int BLOCK, int S, int LOOP>
@@ -179,22 +310,33 @@ __global__ __launch_bounds__(BLOCK) void kTraceAny(const Node* __restrict__ node
 template <int BLOCK, int S, int LOOP>
 static hipError_t launchClosestT(const QueryLaunch& q, hipStream_t s) {
     const uint32_t grid = (q.n + BLOCK - 1) / BLOCK;
-    hipLaunchKernelGGL((kTraceClosest<BLOCK, S, LOOP>), dim3(grid), dim3(BLOCK), 0, s, q.nodes, q.tris,
-                       q.rays, q.n, reinterpret_cast<uint2*>(q.out), q.status);
+    if constexpr (LOOP >= 2)
+        hipLaunchKernelGGL((kTraceClosestWide<BLOCK, S, LOOP>), dim3(grid), dim3(BLOCK), 0, s, q.wide, q.nodes, q.tris,
+                           q.rays, q.n, reinterpret_cast<uint2*>(q.out), q.status);
+    else
+        hipLaunchKernelGGL((kTraceClosest<BLOCK, S, LOOP>), dim3(grid), dim3(BLOCK), 0, s, q.nodes, q.tris,
+                           q.rays, q.n, reinterpret_cast<uint2*>(q.out), q.status);
     return hipGetLastError();
 }
 template <int BLOCK, int S, int LOOP>
 static hipError_t launchAnyT(const QueryLaunch& q, hipStream_t s) {
     const uint32_t grid = (q.n + BLOCK - 1) / BLOCK;
-    hipLaunchKernelGGL((kTraceAny<BLOCK, S, LOOP>), dim3(grid), dim3(BLOCK), 0, s, q.nodes, q.tris,
-                       q.rays, q.n, reinterpret_cast<uint32_t*>(q.out), q.status);
+    if constexpr (LOOP >= 2)
+        hipLaunchKernelGGL((kTraceAnyWide<BLOCK, S, LOOP>), dim3(grid), dim3(BLOCK), 0, s, q.wide, q.nodes, q.tris,
+                           q.rays, q.n, reinterpret_cast<uint32_t*>(q.out), q.status);
+    else
+        hipLaunchKernelGGL((kTraceAny<BLOCK, S, LOOP>), dim3(grid), dim3(BLOCK), 0, s, q.nodes, q.tris,
+                           q.rays, q.n, reinterpret_cast<uint32_t*>(q.out), q.status);
     return hipGetLastError();
 }
 
 #define MRT_QUERY_SHAPES(X)                                                                          \
     X(64, 8, 0) X(64, 16, 0) X(64, 32, 0) X(128, 8, 0) X(128, 16, 0) X(128, 32, 0) X(256, 8, 0)       \
     X(256, 16, 0) X(256, 32, 0) X(64, 8, 1) X(64, 16, 1) X(64, 32, 1) X(128, 8, 1) X(128, 16, 1)     \
-    X(128, 32, 1) X(256, 8, 1) X(256, 16, 1) X(256, 32, 1)
+    X(128, 32, 1) X(256, 8, 1) X(256, 16, 1) X(256, 32, 1)                                            \
+    X(64, 8, 2) X(64, 16, 2) X(64, 32, 2) X(128, 8, 2) X(128, 16, 2) X(128, 32, 2) X(256, 8, 2)       \
+    X(256, 16, 2) X(256, 32, 2) X(64, 8, 3) X(64, 16, 3) X(64, 32, 3) X(128, 8, 3) X(128, 16, 3)     \
+    X(128, 32, 3) X(256, 8, 3) X(256, 16, 3) X(256, 32, 3)
 
 hipError_t launchTraceClosest(const QueryLaunch& q, uint32_t block, uint32_t stack, uint32_t loop,
                               hipStream_t s) {

@@ -152,16 +152,142 @@ __device__ __forceinline__ RayK setupRay(float ox, float oy, float oz, float dx,
 // on a box face) lands in the max over near distances or the min over far
 // distances, where IEEE maxNum/minNum drop it: a vacuous constraint, which is
 // the right answer for a ray inside that slab.
-__device__ __forceinline__ bool slab(const Node& n, const RayK& r, float limit, float& tEnter) {
+__device__ __forceinline__ bool slabBox(float lx, float ly, float lz, float hx, float hy, float hz, const RayK& r,
+                                        float limit, float& tEnter) {
 #pragma clang fp contract(off)
     const bool px = r.ix >= 0.0f, py = r.iy >= 0.0f, pz = r.iz >= 0.0f;
-    const float nx = ((px ? n.a.x : n.b.x) - r.ox) * r.ix, fx = ((px ? n.b.x : n.a.x) - r.ox) * r.ix;
-    const float ny = ((py ? n.a.y : n.b.y) - r.oy) * r.iy, fy = ((py ? n.b.y : n.a.y) - r.oy) * r.iy;
-    const float nz = ((pz ? n.a.z : n.b.z) - r.oz) * r.iz, fz = ((pz ? n.b.z : n.a.z) - r.oz) * r.iz;
+    const float nx = ((px ? lx : hx) - r.ox) * r.ix, fx = ((px ? hx : lx) - r.ox) * r.ix;
+    const float ny = ((py ? ly : hy) - r.oy) * r.iy, fy = ((py ? hy : ly) - r.oy) * r.iy;
+    const float nz = ((pz ? lz : hz) - r.oz) * r.iz, fz = ((pz ? hz : lz) - r.oz) * r.iz;
     const float enter = fmaxf(fmaxf(nx, ny), fmaxf(nz, 0.0f));
     const float far = fminf(fminf(fx, fy), fz) * kFarWiden;
     tEnter = enter;
     return enter <= fminf(far, limit);
+}
+__device__ __forceinline__ bool slab(const Node& n, const RayK& r, float limit, float& tEnter) {
+    return slabBox(n.a.x, n.a.y, n.a.z, n.b.x, n.b.y, n.b.z, r, limit, tEnter);
+}
+
+// ------------------------------------------------------------ wide nodes --
+// The four-wide traversal tree collapsed from the binary BVH on the host
+// (host/bvh_wide.h): 128 bytes per node, eight float4 rows = the four
+// children's lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, first node or primitive,
+// code.  Codes: 0 inner (child = wide node), 1..29 a leaf of that many
+// primitives (child = first primitive), kWideCodeBigLeaf a larger leaf
+// (child = its binary node), kWideCodeEmpty an unused slot.  Stack entries
+// use the encoding of nodeEntry: inner children with index = wide node, big
+// leaves as kEntryRefetch with index = binary node.
+//
+// Per ray, the row offsets of each axis's near and far planes follow the
+// direction's sign (the choice slab() makes per box), so a step loads the
+// four children's near and far planes directly, without a select per plane.
+//
+// A plane's distance is one fused multiply-add, fma(plane, inv, c), with c
+// per ray and axis: c = -o inv - eps for near planes and -o inv + eps for far
+// planes, eps = 2^-22 |o inv|.  The one error this adds to slab()'s
+// (plane - o) * inv, the absolute rounding of o * inv, is at most half an ulp
+// of it, below eps (two ulps) even after c's own rounding, so near distances
+// only come out smaller and far distances larger than with exact o * inv; the
+// single rounding of the fma and the reciprocal's ulp are covered by the far
+// widening as in slab().  The test stays conservative, which is all the
+// exactness of the hits needs.  (Rays parallel to an axis can turn a plane
+// distance into NaN, which maxNum / minNum drop: a vacuous constraint, never
+// a cull.)
+constexpr uint32_t kWideCodeBigLeaf = 0xFEu;
+constexpr uint32_t kWideCodeEmpty = 0xFFu;
+
+struct WideRows {
+    uint32_t nx, ny, nz, fx, fy, fz;  // rows of the near / far planes
+    float cnx, cny, cnz, cfx, cfy, cfz;  // fma offsets of near / far planes
+};
+__device__ __forceinline__ WideRows wideRows(const RayK& r) {
+    WideRows w;
+    w.nx = r.ix >= 0.0f ? 0u : 3u;
+    w.ny = r.iy >= 0.0f ? 1u : 4u;
+    w.nz = r.iz >= 0.0f ? 2u : 5u;
+    w.fx = 3u - w.nx;
+    w.fy = 5u - w.ny;
+    w.fz = 7u - w.nz;
+    const float ax = -r.ox * r.ix, ay = -r.oy * r.iy, az = -r.oz * r.iz;
+    const float ex = fabsf(ax) * (1.0f / 4194304.0f), ey = fabsf(ay) * (1.0f / 4194304.0f),
+                ez = fabsf(az) * (1.0f / 4194304.0f);
+    w.cnx = ax - ex;
+    w.cny = ay - ey;
+    w.cnz = az - ez;
+    w.cfx = ax + ex;
+    w.cfy = ay + ey;
+    w.cfz = az + ez;
+    return w;
+}
+
+struct WideStep {
+    uint32_t key[4];    // entry distance bits of a hit child (t >= 0, -0 as +0), 0xFFFFFFFF for a miss
+    uint32_t entry[4];  // stack entry codes (x) of the children, by slot
+    uint32_t hitMask;
+};
+
+__device__ __forceinline__ float f4At(const float4& v, int k) {
+    return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
+}
+__device__ __forceinline__ uint32_t u4At(const float4& v, int k) { return __float_as_uint(f4At(v, k)); }
+
+// Box tests of the four children of wide node w.
+__device__ __forceinline__ WideStep wideTest(const float4* __restrict__ wide, uint32_t w, const RayK& r,
+                                             const WideRows& rows, float limit) {
+#pragma clang fp contract(off)
+    const float4* p = wide + size_t(w) * 8;
+    const float4 nx = ldGlobal4(p + rows.nx), ny = ldGlobal4(p + rows.ny), nz = ldGlobal4(p + rows.nz);
+    const float4 fx = ldGlobal4(p + rows.fx), fy = ldGlobal4(p + rows.fy), fz = ldGlobal4(p + rows.fz);
+    const float4 ch = ldGlobal4(p + 6), cd = ldGlobal4(p + 7);
+    WideStep s;
+    s.hitMask = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t code = u4At(cd, k);
+        const float tnx = fmaf(f4At(nx, k), r.ix, rows.cnx), tfx = fmaf(f4At(fx, k), r.ix, rows.cfx);
+        const float tny = fmaf(f4At(ny, k), r.iy, rows.cny), tfy = fmaf(f4At(fy, k), r.iy, rows.cfy);
+        const float tnz = fmaf(f4At(nz, k), r.iz, rows.cnz), tfz = fmaf(f4At(fz, k), r.iz, rows.cfz);
+        const float enter = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));  // >= 0 or -0, never NaN
+        const float far = fminf(fminf(tfx, tfy), tfz) * kFarWiden;
+        // bitwise & on the conditions, and selects, keep the step free of
+        // branches (the compiler turns && and nested ?: into exec-mask
+        // branches per child)
+        const bool hit = (code != kWideCodeEmpty) & (enter <= fminf(far, limit));
+        s.key[k] = hit ? (__float_as_uint(enter) & 0x7FFFFFFFu) : 0xFFFFFFFFu;
+        s.hitMask |= uint32_t(hit) << k;
+        const uint32_t child = u4At(ch, k);
+        s.entry[k] = code == kWideCodeBigLeaf ? (kEntryRefetch << kEntryIndexBits) | child
+                                              : (code << kEntryIndexBits) | child;
+    }
+    return s;
+}
+
+// Cursor of a stack entry of the wide walk (big leaves from the binary tree).
+__device__ __forceinline__ Cursor wideEntryCursor(const Node* __restrict__ nodes, uint2 e) {
+    const uint32_t code = entryCode(e), idx = e.x & kEntryIndexMask;
+    if (code == kEntryRefetch) return cursorOf(loadNode(nodes, idx));
+    return Cursor{ idx, code };
+}
+
+// Sorts the children nearest first, misses last (a sorting network over
+// (key, entry) pairs; ties keep slot order), so that key[0..hits-1] and
+// entry[0..hits-1] are the hit children in visiting order.  Returns the
+// number of hits.
+__device__ __forceinline__ int wideSort(WideStep& s) {
+    auto cx = [&](int a, int b) {
+        const bool sw = s.key[b] < s.key[a];
+        const uint32_t ka = s.key[a], kb = s.key[b], ea = s.entry[a], eb = s.entry[b];
+        s.key[a] = sw ? kb : ka;
+        s.key[b] = sw ? ka : kb;
+        s.entry[a] = sw ? eb : ea;
+        s.entry[b] = sw ? ea : eb;
+    };
+    cx(0, 1);
+    cx(2, 3);
+    cx(0, 2);
+    cx(1, 3);
+    cx(1, 2);
+    return __builtin_popcount(s.hitMask);
 }
 
 // Watertight ray/triangle test.  Hit iff tmin < t <= tLimit; writes t and the

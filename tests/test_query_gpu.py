@@ -87,7 +87,8 @@ def test_closest_vs_reference_traversal(c1, oracle):
 
 
 @pytest.mark.parametrize("block,stack,loop", [(64, 8, 0), (64, 32, 0), (128, 16, 0), (256, 8, 0), (256, 32, 0),
-                                              (64, 8, 1), (128, 16, 1), (256, 32, 1)])
+                                              (64, 8, 1), (128, 16, 1), (256, 32, 1), (64, 8, 2), (128, 8, 2),
+                                              (256, 32, 2), (128, 16, 3), (256, 8, 3)])
 def test_all_launch_shapes_identical(c1, oracle, block, stack, loop):
     _print(f"case: c1 shape block={block} stack={stack} loop={loop}")
     base = _trace(c1)
@@ -99,30 +100,39 @@ def test_all_launch_shapes_identical(c1, oracle, block, stack, loop):
     assert np.array_equal(base, other)
 
 
-def test_any_hit_vs_brute(c1, oracle):
-    _print("case: c1 any-hit with finite tmax")
+@pytest.mark.parametrize("loop", [0, 1, 2, 3])
+def test_any_hit_vs_brute(c1, oracle, loop):
+    _print(f"case: c1 any-hit with finite tmax, loop {loop}")
     torch = c1["torch"]
     rays = c1["rays"][:50_000].copy()
     rng = np.random.default_rng(4)
     rays[:, 7] = rng.uniform(0.5, 3.0, size=rays.shape[0]).astype(np.float32)
     d = torch.from_numpy(rays).cuda()
     occ = torch.empty(rays.shape[0], dtype=torch.int32, device="cuda")
-    c1["qs"].trace_any(d, occ)
-    torch.cuda.synchronize()
-    got = occ.cpu().numpy()
-    exp = oracle.brute_any(c1["verts"], rays, cpu_threads())
-    assert np.array_equal(got.astype(np.uint8), exp)
-    # closest with the same tmax agrees with occlusion
-    hits = _trace(c1, rays=rays)
-    assert np.array_equal(hits[:, 0] != 0xFFFFFFFF, exp.astype(bool))
+    c1["qs"].set_config(128, 16, loop)
+    try:
+        c1["qs"].trace_any(d, occ)
+        torch.cuda.synchronize()
+        got = occ.cpu().numpy()
+        exp = oracle.brute_any(c1["verts"], rays, cpu_threads())
+        assert np.array_equal(got.astype(np.uint8), exp)
+        # closest with the same tmax agrees with occlusion
+        hits = _trace(c1, rays=rays)
+        assert np.array_equal(hits[:, 0] != 0xFFFFFFFF, exp.astype(bool))
+    finally:
+        c1["qs"].set_config(128, 16, 0)
 
 
-def test_edge_cases(c1, oracle):
-    _print("case: edge cases (empty, single, coincident ties, axis rays)")
+@pytest.mark.parametrize("loop", [0, 2])
+def test_edge_cases(c1, oracle, loop):
+    """Also a 100-primitive leaf, which the four-wide walk pushes as a
+    refetch entry."""
+    _print(f"case: edge cases (empty, single, coincident ties, axis rays), loop {loop}")
     from miniraytrace_amd.query import QueryScene
     torch = c1["torch"]
     # empty scene
     qs = QueryScene(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32))
+    qs.set_config(128, 8, loop)
     rays = c1["rays"][:1000]
     h = _trace(c1, qs=qs, rays=rays)
     assert np.all(h[:, 0] == 0xFFFFFFFF)
@@ -130,6 +140,7 @@ def test_edge_cases(c1, oracle):
     # one triangle hit head-on along an axis (direction components exactly 0)
     tri = np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0]], np.float32)
     qs = QueryScene.from_soup(tri)
+    qs.set_config(128, 8, loop)
     r = np.array([[0.25, 0.25, 5, 1e-4, 0, 0, -1, np.inf],
                   [0.8, 0.8, 5, 1e-4, 0, 0, -1, np.inf],
                   [0.25, 0.25, 5, 1e-4, 0, 0, -1, 4.0],
@@ -143,6 +154,7 @@ def test_edge_cases(c1, oracle):
     same = np.tile(tri, (100, 1))
     same[37] += np.float32(0.0)
     qs = QueryScene.from_soup(same)
+    qs.set_config(128, 8, loop)
     h = _trace(c1, qs=qs, rays=r)
     assert h[0, 0] == 0 and h[0, 1] == np.float32(5.0).view(np.uint32)
     assert np.array_equal(h, oracle.brute_closest(same, r, 1))
@@ -150,6 +162,7 @@ def test_edge_cases(c1, oracle):
     # duplicated soup: every tie must resolve to the lower copy
     dup = np.concatenate([c1["verts"][:2000], c1["verts"][:2000]])
     qs = QueryScene.from_soup(dup)
+    qs.set_config(128, 8, loop)
     rr = c1["rays"][:20000]
     h = _trace(c1, qs=qs, rays=rr)
     assert np.array_equal(h, oracle.brute_closest(dup, rr, cpu_threads()))
