@@ -41,6 +41,8 @@ struct FrameConsts {
     uint32_t guideTrainPhase;
     float4 guideBoxMin;          // scene box: pos01 = (p - min) * scale
     float4 guideBoxScale;
+    uint32_t tlasWideRoot;       // the TLAS's four-wide copy (SceneView::wide)
+    uint32_t padW0, padW1, padW2;
 };
 
 struct GpuInstance {             // 144 bytes
@@ -56,7 +58,9 @@ static_assert(sizeof(GpuInstance) == 144, "GpuInstance is 144 bytes");
 // the BLAS root (64 bytes, one cache-friendly read per TLAS leaf).
 struct TraceInstance {
     float4 row0, row1, row2;     // worldToObject rows [R | t]
-    uint32_t blasRoot, materialIndex, pad0, pad1;
+    uint32_t blasRoot, materialIndex;
+    uint32_t wideRoot;           // the BLAS's four-wide copy (SceneView::wide)
+    uint32_t pad1;
 };
 static_assert(sizeof(TraceInstance) == 64, "TraceInstance is 64 bytes");
 

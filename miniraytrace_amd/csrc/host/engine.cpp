@@ -340,6 +340,7 @@ uint32_t Engine::packGeometry() {
         for (uint32_t t = 0; t < nt; ++t)
             for (int k = 0; k < 3; ++k) bounds[t].grow(&m.positions[size_t(m.indices[t * 3 + k]) * 4]);
         BvhBuild b = buildBvhOnDevice(gpuBvh_, bounds, 4, stream_);
+        built[j].wide = collapseWide(b.nodes);
         built[j].nodes = std::move(b.nodes);
         built[j].triOrder = std::move(b.primOrder);
         built[j].bounds = b.rootBounds;
@@ -357,6 +358,7 @@ uint32_t Engine::packGeometry() {
                 for (int k = 0; k < 3; ++k) bounds[t].grow(&m.positions[size_t(m.indices[t * 3 + k]) * 4]);
             // Large meshes parallelise inside the builder; small ones across meshes.
             BvhBuild b = buildBvh(bounds, 4, nt >= 65536 ? 0 : 1);
+            built[j].wide = collapseWide(b.nodes);
             built[j].nodes = std::move(b.nodes);
             built[j].triOrder = std::move(b.primOrder);
             built[j].bounds = b.rootBounds;
@@ -389,8 +391,18 @@ uint32_t Engine::packGeometry() {
     return uint32_t(jobs.size());
 }
 
+// Wide node w with local indices -> global: inner children by wideBase,
+// leaves' first primitives by primBase, big leaves' binary nodes by nodeBase.
+static void rebaseWide(WideNode& w, uint32_t wideBase, uint32_t primBase, uint32_t nodeBase) {
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t code = w.code[k];
+        if (code == kWideEmpty) continue;
+        w.child[k] += code == kWideInner ? wideBase : (code == kWideBigLeaf ? nodeBase : primBase);
+    }
+}
+
 // One mesh's vertices, its triangles in BLAS leaf order and its BLAS nodes
-// rebased to the slot's ranges, appended to g.
+// (binary and four-wide) rebased to the slot's ranges, appended to g.
 void Engine::appendMesh(const MeshData& m, Blas& b, const MeshSlot& sl, GeomArrays& g) {
     b.rootGlobal = sl.nb;
     g.pos.insert(g.pos.end(), m.positions.begin(), m.positions.end());
@@ -417,6 +429,11 @@ void Engine::appendMesh(const MeshData& m, Blas& b, const MeshSlot& sl, GeomArra
     for (BvhNode n : b.nodes) {
         n.leftOrPrim += (n.packed >> 2) == 0 ? sl.nb : sl.tb;
         g.nodes.push_back(n);
+    }
+    b.wideRootGlobal = sl.wb;
+    for (WideNode w : b.wide) {
+        rebaseWide(w, sl.wb, sl.tb, sl.nb);
+        g.wide.push_back(w);
     }
 }
 
@@ -449,6 +466,8 @@ void Engine::repackGeometry() {
         sl.tcap = m.triCount();
         sl.nb = uint32_t(g.nodes.size());
         sl.ncap = uint32_t(b.nodes.size());
+        sl.wb = uint32_t(g.wide.size());
+        sl.wcap = uint32_t(b.wide.size());
         appendMesh(m, b, sl, g);
         slots_[id] = sl;
     }
@@ -459,13 +478,18 @@ void Engine::repackGeometry() {
     vertEnd_ = uint32_t(g.pos.size() / 4);
     triEnd_ = uint32_t(g.tris.size());
     nodeEnd_ = tlasRoot_ + tlasCap_;
-    if (size_t(nodeEnd_) > kMaxTraversalIndex)
+    // a wide tree has at most as many nodes as its binary tree
+    tlasWideRoot_ = uint32_t(g.wide.size());
+    tlasWideCap_ = tlasCap_;
+    wideEnd_ = tlasWideRoot_ + tlasWideCap_;
+    if (size_t(nodeEnd_) > kMaxTraversalIndex || size_t(wideEnd_) > kMaxTraversalIndex)
         throw Error(MRT_ERROR_OUT_OF_MEMORY, "scene BVH holds more than 2^27 - 1 nodes");
-    size_t bigV = 0, bigT = 0, bigN = 0;
+    size_t bigV = 0, bigT = 0, bigN = 0, bigW = 0;
     for (auto& [id, m] : meshes) {
         bigV = std::max<size_t>(bigV, m.vertexCount());
         bigT = std::max<size_t>(bigT, m.triCount());
         bigN = std::max<size_t>(bigN, blas_[id].nodes.size());
+        bigW = std::max<size_t>(bigW, blas_[id].wide.size());
     }
     auto roomOf = [](size_t n, size_t big) { return n + std::max(n / 4, big + big / 8 + 8) + 64; };
     auto put = [&](DeviceBuffer& b, const void* data, size_t bytes, size_t capBytes) {
@@ -481,6 +505,7 @@ void Engine::repackGeometry() {
     put(dTris_, g.tris.data(), g.tris.size() * sizeof(GpuTriangle), tRoom * sizeof(GpuTriangle));
     put(dTriVerts_, g.triVerts.data(), g.triVerts.size() * 4, tRoom * 48);
     put(dNodes_, g.nodes.data(), g.nodes.size() * sizeof(BvhNode), nRoom * sizeof(BvhNode));
+    put(dWide_, g.wide.data(), g.wide.size() * sizeof(WideNode), roomOf(wideEnd_, bigW) * sizeof(WideNode));
     layoutValid_ = true;
     ++geometryRepacks_;
     invalidateGraph();
@@ -504,7 +529,7 @@ bool Engine::placeIncremental(const std::vector<uint32_t>& dirty) {
         holes.add(it->second);
         it = slots.erase(it);
     }
-    uint64_t ve = vertEnd_, te = triEnd_, ne = nodeEnd_;
+    uint64_t ve = vertEnd_, te = triEnd_, ne = nodeEnd_, we = wideEnd_;
     struct Place {
         uint32_t id;
         MeshSlot sl;
@@ -514,6 +539,7 @@ bool Engine::placeIncremental(const std::vector<uint32_t>& dirty) {
     for (uint32_t id : dirty) {
         const MeshData& m = meshes.at(id);
         const uint32_t nv = m.vertexCount(), nt = m.triCount(), nn = uint32_t(blas_[id].nodes.size());
+        const uint32_t nw = uint32_t(blas_[id].wide.size());
         auto it = slots.find(id);
         MeshSlot sl;
         const bool had = it != slots.end();
@@ -541,15 +567,21 @@ bool Engine::placeIncremental(const std::vector<uint32_t>& dirty) {
             pad = sl.nb != ne;
             ne = uint64_t(sl.nb) + sl.ncap;
         }
+        if (!had || nw > sl.wcap) {
+            if (had) holes.wides += sl.wcap;
+            sl.wb = uint32_t(we);
+            sl.wcap = spare(nw);
+            we += sl.wcap;
+        }
         slots[id] = sl;
         place.push_back(Place{ id, sl, pad });
     }
-    if (ne > kMaxTraversalIndex || te > kMaxTraversalIndex) return false;
+    if (ne > kMaxTraversalIndex || te > kMaxTraversalIndex || we > kMaxTraversalIndex) return false;
     if (ve * 16 > dPos_.size() || ve * 16 > dNrm_.size() || ve * 16 > dTan_.size() || ve * 8 > dUv_.size() ||
         te * sizeof(GpuTriangle) > dTris_.size() || te * 48 > dTriVerts_.size() ||
-        ne * sizeof(BvhNode) > dNodes_.size())
+        ne * sizeof(BvhNode) > dNodes_.size() || we * sizeof(WideNode) > dWide_.size())
         return false;
-    if (2 * holes.verts > ve || 2 * holes.tris > te || 2 * holes.nodes > ne) return false;
+    if (2 * holes.verts > ve || 2 * holes.tris > te || 2 * holes.nodes > ne || 2 * holes.wides > we) return false;
     for (const Place& pl : place) {
         const MeshSlot& sl = pl.sl;
         GeomArrays g;
@@ -565,8 +597,10 @@ bool Engine::placeIncremental(const std::vector<uint32_t>& dirty) {
             dNodes_.upload(&pad, sizeof(BvhNode), size_t(sl.nb - 1) * sizeof(BvhNode));
         }
         dNodes_.upload(g.nodes.data(), g.nodes.size() * sizeof(BvhNode), size_t(sl.nb) * sizeof(BvhNode));
+        dWide_.upload(g.wide.data(), g.wide.size() * sizeof(WideNode), size_t(sl.wb) * sizeof(WideNode));
         uploadBytes_ += (g.pos.size() + g.nrm.size() + g.uv.size() + g.tan.size() + g.triVerts.size()) * 4 +
-                        g.tris.size() * sizeof(GpuTriangle) + g.nodes.size() * sizeof(BvhNode);
+                        g.tris.size() * sizeof(GpuTriangle) + g.nodes.size() * sizeof(BvhNode) +
+                        g.wide.size() * sizeof(WideNode);
         ++meshesWritten_;
     }
     slots_ = std::move(slots);
@@ -574,6 +608,7 @@ bool Engine::placeIncremental(const std::vector<uint32_t>& dirty) {
     vertEnd_ = uint32_t(ve);
     triEnd_ = uint32_t(te);
     nodeEnd_ = uint32_t(ne);
+    wideEnd_ = uint32_t(we);
     return true;
 }
 
@@ -591,7 +626,8 @@ void Engine::packInstances() {
     }
     BvhBuild tlas = buildBvh(bounds, 1);
     if (tlas.nodes.empty()) tlas.nodes.push_back(sentinelNode());
-    if (!layoutValid_ || tlas.nodes.size() > tlasCap_) repackGeometry();
+    std::vector<WideNode> tlasWide = collapseWide(tlas.nodes);
+    if (!layoutValid_ || tlas.nodes.size() > tlasCap_ || tlasWide.size() > tlasWideCap_) repackGeometry();
     std::vector<GpuInstance> gi;
     std::vector<TraceInstance> ti;
     gi.reserve(flat.size());
@@ -617,6 +653,7 @@ void Engine::packInstances() {
         t.row2 = make_float4(inv[2], inv[6], inv[10], inv[14]);
         t.blasRoot = g.blasRoot;
         t.materialIndex = g.materialIndex;
+        t.wideRoot = blas_[inst->mesh].wideRootGlobal;
         ti.push_back(t);
     }
     for (BvhNode& n : tlas.nodes) {
@@ -643,11 +680,15 @@ void Engine::packInstances() {
                                            std::to_string(tlasDepth_) + ", BLAS depth " +
                                            std::to_string(maxBlasDepth_) + ")");
     dNodes_.upload(tlas.nodes.data(), tlas.nodes.size() * sizeof(BvhNode), size_t(tlasRoot_) * sizeof(BvhNode));
+    // leaves hold instance ranges (TLAS leaf order), big leaves the binary TLAS node
+    for (WideNode& w : tlasWide) rebaseWide(w, tlasWideRoot_, 0u, tlasRoot_);
+    dWide_.upload(tlasWide.data(), tlasWide.size() * sizeof(WideNode), size_t(tlasWideRoot_) * sizeof(WideNode));
     const void* inst0 = dInst_.get();
     const void* trace0 = dTraceInst_.get();
     uploadVec(dInst_, gi.data(), gi.size() * sizeof(GpuInstance));
     uploadVec(dTraceInst_, ti.data(), ti.size() * sizeof(TraceInstance));
-    uploadBytes_ += tlas.nodes.size() * sizeof(BvhNode) + gi.size() * (sizeof(GpuInstance) + sizeof(TraceInstance));
+    uploadBytes_ += tlas.nodes.size() * sizeof(BvhNode) + tlasWide.size() * sizeof(WideNode) +
+                    gi.size() * (sizeof(GpuInstance) + sizeof(TraceInstance));
     if (dInst_.get() != inst0 || dTraceInst_.get() != trace0) invalidateGraph();
 }
 
@@ -874,6 +915,7 @@ void Engine::ensureGuideNet() {
 dev::SceneView Engine::sceneView() const {
     dev::SceneView s;
     s.nodes = dNodes_.as<const dev::Node>();
+    s.wide = dWide_.as<const float4>();
     s.triVerts = dTriVerts_.as<const float4>();
     s.tris = dTris_.as<const GpuTriangle>();
     s.positions = dPos_.as<const float4>();
@@ -972,6 +1014,7 @@ void Engine::fillConsts(FrameConsts& g) const {
     g.fireflyClamp = settings_.fireflyClamp;
     g.tonemap = settings_.tonemap;
     g.tlasRoot = tlasRoot_;
+    g.tlasWideRoot = tlasWideRoot_;
     g.instanceCount = instanceCount_;
     g.seedMix = seed_ ? (seed_ * 0x9E3779B9u) ^ 0x85EBCA6Bu : 0u;
     g.guiding = guide_.enableSampling ? 1u : 0u;

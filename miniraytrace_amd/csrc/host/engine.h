@@ -6,6 +6,7 @@
 
 #include "bvh_build.h"
 #include "bvh_gpu_build.h"
+#include "bvh_wide.h"
 #include "common.h"
 #include "common/gpu_types.h"
 #include "guide.h"
@@ -120,28 +121,32 @@ public:
 private:
     struct Blas {
         std::vector<BvhNode> nodes;
+        std::vector<WideNode> wide;  // four-wide copy (local indices)
         std::vector<uint32_t> triOrder;
         Box3 bounds;
         uint32_t depth = 0;
         uint32_t rootGlobal = 0;
+        uint32_t wideRootGlobal = 0;
     };
-    // A mesh's ranges in the vertex, triangle and node arrays: first index
-    // and capacity (its root is node nb).
+    // A mesh's ranges in the vertex, triangle, node and wide-node arrays:
+    // first index and capacity (its roots are node nb and wide node wb).
     struct MeshSlot {
-        uint32_t vb = 0, vcap = 0, tb = 0, tcap = 0, nb = 0, ncap = 0;
+        uint32_t vb = 0, vcap = 0, tb = 0, tcap = 0, nb = 0, ncap = 0, wb = 0, wcap = 0;
     };
-    struct Holes {  // entries of the three arrays no mesh uses
-        uint64_t verts = 0, tris = 0, nodes = 0;
+    struct Holes {  // entries of the four arrays no mesh uses
+        uint64_t verts = 0, tris = 0, nodes = 0, wides = 0;
         void add(const MeshSlot& s) {
             verts += s.vcap;
             tris += s.tcap;
             nodes += s.ncap;
+            wides += s.wcap;
         }
     };
     struct GeomArrays {
         std::vector<float> pos, nrm, uv, tan, triVerts;
         std::vector<GpuTriangle> tris;
         std::vector<BvhNode> nodes;
+        std::vector<WideNode> wide;
     };
     void appendMesh(const MeshData& m, Blas& b, const MeshSlot& sl, GeomArrays& g);
     void repackGeometry();
@@ -186,6 +191,8 @@ private:
     uint32_t vertEnd_ = 0, triEnd_ = 0;    // first entries behind every slot
     uint32_t nodeEnd_ = 0;                 // first node behind every slot and the TLAS region
     uint32_t tlasCap_ = 0;                 // nodes reserved for the TLAS at tlasRoot_
+    uint32_t wideEnd_ = 0;                 // first wide node behind every slot and the TLAS's wide region
+    uint32_t tlasWideRoot_ = 0, tlasWideCap_ = 0;  // the TLAS's four-wide copy: first node, reserved nodes
     uint32_t geometryRepacks_ = 0, meshesWritten_ = 0;  // since creation
     uint64_t uploadBytes_ = 0;             // by the last commit
     uint32_t tlasRoot_ = 0;
@@ -201,7 +208,7 @@ private:
 
     // scene buffers
     GpuBvhBuilder gpuBvh_;  // large meshes' BLAS builds (workspace kept between commits)
-    DeviceBuffer dNodes_, dTriVerts_, dTris_, dPos_, dNrm_, dUv_, dTan_;
+    DeviceBuffer dNodes_, dWide_, dTriVerts_, dTris_, dPos_, dNrm_, dUv_, dTan_;
     DeviceBuffer dInst_, dTraceInst_, dMats_, dLights_, dEnv_, dAlias_;
     DeviceBuffer dTexTable_, dTex8_, dTex32_;
     size_t tex8Texels_ = 0, tex32Texels_ = 0;  // texels held by the two pools

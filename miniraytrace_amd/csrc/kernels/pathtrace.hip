@@ -265,7 +265,7 @@ __device__ __forceinline__ void extendChunk(const SceneView& sc, const PathView&
         // traverses together (same hits as the per-lane traversal).
         hit = bounce == 0 && (gp->flags & kFlagPacketCamera) ? traceScenePacket(sc, gp->tlasRoot, gp->instanceCount, make_float3(o.x, o.y, o.z),
                                              make_float3(d.x, d.y, d.z), kEpsRay, 1e30f, waveStack, h, overflow)
-                          : traceScene<false, S, BLOCK>(sc, gp->tlasRoot, gp->instanceCount,
+                          : traceSceneWide<false, S, BLOCK>(sc, gp->tlasWideRoot, gp->instanceCount,
                                                         make_float3(o.x, o.y, o.z), make_float3(d.x, d.y, d.z),
                                                         kEpsRay, 1e30f, lds, h, overflow);
         pv.hit[k] = hit ? make_float4(h.t, h.b1, h.b2, __uint_as_float(h.tri))
@@ -548,7 +548,7 @@ __device__ __forceinline__ void shadowChunk(const SceneView& sc, const PathView&
     if (k < count) {
         const float4 o = pv.shadowO[k], d = pv.shadowD[k];
         SceneHit h;
-        const bool occluded = traceScene<true, S, BLOCK>(sc, gp->tlasRoot, gp->instanceCount,
+        const bool occluded = traceSceneWide<true, S, BLOCK>(sc, gp->tlasWideRoot, gp->instanceCount,
                                                          make_float3(o.x, o.y, o.z), make_float3(d.x, d.y, d.z),
                                                          kEpsRay, o.w, lds, h, overflow);
         if (!occluded) applyShadow(pv, *gp, bounce, __float_as_uint(d.w), v3(pv.shadowC[k]));
@@ -630,7 +630,7 @@ __device__ __forceinline__ void tailChunk(const SceneView& sc, const PathView& p
             waveAppend(pv.counters + b * kCntPerBounce + kCntTailShadows, o.emitShadow);
             if (o.emitShadow) {
                 SceneHit sh;
-                const bool occluded = traceScene<true, S, BLOCK, false>(sc, g.tlasRoot, g.instanceCount,
+                const bool occluded = traceSceneWide<true, S, BLOCK, false>(sc, g.tlasWideRoot, g.instanceCount,
                                                                         make_float3(o.shO.x, o.shO.y, o.shO.z),
                                                                         make_float3(o.shD.x, o.shD.y, o.shD.z), kEpsRay,
                                                                         o.shMax, lds, sh, overflow);
@@ -639,7 +639,7 @@ __device__ __forceinline__ void tailChunk(const SceneView& sc, const PathView& p
             if (!o.emitRay) break;
             waveAppend(pv.counters + (b + 1) * kCntPerBounce + kCntTailRays, true);
             SceneHit hh;
-            const bool hit = traceScene<false, S, BLOCK, false>(sc, g.tlasRoot, g.instanceCount,
+            const bool hit = traceSceneWide<false, S, BLOCK, false>(sc, g.tlasWideRoot, g.instanceCount,
                                                                 make_float3(o.newO.x, o.newO.y, o.newO.z),
                                                                 make_float3(o.newD.x, o.newD.y, o.newD.z), kEpsRay, 1e30f,
                                                                 lds, hh, overflow);

@@ -14,6 +14,7 @@ struct Node;
 
 struct SceneView {
     const Node* nodes;               // all BLAS nodes, then the TLAS at FrameConsts::tlasRoot
+    const float4* wide;              // their four-wide copies (8 float4 per node), TLAS at tlasWideRoot
     const float4* triVerts;          // 3 float4 per leaf-ordered triangle (object space)
     const GpuTriangle* tris;         // leaf-ordered vertex indices (global)
     const float4* positions;

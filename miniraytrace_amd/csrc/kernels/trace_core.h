@@ -197,17 +197,12 @@ constexpr uint32_t kWideCodeBigLeaf = 0xFEu;
 constexpr uint32_t kWideCodeEmpty = 0xFFu;
 
 struct WideRows {
-    uint32_t nx, ny, nz, fx, fy, fz;  // rows of the near / far planes
+    uint32_t near;                       // near-plane rows of x, y, z in bytes 0, 1, 2 (far = 3 + a - (near - a))
     float cnx, cny, cnz, cfx, cfy, cfz;  // fma offsets of near / far planes
 };
 __device__ __forceinline__ WideRows wideRows(const RayK& r) {
     WideRows w;
-    w.nx = r.ix >= 0.0f ? 0u : 3u;
-    w.ny = r.iy >= 0.0f ? 1u : 4u;
-    w.nz = r.iz >= 0.0f ? 2u : 5u;
-    w.fx = 3u - w.nx;
-    w.fy = 5u - w.ny;
-    w.fz = 7u - w.nz;
+    w.near = (r.ix >= 0.0f ? 0u : 3u) | (r.iy >= 0.0f ? 1u : 4u) << 8 | (r.iz >= 0.0f ? 2u : 5u) << 16;
     const float ax = -r.ox * r.ix, ay = -r.oy * r.iy, az = -r.oz * r.iz;
     const float ex = fabsf(ax) * (1.0f / 4194304.0f), ey = fabsf(ay) * (1.0f / 4194304.0f),
                 ez = fabsf(az) * (1.0f / 4194304.0f);
@@ -231,22 +226,40 @@ __device__ __forceinline__ float f4At(const float4& v, int k) {
 }
 __device__ __forceinline__ uint32_t u4At(const float4& v, int k) { return __float_as_uint(f4At(v, k)); }
 
-// Box tests of the four children of wide node w.
+// Box tests of the four children of wide node w.  FMA: plane distances as
+// fused multiply-adds with the offsets of `rows` (one op per plane); else as
+// slab()'s (plane - o) * inv, which keeps six fewer values live (the path
+// tracer's trace kernel runs at a register budget).
+template <bool FMA = true>
 __device__ __forceinline__ WideStep wideTest(const float4* __restrict__ wide, uint32_t w, const RayK& r,
                                              const WideRows& rows, float limit) {
 #pragma clang fp contract(off)
     const float4* p = wide + size_t(w) * 8;
-    const float4 nx = ldGlobal4(p + rows.nx), ny = ldGlobal4(p + rows.ny), nz = ldGlobal4(p + rows.nz);
-    const float4 fx = ldGlobal4(p + rows.fx), fy = ldGlobal4(p + rows.fy), fz = ldGlobal4(p + rows.fz);
+    const uint32_t rx = rows.near & 0xFFu, ry = (rows.near >> 8) & 0xFFu, rz = rows.near >> 16;
+    const float4 nx = ldGlobal4(p + rx), ny = ldGlobal4(p + ry), nz = ldGlobal4(p + rz);
+    const float4 fx = ldGlobal4(p + (3u - rx)), fy = ldGlobal4(p + (5u - ry)), fz = ldGlobal4(p + (7u - rz));
     const float4 ch = ldGlobal4(p + 6), cd = ldGlobal4(p + 7);
     WideStep s;
     s.hitMask = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const uint32_t code = u4At(cd, k);
-        const float tnx = fmaf(f4At(nx, k), r.ix, rows.cnx), tfx = fmaf(f4At(fx, k), r.ix, rows.cfx);
-        const float tny = fmaf(f4At(ny, k), r.iy, rows.cny), tfy = fmaf(f4At(fy, k), r.iy, rows.cfy);
-        const float tnz = fmaf(f4At(nz, k), r.iz, rows.cnz), tfz = fmaf(f4At(fz, k), r.iz, rows.cfz);
+        float tnx, tfx, tny, tfy, tnz, tfz;
+        if constexpr (FMA) {
+            tnx = fmaf(f4At(nx, k), r.ix, rows.cnx);
+            tfx = fmaf(f4At(fx, k), r.ix, rows.cfx);
+            tny = fmaf(f4At(ny, k), r.iy, rows.cny);
+            tfy = fmaf(f4At(fy, k), r.iy, rows.cfy);
+            tnz = fmaf(f4At(nz, k), r.iz, rows.cnz);
+            tfz = fmaf(f4At(fz, k), r.iz, rows.cfz);
+        } else {
+            tnx = (f4At(nx, k) - r.ox) * r.ix;
+            tfx = (f4At(fx, k) - r.ox) * r.ix;
+            tny = (f4At(ny, k) - r.oy) * r.iy;
+            tfy = (f4At(fy, k) - r.oy) * r.iy;
+            tnz = (f4At(nz, k) - r.oz) * r.iz;
+            tfz = (f4At(fz, k) - r.oz) * r.iz;
+        }
         const float enter = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));  // >= 0 or -0, never NaN
         const float far = fminf(fminf(tfx, tfy), tfz) * kFarWiden;
         // bitwise & on the conditions, and selects, keep the step free of

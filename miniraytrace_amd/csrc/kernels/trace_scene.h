@@ -193,6 +193,122 @@ __device__ bool traceScene(const SceneView& sc, uint32_t tlasRoot, uint32_t inst
     return true;
 }
 
+// The same walk over the four-wide trees (host/bvh_wide.h): the TLAS and
+// every BLAS collapsed from their binary trees, whose leaves they share.  A
+// step tests the four children of the current wide node (trace_core.h
+// wideTest), goes on with the nearest hit child and pushes the other hit
+// children farthest first; a leaf step is as in traceScene.  The hits are
+// those of traceScene: the box tests are conservative and the tie rule is
+// the same, so neither tree shape nor visiting order changes a result.
+// Camera-ray packets keep the binary walk (traceScenePacket).
+template <bool ANY, int S, int BLOCK, bool TRI_STEP = true>
+__device__ bool traceSceneWide(const SceneView& sc, uint32_t tlasWideRoot, uint32_t instanceCount,
+                               float3 o, float3 d, float tmin, float tmax, uint2* ldsLane,
+                               SceneHit& out, bool& overflow) {
+    if (instanceCount == 0) return false;
+    RayK r = setupRayBox(o.x, o.y, o.z, d.x, d.y, d.z, tmin);
+    WideRows rows = wideRows(r);
+    uint2 spill[kSpill];
+    ShortStack<S, BLOCK> stack(ldsLane, spill);
+    bool inBlas = false;
+    int blasBase = 0;
+    uint32_t curInst = 0;
+    float best = tmax;
+    bool found = false;
+    uint32_t bestTri = kNoHit, bestInst = kNoHit;
+    float bestB1 = 0.0f, bestB2 = 0.0f;
+
+    Cursor cur{ tlasWideRoot, 0u };
+    bool have = true;
+    bool pendingInst = false;
+    uint32_t pendingIdx = 0;
+
+    while (true) {
+        if (pendingInst) {
+            // Enter an instance: object-space ray, BLAS root (a wide node).
+            pendingInst = false;
+            const TraceInstance* tp = sc.traceInst + pendingIdx;
+            TraceInstance ti;
+            ti.row0 = ldGlobal4(&tp->row0);
+            ti.row1 = ldGlobal4(&tp->row1);
+            ti.row2 = ldGlobal4(&tp->row2);
+            const float4 tail = ldGlobal4(reinterpret_cast<const float4*>(&tp->blasRoot));
+            float3 oo, dd;
+            toObject(ti, o.x, o.y, o.z, d.x, d.y, d.z, oo, dd);
+            r = setupRay(oo.x, oo.y, oo.z, dd.x, dd.y, dd.z, tmin);
+            rows = wideRows(r);
+            inBlas = true;
+            blasBase = stack.sp;
+            curInst = pendingIdx;
+            cur = Cursor{ __float_as_uint(tail.z), 0u };  // TraceInstance::wideRoot
+            have = true;
+        }
+        if (have) {
+            const uint32_t lp = cur.index;
+            const uint32_t cnt = cur.count;
+            if (cnt == 0) {
+                WideStep ws = wideTest<false>(sc.wide, lp, r, rows, best * kLimitWiden);
+                const int hits = wideSort(ws);
+                if (hits > 0) {
+                    if (hits > 3) stack.push(make_uint2(ws.entry[3], ws.key[3]));
+                    if (hits > 2) stack.push(make_uint2(ws.entry[2], ws.key[2]));
+                    if (hits > 1) stack.push(make_uint2(ws.entry[1], ws.key[1]));
+                    cur = wideEntryCursor(sc.nodes, make_uint2(ws.entry[0], ws.key[0]));
+                    continue;
+                }
+            } else if (inBlas) {
+                if constexpr (TRI_STEP) {
+                    if (blasTriangle<ANY>(sc, r, lp, curInst, tmax, best, found, bestTri, bestInst, bestB1, bestB2))
+                        return true;
+                    if (cnt > 1) {
+                        cur = Cursor{ lp + 1, cnt - 1 };
+                        continue;
+                    }
+                } else {
+                    for (uint32_t k = lp; k < lp + cnt; ++k)
+                        if (blasTriangle<ANY>(sc, r, k, curInst, tmax, best, found, bestTri, bestInst, bestB1, bestB2))
+                            return true;
+                }
+            } else {
+                // TLAS leaf: one or more instances [lp, lp + cnt).
+                for (uint32_t j = cnt - 1; j >= 1; --j) stack.push(instanceEntry(lp + j));
+                pendingInst = true;
+                pendingIdx = lp;
+                continue;
+            }
+        }
+        have = false;
+        while (true) {
+            if (inBlas && stack.sp == blasBase) {
+                inBlas = false;
+                r = setupRayBox(o.x, o.y, o.z, d.x, d.y, d.z, tmin);
+                rows = wideRows(r);
+            }
+            if (stack.empty()) break;
+            const uint2 e = stack.pop();
+            if (entryCode(e) == kEntryInstance) {
+                pendingInst = true;
+                pendingIdx = e.x & kEntryIndexMask;
+                break;
+            }
+            if (__uint_as_float(e.y) <= best * kLimitWiden) {
+                cur = wideEntryCursor(sc.nodes, e);
+                have = true;
+                break;
+            }
+        }
+        if (!have && !pendingInst) break;
+    }
+    if (stack.overflow) overflow = true;
+    if (ANY || !found) return false;
+    out.t = best;
+    out.b1 = bestB1;
+    out.b2 = bestB2;
+    out.tri = bestTri;
+    out.inst = bestInst;
+    return true;
+}
+
 // ------------------------------------------------------------ wave packets --
 // Closest hit for rays that arrive in coherent groups (camera rays: a wave
 // holds an 8 x 8 pixel tile).  The wave walks the trees together: it enters a
