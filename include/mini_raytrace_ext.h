@@ -290,6 +290,12 @@ typedef struct mrtxGuidingStats {
     uint32_t poolRecords;     /* training records collected in the last frame */
     uint32_t candidates;      /* positive-weight path records found in the last frame */
     uint32_t lastFrameRows;   /* network queries (inferred rows) of the last profiled frame */
+    /* Appended fields, written when structSize covers them.  Guided
+     * environment NEE draws from two strata (bright texels, the rest) with a
+     * bright-stratum probability learned per cell of the scene: */
+    float    envBrightShare;  /* the bright stratum's share of the environment (0: no strata) */
+    uint32_t envCellsLearned; /* cells whose probability has left the share (both strata sampled) */
+    float    envMeanSplit;    /* mean learned bright-stratum probability over those cells */
 } mrtxGuidingStats;
 
 /* Changing the options re-captures the frame graph; it does not reset the

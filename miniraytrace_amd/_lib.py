@@ -169,7 +169,8 @@ class mrtxGuidingOptions(_Sized):
 class mrtxGuidingStats(_Sized):
     _fields_ = [("structSize", C.c_size_t), ("steps", C.c_uint32), ("lastBatch", C.c_uint32),
                 ("lastLoss", C.c_float), ("poolRecords", C.c_uint32), ("candidates", C.c_uint32),
-                ("lastFrameRows", C.c_uint32)]
+                ("lastFrameRows", C.c_uint32), ("envBrightShare", C.c_float), ("envCellsLearned", C.c_uint32),
+                ("envMeanSplit", C.c_float)]
 
 
 class mrtxSceneExport(_Sized):

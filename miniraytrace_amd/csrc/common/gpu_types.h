@@ -42,7 +42,11 @@ struct FrameConsts {
     float4 guideBoxMin;          // scene box: pos01 = (p - min) * scale
     float4 guideBoxScale;
     uint32_t tlasWideRoot;       // the TLAS's four-wide copy (SceneView::wide)
-    uint32_t padW0, padW1, padW2;
+    // Environment strata (SceneView::envA / envB): the bright texels (each at
+    // least 64 x the mean weight: a sun) and the rest.  envSplitA = the bright
+    // stratum's share of the environment's weight, 0 when either is empty.
+    float envSplitA;
+    uint32_t envCountA, envCountB;
 };
 
 struct GpuInstance {             // 144 bytes
@@ -96,5 +100,19 @@ struct AliasEntry {              // environment alias table (Vose)
     float prob;                  // probability of keeping this texel
     uint32_t alias;              // texel taken otherwise
 };
+
+struct EnvStratumEntry {         // alias table over one stratum's texels
+    float prob;                  // probability of keeping this entry
+    uint32_t alias;              // entry taken otherwise
+    uint32_t texel;              // the entry's texel
+    uint32_t pad;
+};
+
+// Learned environment-stratum selection of guided shading: per cell (the
+// shading point's position on a kNeeGrid^3 grid over the scene box, times the
+// normal's dominant direction) the running sums of squared stratum-
+// conditional NEE contributions and sample counts: (sumA, nA, sumB, nB).
+constexpr uint32_t kNeeGrid = 32;
+constexpr uint32_t kNeeCells = kNeeGrid * kNeeGrid * kNeeGrid * 6;
 
 } // namespace mrt

@@ -305,6 +305,8 @@ void Engine::sync(mrtCommitStats* out) {
     if (tlas) packInstances();
     if (d & (kDirtyLights | kDirtyEnv)) packLights();
     if (d & kDirtyEnv) packEnvironment();
+    // the learned environment-stratum statistics describe the old scene
+    if (dNeeStats_.get()) dNeeStats_.zero(stream_);
     scene_.clearDirty();
     resetAccumulation();
     stats_.commitMs = msSince(t0);
@@ -773,6 +775,8 @@ void Engine::packEnvironment() {
     hasEnv_ = scene_.hasEnvironment();
     if (!hasEnv_) {
         envW_ = envH_ = 0;
+        envSplitA_ = 0.0f;
+        envCountA_ = envCountB_ = 0;
         invalidateGraph();
         return;
     }
@@ -803,6 +807,48 @@ void Engine::packEnvironment() {
     envIntegral_ = std::max(float(integral), 1e-8f);
     uploadVec(dEnv_, px, size_t(w) * h * 16);
     uploadVec(dAlias_, alias.data(), alias.size() * sizeof(AliasEntry));
+
+    // Strata for guided shading's environment NEE: texels of at least 64 x
+    // the mean weight (a sun) and the rest, each with its own alias table over
+    // the same weights, so that sampling a stratum with probability s and
+    // then its table gives texel t the probability P(t) s / share (bright) or
+    // P(t) (1 - s) / (1 - share).
+    const size_t n = weights.size();
+    double total = 0.0;
+    for (double x : weights) total += x;
+    const double cut = 64.0 * total / double(std::max<size_t>(n, 1));
+    std::vector<uint8_t> strat(n, 0);
+    std::vector<double> wa, wb;
+    std::vector<uint32_t> ta, tb;
+    double sumA = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        if (weights[i] >= cut && weights[i] > 0.0) {
+            strat[i] = 1;
+            wa.push_back(weights[i]);
+            ta.push_back(uint32_t(i));
+            sumA += weights[i];
+        } else {
+            wb.push_back(weights[i]);
+            tb.push_back(uint32_t(i));
+        }
+    }
+    const bool split = !wa.empty() && !wb.empty() && total > 0.0 && sumA < total;
+    envSplitA_ = split ? float(sumA / total) : 0.0f;
+    envCountA_ = split ? uint32_t(wa.size()) : 0u;
+    envCountB_ = split ? uint32_t(wb.size()) : 0u;
+    auto table = [](const std::vector<double>& wts, const std::vector<uint32_t>& tex) {
+        std::vector<AliasEntry> al;
+        buildAlias(wts, al);
+        std::vector<EnvStratumEntry> out(al.size());
+        for (size_t i = 0; i < al.size(); ++i) out[i] = EnvStratumEntry{ al[i].prob, al[i].alias, tex[i], 0u };
+        return out;
+    };
+    if (split) {
+        const auto a = table(wa, ta), b = table(wb, tb);
+        uploadVec(dEnvStrat_, strat.data(), strat.size());
+        uploadVec(dEnvA_, a.data(), a.size() * sizeof(EnvStratumEntry));
+        uploadVec(dEnvB_, b.data(), b.size() * sizeof(EnvStratumEntry));
+    }
     invalidateGraph();
 }
 
@@ -897,6 +943,10 @@ void Engine::allocGuide() {
     dPathRec_.allocate(size_t(levels) * trainPaths * 4 * 16);
     dPathRec_.zero(stream_);  // stamp 0 never matches a frame
     dGuidePool_.allocate(size_t(poolCap) * 4 * 16);
+    dNeeSplit_.allocate(size_t(n) * 4);
+    dShX_.allocate(size_t(n) * 8);
+    dNeeStats_.allocate(size_t(kNeeCells) * 16);
+    dNeeStats_.zero(stream_);
     MRT_HIP(hipStreamSynchronize(stream_));
     guidePaths_ = n;
     guideLevels_ = levels;
@@ -928,6 +978,9 @@ dev::SceneView Engine::sceneView() const {
     s.lights = dLights_.as<const GpuLight>();
     s.envMap = dEnv_.as<const float4>();
     s.envAlias = dAlias_.as<const AliasEntry>();
+    s.envStratum = dEnvStrat_.as<const uint8_t>();
+    s.envA = dEnvA_.as<const EnvStratumEntry>();
+    s.envB = dEnvB_.as<const EnvStratumEntry>();
     s.texTable = dTexTable_.as<const GpuTexture>();
     s.tex8 = dTex8_.as<const uint32_t>();
     s.tex32 = dTex32_.as<const float4>();
@@ -958,6 +1011,10 @@ dev::PathView Engine::pathView() const {
     p.guideRec = guide ? dGuideRec_.as<float4>() : nullptr;
     p.guideMix = guide ? dGuideMix_.as<float4>() : nullptr;
     p.pathRec = guide && guide_.enableTraining ? dPathRec_.as<float4>() : nullptr;
+    const bool nee = guide && guide_.enableSampling && envSplitA_ > 0.0f;
+    p.neeSplit = nee ? dNeeSplit_.as<float>() : nullptr;
+    p.shadowX = nee ? dShX_.as<uint2>() : nullptr;
+    p.neeStats = nee ? dNeeStats_.as<float4>() : nullptr;
     return p;
 }
 
@@ -1015,6 +1072,9 @@ void Engine::fillConsts(FrameConsts& g) const {
     g.tonemap = settings_.tonemap;
     g.tlasRoot = tlasRoot_;
     g.tlasWideRoot = tlasWideRoot_;
+    g.envSplitA = hasEnv_ ? envSplitA_ : 0.0f;
+    g.envCountA = hasEnv_ ? envCountA_ : 0u;
+    g.envCountB = hasEnv_ ? envCountB_ : 0u;
     g.instanceCount = instanceCount_;
     g.seedMix = seed_ ? (seed_ * 0x9E3779B9u) ^ 0x85EBCA6Bu : 0u;
     g.guiding = guide_.enableSampling ? 1u : 0u;
@@ -1419,6 +1479,23 @@ void Engine::guidingStats(mrtxGuidingStats& o) {
     o.poolRecords = std::min(w[dev::kGwPool], guide_.batchSize * guide_.stepsPerFrame);
     o.candidates = w[dev::kGwCand0 + (guideStamp_ & 1u)];
     o.lastFrameRows = stats_.guideRows;
+    o.envBrightShare = hasEnv_ ? envSplitA_ : 0.0f;
+    if (dNeeStats_.get() && envSplitA_ > 0.0f) {
+        // the rule of neeSelect (pt_device.h)
+        std::vector<float> c(size_t(kNeeCells) * 4);
+        dNeeStats_.download(c.data(), c.size() * 4, 0, stream_);
+        MRT_HIP(hipStreamSynchronize(stream_));
+        double sum = 0.0;
+        for (uint32_t i = 0; i < kNeeCells; ++i) {
+            const float* q = &c[size_t(i) * 4];
+            if (q[1] < 16.0f || q[3] < 16.0f) continue;
+            const float a = std::sqrt(q[0] / q[1]), b = std::sqrt(q[2] / q[3]);
+            if (!(a + b > 0.0f)) continue;
+            sum += std::clamp(a / (a + b), 0.05f, 0.95f);
+            ++o.envCellsLearned;
+        }
+        o.envMeanSplit = o.envCellsLearned ? float(sum / o.envCellsLearned) : 0.0f;
+    }
 }
 
 void Engine::getGuideParams(float* dst) {

@@ -7,6 +7,8 @@
 #include "mini_raytrace.h"
 #include "mini_raytrace_ext.h"
 
+#include <algorithm>
+#include <cstddef>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -81,9 +83,13 @@ MRT_API mrtResult mrtxGetGuiding(mrtEngine* e, mrtxGuidingOptions* o) {
 MRT_API mrtResult mrtxGetGuidingStats(mrtEngine* e, mrtxGuidingStats* o) {
     mrt::Engine* eng = unwrap(e);
     if (!eng) return MRT_ERROR_INVALID_HANDLE;
-    if (!validStruct(o)) return MRT_ERROR_INVALID_ARGUMENT;
+    if (!o || o->structSize < offsetof(mrtxGuidingStats, envBrightShare)) return MRT_ERROR_INVALID_ARGUMENT;
     return mrt::guarded([&] {
-        eng->guidingStats(*o);
+        mrtxGuidingStats full;
+        eng->guidingStats(full);
+        const size_t n = std::min(o->structSize, sizeof(full));
+        full.structSize = o->structSize;
+        std::memcpy(o, &full, n);
         return MRT_SUCCESS;
     });
 }

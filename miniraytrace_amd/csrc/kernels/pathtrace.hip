@@ -301,6 +301,7 @@ struct ShadeOut {
     bool emitRay, emitShadow;
     V3 newO, newD, shO, shD, shC;
     float shMax;
+    uint2 shX;  // stratum-statistics record of the shadow ray (PathView::shadowX)
 };
 
 // Shades path p at the hit `h` (t, b1, b2, triangle bits; instance hitInst)
@@ -324,6 +325,7 @@ __device__ __forceinline__ ShadeOut shadePath(const SceneView& sc, const PathVie
     o.emitRay = false;
     o.emitShadow = false;
     o.shMax = 0.0f;
+    o.shX = make_uint2(~0u, 0u);
     const uint32_t tri = __float_as_uint(h.w);
     float4 beta4 = pv.beta[p];
     V3 beta = v3(beta4);
@@ -337,7 +339,8 @@ __device__ __forceinline__ ShadeOut shadePath(const SceneView& sc, const PathVie
         if (maxComp(env) > 0.0f) {
             float w = 1.0f;
             if (bounce > 0) {
-                const float pl = envPdf(sc, g, dir) * envPickPdf(g);
+                float pl = envPdf(sc, g, dir) * envPickPdf(g);
+                if (pv.neeSplit) pl *= envStrataRatio(sc, g, dir, pv.neeSplit[p]);  // the previous vertex's NEE
                 w = balanceHeuristic(prevPdf, pl);
                 if (pv.pathRec && g.guideTrain && p % g.guideTrainStride == g.guideTrainPhase) {
                     // Training target of the previous vertex: its own
@@ -386,8 +389,18 @@ __device__ __forceinline__ ShadeOut shadePath(const SceneView& sc, const PathVie
                     }
                 }
             }
-            // NEE (pathtrace.comp:181-197)
-            const LightSample ls = sampleOneLight(sc, g, s.pos, rng);
+            // NEE (pathtrace.comp:181-197).  Guided shading with stratum
+            // statistics draws the environment over its strata.
+            const bool strata = GUIDE && pv.neeStats && g.envSplitA > 0.0f;
+            uint32_t cell = 0;
+            float split = g.envSplitA;
+            if (strata) {
+                cell = neeCell(g, s.pos, s.nShade);
+                split = neeSelect(pv.neeStats, cell, g.envSplitA);
+            }
+            if (pv.neeSplit) pv.neeSplit[p] = split;
+            const LightSample ls = strata ? sampleOneLight<true>(sc, g, s.pos, rng, split)
+                                          : sampleOneLight<false>(sc, g, s.pos, rng);
             if (ls.valid && ls.pdf > 1e-9f) {
                 const V3 wi = v3(dot(ls.wi, T), dot(ls.wi, B), dot(ls.wi, s.nShade));
                 V3 f;
@@ -406,6 +419,11 @@ __device__ __forceinline__ ShadeOut shadePath(const SceneView& sc, const PathVie
                     o.shMax = ls.dist - 2.0f * kEpsRay;
                     o.shC = beta * f * ls.radiance * (fabsf(wi.z) * w / ls.pdf);
                     o.emitShadow = true;
+                    if (strata && ls.stratum >= 0) {
+                        // the contribution as its stratum's own estimator would give it
+                        const float c = luminance(o.shC) * (ls.stratum ? split : 1.0f - split);
+                        o.shX = make_uint2((cell << 1) | uint32_t(ls.stratum), __float_as_uint(c));
+                    }
                 }
             }
             // BSDF bounce and Russian roulette (pathtrace.comp:200-216)
@@ -498,6 +516,7 @@ __device__ __forceinline__ void shadeChunk(const SceneView& sc, const PathView& 
         pv.shadowO[ss] = f4(o.shO, o.shMax);
         pv.shadowD[ss] = f4(o.shD, __uint_as_float(p));
         pv.shadowC[ss] = f4(o.shC, 0.0f);
+        if (pv.shadowX) pv.shadowX[ss] = o.shX;
     }
 }
 
@@ -552,6 +571,17 @@ __device__ __forceinline__ void shadowChunk(const SceneView& sc, const PathView&
                                                          make_float3(o.x, o.y, o.z), make_float3(d.x, d.y, d.z),
                                                          kEpsRay, o.w, lds, h, overflow);
         if (!occluded) applyShadow(pv, *gp, bounce, __float_as_uint(d.w), v3(pv.shadowC[k]));
+        if (pv.neeStats) {
+            // second moment and count of the stratum's contributions (an
+            // occluded sample counts as zero)
+            const uint2 x = pv.shadowX[k];
+            if (x.x != ~0u) {
+                const float c = occluded ? 0.0f : __uint_as_float(x.y);
+                float* cellSums = reinterpret_cast<float*>(pv.neeStats + (x.x >> 1)) + ((x.x & 1u) ? 0 : 2);
+                atomicAdd(cellSums, c * c);
+                atomicAdd(cellSums + 1, 1.0f);
+            }
+        }
     }
     if (overflow) atomicOr(pv.counters + kStatusWord, 1u);
 }

@@ -221,6 +221,79 @@ __device__ __forceinline__ bool sampleEnv(const SceneView& sc, const FrameConsts
     return pdf > 1e-9f;
 }
 
+// ----------------------------------------------- environment strata (guided) --
+// Guided shading may draw environment NEE from two strata (FrameConsts::
+// envSplitA): the bright texels (a sun) with probability s and the rest with
+// 1 - s, each through its own alias table over the same texel weights.  A
+// texel then has the probability of sampleEnv times s / share or
+// (1 - s) / (1 - share), and so has the pdf of every direction in it: s =
+// share is sampleEnv's distribution.  s comes from per-cell statistics of the
+// shading point (neeSelect), and every path records the s of its last
+// vertex, so the MIS weight of a later environment hit uses the pdf that
+// vertex's NEE had.  Any s in (0, 1) keeps the estimator unbiased.
+__device__ __forceinline__ uint32_t envTexelOf(const FrameConsts& g, V3 d) {
+    float u, v;
+    dirToEquirect(d, g.envRotation, u, v);
+    const uint32_t col = min(uint32_t(u * float(g.envWidth)), g.envWidth - 1);
+    const uint32_t row = min(uint32_t(v * float(g.envHeight)), g.envHeight - 1);
+    return row * g.envWidth + col;
+}
+// pdf of stratified environment sampling with bright-stratum probability s
+// over the plain pdf, for direction d.
+__device__ __forceinline__ float envStrataRatio(const SceneView& sc, const FrameConsts& g, V3 d, float s) {
+    if (!(g.envSplitA > 0.0f) || !(g.flags & kFlagHasEnv)) return 1.0f;
+    const bool bright = sc.envStratum[envTexelOf(g, d)] != 0;
+    return bright ? s / g.envSplitA : (1.0f - s) / (1.0f - g.envSplitA);
+}
+// sampleEnv over the strata: the same two draws (the first chooses the
+// stratum and, rescaled, its entry; the second keeps the entry or takes its
+// alias).  stratum: 1 bright, 0 the rest.
+__device__ __forceinline__ bool sampleEnvStrata(const SceneView& sc, const FrameConsts& g, Rng& rng, float s,
+                                                V3& dir, V3& rad, float& pdf, int& stratum) {
+    const uint32_t w = g.envWidth, h = g.envHeight;
+    const float u1 = rng.next1();
+    const float u2 = rng.next1();
+    if (w == 0 || h == 0) return false;
+    const bool bright = u1 < s;
+    const float uu = bright ? u1 / s : (u1 - s) / (1.0f - s);
+    const uint32_t n = bright ? g.envCountA : g.envCountB;
+    const EnvStratumEntry* t = bright ? sc.envA : sc.envB;
+    uint32_t i = uint32_t(uu * float(n));
+    if (i >= n) i = n - 1;
+    const EnvStratumEntry e = t[i];
+    const uint32_t texel = u2 < e.prob ? e.texel : t[e.alias].texel;
+    const uint32_t row = texel / w, col = texel - row * w;
+    const float u = (float(col) + 0.5f) / float(w), v = (float(row) + 0.5f) / float(h);
+    dir = equirectToDir(u, v, g.envRotation);
+    rad = v3(envLookup(sc, g, u, v)) * g.envIntensity;
+    pdf = envPdf(sc, g, dir) * (bright ? s / g.envSplitA : (1.0f - s) / (1.0f - g.envSplitA));
+    stratum = bright ? 1 : 0;
+    return pdf > 1e-9f;
+}
+// Cell of a shading point for the stratum statistics.
+__device__ __forceinline__ uint32_t neeCell(const FrameConsts& g, V3 pos, V3 n) {
+    const V3 q = (pos - v3(g.guideBoxMin)) * v3(g.guideBoxScale);
+    const int G = int(kNeeGrid);
+    const int ix = min(max(int(q.x * float(G)), 0), G - 1);
+    const int iy = min(max(int(q.y * float(G)), 0), G - 1);
+    const int iz = min(max(int(q.z * float(G)), 0), G - 1);
+    const float ax = fabsf(n.x), ay = fabsf(n.y), az = fabsf(n.z);
+    const int axis = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+    const float c = axis == 0 ? n.x : (axis == 1 ? n.y : n.z);
+    return uint32_t(((ix * G + iy) * G + iz) * 6 + axis * 2 + (c < 0.0f ? 1 : 0));
+}
+// Bright-stratum probability of a cell: the variance-minimising choice of a
+// one-sample estimator over two strata, s = sqrt(M_A) / (sqrt(M_A) +
+// sqrt(M_B)) with M the mean squared stratum-conditional contribution, kept
+// in [0.05, 0.95]; the plain share until both strata have 16 samples.
+__device__ __forceinline__ float neeSelect(const float4* __restrict__ stats, uint32_t cell, float share) {
+    const float4 c = stats[cell];
+    if (c.y < 16.0f || c.w < 16.0f) return share;
+    const float a = sqrtf(c.x / c.y), b = sqrtf(c.z / c.w);
+    if (!(a + b > 0.0f)) return share;
+    return clampf(a / (a + b), 0.05f, 0.95f);
+}
+
 // ------------------------------------------------------------------- lights --
 struct LightSample {
     V3 wi;
@@ -229,17 +302,21 @@ struct LightSample {
     float dist;
     bool delta;
     bool valid;
+    int stratum;  // environment samples of sampleEnvStrata: 1 bright, 0 the rest; -1 otherwise
 };
 
 // Uniform pick over the light list (environment at slot 0 when present),
-// lights.glsl:70-134.
+// lights.glsl:70-134.  STRATA: the environment is drawn over its strata with
+// bright-stratum probability s (guided shading).
+template <bool STRATA = false>
 __device__ __forceinline__ LightSample sampleOneLight(const SceneView& sc, const FrameConsts& g, V3 p,
-                                                      Rng& rng) {
+                                                      Rng& rng, float s = 0.0f) {
     LightSample ls;
     ls.valid = false;
     ls.delta = false;
     ls.pdf = 0.0f;
     ls.dist = 0.0f;
+    ls.stratum = -1;
     if (g.lightCount == 0) return ls;
     uint32_t idx = uint32_t(rng.next1() * float(g.lightCount));
     if (idx > g.lightCount - 1) idx = g.lightCount - 1;
@@ -248,7 +325,11 @@ __device__ __forceinline__ LightSample sampleOneLight(const SceneView& sc, const
     if (l.type == kLightEnv) {
         V3 dir, rad;
         float pdf;
-        if (!sampleEnv(sc, g, rng, dir, rad, pdf)) return ls;
+        if (STRATA) {
+            if (!sampleEnvStrata(sc, g, rng, s, dir, rad, pdf, ls.stratum)) return ls;
+        } else if (!sampleEnv(sc, g, rng, dir, rad, pdf)) {
+            return ls;
+        }
         ls.wi = dir;
         ls.radiance = rad;
         ls.pdf = pdf * pick;

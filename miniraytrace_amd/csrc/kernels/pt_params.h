@@ -27,6 +27,9 @@ struct SceneView {
     const GpuLight* lights;
     const float4* envMap;            // envWidth x envHeight RGBA32F, row 0 = top (theta = 0)
     const AliasEntry* envAlias;      // envWidth * envHeight entries
+    const uint8_t* envStratum;       // per texel: 1 in the bright stratum
+    const EnvStratumEntry* envA;     // FrameConsts::envCountA entries
+    const EnvStratumEntry* envB;     // FrameConsts::envCountB entries
     const GpuTexture* texTable;      // 64 slots
     const uint32_t* tex8;            // RGBA8 texel pool
     const float4* tex32;             // RGBA32F texel pool
@@ -64,6 +67,10 @@ struct PathView {
     float4* guideRec;                // 3 float4 per guided row: (pos01), (n), (wo)
     float4* guideMix;                // 12 float4 per guided row: decoded mixture
     float4* pathRec;                 // [level][path] x 4 float4 training records
+    // Learned environment-stratum selection (null when guided sampling is off).
+    float* neeSplit;                 // per path: bright-stratum probability at its last vertex
+    uint2* shadowX;                  // per shadow slot: (cell << 1 | stratum, contribution bits) or ~0u
+    float4* neeStats;                // kNeeCells cells
 };
 
 // One request of a batched frame (requests keep their own queues, counters

@@ -18,6 +18,7 @@ Rendering:
 * guided sampling is unbiased (block means agree with plain within noise);
 * guided sampling lowers the error at equal sample count on a room lit
   through a window (against a plain reference render);
+* guided environment NEE learns its per-cell stratum probabilities;
 * at 1280x720 both Adam steps of a frame train on full 65,536-record
   batches.
 """
@@ -221,7 +222,10 @@ def test_sampling_matches_oracle(gpu):
 
 
 # ------------------------------------------------------------------ rendering --
-def _room_engine(w, h, spp_limit, seed=1, depth=8):
+def _room_engine(w, h, spp_limit, seed=1, depth=8, sky=256):
+    """The window-lit room at reduced resolution.  sky=256: a 256 x 128 sky,
+    whose texel centres miss the 0.5 deg sun disc (no bright stratum);
+    sky=2048: the full 2048 x 1024 sky with the sun in a few texels."""
     from miniraytrace_amd import scenes
     from miniraytrace_amd.engine import Engine
     e = Engine(width=w, height=h)
@@ -229,7 +233,7 @@ def _room_engine(w, h, spp_limit, seed=1, depth=8):
                           denoise=False)
     cfg = dict(scenes.CONFIG4)
     cfg.update(n=24)
-    scenes.load_room_scene(e, cfg, seed=seed, sky_img=scenes.sky(256, 128))
+    scenes.load_room_scene(e, cfg, seed=seed, sky_img=scenes.sky(sky, sky // 2))
     e.commit()
     return e
 
@@ -254,16 +258,18 @@ def test_training_only_is_plain(gpu):
     assert np.isfinite(st["lastLoss"])
 
 
-@pytest.mark.parametrize("depth", [1, 8])
-def test_guided_is_unbiased(gpu, depth):
+@pytest.mark.parametrize("depth,sky", [(1, 256), (8, 256), (1, 2048), (8, 2048)])
+def test_guided_is_unbiased(gpu, depth, sky):
     """Guided and plain tracing converge to the same image: per-block means of
     independent batches agree within noise.  Depth 1 makes the last bounce's
     NEE weight matter (plain weights NEE against the BSDF pdf there; a
-    guided tracer that used its mixture pdf converged 1.2 % darker)."""
-    _print(f"case: guided vs plain block means, depth {depth}")
+    guided tracer that used its mixture pdf converged 1.2 % darker).  The
+    2048 x 1024 sky holds the sun, so guided environment NEE draws over
+    learned strata there."""
+    _print(f"case: guided vs plain block means, depth {depth}, sky {sky}")
     W, H, F, B = 96, 54, 64, 8
-    plain = _room_engine(W, H, 10**6, depth=depth)
-    guided = _room_engine(W, H, 10**6, depth=depth)
+    plain = _room_engine(W, H, 10**6, depth=depth, sky=sky)
+    guided = _room_engine(W, H, 10**6, depth=depth, sky=sky)
     guided.set_guiding(sampling=True, training=True, batch_size=4096, steps_per_frame=2)
     _render(guided, 32, W, H)  # train a little first
     imgs = {"plain": [], "guided": []}
@@ -314,6 +320,34 @@ def test_guided_lowers_error(gpu):
     _print(f"MSE at {N} spp: plain {mse_p:.4e}, guided {mse_g:.4e}, ratio {mse_g / mse_p:.3f}; "
            f"stats {guided.guiding_stats()}")
     assert mse_g < 0.95 * mse_p
+
+
+def test_environment_strata_are_learned(gpu):
+    """Guided environment NEE over two strata (the sun's texels, the rest):
+    the per-cell bright-stratum probabilities are learned from the shadow
+    rays' outcomes and move away from the plain share, and a plain engine
+    keeps none (its images stay the reference's).  Unbiasedness of the
+    resulting estimator is test_guided_is_unbiased's (the same room)."""
+    from miniraytrace_amd import scenes
+    _print("case: learned environment strata")
+    W, H = 96, 54
+    plain = _room_engine(W, H, 10**6, sky=2048)
+    _render(plain, 4, W, H)
+    ps = plain.guiding_stats()
+    assert ps["envCellsLearned"] == 0
+    guided = _room_engine(W, H, 10**6, sky=2048)
+    guided.set_guiding(sampling=True, training=True, batch_size=4096, steps_per_frame=2)
+    _render(guided, 48, W, H)
+    st = guided.guiding_stats()
+    _print("stats:", st)
+    share = st["envBrightShare"]
+    assert 0.0 < share < 1.0
+    assert st["envCellsLearned"] > 100
+    assert abs(st["envMeanSplit"] - share) > 0.02
+    # a scene edit (the environment's intensity) clears the statistics
+    guided.set_environment(scenes.sky(2048, 1024), 0.0, 0.9)
+    guided.commit()
+    assert guided.guiding_stats()["envCellsLearned"] == 0
 
 
 def test_training_batches_are_full_at_720p(gpu):
