@@ -944,6 +944,7 @@ void Engine::allocGuide() {
     dPathRec_.zero(stream_);  // stamp 0 never matches a frame
     dGuidePool_.allocate(size_t(poolCap) * 4 * 16);
     dNeeSplit_.allocate(size_t(n) * 4);
+    dRrScale_.allocate(size_t(n) * 4);
     dShX_.allocate(size_t(n) * 8);
     dNeeStats_.allocate(size_t(kNeeCells) * 16);
     dNeeStats_.zero(stream_);
@@ -1015,6 +1016,7 @@ dev::PathView Engine::pathView() const {
     p.neeSplit = nee ? dNeeSplit_.as<float>() : nullptr;
     p.shadowX = nee ? dShX_.as<uint2>() : nullptr;
     p.neeStats = nee ? dNeeStats_.as<float4>() : nullptr;
+    p.rrScale = guide && guide_.enableSampling ? dRrScale_.as<float>() : nullptr;
     return p;
 }
 
@@ -1170,11 +1172,13 @@ void Engine::enqueueGuideInfer(hipStream_t s, uint32_t bounce) {
 }
 
 // Vertices the tail kernel shades are not guided, so a guided request hands
-// a level over only once it holds at most a quarter of its pixels (besides
-// the batch-wide tailPaths bound).  Measured on the 1280x720 room: 1/4 gave
-// the best equal-RMSE time (0.74 x plain's), ahead of 1/8 (0.79), 1/16 and
-// 1/32 (0.81, fewer cheap tail levels) and 1/2 (0.77, too few guided ones).
-uint32_t Engine::tailAt() const { return guideFrame() ? frameW_ * frameH_ / 4 : ~0u; }
+// a level over only once it holds at most a sixteenth of its pixels (besides
+// the batch-wide tailPaths bound).  Measured on the 1280x720 room after 256
+// training frames (per-sample variance guided / plain, ms per frame): 1/4
+// 0.383 at 6.99 ms, 1/16 0.220 at 7.30, 1/64 0.219 at 7.38, never 0.217 at
+// 7.38.  The deep levels pay since roulette reads the BSDF-only throughput
+// (PathView::rrScale).
+uint32_t Engine::tailAt() const { return guideFrame() ? frameW_ * frameH_ / 16 : ~0u; }
 
 dev::TraceShape Engine::traceShape(bool guidePrep) const {
     dev::TraceShape t = extendShape_;

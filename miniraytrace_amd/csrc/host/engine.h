@@ -236,6 +236,7 @@ private:
     std::unique_ptr<GuideNetwork> guideNet_;
     DeviceBuffer dGuideIdx_, dGuideRec_, dGuideMix_, dPathRec_, dGuidePool_;
     DeviceBuffer dNeeSplit_, dShX_, dNeeStats_;  // learned environment-stratum selection
+    DeviceBuffer dRrScale_;                      // guided paths: roulette throughput scale
     uint32_t guidePaths_ = 0, guideLevels_ = 0;
     uint32_t guideTrainStride_ = 1;  // every guideTrainStride_-th path writes training records
     uint32_t guideStamp_ = 0;

@@ -139,6 +139,7 @@ __device__ __forceinline__ void generatePixel(const PathView& pv, const FrameCon
     pv.beta[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     pv.radiance[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     pv.rngCounter[p] = rng.counter;
+    if (pv.rrScale) pv.rrScale[p] = 1.0f;
 }
 
 // ------------------------------------------------------------ batch helpers --
@@ -459,12 +460,22 @@ __device__ __forceinline__ ShadeOut shadePath(const SceneView& sc, const PathVie
                     beta = beta * f * (fabsf(wiL.z) / pdf);
                     prevPdf = pdf;
                     if (GUIDE) L4.w = pBsdf;  // BSDF pdf of this continuation
+                    // Guided engines: roulette on the throughput of BSDF
+                    // sampling alone.  The guided throughput is small exactly
+                    // on paths drawn towards bright regions (high mixture
+                    // pdf); rouletting on it would end those paths most often.
+                    // Any survival probability in (0, 1] keeps the estimator
+                    // unbiased.  Plain engines have no rrScale.
+                    float rrS = 1.0f;
+                    if (pv.rrScale) rrS = clampf(pv.rrScale[p] * (pdf / fmaxf(pBsdf, 1e-20f)), 1e-6f, 1e6f);
                     bool alive = true;
                     if (bounce >= 3) {
-                        const float q = clampf(maxComp(beta), 0.05f, 0.95f);
+                        const float q = pv.rrScale ? clampf(maxComp(beta) * rrS, 0.05f, 0.95f)
+                                                   : clampf(maxComp(beta), 0.05f, 0.95f);
                         if (rng.next1() >= q) alive = false;
                         else beta = beta * (1.0f / q);
                     }
+                    if (pv.rrScale) pv.rrScale[p] = rrS;
                     if (alive) {
                         o.newO = s.pos + s.nGeom * (wiL.z > 0.0f ? kEpsRay : -kEpsRay);
                         o.newD = wiW;

@@ -71,6 +71,11 @@ struct PathView {
     float* neeSplit;                 // per path: bright-stratum probability at its last vertex
     uint2* shadowX;                  // per shadow slot: (cell << 1 | stratum, contribution bits) or ~0u
     float4* neeStats;                // kNeeCells cells
+    // Guided sampling (null when off): per path, the product over its
+    // vertices of (pdf the continuation was drawn with) / (its BSDF pdf), so
+    // beta * rrScale is the throughput BSDF sampling alone would carry.
+    // Russian roulette reads that throughput.
+    float* rrScale;
 };
 
 // One request of a batched frame (requests keep their own queues, counters
