@@ -275,7 +275,7 @@ typedef struct mrtxGuidingOptions {
     float    selectProb;      /* probability of a mixture draw at a guided vertex (0.5) */
     float    learningRate;    /* Adam step size (1e-3) */
     uint32_t batchSize;       /* training records per Adam step (65,536) */
-    uint32_t stepsPerFrame;   /* Adam steps per rendered frame (2) */
+    uint32_t stepsPerFrame;   /* Adam steps per rendered frame (4) */
     float    minRoughness;    /* less rough or transmissive vertices use the BSDF only (0.2) */
     uint32_t seed;            /* network initialisation seed (used when the network is created) */
     float    positionScale;   /* network position input = (p - scene box min) / largest box extent
@@ -339,7 +339,7 @@ MRT_API mrtResult mrtxGuideNetworkTrain(mrtxGuideNetwork* net, const float* reco
  * stream; another stream is ordered against the network's other calls by the
  * caller).  It calls no hipMalloc, hipFree or other allocating routine: the
  * training workspace lives in one fixed device array per GPU, sized for
- * MRTX_GUIDE_NOALLOC_MAX_RECORDS records (about 13 MB, held until the process
+ * MRTX_GUIDE_NOALLOC_MAX_RECORDS records (about 12 MB, held until the process
  * ends); more records return MRT_ERROR_INVALID_ARGUMENT.  Every network on a
  * GPU shares that GPU's array, so calls that overlap on one GPU (other
  * streams or threads) race.  loss may be NULL; when given, the call waits for

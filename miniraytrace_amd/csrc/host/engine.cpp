@@ -149,7 +149,7 @@ Engine::Engine(const mrtEngineDesc& desc) {
     guide_.selectProb = 0.5f;
     guide_.learningRate = 1e-3f;
     guide_.batchSize = 65536;
-    guide_.stepsPerFrame = 2;
+    guide_.stepsPerFrame = 4;
     guide_.minRoughness = 0.2f;
     guide_.seed = 1;
     guide_.positionScale = 1.0f;
@@ -942,6 +942,7 @@ void Engine::allocGuide() {
     const uint32_t trainPaths = (n + guideTrainStride_ - 1) / guideTrainStride_;
     dPathRec_.allocate(size_t(levels) * trainPaths * 4 * 16);
     dPathRec_.zero(stream_);  // stamp 0 never matches a frame
+    dRecMask_.allocate(size_t(trainPaths) * 4);  // reset per frame by ray generation
     dGuidePool_.allocate(size_t(poolCap) * 4 * 16);
     dNeeSplit_.allocate(size_t(n) * 4);
     dRrScale_.allocate(size_t(n) * 4);
@@ -1012,6 +1013,7 @@ dev::PathView Engine::pathView() const {
     p.guideRec = guide ? dGuideRec_.as<float4>() : nullptr;
     p.guideMix = guide ? dGuideMix_.as<float4>() : nullptr;
     p.pathRec = guide && guide_.enableTraining ? dPathRec_.as<float4>() : nullptr;
+    p.recMask = p.pathRec ? dRecMask_.as<uint32_t>() : nullptr;
     const bool nee = guide && guide_.enableSampling && envSplitA_ > 0.0f;
     p.neeSplit = nee ? dNeeSplit_.as<float>() : nullptr;
     p.shadowX = nee ? dShX_.as<uint2>() : nullptr;
@@ -1194,6 +1196,7 @@ void Engine::enqueueGuideTraining(hipStream_t s, const FrameConsts* g) {
     const uint32_t poolCap = guide_.batchSize * steps;
     dev::GuideCollectArgs ca;
     ca.pathRec = dPathRec_.as<const float4>();
+    ca.recMask = dRecMask_.as<const uint32_t>();
     ca.radiance = dRad_.as<const float4>();
     ca.paths = guidePaths_;
     ca.trainPaths = (guidePaths_ + guideTrainStride_ - 1) / guideTrainStride_;
@@ -1214,8 +1217,7 @@ void Engine::enqueueGuideTraining(hipStream_t s, const FrameConsts* g) {
         b.phase = t;
         b.period = steps;
         b.cap = guide_.batchSize;
-        MRT_HIP(dev::launchGuideGradient(b, net, true, s));
-        MRT_HIP(dev::launchGuideAdam(net, h, s));
+        MRT_HIP(dev::launchGuideStep(b, net, h, s));
     }
 }
 
@@ -1445,7 +1447,7 @@ void Engine::setGuiding(const mrtxGuidingOptions& in) {
     if (o.selectProb == 0.0f) o.selectProb = 0.5f;
     if (o.learningRate == 0.0f) o.learningRate = 1e-3f;
     if (o.batchSize == 0) o.batchSize = 65536;
-    if (o.stepsPerFrame == 0) o.stepsPerFrame = 2;
+    if (o.stepsPerFrame == 0) o.stepsPerFrame = 4;
     if (o.positionScale == 0.0f) o.positionScale = 1.0f;
     if (!(o.positionScale > 0.0f) || !std::isfinite(o.positionScale))
         throw Error(MRT_ERROR_INVALID_ARGUMENT, "guiding positionScale must be > 0");

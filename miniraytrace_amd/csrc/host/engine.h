@@ -234,7 +234,7 @@ private:
     // path guiding
     mrtxGuidingOptions guide_{};
     std::unique_ptr<GuideNetwork> guideNet_;
-    DeviceBuffer dGuideIdx_, dGuideRec_, dGuideMix_, dPathRec_, dGuidePool_;
+    DeviceBuffer dGuideIdx_, dGuideRec_, dGuideMix_, dPathRec_, dRecMask_, dGuidePool_;
     DeviceBuffer dNeeSplit_, dShX_, dNeeStats_;  // learned environment-stratum selection
     DeviceBuffer dRrScale_;                      // guided paths: roulette throughput scale
     uint32_t guidePaths_ = 0, guideLevels_ = 0;

@@ -22,32 +22,33 @@ GuideNetwork::GuideNetwork(uint32_t cap) {
     m_.allocate(kParamBytes);
     v_.allocate(kParamBytes);
     grad_.allocate(kParamBytes);
-    image_.allocate(dev::kGuideImageBytes);
+    image_.allocate(dev::kGuideImageBytes + dev::kGuideImageTBytes);
     words_.allocate(dev::kGwWords * sizeof(uint32_t));
     grad_.zero();
     words_.zero();
     params_.zero();
     allocWorkspace(cap);
     resetAdam();
-    MRT_HIP(dev::launchGuidePack(params_.as<float>(), image_.get(), nullptr));
+    MRT_HIP(dev::launchGuidePack(params_.as<float>(), image_.get(), imageT(), nullptr));
     MRT_HIP(hipDeviceSynchronize());
 }
 
-// Feature-major workspace, rows padded to a multiple of 32.  The input
-// block has 96 feature rows so the weight-gradient tiles of layer 0 read
-// zeros past feature 79 (rows 80..95 are never written).
+// Row-block-tiled workspace (dev::GuideNet), rows padded to a multiple of
+// 32.  The input block has 96 features so the weight-gradient tiles of
+// layer 0 cover it with three 32-feature tiles; the training kernel writes
+// features 80..95 as zeros.
 void GuideNetwork::allocWorkspace(uint32_t cap) {
     if (cap == 0) throw Error(MRT_ERROR_INVALID_ARGUMENT, "guide batch capacity must be > 0");
     const uint32_t stride = dev::guideRowStride(cap);
     actIn_.allocate(size_t(96) * stride * sizeof(uint16_t));
     actIn_.zero();
     act_.allocate(size_t(4) * 64 * stride * sizeof(uint16_t));
-    raw_.allocate(size_t(dev::kGuideOutputs) * stride * sizeof(float));
+    tile_.allocate(size_t(stride / 32) * 2 * sizeof(float));
     // one delta block per layer, so that one launch forms every layer's
     // weight gradient after the backward pass
     for (DeviceBuffer& d : delta_) d.allocate(size_t(64) * stride * sizeof(uint16_t));
-    // weight-gradient partials, then the per-chunk weight sums
-    partial_.allocate(size_t(dev::guideGradChunks(cap)) * (dev::kGuideParamCount + 1) * sizeof(float));
+    // weight-gradient partials, then the per-chunk weight and loss sums
+    partial_.allocate(size_t(dev::guideGradChunks(cap)) * (dev::kGuideParamCount + 2) * sizeof(float));
     MRT_HIP(hipStreamSynchronize(nullptr));
     cap_ = cap;
     stride_ = stride;
@@ -72,7 +73,7 @@ void GuideNetwork::initParams(uint32_t seed) {
 
 void GuideNetwork::setParams(const float* src) {
     params_.upload(src, kParamBytes);
-    MRT_HIP(dev::launchGuidePack(params_.as<float>(), image_.get(), nullptr));
+    MRT_HIP(dev::launchGuidePack(params_.as<float>(), image_.get(), imageT(), nullptr));
     resetAdam();
 }
 
@@ -103,9 +104,10 @@ dev::GuideNet GuideNetwork::view() const {
     n.adamV = v_.as<float>();
     n.grad = grad_.as<float>();
     n.image = image_.get();
+    n.imageT = imageT();
     n.actIn = actIn_.as<uint16_t>();
     n.act = act_.as<uint16_t>();
-    n.raw = raw_.as<float>();
+    n.tilePart = tile_.as<float>();
     for (int l = 0; l < 5; ++l) n.delta[l] = delta_[l].as<uint16_t>();
     n.partial = partial_.as<float>();
     n.weightPart = n.partial + size_t(dev::guideGradChunks(cap_)) * dev::kGuideParamCount;

@@ -30,6 +30,7 @@ public:
 
     dev::GuideNet view() const;
     const void* deviceImage() const { return image_.get(); }
+    void* imageT() const { return static_cast<char*>(image_.get()) + dev::kGuideImageBytes; }
     uint32_t* deviceWords() const { return words_.as<uint32_t>(); }
     float* deviceGrad() const { return grad_.as<float>(); }
 
@@ -38,7 +39,7 @@ private:
 
     uint32_t cap_ = 0, stride_ = 0;
     DeviceBuffer params_, m_, v_, grad_, image_, words_;
-    DeviceBuffer actIn_, act_, raw_, delta_[5], partial_;
+    DeviceBuffer actIn_, act_, tile_, delta_[5], partial_;
 };
 
 } // namespace mrt

@@ -227,8 +227,7 @@ MRT_API mrtResult mrtxGuideNetworkTrain(mrtxGuideNetwork* n, const float* record
         const mrt::dev::GuideNet net = n->net->view();
         mrt::dev::GuideAdam h;
         h.lr = learningRate;
-        MRT_HIP(mrt::dev::launchGuideGradient(wholeBatch(n, count), net, true, n->stream));
-        MRT_HIP(mrt::dev::launchGuideAdam(net, h, n->stream));
+        MRT_HIP(mrt::dev::launchGuideStep(wholeBatch(n, count), net, h, n->stream));
         uint32_t w[mrt::dev::kGwWords];
         n->net->readWords(w, n->stream);
         if (loss) *loss = wordAsFloat(w, mrt::dev::kGwLastLoss);

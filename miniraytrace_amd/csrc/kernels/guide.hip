@@ -15,21 +15,20 @@
 // Inference keeps one wave's 32 rows in registers between layers (lane-half
 // swaps turn a layer's outputs into the next layer's operands); the training
 // forward passes them through LDS (row-major), from where it also stores
-// them.  Training stores activations and deltas feature-major ([feature][row],
-// rows padded to 32) so every operand of the backward and weight-gradient
-// products is a contiguous 16-byte fragment in HBM.
+// them.  Training stores inputs, activations and deltas row-block-tiled
+// (storeTiled) so every operand load of the weight-gradient products is one
+// contiguous kilobyte in HBM.
 //
 // Kernels:
-//   kGuidePack        FP32 params -> packed bf16 forward image
+//   kGuidePack        FP32 params -> packed bf16 forward image and transposed image
 //   kGuideInfer       fused forward for rendering (features -> mixture)
-//   kGuideStepBegin   batch size, counters and per-chunk weight sums of one step
-//   kGuideFwdTrain    forward over the batch, keeping the activations
-//   kGuideOutGrad     d(weighted NLL)/d(raw outputs) per row
-//   kGuideBackward    delta_{l-1} = (W_l^T delta_l) * relu'(a_{l-1}), l = 4 .. 1
+//   kGuideTrainRows   per 32-row tile of a training batch: encoding, forward,
+//                     d(weighted NLL)/d(raw outputs) and the backward pass
+//                     delta_{l-1} = (W_l^T delta_l) * relu'(a_{l-1}), l = 4 .. 1
 //   kGuideWeightGrad  per row chunk, all layers in one launch after the
 //                     backward pass: grad W_l = delta_l a_{l-1}^T, grad b_l
-//   kGuideGradReduce  sum of the chunk partials (fixed order, no atomics)
-//   kGuideAdam        Adam update of the FP32 master weights
+//   kGuideReduceStep  sum of the chunk partials (fixed order, no atomics),
+//                     then the Adam update and the bf16 copies of the weights
 //   kGuideCollect     path records -> weighted training records
 // All counts are read from device memory, so a captured graph replays the
 // same launches whatever the batch sizes turn out to be.
@@ -120,9 +119,25 @@ constexpr int kActStride = 88;
 constexpr int kRawStride = 44;
 static_assert(kActStride >= kGuideInPad && kRawStride >= kGuideOut && kRawStride * 4 <= kActStride * 2, "row buffers");
 
-__global__ __launch_bounds__(256) void kGuidePack(const float* __restrict__ params, uint16_t* __restrict__ image) {
+// Transposed image: W_l^T of layers 1..4 (inputs x outputs, the output
+// layer's 40 outputs padded with zeros) at the forward image's hidden row
+// stride, the A operand of the backward products.
+constexpr int kWTTotal = 4 * 64 * kWRow;  // 18,432 bf16
+constexpr int kImageTU4 = kWTTotal * 2 / 16;
+static_assert(kWTTotal * 2 == int(kGuideImageTBytes), "transposed image size");
+__host__ __device__ constexpr int wtOff(int l) { return (l - 1) * 64 * kWRow; }
+
+__global__ __launch_bounds__(256) void kGuidePack(const float* __restrict__ params, uint16_t* __restrict__ image,
+                                                  uint16_t* __restrict__ imageT) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < kWTotal) {
+    if (i >= kWTotal + 320) {
+        const int t = i - (kWTotal + 320);
+        if (t >= kWTTotal) return;
+        const int l = 1 + t / (64 * kWRow), in = (t % (64 * kWRow)) / kWRow, o = t % kWRow;
+        float v = 0.0f;
+        if (o < guideLayerOut(l) && in < guideLayerIn(l)) v = params[guideWeightOffset(l) + o * guideLayerIn(l) + in];
+        imageT[t] = toBf16(v);
+    } else if (i < kWTotal) {
         const int l = i < kWOff1 ? 0 : 1 + (i - kWOff1) / (64 * kWRow);
         const int r = (i - wOff(l)) / wStride(l), c = (i - wOff(l)) % wStride(l);
         float v = 0.0f;
@@ -282,16 +297,24 @@ __device__ __forceinline__ void stashOutputs(const f32x16& c0, const f32x16& c1,
     waveHandoff();
 }
 
-// Feature-major copy of the wave's 32 rows: features [0, nfeat) of act to
-// dst[f * stride + row0 + 0..31] (64 contiguous bytes per feature).
-__device__ __forceinline__ void storeFeatureMajor(const uint16_t* act, int nfeat, uint16_t* __restrict__ dst,
-                                                  uint32_t stride, uint32_t row0) {
+// Copy of the wave's 32 rows into a row-block-tiled array of F features:
+// element (f, row) at ((row / 16) F + f) 16 + row % 16, so the 16 rows of
+// one feature in a block are 32 contiguous bytes and one MFMA operand load
+// of the weight gradient (32 features x 16 rows) is one contiguous
+// kilobyte.  Features [0, nfeat) come from act (row-major, stride
+// kActStride); features [nfeat, F) are written as zeros.
+__device__ __forceinline__ void storeTiled(const uint16_t* act, int nfeat, int F, uint16_t* __restrict__ dst,
+                                           uint32_t row0) {
     const int lane = threadIdx.x & 63;
-    for (int f = lane; f < nfeat; f += 64) {
+    for (int f = lane; f < F; f += 64) {
         uint32_t w[16];
-        for (int m = 0; m < 16; ++m) w[m] = pack2(act[(2 * m) * kActStride + f], act[(2 * m + 1) * kActStride + f]);
-        u32x4* d = reinterpret_cast<u32x4*>(dst + size_t(f) * stride + row0);
-        for (int q = 0; q < 4; ++q) d[q] = u32x4{ w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3] };
+        for (int m = 0; m < 16; ++m)
+            w[m] = f < nfeat ? pack2(act[(2 * m) * kActStride + f], act[(2 * m + 1) * kActStride + f]) : 0u;
+        for (int blk = 0; blk < 2; ++blk) {
+            u32x4* d = reinterpret_cast<u32x4*>(dst + (size_t(row0 / 16 + blk) * F + f) * 16);
+            d[0] = u32x4{ w[8 * blk], w[8 * blk + 1], w[8 * blk + 2], w[8 * blk + 3] };
+            d[1] = u32x4{ w[8 * blk + 4], w[8 * blk + 5], w[8 * blk + 6], w[8 * blk + 7] };
+        }
     }
 }
 
@@ -459,352 +482,431 @@ __global__ __launch_bounds__(kFwdWaves * 64) void kGuideInfer(RecView rv, const 
 }
 
 // ------------------------------------------------------------------- training --
-// Start of a training step.  Block 0 publishes the batch size, clears the
-// loss and advances the Adam step counter (countStep, non-empty batch); the
-// later kernels of the step read them.  Every block sums the weights of its
-// row chunk in a fixed order: block k sums chunk k (kGuideGradChunk rows, four
-// per thread, then an LDS tree) into weightPart[k], and kGuideOutGrad adds
-// the chunk sums in index order.  The sum scales every delta before its bf16
-// rounding, so an order-dependent sum would make the gradient differ from run
-// to run.
-constexpr int kWeightSumBlock = 256;
-__global__ __launch_bounds__(kWeightSumBlock) void kGuideStepBegin(GuideBatch b, uint32_t* __restrict__ words,
-                                                                   float* __restrict__ weightPart, int countStep) {
-    __shared__ float part[kWeightSumBlock];
-    const uint32_t n = batchCount(b);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        words[kGwBatch] = n;
-        *wordF(words, kGwLoss) = 0.0f;
-        if (countStep && n > 0) words[kGwStep] += 1u;
-    }
-    const RecView rv = batchView(b);
-    const uint32_t row0 = blockIdx.x * kGuideGradChunk + threadIdx.x;
-    float s = 0.0f;
-    for (uint32_t i = 0; i < kGuideGradChunk / kWeightSumBlock; ++i) {
-        const uint32_t row = row0 + i * kWeightSumBlock;
-        if (row < n) s += recRow(rv, row)[0].w;
-    }
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int half = kWeightSumBlock / 2; half > 0; half >>= 1) {
-        if (threadIdx.x < half) part[threadIdx.x] += part[threadIdx.x + half];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) weightPart[blockIdx.x] = part[0];
+// One Adam step is three launches:
+//   kGuideTrainRows   per 32-row tile, one wave: encoding, forward, output
+//                     gradient and the backward pass through the hidden
+//                     layers; stores the inputs, activations and deltas
+//                     feature-major for the weight gradient, and the tile's
+//                     weight and loss sums
+//   kGuideWeightGrad  weight gradients of all five layers, per row chunk
+//   kGuideReduceStep  chunk partials summed in chunk order and divided by the
+//                     batch weight; with Adam, the update of the FP32 master
+//                     weights and of their bf16 copies in both images
+// The deltas are derivatives of the unnormalised sum_j w_j (-log q(wi_j)),
+// rounded to bf16; the gradient is divided by the batch's weight sum once,
+// after the reduction.  Every sum runs in a fixed order, so a step is
+// bit-reproducible.
+
+__device__ __forceinline__ void loadImageT(const u32x4* __restrict__ imageT, uint16_t* WT) {
+    u32x4* w4 = reinterpret_cast<u32x4*>(WT);
+    for (int i = threadIdx.x; i < kImageTU4; i += blockDim.x) w4[i] = imageT[i];
 }
 
-// Sum of the chunk weight sums in index order, the same in every block
-// (threads add strided slices in order, then an LDS tree).
-__device__ float batchWeightSum(const float* __restrict__ weightPart, uint32_t chunks) {
-    __shared__ float part[256];
-    float s = 0.0f;
-    for (uint32_t i = threadIdx.x; i < chunks; i += 256) s += weightPart[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int half = 128; half > 0; half >>= 1) {
-        if (threadIdx.x < half) part[threadIdx.x] += part[threadIdx.x + half];
-        __syncthreads();
-    }
-    const float total = part[0];
-    __syncthreads();
-    return total;
+// A training row: the query (pos01 + weight, normal, wo) and the direction wi.
+struct TrainRow {
+    RowIn q;
+    float4 wi;
+};
+__device__ __forceinline__ TrainRow loadTrainRow(const RecView& rv, uint32_t row, uint32_t count) {
+    TrainRow t;
+    t.q = loadRow(rv, row, count);
+    t.wi = row < count ? recRow(rv, row)[3] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return t;
 }
 
-// Forward over the batch's padded rows [0, pad32(count)): keeps the encoded
-// inputs (features 0..79 of actIn), the hidden outputs and the raw outputs,
-// feature-major.  Padding rows encode to zero inputs (finite activations).
-__global__ __launch_bounds__(kFwdWaves * 64) void kGuideFwdTrain(GuideBatch b, const uint32_t* __restrict__ words,
-                                                                 const u32x4* __restrict__ image, GuideNet net) {
+// ReLU bits of the hidden outputs a lane holds after layerForward (which
+// stored bf16(max(c, 0))): bit 4 j + i for output 8 j + 4 h + i (c0), bit
+// 16 + 4 j + i for output 32 + 8 j + 4 h + i (c1), set where the stored
+// activation is > 0.
+__device__ __forceinline__ uint32_t reluBits(const f32x16& c0, const f32x16& c1) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+        m |= uint32_t(fromBf16(toBf16(fmaxf(c0[s], 0.0f))) > 0.0f) << s;
+        m |= uint32_t(fromBf16(toBf16(fmaxf(c1[s], 0.0f))) > 0.0f) << (16 + s);
+    }
+    return m;
+}
+
+// d/d raw of wgt (-log q(wi)) for lobes 4 h .. 4 h + 3 of the lane's row
+// (raw outputs 20 h .. 20 h + 19 into d), q the row's decoded mixture.
+// Lanes r and r + 32 hold the same row and share the softmax and the
+// log-sum-exp through shuffles, so every lane of the wave must call it.
+// Returns -wgt log q.
+__device__ __forceinline__ float outputGradHalf(const float* raw, int h, float wgt, V3 wi, float (&d)[20]) {
+    float mxl = -1e30f;
+    for (int k = 0; k < kGuideLobes; ++k) mxl = fmaxf(mxl, raw[5 * k + 4]);
+    float e[4], s = 0.0f;
+    for (int i = 0; i < 4; ++i) {
+        e[i] = expf(raw[5 * (4 * h + i) + 4] - mxl);
+        s += e[i];
+    }
+    const float other = __shfl_xor(s, 32, 64);
+    const float inv = 1.0f / (h == 0 ? s + other : other + s);  // lobes 0..3, then 4..7 (decodeMix's order)
+    V3 mu[4];
+    float kap[4], c[4], lf[4], w[4];
+    float mx = -1e30f;
+    for (int i = 0; i < 4; ++i) {
+        decodeLobe(raw, 4 * h + i, mu[i], kap[i]);
+        w[i] = e[i] * inv;
+        c[i] = dot(mu[i], wi);
+        lf[i] = logf(w[i]) + logf(kap[i]) - logf(kTwoPi) - logf(-expm1f(-2.0f * kap[i])) + kap[i] * (c[i] - 1.0f);
+        mx = fmaxf(mx, lf[i]);
+    }
+    mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
+    float se = 0.0f;
+    for (int i = 0; i < 4; ++i) se += expf(lf[i] - mx);
+    const float seo = __shfl_xor(se, 32, 64);
+    const float logq = mx + logf(h == 0 ? se + seo : seo + se);
+    for (int i = 0; i < 4; ++i) {
+        const int k = 4 * h + i;
+        const float resp = expf(lf[i] - logq);
+        const float lk = raw[5 * k + 3];
+        const bool inside = lk > kLogKappaLo && lk < kLogKappaHi;
+        const float dlogf = 1.0f / kap[i] - 2.0f / expm1f(2.0f * kap[i]) + (c[i] - 1.0f);
+        const V3 v = v3(raw[5 * k], raw[5 * k + 1], raw[5 * k + 2]);
+        const float vn = fmaxf(sqrtf(dot(v, v)), 1e-12f);
+        const float gmu = -wgt * resp * kap[i] / vn;
+        d[5 * i + 0] = gmu * (wi.x - c[i] * mu[i].x);
+        d[5 * i + 1] = gmu * (wi.y - c[i] * mu[i].y);
+        d[5 * i + 2] = gmu * (wi.z - c[i] * mu[i].z);
+        d[5 * i + 3] = inside ? -wgt * resp * dlogf * kap[i] : 0.0f;
+        d[5 * i + 4] = -wgt * (resp - w[i]);
+    }
+    return -wgt * logq;
+}
+
+// delta_{l-1} of the wave's 32 rows (l = 4 .. 1): layer l's inputs are the A
+// operand's rows (W_l^T from WT), the rows of delta_l in act (row-major)
+// the B operand, so lane (r, h) ends with inputs 8 j + 4 h + i (c0) and
+// 32 + 8 j + 4 h + i (c1) of row r: the layout reluBits recorded layer
+// l - 1's outputs in.  Masked, rounded to bf16 and written back into act.
+__device__ __forceinline__ void layerBackward(int l, uint16_t* act, const uint16_t* WT, uint32_t mask) {
+    const int lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
+    const uint16_t* Wl = WT + wtOff(l);
+    f32x16 c0 = zero16(), c1 = zero16();
+    waveHandoff();
+    for (int kb = 0; kb < 64; kb += 16) {
+        const bf16x8 x = ld8(act + r * kActStride + kb + 8 * h);
+        c0 = mfma(ld8(Wl + r * kWRow + kb + 8 * h), x, c0);
+        c1 = mfma(ld8(Wl + (32 + r) * kWRow + kb + 8 * h), x, c1);
+    }
+    auto masked4 = [mask](const f32x16& c, int j, int bit0) {
+        float v[4];
+        for (int i = 0; i < 4; ++i) v[i] = (mask >> (bit0 + i)) & 1u ? c[4 * j + i] : 0.0f;
+        return make_uint2(pack2f(v[0], v[1]), pack2f(v[2], v[3]));
+    };
+    for (int j = 0; j < 4; ++j) {
+        *reinterpret_cast<uint2*>(act + r * kActStride + 8 * j + 4 * h) = masked4(c0, j, 4 * j);
+        *reinterpret_cast<uint2*>(act + r * kActStride + 32 + 8 * j + 4 * h) = masked4(c1, j, 16 + 4 * j);
+    }
+    waveHandoff();
+}
+
+// Everything of a training step that is per row, one wave per 32-row tile
+// of the batch's padded rows [0, pad32(count)).  Block 0 publishes the batch
+// size and advances the Adam step counter (countStep, non-empty batch).
+// Padding rows encode to zero inputs and get zero weight and deltas.
+constexpr int kTrainWaves = 8;  // 47 KB forward image + 36 KB transposed image + 8 x 5.5 KB rows
+__global__ __launch_bounds__(kTrainWaves * 64) void kGuideTrainRows(GuideBatch b, GuideNet net, int countStep) {
     __shared__ __attribute__((aligned(16))) uint16_t W[kWTotal];
     __shared__ __attribute__((aligned(16))) float bias[5 * 64];
-    __shared__ __attribute__((aligned(16))) uint16_t actAll[kFwdWaves][32 * kActStride];
-    const uint32_t count = words[kGwBatch];
+    __shared__ __attribute__((aligned(16))) uint16_t WT[kWTTotal];
+    __shared__ __attribute__((aligned(16))) uint16_t actAll[kTrainWaves][32 * kActStride];
+    const uint32_t count = batchCount(b);
     const uint32_t rows = pad32(count);
-    if (blockIdx.x * kFwdWaves * 32 >= rows) return;  // uniform per block
-    loadImage(image, W, bias);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        net.words[kGwBatch] = count;
+        if (countStep && count > 0) net.words[kGwStep] += 1u;
+    }
+    if (blockIdx.x * kTrainWaves * 32 >= rows) return;  // uniform per block
+    loadImage(static_cast<const u32x4*>(net.image), W, bias);
+    loadImageT(static_cast<const u32x4*>(net.imageT), WT);
     __syncthreads();
     const RecView rv = batchView(b);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
     uint16_t* act = actAll[wave];
     const float* fo = reinterpret_cast<const float*>(act);
     const uint32_t stride = net.stride;
-    const uint32_t totalWaves = gridDim.x * kFwdWaves;
-    uint32_t blk = blockIdx.x * kFwdWaves + wave;
-    RowIn next = loadRow(rv, blk * 32 + (lane & 31), count);
+    const uint32_t totalWaves = gridDim.x * kTrainWaves;
+    uint32_t blk = blockIdx.x * kTrainWaves + wave;
+    TrainRow next = loadTrainRow(rv, blk * 32 + r, count);
     for (; blk * 32 < rows; blk += totalWaves) {
         const uint32_t row0 = blk * 32;
-        const RowIn cur = next;
-        next = loadRow(rv, (blk + totalWaves) * 32 + (lane & 31), count);
-        encodeRows(cur, row0 + (lane & 31) < count, act);
-        storeFeatureMajor(act, kGuideInPad, net.actIn, stride, row0);
+        const TrainRow cur = next;
+        next = loadTrainRow(rv, (blk + totalWaves) * 32 + r, count);
+        const bool valid = row0 + r < count;
+        encodeRows(cur.q, valid, act);
+        storeTiled(act, kGuideInPad, 96, net.actIn, row0);
+        uint32_t mask[4];
         f32x16 c0, c1;
         for (int l = 0; l < 5; ++l) {
             layerForward(l, act, W, bias, c0, c1);
-            if (l < 4) storeFeatureMajor(act, 64, net.act + size_t(l) * 64 * stride, stride, row0);
+            if (l < 4) {
+                mask[l] = reluBits(c0, c1);
+                storeTiled(act, 64, 64, net.act + size_t(l) * 64 * stride, row0);
+            }
         }
+        waveHandoff();  // layer 3's stores have read act
         stashOutputs(c0, c1, act);
-        if (lane < kGuideOut) {
-            float* d = net.raw + size_t(lane) * stride + row0;
-            for (int m = 0; m < 32; m += 4)
-                *reinterpret_cast<float4*>(d + m) = make_float4(fo[m * kRawStride + lane], fo[(m + 1) * kRawStride + lane],
-                                                                fo[(m + 2) * kRawStride + lane],
-                                                                fo[(m + 3) * kRawStride + lane]);
-        }
-    }
-}
-
-// delta (feature-major bf16, 64 features, 40..63 zero) = d L / d raw with
-// L = sum_j w_j (-log q(wi_j)) / sum_j w_j; padding rows get zero deltas.
-__global__ __launch_bounds__(256) void kGuideOutGrad(GuideBatch b, uint32_t* __restrict__ words, GuideNet net,
-                                                     uint32_t chunks) {
-    const uint32_t count = words[kGwBatch];
-    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t stride = net.stride;
-    const float wsum = batchWeightSum(net.weightPart, chunks);
-    if (blockIdx.x == 0 && threadIdx.x == 0) *wordF(words, kGwWeightSum) = wsum;
-    float loss = 0.0f;
-    if (row < pad32(count)) {
-        uint16_t* dcol = net.delta[4] + row;
-        if (row < count) {
-            const float scale = 1.0f / fmaxf(wsum, 1e-30f);
-            const float4* q = recRow(batchView(b), row);
-            const float wgt = q[0].w * scale;
-            const V3 wi = v3(q[3]);
-            float o[kGuideOut];
-            for (int k = 0; k < kGuideOut; ++k) o[k] = net.raw[size_t(k) * stride + row];
-            GuideMix m;
-            decodeMix(o, m);
-            float lf[kGuideLobes], mx = -1e30f;
-            for (int k = 0; k < kGuideLobes; ++k) {
-                const float c = dot(m.mu[k], wi);
-                lf[k] = logf(m.w[k]) + logf(m.kappa[k]) - logf(kTwoPi) - logf(-expm1f(-2.0f * m.kappa[k])) +
-                        m.kappa[k] * (c - 1.0f);
-                mx = fmaxf(mx, lf[k]);
-            }
-            float s = 0.0f;
-            for (int k = 0; k < kGuideLobes; ++k) s += expf(lf[k] - mx);
-            const float logq = mx + logf(s);
-            loss = -wgt * logq;
-            for (int k = 0; k < kGuideLobes; ++k) {
-                const float resp = expf(lf[k] - logq);
-                const float kap = m.kappa[k];
-                const float c = dot(m.mu[k], wi);
-                const float lk = o[5 * k + 3];
-                const bool inside = lk > kLogKappaLo && lk < kLogKappaHi;
-                const float dlogf = 1.0f / kap - 2.0f / expm1f(2.0f * kap) + (c - 1.0f);
-                const V3 v = v3(o[5 * k], o[5 * k + 1], o[5 * k + 2]);
-                const float vn = fmaxf(sqrtf(dot(v, v)), 1e-12f);
-                const float gmu = -wgt * resp * kap / vn;
-                dcol[size_t(5 * k + 0) * stride] = toBf16(gmu * (wi.x - c * m.mu[k].x));
-                dcol[size_t(5 * k + 1) * stride] = toBf16(gmu * (wi.y - c * m.mu[k].y));
-                dcol[size_t(5 * k + 2) * stride] = toBf16(gmu * (wi.z - c * m.mu[k].z));
-                dcol[size_t(5 * k + 3) * stride] = toBf16(inside ? -wgt * resp * dlogf * kap : 0.0f);
-                dcol[size_t(5 * k + 4) * stride] = toBf16(-wgt * (resp - m.w[k]));
-            }
-        } else {
-            for (int k = 0; k < kGuideOut; ++k) dcol[size_t(k) * stride] = 0;
-        }
-        for (int k = kGuideOut; k < 64; ++k) dcol[size_t(k) * stride] = 0;
-    }
-    for (int off = 32; off > 0; off >>= 1) loss += __shfl_down(loss, off, 64);
-    if ((threadIdx.x & 63) == 0 && loss != 0.0f) atomicAdd(wordF(words, kGwLoss), loss);
-}
-
-// Backward pass through the hidden layers in one launch: for l = 4 .. 1,
-// delta_{l-1}[in][row] = (sum_o W_l[o][in] delta_l[o][row]) * (a_{l-1}[in][row] > 0),
-// one wave per 32-row tile.  The four W_l^T are staged in LDS once per block
-// (WT[l-1][in][o]); the wave's tile of delta_4 is read from HBM and staged
-// transposed (sD[row][o]), and each following layer's tile is written
-// straight into sD from the accumulators, so both MFMA operands are 16-byte
-// LDS loads (144-byte row stride) and only delta_4 is read back.  Every
-// delta_{l-1} is also written out for the weight gradient.
-__global__ __launch_bounds__(256) void kGuideBackward(GuideNet net) {
-    // W^T rows at the forward image's hidden row stride (144 bytes): the 32
-    // rows of one operand load fall on 16 LDS bank offsets.
-    __shared__ __attribute__((aligned(16))) uint16_t WT[4][64 * kWRow];
-    __shared__ __attribute__((aligned(16))) uint16_t sDAll[4][32 * kWRow];
-    const uint32_t rows = pad32(net.words[kGwBatch]);
-    if (blockIdx.x * 128 >= rows) return;  // uniform per block
-    const uint32_t stride = net.stride;
-    for (int l = 1; l <= 4; ++l) {
-        const int nin = guideLayerIn(l), nout = guideLayerOut(l);
-        for (int i = threadIdx.x; i < 64 * 64; i += blockDim.x) {
-            const int in = i / 64, o = i % 64;
-            WT[l - 1][in * kWRow + o] =
-                (o < nout && in < nin) ? toBf16(net.params[guideWeightOffset(l) + o * nin + in]) : uint16_t(0);
-        }
-    }
-    __syncthreads();
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
-    uint16_t* sD = sDAll[wave];
-    const uint32_t totalWaves = gridDim.x * 4;
-    for (uint32_t blk = blockIdx.x * 4 + wave; blk * 32 < rows; blk += totalWaves) {
-        const uint32_t row0 = blk * 32;
-        // stage delta_4[0..63][row0 .. row0 + 31] transposed: 256 16-byte
-        // pieces (8 rows of one feature), 4 per lane
-        for (int q = lane; q < 256; q += 64) {
-            const int o = q >> 2, part = q & 3;
-            const u32x4 v = *reinterpret_cast<const u32x4*>(net.delta[4] + size_t(o) * stride + row0 + 8 * part);
-            const uint32_t w[4] = { v.x, v.y, v.z, v.w };
-            for (int t = 0; t < 8; ++t) sD[(8 * part + t) * kWRow + o] = uint16_t(w[t >> 1] >> (16 * (t & 1)));
-        }
+        const float wgt = valid ? cur.q.a.w : 0.0f;
+        float d[20];
+        const float loss = outputGradHalf(fo + r * kRawStride, h, wgt, v3(cur.wi), d);
+        waveHandoff();  // every lane has read its raw outputs
+        // output-layer deltas, row-major: lane half h writes outputs 20 h ..
+        // 20 h + 19 of row r, half 1 also the zero features 40..63
+        uint32_t* drow = reinterpret_cast<uint32_t*>(act + r * kActStride);
+        for (int m = 0; m < 10; ++m) drow[10 * h + m] = valid ? pack2f(d[2 * m], d[2 * m + 1]) : 0u;
+        if (h == 1)
+            for (int m = 20; m < 32; ++m) drow[m] = 0u;
         waveHandoff();
+        storeTiled(act, 64, 64, net.delta[4], row0);
         for (int l = 4; l >= 1; --l) {
-            bf16x8 b[4];
-            for (int k = 0; k < 4; ++k) b[k] = ld8(sD + r * kWRow + 16 * k + 8 * h);
-            waveHandoff();  // every lane has its operand before sD is overwritten
-            const uint16_t* actPrev = net.act + size_t(l - 1) * 64 * stride;
-            uint16_t* deltaPrev = net.delta[l - 1];
-            const uint16_t* W = WT[l - 1];
-            // The delta rows are the A operand and W^T the B operand, so lane
-            // (r, h) ends with rows 8 j + 4 h + i of input feature 32 mt + r:
-            // four consecutive rows of deltaPrev and actPrev per 8-byte access.
-            for (int mt = 0; mt < 2; ++mt) {
-                f32x16 acc = zero16();
-                for (int k = 0; k < 4; ++k) acc = mfma(b[k], ld8(W + (32 * mt + r) * kWRow + 16 * k + 8 * h), acc);
-                for (int j = 0; j < 4; ++j) {
-                    const size_t at = size_t(32 * mt + r) * stride + row0 + 8 * j + 4 * h;
-                    const uint2 ap = *reinterpret_cast<const uint2*>(actPrev + at);
-                    const uint32_t a4[4] = { ap.x & 0xFFFFu, ap.x >> 16, ap.y & 0xFFFFu, ap.y >> 16 };
-                    uint16_t d4[4];
-                    for (int i = 0; i < 4; ++i)
-                        d4[i] = fromBf16(uint16_t(a4[i])) > 0.0f ? toBf16(acc[4 * j + i]) : uint16_t(0);
-                    *reinterpret_cast<uint2*>(deltaPrev + at) = make_uint2(pack2(d4[0], d4[1]), pack2(d4[2], d4[3]));
-                    if (l > 1)
-                        for (int i = 0; i < 4; ++i) sD[(8 * j + 4 * h + i) * kWRow + 32 * mt + r] = d4[i];
-                }
-            }
-            waveHandoff();  // the next layer (or tile) reads sD
+            layerBackward(l, act, WT, mask[l - 1]);
+            storeTiled(act, 64, 64, net.delta[l - 1], row0);
         }
+        // the tile's weight and loss sums (each row counted by its h = 0 lane)
+        float sw = h == 0 ? wgt : 0.0f, sl = h == 0 && valid ? loss : 0.0f;
+        for (int off = 32; off > 0; off >>= 1) {
+            sw += __shfl_xor(sw, off, 64);
+            sl += __shfl_xor(sl, off, 64);
+        }
+        if (lane == 0) *reinterpret_cast<float2*>(net.tilePart + 2 * size_t(blk)) = make_float2(sw, sl);
     }
 }
 
 // One launch forms the weight gradients of all five layers.  Block job =
-// (chunk, layer, tile); tile (mt, nt) covers outputs 32 mt.. and inputs
-// 32 nt.. (layer 0: 3 input tiles over the 96 zero-padded input features).
-// The block's 4 waves take consecutive quarters of the chunk's rows, and
-// wave 0 adds their accumulators in wave order (fixed summation order, no
-// atomics).  Tiles with nt == 0 also form the bias sums with an all-ones B
-// operand.
-constexpr uint32_t kGradWaves = 4;
+// (row chunk, layer): the block's 8 waves take consecutive eighths of the
+// chunk's rows, and each wave forms every 32 x 32 tile of the layer's
+// gradient (outputs 32 mt.., inputs 32 nt..; layer 0: 3 input tiles over the
+// 96 zero-padded input features) and the bias sums (an all-ones B operand),
+// so each delta and activation fragment is read once.  Wave 0 then adds the
+// other waves' accumulators in wave order through LDS, two tiles at a time
+// (fixed summation order, no atomics).  The layer-0 job of each chunk also
+// adds the chunk's 64 tile weight and loss sums, in tile order, into
+// weightPart.
+constexpr uint32_t kGradWaves = 8;
 constexpr uint32_t kGradSlice = kGuideGradChunk / kGradWaves;  // rows per wave
-constexpr uint32_t kGradJobsPerChunk = 2 * 3 + 4 * (2 * 2);    // layer 0: 6 tiles, layers 1..4: 4 each
-__global__ __launch_bounds__(kGradWaves * 64) void kGuideWeightGrad(GuideNet net) {
-    // per wave and lane, the 16 accumulator values contiguous (16-byte LDS accesses)
-    __shared__ __attribute__((aligned(16))) float sAcc[kGradWaves][64][16];
-    __shared__ __attribute__((aligned(16))) float sBias[kGradWaves][64][16];
-    const uint32_t rows = pad32(net.words[kGwBatch]);
-    const uint32_t chunk = blockIdx.x / kGradJobsPerChunk, job = blockIdx.x % kGradJobsPerChunk;
-    const uint32_t r0 = chunk * kGuideGradChunk;
-    if (r0 >= rows) return;  // whole block
-    const int l = job < 6 ? 0 : 1 + int(job - 6) / 4;
-    const uint32_t tile = job < 6 ? job : (job - 6) % 4;
-    const uint32_t stride = net.stride;
+static_assert(kGradSlice % 16 == 0, "whole MFMA k steps per wave");
+
+template <int NT>
+__device__ __forceinline__ void weightGradJob(const GuideNet& net, int l, uint32_t chunk, uint32_t rows,
+                                              float (*red)[64][32]) {
+    constexpr int kTiles = 2 * NT + 2;  // (mt, nt) weight tiles, then the two bias tiles
     const uint16_t* delta = net.delta[l];
-    const uint16_t* actPrev = l == 0 ? net.actIn : net.act + size_t(l - 1) * 64 * stride;
-    const int nin = guideLayerIn(l), nout = guideLayerOut(l);
-    const int ntIn = l == 0 ? 3 : 2;
+    const uint16_t* actPrev = l == 0 ? net.actIn : net.act + size_t(l - 1) * 64 * net.stride;
     const uint32_t w = threadIdx.x >> 6;
-    const uint32_t s0 = r0 + w * kGradSlice, s1 = min(s0 + kGradSlice, rows);
-    const int mt = int(tile) / ntIn, nt = int(tile) % ntIn;
+    const uint32_t s0 = chunk * kGuideGradChunk + w * kGradSlice, s1 = min(s0 + kGradSlice, rows);
     const int lane = threadIdx.x & 63, lr = lane & 31, h = lane >> 5;
-    const uint16_t* aRow = delta + size_t(32 * mt + lr) * stride + 8 * h;
-    const uint16_t* bRow = actPrev + size_t(32 * nt + lr) * stride + 8 * h;
-    const uint32_t one2 = 0x3F803F80u;  // two bf16 1.0
+    // row-block-tiled operands (storeTiled): the 16-row block of row r starts
+    // at (r / 16) F 16, feature f of it at + 16 f
+    constexpr int F = NT == 3 ? 96 : 64;
+    const uint16_t* aRow0 = delta + lr * 16 + 8 * h;
+    const uint16_t* aRow1 = aRow0 + 32 * 16;
+    const uint16_t* bRow = actPrev + lr * 16 + 8 * h;
+    constexpr int bStep = 32 * 16;  // next input tile
+    const uint32_t one2 = 0x3F803F80u;         // two bf16 1.0
     const bf16x8 ones = __builtin_bit_cast(bf16x8, u32x4{ one2, one2, one2, one2 });
-    f32x16 acc = zero16(), accB = zero16();
-    // Eight row steps' operands are loaded before their products, so a wave
-    // waits once per eight steps instead of once per step; the products are
-    // accumulated in row order either way.
-    constexpr int kAhead = 8;
+    f32x16 T[kTiles];
+#pragma unroll
+    for (int t = 0; t < kTiles; ++t) T[t] = zero16();
+    // Four row steps' operands are loaded before their products (one wait
+    // per four steps); the products accumulate in row order either way.
+    constexpr int kAhead = 4;
     uint32_t r = s0;
     for (; r + 16 * kAhead <= s1; r += 16 * kAhead) {
-        bf16x8 a[kAhead], b[kAhead];
+        bf16x8 a0[kAhead], a1[kAhead], b[kAhead][NT];
 #pragma unroll
         for (int u = 0; u < kAhead; ++u) {
-            a[u] = ld8(aRow + r + 16 * u);
-            b[u] = ld8(bRow + r + 16 * u);
+            const size_t blk = r / 16 + u;
+            a0[u] = ld8(aRow0 + blk * 64 * 16);
+            a1[u] = ld8(aRow1 + blk * 64 * 16);
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) b[u][nt] = ld8(bRow + blk * F * 16 + nt * bStep);
         }
 #pragma unroll
         for (int u = 0; u < kAhead; ++u) {
-            acc = mfma(a[u], b[u], acc);
-            if (nt == 0) accB = mfma(a[u], ones, accB);
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                T[nt] = mfma(a0[u], b[u][nt], T[nt]);
+                T[NT + nt] = mfma(a1[u], b[u][nt], T[NT + nt]);
+            }
+            T[2 * NT] = mfma(a0[u], ones, T[2 * NT]);
+            T[2 * NT + 1] = mfma(a1[u], ones, T[2 * NT + 1]);
         }
     }
     for (; r < s1; r += 16) {
-        const bf16x8 a = ld8(aRow + r);
-        acc = mfma(a, ld8(bRow + r), acc);
-        if (nt == 0) accB = mfma(a, ones, accB);
+        const size_t blk = r / 16;
+        const bf16x8 a0 = ld8(aRow0 + blk * 64 * 16), a1 = ld8(aRow1 + blk * 64 * 16);
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const bf16x8 b = ld8(bRow + blk * F * 16 + nt * bStep);
+            T[nt] = mfma(a0, b, T[nt]);
+            T[NT + nt] = mfma(a1, b, T[NT + nt]);
+        }
+        T[2 * NT] = mfma(a0, ones, T[2 * NT]);
+        T[2 * NT + 1] = mfma(a1, ones, T[2 * NT + 1]);
     }
-    for (int q = 0; q < 4; ++q) {
-        reinterpret_cast<float4*>(sAcc[w][lane])[q] = make_float4(acc[4 * q], acc[4 * q + 1], acc[4 * q + 2],
-                                                                  acc[4 * q + 3]);
-        if (nt == 0)
-            reinterpret_cast<float4*>(sBias[w][lane])[q] = make_float4(accB[4 * q], accB[4 * q + 1],
-                                                                       accB[4 * q + 2], accB[4 * q + 3]);
-    }
-    __syncthreads();
-    if (w != 0) return;
-    for (uint32_t v = 1; v < kGradWaves; ++v)
-        for (int q = 0; q < 4; ++q) {
-            const float4 x = reinterpret_cast<const float4*>(sAcc[v][lane])[q];
-            acc[4 * q] += x.x;
-            acc[4 * q + 1] += x.y;
-            acc[4 * q + 2] += x.z;
-            acc[4 * q + 3] += x.w;
-            if (nt == 0) {
-                const float4 y = reinterpret_cast<const float4*>(sBias[v][lane])[q];
-                accB[4 * q] += y.x;
-                accB[4 * q + 1] += y.y;
-                accB[4 * q + 2] += y.z;
-                accB[4 * q + 3] += y.w;
+#pragma unroll
+    for (int p = 0; p < kTiles / 2; ++p) {
+        if (w != 0) {
+            float4* dst = reinterpret_cast<float4*>(red[w - 1][lane]);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                dst[q] = make_float4(T[2 * p][4 * q], T[2 * p][4 * q + 1], T[2 * p][4 * q + 2], T[2 * p][4 * q + 3]);
+                dst[4 + q] = make_float4(T[2 * p + 1][4 * q], T[2 * p + 1][4 * q + 1], T[2 * p + 1][4 * q + 2],
+                                         T[2 * p + 1][4 * q + 3]);
             }
         }
+        __syncthreads();
+        if (w == 0)
+            for (uint32_t v = 1; v < kGradWaves; ++v) {
+                const float4* src = reinterpret_cast<const float4*>(red[v - 1][lane]);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float4 x = src[q], y = src[4 + q];
+                    T[2 * p][4 * q] += x.x;
+                    T[2 * p][4 * q + 1] += x.y;
+                    T[2 * p][4 * q + 2] += x.z;
+                    T[2 * p][4 * q + 3] += x.w;
+                    T[2 * p + 1][4 * q] += y.x;
+                    T[2 * p + 1][4 * q + 1] += y.y;
+                    T[2 * p + 1][4 * q + 2] += y.z;
+                    T[2 * p + 1][4 * q + 3] += y.w;
+                }
+            }
+        __syncthreads();
+    }
+    if (w != 0) return;
+    const int nin = guideLayerIn(l), nout = guideLayerOut(l);
     float* dst = net.partial + size_t(chunk) * kGuideParams;
-    for (int j = 0; j < 4; ++j)
-        for (int i = 0; i < 4; ++i) {
-            const int o = 32 * mt + 8 * j + 4 * h + i, in = 32 * nt + lr;
-            if (o < nout && in < nin) dst[guideWeightOffset(l) + o * nin + in] = acc[4 * j + i];
-            if (nt == 0 && lr == 0 && o < nout) dst[guideBiasOffset(l) + o] = accB[4 * j + i];
-        }
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int o = 32 * mt + 8 * j + 4 * h + i;
+                if (o >= nout) continue;
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) {
+                    const int in = 32 * nt + lr;
+                    if (in < nin) dst[guideWeightOffset(l) + o * nin + in] = T[mt * NT + nt][4 * j + i];
+                }
+                if (lr == 0) dst[guideBiasOffset(l) + o] = T[2 * NT + mt][4 * j + i];
+            }
 }
 
-// 64 parameters per block; thread (q, j) sums chunks q, q + 4, ... of
-// parameter j in order, then the four slices are added in q order.
-constexpr int kReduceParams = 64;
-__global__ __launch_bounds__(256) void kGuideGradReduce(const uint32_t* __restrict__ words,
-                                                        const float* __restrict__ partial, float* __restrict__ grad) {
-    __shared__ float slice[4][kReduceParams];
-    const int j = threadIdx.x & (kReduceParams - 1), q = threadIdx.x / kReduceParams;
-    const int i = blockIdx.x * kReduceParams + j;
-    const uint32_t chunks = (pad32(words[kGwBatch]) + kGuideGradChunk - 1) / kGuideGradChunk;
+__global__ __launch_bounds__(kGradWaves * 64) void kGuideWeightGrad(GuideNet net) {
+    __shared__ __attribute__((aligned(16))) float red[kGradWaves - 1][64][32];
+    const uint32_t rows = pad32(net.words[kGwBatch]);
+    const uint32_t chunk = blockIdx.x / 5;
+    const int l = int(blockIdx.x % 5);
+    const uint32_t r0 = chunk * kGuideGradChunk;
+    if (r0 >= rows) return;  // whole block
+    if (l == 0 && threadIdx.x < 64) {
+        constexpr uint32_t kTiles = kGuideGradChunk / 32;
+        static_assert(kTiles <= 64, "one lane per tile");
+        const uint32_t t0 = r0 / 32, tn = min(kTiles, (rows - r0) / 32);
+        float2 v = threadIdx.x < tn ? *reinterpret_cast<const float2*>(net.tilePart + 2 * size_t(t0 + threadIdx.x))
+                                    : make_float2(0.0f, 0.0f);
+        for (int off = 32; off > 0; off >>= 1) {
+            v.x += __shfl_down(v.x, off, 64);
+            v.y += __shfl_down(v.y, off, 64);
+        }
+        if (threadIdx.x == 0) *reinterpret_cast<float2*>(net.weightPart + 2 * size_t(chunk)) = v;
+    }
+    if (l == 0) weightGradJob<3>(net, 0, chunk, rows, red);
+    else weightGradJob<2>(net, l, chunk, rows, red);
+}
+
+// Sum of the chunks' weight and loss sums in chunk order, the same in every
+// block (threads add strided slices in order, then an LDS tree).
+__device__ float2 batchSums(const float* __restrict__ weightPart, uint32_t chunks) {
+    __shared__ float2 part[256];
+    float2 s = make_float2(0.0f, 0.0f);
+    for (uint32_t i = threadIdx.x; i < chunks; i += 256) {
+        s.x += weightPart[2 * i];
+        s.y += weightPart[2 * i + 1];
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int half = 128; half > 0; half >>= 1) {
+        if (threadIdx.x < half) {
+            part[threadIdx.x].x += part[threadIdx.x + half].x;
+            part[threadIdx.x].y += part[threadIdx.x + half].y;
+        }
+        __syncthreads();
+    }
+    const float2 total = part[0];
+    __syncthreads();
+    return total;
+}
+
+// 64 parameters per block.  Thread (q, j) sums the partials of chunks q,
+// q + 4, ... of parameter j in chunk order (eight loads in flight), the four
+// slices are added in q order, and grad = that sum / batch weight sum.  With
+// adam, threads q = 0 then apply the Adam update to the FP32 master weight
+// and write its bf16 copies into the forward image and, for layers 1..4,
+// the transposed image (the images' padding stays zero from
+// launchGuidePack).  Block 0 publishes the normalised loss.
+constexpr int kStepParams = 64;
+constexpr int kStepBlock = 4 * kStepParams;
+__global__ __launch_bounds__(kStepBlock) void kGuideReduceStep(GuideNet net, GuideAdam hp, int adam) {
+    __shared__ float slice[4][kStepParams];
+    const uint32_t n = net.words[kGwBatch];
+    const uint32_t chunks = (pad32(n) + kGuideGradChunk - 1) / kGuideGradChunk;
+    const float2 sums = batchSums(net.weightPart, chunks);
+    const float inv = sums.x > 0.0f ? 1.0f / sums.x : 0.0f;
+    const int j = threadIdx.x % kStepParams, q = threadIdx.x / kStepParams;
+    const int i = blockIdx.x * kStepParams + j;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *wordF(net.words, kGwWeightSum) = sums.x;
+        *wordF(net.words, kGwLoss) = sums.y * inv;
+        if (adam && n > 0) {
+            *wordF(net.words, kGwLastLoss) = sums.y * inv;
+            net.words[kGwLastBatch] = n;
+        }
+    }
     float s = 0.0f;
-    if (i < kGuideParams)
-        for (uint32_t c = q; c < chunks; c += 4) s += partial[size_t(c) * kGuideParams + i];
+    if (i < kGuideParams) {
+        const float* src = net.partial + i;
+        uint32_t c = q;
+        for (; c + 28 < chunks; c += 32) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = src[size_t(c + 4 * u) * kGuideParams];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s += v[u];
+        }
+        for (; c < chunks; c += 4) s += src[size_t(c) * kGuideParams];
+    }
     slice[q][j] = s;
     __syncthreads();
-    if (q == 0 && i < kGuideParams) grad[i] = ((slice[0][j] + slice[1][j]) + slice[2][j]) + slice[3][j];
-}
-
-__global__ __launch_bounds__(256) void kGuideAdam(float* __restrict__ params, float* __restrict__ m,
-                                                  float* __restrict__ v, const float* __restrict__ grad,
-                                                  uint32_t* __restrict__ words, GuideAdam h) {
-    const uint32_t n = words[kGwBatch];
-    if (n == 0) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        words[kGwLastLoss] = words[kGwLoss];
-        words[kGwLastBatch] = n;
+    if (q != 0 || i >= kGuideParams) return;
+    const float g = (((slice[0][j] + slice[1][j]) + slice[2][j]) + slice[3][j]) * inv;
+    net.grad[i] = g;
+    if (!adam || n == 0) return;
+    const float t = float(net.words[kGwStep] > 0 ? net.words[kGwStep] : 1u);
+    const float c1 = 1.0f - powf(hp.beta1, t), c2 = 1.0f - powf(hp.beta2, t);
+    const float mi = hp.beta1 * net.adamM[i] + (1.0f - hp.beta1) * g;
+    const float vi = hp.beta2 * net.adamV[i] + (1.0f - hp.beta2) * g * g;
+    net.adamM[i] = mi;
+    net.adamV[i] = vi;
+    const float p = net.params[i] - hp.lr * (mi / c1) / (sqrtf(vi / c2) + hp.eps);
+    net.params[i] = p;
+    int l = 0;
+    while (l < 4 && i >= guideWeightOffset(l + 1)) ++l;
+    const int k = i - guideWeightOffset(l), nin = guideLayerIn(l);
+    uint16_t* image = static_cast<uint16_t*>(net.image);
+    if (k < nin * guideLayerOut(l)) {
+        const int o = k / nin, in = k % nin;
+        const uint16_t v = toBf16(p);
+        image[wOff(l) + o * wStride(l) + in] = v;
+        if (l > 0) static_cast<uint16_t*>(net.imageT)[wtOff(l) + in * kWRow + o] = v;
+    } else {
+        reinterpret_cast<float*>(image + kWTotal)[l * 64 + (k - nin * guideLayerOut(l))] = p;
     }
-    if (i >= kGuideParams) return;
-    const float t = float(words[kGwStep] > 0 ? words[kGwStep] : 1u);
-    const float c1 = 1.0f - powf(h.beta1, t), c2 = 1.0f - powf(h.beta2, t);
-    const float g = grad[i];
-    const float mi = h.beta1 * m[i] + (1.0f - h.beta1) * g;
-    const float vi = h.beta2 * v[i] + (1.0f - h.beta2) * g * g;
-    m[i] = mi;
-    v[i] = vi;
-    params[i] -= h.lr * (mi / c1) / (sqrtf(vi / c2) + h.eps);
 }
 
 // ------------------------------------------------------- sampling queries --
@@ -850,40 +952,58 @@ __device__ __forceinline__ float hashUnit(uint32_t a, uint32_t b, uint32_t c) {
 }
 
 // One thread per training path: a lane gathers the levels of its path that
-// hold a record of this frame with positive weight (the candidates) and those
-// kept for the pool, then the workgroup reserves its pool slots with one
-// atomic (prefix sums over lanes and waves give each record its slot).  Every
-// lane of the workgroup runs the same number of iterations (the barriers need
-// all of them); the per-wave totals are double buffered by iteration parity,
-// so one iteration's writes never meet the previous one's reads.
-__global__ __launch_bounds__(256) void kGuideCollect(GuideCollectArgs a) {
+// hold a record of this frame (its bit mask, set by the shade kernel) with
+// positive weight (the candidates) and those kept for the pool, then the
+// workgroup reserves its pool slots with one atomic (prefix sums over lanes
+// and waves give each record its slot).  The loops over levels are unrolled
+// to the largest level count and predicated by the mask, so a lane's record
+// reads are all in flight at once.  Every lane of the workgroup runs the
+// same number of iterations (the barriers need all of them); the per-wave
+// totals are double buffered by iteration parity, so one iteration's writes
+// never meet the previous one's reads.
+constexpr uint32_t kCollectLevels = 17;  // bounce levels 0..16
+constexpr uint32_t kCollectBlock = 1024;  // one pair of pool atomics per 1,024 paths
+__global__ __launch_bounds__(kCollectBlock) void kGuideCollect(GuideCollectArgs a) {
     const uint32_t stamp = a.g->guideStamp;
     const uint32_t kStride = a.g->guideTrainStride, phase = a.g->guideTrainPhase;
     uint32_t* cand = a.words + kGwCand0 + (stamp & 1u);
     const uint32_t prev = a.words[kGwCand0 + ((stamp + 1u) & 1u)];
     const float keep = prev > a.poolCap ? float(a.poolCap) / float(prev) : 1.0f;
     const uint32_t stride = gridDim.x * blockDim.x;
-    constexpr uint32_t kWaves = 256 / 64;
+    constexpr uint32_t kWaves = kCollectBlock / 64;
     __shared__ uint32_t sKeep[2][kWaves], sTake[2][kWaves], sBase[2];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t levelMask = a.levels >= 32 ? ~0u : (1u << a.levels) - 1u;
     uint32_t buf = 0;
     for (uint32_t base = blockIdx.x * blockDim.x; base < a.trainPaths; base += stride) {
         const uint32_t q = base + threadIdx.x;
         const uint32_t p = q * kStride + phase;
         const bool valid = q < a.trainPaths && p < a.paths;
-        const float lf = valid ? luminance(v3(a.radiance[p])) : 0.0f;
+        const uint32_t m = valid ? a.recMask[q] & levelMask : 0u;
+        const float lf = m ? luminance(v3(a.radiance[p])) : 0.0f;
         // Levels of this path with a record of this frame and positive
         // weight (candidates), and those kept for the pool.
-        uint32_t takeMask = 0, keepMask = 0;
-        if (valid) {
-            for (uint32_t lv = 0; lv < a.levels; ++lv) {
+        float r0w[kCollectLevels], r1w[kCollectLevels];
+        uint32_t st[kCollectLevels];
+#pragma unroll
+        for (uint32_t lv = 0; lv < kCollectLevels; ++lv) {
+            r0w[lv] = r1w[lv] = 0.0f;
+            st[lv] = 0u;
+            if (m >> lv & 1u) {
                 const float4* rec = a.pathRec + (size_t(lv) * a.trainPaths + q) * 4;
-                if (__float_as_uint(rec[2].w) != stamp) continue;
-                const float w = (lf - rec[1].w) / fmaxf(rec[0].w, 1e-20f);
-                if (!(w > 0.0f && isfinite(w))) continue;
-                takeMask |= 1u << lv;
-                if (keep >= 1.0f || hashUnit(p, lv, stamp) < keep) keepMask |= 1u << lv;
+                r0w[lv] = rec[0].w;
+                r1w[lv] = rec[1].w;
+                st[lv] = __float_as_uint(rec[2].w);
             }
+        }
+        uint32_t takeMask = 0, keepMask = 0;
+        float wv[kCollectLevels];
+#pragma unroll
+        for (uint32_t lv = 0; lv < kCollectLevels; ++lv) {
+            wv[lv] = (lf - r1w[lv]) / fmaxf(r0w[lv], 1e-20f);
+            if (!(m >> lv & 1u) || st[lv] != stamp || !(wv[lv] > 0.0f && isfinite(wv[lv]))) continue;
+            takeMask |= 1u << lv;
+            if (keep >= 1.0f || hashUnit(p, lv, stamp) < keep) keepMask |= 1u << lv;
         }
         // Inclusive scan of the lanes' kept counts within the wave.
         const uint32_t nKeep = __builtin_popcount(keepMask);
@@ -909,23 +1029,22 @@ __global__ __launch_bounds__(256) void kGuideCollect(GuideCollectArgs a) {
             sBase[buf] = kept ? atomicAdd(a.words + kGwPool, kept) : 0u;
         }
         __syncthreads();
-        uint32_t slot = sBase[buf] + incl - nKeep;
-        for (uint32_t w = 0; w < wave; ++w) slot += sKeep[buf][w];
+        uint32_t slot0 = sBase[buf] + incl - nKeep;
+        for (uint32_t w = 0; w < wave; ++w) slot0 += sKeep[buf][w];
         buf ^= 1u;
-        while (keepMask) {
-            const uint32_t lv = uint32_t(__builtin_ctz(keepMask));
-            keepMask &= keepMask - 1u;
-            if (slot < a.poolCap) {
-                const float4* rec = a.pathRec + (size_t(lv) * a.trainPaths + q) * 4;
-                const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
-                const float w = (lf - r1.w) / fmaxf(r0.w, 1e-20f);
-                float4* dst = a.pool + size_t(slot) * 4;
-                dst[0] = make_float4(r0.x, r0.y, r0.z, w);
-                dst[1] = make_float4(r1.x, r1.y, r1.z, 0.0f);
-                dst[2] = make_float4(r2.x, r2.y, r2.z, 0.0f);
-                dst[3] = make_float4(r3.x, r3.y, r3.z, 0.0f);
-            }
-            ++slot;
+#pragma unroll
+        for (uint32_t lv = 0; lv < kCollectLevels; ++lv) {
+            if (!(keepMask >> lv & 1u)) continue;
+            const uint32_t slot = slot0 + uint32_t(__builtin_popcount(keepMask & ((1u << lv) - 1u)));
+            if (slot >= a.poolCap) continue;
+            const float4* rec = a.pathRec + (size_t(lv) * a.trainPaths + q) * 4;
+            const float4 r0 = rec[0], r2 = rec[2], r3 = rec[3];
+            const float4 r1 = rec[1];
+            float4* dst = a.pool + size_t(slot) * 4;
+            dst[0] = make_float4(r0.x, r0.y, r0.z, wv[lv]);
+            dst[1] = make_float4(r1.x, r1.y, r1.z, 0.0f);
+            dst[2] = make_float4(r2.x, r2.y, r2.z, 0.0f);
+            dst[3] = make_float4(r3.x, r3.y, r3.z, 0.0f);
         }
     }
 }
@@ -937,9 +1056,9 @@ static uint32_t cappedGrid(uint32_t rows, uint32_t rowsPerBlock, uint32_t cap) {
     return g ? g : 1;
 }
 
-hipError_t launchGuidePack(const float* params, void* image, hipStream_t s) {
-    hipLaunchKernelGGL(kGuidePack, dim3((kWTotal + 320 + 255) / 256), dim3(256), 0, s, params,
-                       static_cast<uint16_t*>(image));
+hipError_t launchGuidePack(const float* params, void* image, void* imageT, hipStream_t s) {
+    hipLaunchKernelGGL(kGuidePack, dim3((kWTotal + 320 + kWTTotal + 255) / 256), dim3(256), 0, s, params,
+                       static_cast<uint16_t*>(image), static_cast<uint16_t*>(imageT));
     return hipGetLastError();
 }
 
@@ -950,36 +1069,33 @@ hipError_t launchGuideInfer(const RecView& rec, const uint32_t* countPtr, uint32
     return hipGetLastError();
 }
 
-// b.cap bounds the batch; the workspace is sized for net.cap rows.
-hipError_t launchGuideGradient(const GuideBatch& b, const GuideNet& net, bool countStep, hipStream_t s) {
+// The per-row launch and the weight gradient; b.cap bounds the batch, the
+// workspace is sized for net.cap rows.
+static hipError_t launchRowsAndWeightGrad(const GuideBatch& b, const GuideNet& net, bool countStep, hipStream_t s) {
     if (b.cap > net.cap || b.cap == 0 || net.stride != guideRowStride(net.cap)) return hipErrorInvalidValue;
     const uint32_t rows = guideRowStride(b.cap);
-    hipError_t e;
-    const uint32_t chunks = guideGradChunks(b.cap);
-    hipLaunchKernelGGL(kGuideStepBegin, dim3(chunks), dim3(kWeightSumBlock), 0, s, b, net.words, net.weightPart,
-                       countStep ? 1 : 0);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(kGuideFwdTrain, dim3(cappedGrid(rows, kFwdWaves * 32, 1024)), dim3(kFwdWaves * 64), 0, s, b,
-                       net.words, static_cast<const u32x4*>(net.image), net);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(kGuideOutGrad, dim3(cappedGrid(rows, 256, 1u << 20)), dim3(256), 0, s, b, net.words, net,
-                       chunks);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(kGuideBackward, dim3(cappedGrid(rows, 128, 1024)), dim3(256), 0, s, net);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(kGuideWeightGrad, dim3(chunks * kGradJobsPerChunk), dim3(kGradWaves * 64), 0, s, net);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(kGuideGradReduce, dim3((kGuideParams + kReduceParams - 1) / kReduceParams), dim3(256), 0, s,
-                       net.words, net.partial, net.grad);
+    hipLaunchKernelGGL(kGuideTrainRows, dim3(cappedGrid(rows, kTrainWaves * 32, 256)), dim3(kTrainWaves * 64), 0, s, b,
+                       net, countStep ? 1 : 0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kGuideWeightGrad, dim3(guideGradChunks(b.cap) * 5), dim3(kGradWaves * 64), 0, s, net);
     return hipGetLastError();
 }
 
-hipError_t launchGuideAdam(const GuideNet& net, const GuideAdam& h, hipStream_t s) {
-    hipLaunchKernelGGL(kGuideAdam, dim3((kGuideParams + 255) / 256), dim3(256), 0, s, net.params, net.adamM,
-                       net.adamV, net.grad, net.words, h);
-    hipError_t e = hipGetLastError();
+hipError_t launchGuideGradient(const GuideBatch& b, const GuideNet& net, bool countStep, hipStream_t s) {
+    hipError_t e = launchRowsAndWeightGrad(b, net, countStep, s);
     if (e != hipSuccess) return e;
-    return launchGuidePack(net.params, net.image, s);
+    hipLaunchKernelGGL(kGuideReduceStep, dim3((kGuideParams + kStepParams - 1) / kStepParams), dim3(kStepBlock), 0, s,
+                       net, GuideAdam{}, 0);
+    return hipGetLastError();
+}
+
+hipError_t launchGuideStep(const GuideBatch& b, const GuideNet& net, const GuideAdam& h, hipStream_t s) {
+    hipError_t e = launchRowsAndWeightGrad(b, net, true, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kGuideReduceStep, dim3((kGuideParams + kStepParams - 1) / kStepParams), dim3(kStepBlock), 0, s,
+                       net, h, 1);
+    return hipGetLastError();
 }
 
 hipError_t launchGuideSample(const float* raw, const float* u, uint32_t count, float4* out, hipStream_t s) {
@@ -1001,7 +1117,7 @@ hipError_t launchGuideFrameBegin(uint32_t* words, const FrameConsts* g, hipStrea
 
 hipError_t launchGuideCollect(const GuideCollectArgs& a, hipStream_t s) {
     if (a.trainPaths == 0) return hipSuccess;
-    hipLaunchKernelGGL(kGuideCollect, dim3(cappedGrid(a.trainPaths, 256, 2048)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(kGuideCollect, dim3(cappedGrid(a.trainPaths, kCollectBlock, 1024)), dim3(kCollectBlock), 0, s, a);
     return hipGetLastError();
 }
 

@@ -22,7 +22,10 @@ constexpr int kGuideInputsPadded = 80;   // bf16 features per row (78 used)
 constexpr int kGuideMixFloat4 = 12;      // decoded mixture per query (see guide_device.h)
 // Packed forward image: bf16 weights (24,064 with row padding) then FP32 biases (5 x 64), in bytes.
 constexpr size_t kGuideImageBytes = 24064 * 2 + 320 * 4;
-constexpr uint32_t kGuideGradChunk = 1024;  // training rows per weight-gradient partial (4 waves x 256 rows)
+// Transposed image that follows it: bf16 W_l^T of layers 1..4 (64 inputs x
+// 72-element rows of outputs), the backward pass's operand, in bytes.
+constexpr size_t kGuideImageTBytes = 4 * 64 * 72 * 2;
+constexpr uint32_t kGuideGradChunk = 2048;  // training rows per weight-gradient partial (8 waves x 256 rows)
 
 __host__ __device__ constexpr uint32_t guideRowStride(uint32_t cap) { return (cap + 31u) & ~31u; }
 __host__ __device__ constexpr uint32_t guideGradChunks(uint32_t cap) {
@@ -63,21 +66,23 @@ struct GuideBatch {
 };
 
 // Network parameters, Adam moments and the training workspace for batches of
-// up to `cap` rows.  Activations and deltas are stored feature-major
-// ([feature][row], row stride `stride` = cap rounded up to 32) so the matrix
-// kernels read 16-byte fragments.
+// up to `cap` rows (`stride` = cap rounded up to 32).  Inputs, activations
+// and deltas are stored row-block-tiled: element (feature f, row r) of an
+// array of F features at ((r / 16) F + f) 16 + r % 16, so the weight-gradient
+// kernel reads each operand fragment of a wave as one contiguous kilobyte.
 struct GuideNet {
     float* params = nullptr;     // kGuideParamCount
     float* adamM = nullptr;
     float* adamV = nullptr;
     float* grad = nullptr;       // written by the gradient reduction
     void* image = nullptr;       // packed forward image (kGuideImageBytes), kept in step with params
-    uint16_t* actIn = nullptr;   // bf16 [96][stride] encoded inputs (rows 78..95 zero)
-    uint16_t* act = nullptr;     // bf16 [4][64][stride] hidden-layer outputs
-    float* raw = nullptr;        // FP32 [40][stride] raw outputs
-    uint16_t* delta[5] = {};     // bf16 [64][stride] per layer: error at layer l's outputs
+    void* imageT = nullptr;      // transposed image (kGuideImageTBytes), kept in step with params
+    uint16_t* actIn = nullptr;   // bf16, 96 features x stride rows: encoded inputs (features 78..95 zero)
+    uint16_t* act = nullptr;     // bf16, 4 x (64 features x stride rows): hidden-layer outputs
+    uint16_t* delta[5] = {};     // bf16, 64 features x stride rows per layer: error at layer l's outputs (unnormalised)
+    float* tilePart = nullptr;   // FP32 [stride / 32][2]: weight and loss sums per 32-row tile
     float* partial = nullptr;    // FP32 [guideGradChunks(cap)][kGuideParamCount]
-    float* weightPart = nullptr; // FP32 [guideGradChunks(cap)]: batch weight sum per row chunk
+    float* weightPart = nullptr; // FP32 [guideGradChunks(cap)][2]: weight and loss sums per row chunk
     uint32_t* words = nullptr;   // kGwWords
     uint32_t cap = 0;
     uint32_t stride = 0;
@@ -87,8 +92,9 @@ struct GuideAdam {
     float lr = 1e-3f, beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
 };
 
-// Packs params into the forward image (bf16 weights, padded, and biases).
-hipError_t launchGuidePack(const float* params, void* image, hipStream_t s);
+// Packs params into the forward image (bf16 weights, padded, and biases)
+// and the transposed image.
+hipError_t launchGuidePack(const float* params, void* image, void* imageT, hipStream_t s);
 
 // Network outputs for the first *countPtr rows of rec (countPtr in device
 // memory, at most maxCount).  decoded: kGuideMixFloat4 float4 per row
@@ -100,9 +106,10 @@ hipError_t launchGuideInfer(const RecView& rec, const uint32_t* countPtr, uint32
 // countStep: advance the Adam step counter when the batch is not empty.
 hipError_t launchGuideGradient(const GuideBatch& b, const GuideNet& net, bool countStep, hipStream_t s);
 
-// Adam update from grad (skipped when the last batch was empty), then repacks
-// the forward image.
-hipError_t launchGuideAdam(const GuideNet& net, const GuideAdam& h, hipStream_t s);
+// One training step: the gradient as launchGuideGradient (advancing the
+// step counter), then the Adam update (skipped for an empty batch) and the
+// images' bf16 weights.  Three launches.
+hipError_t launchGuideStep(const GuideBatch& b, const GuideNet& net, const GuideAdam& h, hipStream_t s);
 
 // One training step (gradient + Adam) on `count` device-resident training
 // records (4 float4 each) with the workspace in the library's fixed device
@@ -132,6 +139,7 @@ hipError_t launchGuideFrameBegin(uint32_t* words, const FrameConsts* g, hipStrea
 //   (wo.xyz, FrameConsts::guideStamp as bits), (wi.xyz, -)
 struct GuideCollectArgs {
     const float4* pathRec = nullptr;  // [level][trainPaths] x 4 float4
+    const uint32_t* recMask = nullptr; // per training path: levels that wrote a record this frame
     const float4* radiance = nullptr; // per path: rgb radiance of this frame
     uint32_t paths = 0;               // all paths of the frame
     uint32_t trainPaths = 0;          // ceil(paths / guideTrainStride)

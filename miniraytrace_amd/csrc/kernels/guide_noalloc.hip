@@ -3,11 +3,11 @@
 // allocate device memory (work under HIP graph capture, tight serving loops).
 //
 // The array `scratch` holds, for batches of up to kNoAllocRows records:
-// the encoded inputs, the hidden activations, the raw outputs, the five
-// per-layer delta blocks, the per-chunk weight-gradient partials, the
-// per-chunk batch-weight sums and the batch-size word that the training
+// the encoded inputs, the hidden activations, the five per-layer delta
+// blocks, the per-tile and per-chunk weight and loss sums, the per-chunk
+// weight-gradient partials and the batch-size word that the training
 // kernels read.  It is a file-scope device variable, so every GPU the
-// library runs on holds its own copy (about 13 MB) until the process ends.
+// library runs on holds its own copy (about 12 MB) until the process ends.
 // Host code never passes the array's name to a kernel: on the first call on
 // a device, with that device current, hipGetSymbolAddress yields that
 // device's address, which is kept per device and used for every kernel
@@ -15,9 +15,10 @@
 //
 // Every call on one device shares that device's array and it keeps the last
 // call's contents; calls that overlap on one device (on different streams,
-// or from different threads) race on it.  Each call clears what must start
-// at zero (the unused input-feature rows 80..95) with a fill kernel, so the
-// call can be captured into a graph (no memset node).
+// or from different threads) race on it.  Nothing in it must start at zero:
+// the training kernels write every element they read (the zero input
+// features 80..95 included), and the batch size is published by a small
+// kernel, so the call can be captured into a graph (no memset node).
 #include "guide_launch.h"
 #include "mini_raytrace_ext.h"
 
@@ -36,22 +37,17 @@ constexpr uint32_t kChunks = guideGradChunks(kNoAllocRows);
 
 constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
-// Byte offsets of the workspace blocks inside the array (feature-major, row
-// stride kStride, as GuideNet describes them).
+// Byte offsets of the workspace blocks inside the array (row-block-tiled,
+// kStride rows, as GuideNet describes them).
 constexpr size_t kOffActIn = 0;                                                   // bf16 [96][stride]
 constexpr size_t kOffAct = kOffActIn + align256(size_t(96) * kStride * 2);        // bf16 [4][64][stride]
-constexpr size_t kOffRaw = kOffAct + align256(size_t(4) * 64 * kStride * 2);      // f32 [40][stride]
+constexpr size_t kOffTile = kOffAct + align256(size_t(4) * 64 * kStride * 2);     // f32 [stride / 32][2]
 constexpr size_t kDeltaBytes = align256(size_t(64) * kStride * 2);                // bf16 [64][stride]
-constexpr size_t kOffDelta = kOffRaw + align256(size_t(kGuideOutputs) * kStride * 4);  // x5
+constexpr size_t kOffDelta = kOffTile + align256(size_t(kStride / 32) * 2 * 4);   // x5
 constexpr size_t kOffPartial = kOffDelta + 5 * kDeltaBytes;                       // f32 [chunks][params]
-constexpr size_t kOffWeightPart = kOffPartial + align256(size_t(kChunks) * kGuideParamCount * 4);
-constexpr size_t kOffCount = kOffWeightPart + align256(size_t(kChunks) * 4);      // u32 batch size
+constexpr size_t kOffWeightPart = kOffPartial + align256(size_t(kChunks) * kGuideParamCount * 4);  // f32 [chunks][2]
+constexpr size_t kOffCount = kOffWeightPart + align256(size_t(kChunks) * 2 * 4);  // u32 batch size
 constexpr size_t kScratchBytes = kOffCount + 256;
-
-// Input-feature rows that no kernel writes but the layer-0 weight-gradient
-// tiles read (as zeros).
-constexpr uint32_t kZeroRowFirst = 80, kZeroRows = 16;
-constexpr uint32_t kZeroWords = kZeroRows * kStride / 2;  // u32 words (two bf16 each)
 
 constexpr int kMaxDevices = 64;
 std::mutex gScratchMutex;
@@ -63,13 +59,8 @@ static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 namespace {
 
-// Clears the unused input-feature rows and publishes the batch size.
-__global__ __launch_bounds__(256) void kNoAllocBegin(uint32_t* __restrict__ zeroRows, uint32_t* __restrict__ countWord,
-                                                     uint32_t count) {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i < kZeroWords) zeroRows[i] = 0u;
-    if (i == 0) *countWord = count;
-}
+// Publishes the batch size (the training kernels read it from device memory).
+__global__ void kNoAllocBegin(uint32_t* __restrict__ countWord, uint32_t count) { *countWord = count; }
 
 // This device's address of `scratch` (the device must be current).
 hipError_t scratchAddress(int device, unsigned char** out) {
@@ -100,10 +91,10 @@ hipError_t launchGuideTrainNoAlloc(const GuideNet& state, const float4* records,
     hipError_t e = scratchAddress(device, &base);
     if (e != hipSuccess) return e;
 
-    GuideNet net = state;  // parameters, Adam moments, gradient, image and state words of the network
+    GuideNet net = state;  // parameters, Adam moments, gradient, images and state words of the network
     net.actIn = reinterpret_cast<uint16_t*>(base + kOffActIn);
     net.act = reinterpret_cast<uint16_t*>(base + kOffAct);
-    net.raw = reinterpret_cast<float*>(base + kOffRaw);
+    net.tilePart = reinterpret_cast<float*>(base + kOffTile);
     for (int l = 0; l < 5; ++l) net.delta[l] = reinterpret_cast<uint16_t*>(base + kOffDelta + l * kDeltaBytes);
     net.partial = reinterpret_cast<float*>(base + kOffPartial);
     net.weightPart = reinterpret_cast<float*>(base + kOffWeightPart);
@@ -111,8 +102,7 @@ hipError_t launchGuideTrainNoAlloc(const GuideNet& state, const float4* records,
     net.stride = kStride;
     uint32_t* countWord = reinterpret_cast<uint32_t*>(base + kOffCount);
 
-    uint32_t* zeroRows = reinterpret_cast<uint32_t*>(net.actIn + size_t(kZeroRowFirst) * kStride);
-    hipLaunchKernelGGL(kNoAllocBegin, dim3((kZeroWords + 255) / 256), dim3(256), 0, s, zeroRows, countWord, count);
+    hipLaunchKernelGGL(kNoAllocBegin, dim3(1), dim3(1), 0, s, countWord, count);
     if ((e = hipGetLastError()) != hipSuccess) return e;
 
     GuideBatch b;
@@ -122,8 +112,7 @@ hipError_t launchGuideTrainNoAlloc(const GuideNet& state, const float4* records,
     b.phase = 0;
     b.period = 1;
     b.cap = count;
-    if ((e = launchGuideGradient(b, net, true, s)) != hipSuccess) return e;
-    return launchGuideAdam(net, h, s);
+    return launchGuideStep(b, net, h, s);
 }
 
 } // namespace dev

@@ -140,6 +140,7 @@ __device__ __forceinline__ void generatePixel(const PathView& pv, const FrameCon
     pv.radiance[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     pv.rngCounter[p] = rng.counter;
     if (pv.rrScale) pv.rrScale[p] = 1.0f;
+    if (pv.recMask && p % g.guideTrainStride == g.guideTrainPhase) pv.recMask[p / g.guideTrainStride] = 0u;
 }
 
 // ------------------------------------------------------------ batch helpers --
@@ -487,6 +488,7 @@ __device__ __forceinline__ ShadeOut shadePath(const SceneView& sc, const PathVie
                             r[1] = f4(s.nShade, luminance(L));
                             r[2] = f4(-dir, __uint_as_float(g.guideStamp));
                             r[3] = f4(wiW, 0.0f);
+                            pv.recMask[p / g.guideTrainStride] |= 1u << bounce;
                         }
                     }
                 }

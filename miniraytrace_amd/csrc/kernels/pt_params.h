@@ -67,6 +67,7 @@ struct PathView {
     float4* guideRec;                // 3 float4 per guided row: (pos01), (n), (wo)
     float4* guideMix;                // 12 float4 per guided row: decoded mixture
     float4* pathRec;                 // [level][path] x 4 float4 training records
+    uint32_t* recMask;               // per training path: bit b set when level b holds a record of this frame
     // Learned environment-stratum selection (null when guided sampling is off).
     float* neeSplit;                 // per path: bright-stratum probability at its last vertex
     uint2* shadowX;                  // per shadow slot: (cell << 1 | stratum, contribution bits) or ~0u

@@ -223,7 +223,7 @@ class Engine:
 
     # -- path guiding (mini_raytrace_ext.h section 5) --
     def set_guiding(self, sampling: bool = True, training: bool = True, select_prob: float = 0.5,
-                    learning_rate: float = 1e-3, batch_size: int = 65536, steps_per_frame: int = 2,
+                    learning_rate: float = 1e-3, batch_size: int = 65536, steps_per_frame: int = 4,
                     min_roughness: float = 0.2, seed: int = 1, position_scale: float = 1.0) -> None:
         o = L.mrtxGuidingOptions(enableSampling=int(sampling), enableTraining=int(training),
                                  selectProb=select_prob, learningRate=learning_rate, batchSize=batch_size,

@@ -251,9 +251,11 @@ def mixture_sample(raw, u):
 def backward_bf16(theta, x, w, weight):
     """backward() with the device's rounding points: the forward of
     forward_bf16 (bf16 inputs, weights and hidden activations), the raw-output
-    gradient rounded to bf16, and every back-propagated delta rounded to bf16
-    after the ReLU mask; products and sums exact.  Returns (loss, grad) of
-    the loss normalised by the batch's total weight."""
+    gradient of the unnormalised sum_j weight_j (-log q_j) rounded to bf16,
+    and every back-propagated delta rounded to bf16 after the ReLU mask;
+    products and sums exact, the summed gradient divided by the batch's total
+    weight at the end (guide.hip kGuideTrainRows / kGuideReduceStep).
+    Returns (loss, grad) of the loss normalised by the total weight."""
     layers = unpack(np.asarray(theta, np.float32))
     acts = [bf16(x)]
     h = acts[0]
@@ -266,7 +268,7 @@ def backward_bf16(theta, x, w, weight):
             h = z
     raw = h.astype(np.float32).astype(np.float64)
     wsum = weight.sum()
-    L, d = loss_and_grad_raw(raw, w, weight / wsum)
+    L, d = loss_and_grad_raw(raw, w, weight)
     d = bf16(d)
     grads = []
     for li in range(len(layers) - 1, -1, -1):
@@ -275,7 +277,7 @@ def backward_bf16(theta, x, w, weight):
         if li > 0:
             d = bf16((d @ bf16(W)) * (acts[li] > 0))
     grads.reverse()
-    return L, np.concatenate([np.concatenate([gW.reshape(-1), gb]) for gW, gb in grads])
+    return L / wsum, np.concatenate([np.concatenate([gW.reshape(-1), gb]) for gW, gb in grads]) / wsum
 
 
 def adam_step(theta, m, v, g, t, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):

@@ -354,8 +354,8 @@ def test_training_batches_are_full_at_720p(gpu):
     """The guided room workload (495k-triangle room, 2048x1024 sky, 1280x720):
     every path records its guided vertices.  Once the network guides paths
     to the light (64 frames here), a frame yields more positive-weight records
-    than two 65,536-record Adam batches need, and both steps of a frame train
-    on (nearly) full batches: the pool keeps each candidate with probability
+    than four 65,536-record Adam batches need, and every step of a frame trains
+    on a (nearly) full batch: the pool keeps each candidate with probability
     pool / previous frame's candidates, so it can fall a little short.  The
     untrained network's first frames find fewer records."""
     from miniraytrace_amd import scenes
@@ -368,14 +368,14 @@ def test_training_batches_are_full_at_720p(gpu):
     e.commit()
     e.set_guiding(sampling=True, training=True)
     opts = e.guiding()
-    assert opts["batchSize"] == 65536 and opts["stepsPerFrame"] == 2
+    assert opts["batchSize"] == 65536 and opts["stepsPerFrame"] == 4
     for f in range(64):
         e.render_frame()
         if f in (0, 7, 31, 63):
             _print(f"frame {f}: {e.guiding_stats()}")
     st = e.guiding_stats()
-    assert st["candidates"] >= 2 * 65536
-    assert st["poolRecords"] >= 0.95 * 2 * 65536
+    assert st["candidates"] >= 4 * 65536
+    assert st["poolRecords"] >= 0.95 * 4 * 65536
     assert st["lastBatch"] >= 0.95 * 65536
 
 
