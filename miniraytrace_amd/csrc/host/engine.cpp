@@ -817,13 +817,11 @@ void Engine::packEnvironment() {
     double total = 0.0;
     for (double x : weights) total += x;
     const double cut = 64.0 * total / double(std::max<size_t>(n, 1));
-    std::vector<uint8_t> strat(n, 0);
     std::vector<double> wa, wb;
     std::vector<uint32_t> ta, tb;
     double sumA = 0.0;
     for (size_t i = 0; i < n; ++i) {
         if (weights[i] >= cut && weights[i] > 0.0) {
-            strat[i] = 1;
             wa.push_back(weights[i]);
             ta.push_back(uint32_t(i));
             sumA += weights[i];
@@ -845,7 +843,6 @@ void Engine::packEnvironment() {
     };
     if (split) {
         const auto a = table(wa, ta), b = table(wb, tb);
-        uploadVec(dEnvStrat_, strat.data(), strat.size());
         uploadVec(dEnvA_, a.data(), a.size() * sizeof(EnvStratumEntry));
         uploadVec(dEnvB_, b.data(), b.size() * sizeof(EnvStratumEntry));
     }
@@ -944,7 +941,6 @@ void Engine::allocGuide() {
     dPathRec_.zero(stream_);  // stamp 0 never matches a frame
     dRecMask_.allocate(size_t(trainPaths) * 4);  // reset per frame by ray generation
     dGuidePool_.allocate(size_t(poolCap) * 4 * 16);
-    dNeeSplit_.allocate(size_t(n) * 4);
     dRrScale_.allocate(size_t(n) * 4);
     dShX_.allocate(size_t(n) * 8);
     dNeeStats_.allocate(size_t(kNeeCells) * 16);
@@ -980,7 +976,6 @@ dev::SceneView Engine::sceneView() const {
     s.lights = dLights_.as<const GpuLight>();
     s.envMap = dEnv_.as<const float4>();
     s.envAlias = dAlias_.as<const AliasEntry>();
-    s.envStratum = dEnvStrat_.as<const uint8_t>();
     s.envA = dEnvA_.as<const EnvStratumEntry>();
     s.envB = dEnvB_.as<const EnvStratumEntry>();
     s.texTable = dTexTable_.as<const GpuTexture>();
@@ -1015,7 +1010,6 @@ dev::PathView Engine::pathView() const {
     p.pathRec = guide && guide_.enableTraining ? dPathRec_.as<float4>() : nullptr;
     p.recMask = p.pathRec ? dRecMask_.as<uint32_t>() : nullptr;
     const bool nee = guide && guide_.enableSampling && envSplitA_ > 0.0f;
-    p.neeSplit = nee ? dNeeSplit_.as<float>() : nullptr;
     p.shadowX = nee ? dShX_.as<uint2>() : nullptr;
     p.neeStats = nee ? dNeeStats_.as<float4>() : nullptr;
     p.rrScale = guide && guide_.enableSampling ? dRrScale_.as<float>() : nullptr;

@@ -209,7 +209,7 @@ private:
     // scene buffers
     GpuBvhBuilder gpuBvh_;  // large meshes' BLAS builds (workspace kept between commits)
     DeviceBuffer dNodes_, dWide_, dTriVerts_, dTris_, dPos_, dNrm_, dUv_, dTan_;
-    DeviceBuffer dInst_, dTraceInst_, dMats_, dLights_, dEnv_, dAlias_, dEnvStrat_, dEnvA_, dEnvB_;
+    DeviceBuffer dInst_, dTraceInst_, dMats_, dLights_, dEnv_, dAlias_, dEnvA_, dEnvB_;
     float envSplitA_ = 0.0f;
     uint32_t envCountA_ = 0, envCountB_ = 0;
     DeviceBuffer dTexTable_, dTex8_, dTex32_;
@@ -235,7 +235,7 @@ private:
     mrtxGuidingOptions guide_{};
     std::unique_ptr<GuideNetwork> guideNet_;
     DeviceBuffer dGuideIdx_, dGuideRec_, dGuideMix_, dPathRec_, dRecMask_, dGuidePool_;
-    DeviceBuffer dNeeSplit_, dShX_, dNeeStats_;  // learned environment-stratum selection
+    DeviceBuffer dShX_, dNeeStats_;  // learned environment-stratum selection
     DeviceBuffer dRrScale_;                      // guided paths: roulette throughput scale
     uint32_t guidePaths_ = 0, guideLevels_ = 0;
     uint32_t guideTrainStride_ = 1;  // every guideTrainStride_-th path writes training records

@@ -341,8 +341,9 @@ __device__ __forceinline__ ShadeOut shadePath(const SceneView& sc, const PathVie
         if (maxComp(env) > 0.0f) {
             float w = 1.0f;
             if (bounce > 0) {
-                float pl = envPdf(sc, g, dir) * envPickPdf(g);
-                if (pv.neeSplit) pl *= envStrataRatio(sc, g, dir, pv.neeSplit[p]);  // the previous vertex's NEE
+                // NEE's pdf of the plain light distribution (stratified
+                // guided NEE weighs MIS with it too)
+                const float pl = envPdf(sc, g, dir) * envPickPdf(g);
                 w = balanceHeuristic(prevPdf, pl);
                 if (pv.pathRec && g.guideTrain && p % g.guideTrainStride == g.guideTrainPhase) {
                     // Training target of the previous vertex: its own
@@ -400,10 +401,9 @@ __device__ __forceinline__ ShadeOut shadePath(const SceneView& sc, const PathVie
                 cell = neeCell(g, s.pos, s.nShade);
                 split = neeSelect(pv.neeStats, cell, g.envSplitA);
             }
-            if (pv.neeSplit) pv.neeSplit[p] = split;
             const LightSample ls = strata ? sampleOneLight<true>(sc, g, s.pos, rng, split)
                                           : sampleOneLight<false>(sc, g, s.pos, rng);
-            if (ls.valid && ls.pdf > 1e-9f) {
+            if (ls.valid && ls.misPdf > 1e-9f) {
                 const V3 wi = v3(dot(ls.wi, T), dot(ls.wi, B), dot(ls.wi, s.nShade));
                 V3 f;
                 float bsdfPdf;
@@ -415,7 +415,7 @@ __device__ __forceinline__ ShadeOut shadePath(const SceneView& sc, const PathVie
                     // so both tracers converge to the same image.
                     float contPdf = bsdfPdf;
                     if (GUIDE && mix && !last) contPdf = sp * storedMixPdf(mix, ls.wi) + (1.0f - sp) * bsdfPdf;
-                    const float w = ls.delta ? 1.0f : balanceHeuristic(ls.pdf, contPdf);
+                    const float w = ls.delta ? 1.0f : balanceHeuristic(ls.misPdf, contPdf);
                     o.shO = s.pos + s.nGeom * (wi.z > 0.0f ? kEpsRay : -kEpsRay);
                     o.shD = ls.wi;
                     o.shMax = ls.dist - 2.0f * kEpsRay;

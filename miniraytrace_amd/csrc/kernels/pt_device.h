@@ -225,31 +225,22 @@ __device__ __forceinline__ bool sampleEnv(const SceneView& sc, const FrameConsts
 // Guided shading may draw environment NEE from two strata (FrameConsts::
 // envSplitA): the bright texels (a sun) with probability s and the rest with
 // 1 - s, each through its own alias table over the same texel weights.  A
-// texel then has the probability of sampleEnv times s / share or
-// (1 - s) / (1 - share), and so has the pdf of every direction in it: s =
-// share is sampleEnv's distribution.  s comes from per-cell statistics of the
-// shading point (neeSelect), and every path records the s of its last
-// vertex, so the MIS weight of a later environment hit uses the pdf that
-// vertex's NEE had.  Any s in (0, 1) keeps the estimator unbiased.
-__device__ __forceinline__ uint32_t envTexelOf(const FrameConsts& g, V3 d) {
-    float u, v;
-    dirToEquirect(d, g.envRotation, u, v);
-    const uint32_t col = min(uint32_t(u * float(g.envWidth)), g.envWidth - 1);
-    const uint32_t row = min(uint32_t(v * float(g.envHeight)), g.envHeight - 1);
-    return row * g.envWidth + col;
-}
-// pdf of stratified environment sampling with bright-stratum probability s
-// over the plain pdf, for direction d.
-__device__ __forceinline__ float envStrataRatio(const SceneView& sc, const FrameConsts& g, V3 d, float s) {
-    if (!(g.envSplitA > 0.0f) || !(g.flags & kFlagHasEnv)) return 1.0f;
-    const bool bright = sc.envStratum[envTexelOf(g, d)] != 0;
-    return bright ? s / g.envSplitA : (1.0f - s) / (1.0f - g.envSplitA);
-}
+// texel is then drawn with sampleEnv's probability times r = s / share or
+// (1 - s) / (1 - share); s comes from per-cell statistics of the shading
+// point (neeSelect).  The sample's importance weight divides by that
+// probability, but its MIS weight (and that of a continuation reaching the
+// environment) uses sampleEnv's pdf: NEE returns texel-centre directions
+// while a continuation's pdf is the bilinear lookup, and a ratio that jumps
+// at the stratum border would weigh the two differently across the sun's
+// edge texels (guided rendered the room 0.14 % brighter than plain).  With
+// plain MIS weights, each technique's expectation is the plain tracer's for
+// any s in (0, 1).
 // sampleEnv over the strata: the same two draws (the first chooses the
 // stratum and, rescaled, its entry; the second keeps the entry or takes its
-// alias).  stratum: 1 bright, 0 the rest.
+// alias).  pdf: of the draw; plainPdf: sampleEnv's.  stratum: 1 bright, 0
+// the rest.
 __device__ __forceinline__ bool sampleEnvStrata(const SceneView& sc, const FrameConsts& g, Rng& rng, float s,
-                                                V3& dir, V3& rad, float& pdf, int& stratum) {
+                                                V3& dir, V3& rad, float& pdf, float& plainPdf, int& stratum) {
     const uint32_t w = g.envWidth, h = g.envHeight;
     const float u1 = rng.next1();
     const float u2 = rng.next1();
@@ -266,9 +257,10 @@ __device__ __forceinline__ bool sampleEnvStrata(const SceneView& sc, const Frame
     const float u = (float(col) + 0.5f) / float(w), v = (float(row) + 0.5f) / float(h);
     dir = equirectToDir(u, v, g.envRotation);
     rad = v3(envLookup(sc, g, u, v)) * g.envIntensity;
-    pdf = envPdf(sc, g, dir) * (bright ? s / g.envSplitA : (1.0f - s) / (1.0f - g.envSplitA));
+    plainPdf = envPdf(sc, g, dir);
+    pdf = plainPdf * (bright ? s / g.envSplitA : (1.0f - s) / (1.0f - g.envSplitA));
     stratum = bright ? 1 : 0;
-    return pdf > 1e-9f;
+    return plainPdf > 1e-9f;
 }
 // Cell of a shading point for the stratum statistics.
 __device__ __forceinline__ uint32_t neeCell(const FrameConsts& g, V3 pos, V3 n) {
@@ -298,7 +290,8 @@ __device__ __forceinline__ float neeSelect(const float4* __restrict__ stats, uin
 struct LightSample {
     V3 wi;
     V3 radiance;
-    float pdf;
+    float pdf;     // of the draw (the contribution divides by it)
+    float misPdf;  // of the plain light distribution (MIS weights and validity use it)
     float dist;
     bool delta;
     bool valid;
@@ -315,6 +308,7 @@ __device__ __forceinline__ LightSample sampleOneLight(const SceneView& sc, const
     ls.valid = false;
     ls.delta = false;
     ls.pdf = 0.0f;
+    ls.misPdf = 0.0f;
     ls.dist = 0.0f;
     ls.stratum = -1;
     if (g.lightCount == 0) return ls;
@@ -324,15 +318,18 @@ __device__ __forceinline__ LightSample sampleOneLight(const SceneView& sc, const
     const GpuLight l = sc.lights[idx];
     if (l.type == kLightEnv) {
         V3 dir, rad;
-        float pdf;
+        float pdf, plainPdf;
         if (STRATA) {
-            if (!sampleEnvStrata(sc, g, rng, s, dir, rad, pdf, ls.stratum)) return ls;
+            if (!sampleEnvStrata(sc, g, rng, s, dir, rad, pdf, plainPdf, ls.stratum)) return ls;
         } else if (!sampleEnv(sc, g, rng, dir, rad, pdf)) {
             return ls;
+        } else {
+            plainPdf = pdf;
         }
         ls.wi = dir;
         ls.radiance = rad;
         ls.pdf = pdf * pick;
+        ls.misPdf = plainPdf * pick;
         ls.dist = 1e30f;
         ls.valid = true;
     } else if (l.type == kLightSun) {
@@ -378,6 +375,7 @@ __device__ __forceinline__ LightSample sampleOneLight(const SceneView& sc, const
         ls.delta = true;
         ls.valid = true;
     }
+    if (l.type != kLightEnv) ls.misPdf = ls.pdf;
     return ls;
 }
 

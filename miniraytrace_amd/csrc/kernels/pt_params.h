@@ -27,7 +27,6 @@ struct SceneView {
     const GpuLight* lights;
     const float4* envMap;            // envWidth x envHeight RGBA32F, row 0 = top (theta = 0)
     const AliasEntry* envAlias;      // envWidth * envHeight entries
-    const uint8_t* envStratum;       // per texel: 1 in the bright stratum
     const EnvStratumEntry* envA;     // FrameConsts::envCountA entries
     const EnvStratumEntry* envB;     // FrameConsts::envCountB entries
     const GpuTexture* texTable;      // 64 slots
@@ -69,7 +68,6 @@ struct PathView {
     float4* pathRec;                 // [level][path] x 4 float4 training records
     uint32_t* recMask;               // per training path: bit b set when level b holds a record of this frame
     // Learned environment-stratum selection (null when guided sampling is off).
-    float* neeSplit;                 // per path: bright-stratum probability at its last vertex
     uint2* shadowX;                  // per shadow slot: (cell << 1 | stratum, contribution bits) or ~0u
     float4* neeStats;                // kNeeCells cells
     // Guided sampling (null when off): per path, the product over its

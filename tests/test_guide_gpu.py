@@ -295,6 +295,47 @@ def test_guided_is_unbiased(gpu, depth, sky):
     assert (np.abs(z) > 4.5).sum() <= 2
 
 
+def test_guided_mean_matches_plain_at_1080p(gpu):
+    """The room (495k triangles, 2048 x 1024 sky with the sun) at 1920x1080:
+    64 independent 2-frame image means per tracer pin the guided estimator's
+    expectation to about 0.03 % of plain's.  Guided environment NEE draws the
+    sun's stratum more often than plain does, and its MIS weights must be the
+    plain distribution's: with the stratified pdf in them guided converged
+    0.14 % brighter (z about 6 here), which the 96x54 block tests cannot see."""
+    from miniraytrace_amd import scenes
+    from miniraytrace_amd.engine import Engine
+    _print("case: guided vs plain image mean, room 1920x1080, depth 8")
+    W, H, B, F = 1920, 1080, 64, 2
+    sky = scenes.sky()
+
+    def room():
+        e = Engine(width=W, height=H)
+        e.set_render_settings(width=W, height=H, max_bounces=8, spp_limit=10**6, firefly_clamp=0.0, denoise=False)
+        scenes.load_room_scene(e, scenes.CONFIG4, seed=1, sky_img=sky)
+        e.commit()
+        return e
+
+    plain, guided = room(), room()
+    guided.set_guiding(sampling=True, training=True)
+    for _ in range(8):
+        guided.render_frame()
+    means = {"plain": [], "guided": []}
+    for i in range(B):
+        for name, e in (("plain", plain), ("guided", guided)):
+            e.set_path_options(seed=5000 + 31 * i + (name == "guided"))
+            e.reset_accumulation()
+            means[name].append(_render(e, F, W, H).mean())
+    p, q = np.array(means["plain"]), np.array(means["guided"])
+    z = (q.mean() - p.mean()) / np.sqrt(q.var(ddof=1) / B + p.var(ddof=1) / B)
+    _print(f"mean plain {p.mean():.6f} guided {q.mean():.6f} (relative {q.mean() / p.mean() - 1:+.5f}, z {z:.2f}); "
+           f"stats {guided.guiding_stats()}")
+    assert guided.guiding_stats()["envCellsLearned"] > 100  # the strata are in use
+    # |t| > 4.5 at 126 degrees of freedom: probability about 2e-5
+    assert abs(z) < 4.5
+    plain.close()
+    guided.close()
+
+
 def test_guided_lowers_error(gpu):
     """Error against a 4096-spp plain render (the unbiased estimator the
     oracle pins, so a bias of the guided estimator is not shared by the
