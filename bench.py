@@ -364,9 +364,11 @@ def convergence_workload(args, torch, sky) -> dict:
         "gpu_s": t_gpu,
         "reference_cpu_s": t_ref,
     }
+    ratio = rmse / model
+    worst = int(np.abs(ratio - 1).argmax())
     _log("convergence: RMSE " + ", ".join(f"{s_}:{v:.4g}" for s_, v in zip(checkpoints, rmse)) +
-         f"; sigma {out['sigma']:.4f}; worst ratio to the 1/sqrt(spp) model "
-         f"{float(np.abs(rmse / model - 1).max()):.3f}; FP64 reference {t_ref:.1f} s")
+         f"; sigma {out['sigma']:.4f}; RMSE / (1/sqrt(spp) model) from {ratio.min():.3f} to {ratio.max():.3f} "
+         f"(farthest from 1 at {checkpoints[worst]} spp); FP64 reference {t_ref:.1f} s")
     return out
 
 
