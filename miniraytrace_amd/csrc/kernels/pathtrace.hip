@@ -610,14 +610,17 @@ __device__ __forceinline__ void shadowChunk(const SceneView& sc, const PathView&
 // the registers it costs; at 8 waves (64 VGPRs) the spills outweigh it.
 // Shapes whose LDS stacks allow fewer waves per SIMD (160 KB of LDS per CU,
 // S x 8 bytes per lane and a camera-ray packet stack per wave) get the
-// registers of the waves they can hold.
+// registers of the waves they can hold.  The variant that also writes the
+// guided query records (PREP) runs at most 5 waves (96 VGPRs): at 6 it
+// spilled 27 VGPRs; on the guided room its launches took 467 us at 6 and at
+// 4 waves (113 VGPRs, no spills), 454 us at 5 (13 spilled).
 constexpr int traceWavesPerSimd(int block, int stack) {
     const int blocksPerCU = (160 * 1024) / (stack * 8 * block + (block / 64) * kPacketStack * 4);
     const int waves = blocksPerCU * (block / 64) / 4;
     return waves < 1 ? 1 : (waves > 6 ? 6 : waves);
 }
 template <int BLOCK, int S, bool PREP>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(traceWavesPerSimd(BLOCK, S)))) void kTraceBatch(BatchView bv, uint32_t extendLevel, uint32_t shadowLevel,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PREP ? (traceWavesPerSimd(BLOCK, S) > 5 ? 5 : traceWavesPerSimd(BLOCK, S)) : traceWavesPerSimd(BLOCK, S)))) void kTraceBatch(BatchView bv, uint32_t extendLevel, uint32_t shadowLevel,
                                                      uint32_t tailPaths) {
     __shared__ uint2 ldsStack[S * BLOCK];
     __shared__ uint32_t packetStack[BLOCK / 64][kPacketStack];  // per wave, for camera rays
