@@ -280,6 +280,15 @@ typedef struct mrtxGuidingOptions {
     uint32_t seed;            /* network initialisation seed (used when the network is created) */
     float    positionScale;   /* network position input = (p - scene box min) / largest box extent
                                  x positionScale, encoded as sin/cos(2^b pi x), b < 12 (0 = 1) */
+    /* Appended fields, read and written when structSize covers them (0 =
+     * default).  The first warmupFrames trained frames of a network take
+     * warmupStepsPerFrame Adam steps each instead of stepsPerFrame (the
+     * pool then holds batchSize x warmupStepsPerFrame records, split over
+     * the steps): an untrained network learns from its first, smaller
+     * pools in more, smaller steps.  warmupStepsPerFrame = stepsPerFrame
+     * turns the warm-up off. */
+    uint32_t warmupFrames;        /* (32) */
+    uint32_t warmupStepsPerFrame; /* (16) */
 } mrtxGuidingOptions;
 
 typedef struct mrtxGuidingStats {

@@ -163,7 +163,7 @@ class mrtxGuidingOptions(_Sized):
     _fields_ = [("structSize", C.c_size_t), ("enableSampling", C.c_uint32), ("enableTraining", C.c_uint32),
                 ("selectProb", C.c_float), ("learningRate", C.c_float), ("batchSize", C.c_uint32),
                 ("stepsPerFrame", C.c_uint32), ("minRoughness", C.c_float), ("seed", C.c_uint32),
-                ("positionScale", C.c_float)]
+                ("positionScale", C.c_float), ("warmupFrames", C.c_uint32), ("warmupStepsPerFrame", C.c_uint32)]
 
 
 class mrtxGuidingStats(_Sized):

@@ -118,7 +118,7 @@ void renderMerged(const std::vector<mrt::Engine*>& engines, std::vector<uint32_t
             }
             MRT_HIP(mrt::dev::launchTraceBatch(bv, mrt::dev::kNoLevel, levels - 1, sha, tailPaths, false, s));
             for (size_t i = 0; i < n; ++i)
-                if (f < frames[i]) engines[i]->enqueueGuideTraining(s, consts[i] + f);
+                if (f < frames[i]) engines[i]->enqueueGuideTraining(s, consts[i] + f, f);
             MRT_HIP(mrt::dev::launchFinishBatch(bv, s));
         }
     } catch (...) {

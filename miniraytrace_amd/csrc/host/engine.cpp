@@ -150,6 +150,8 @@ Engine::Engine(const mrtEngineDesc& desc) {
     guide_.learningRate = 1e-3f;
     guide_.batchSize = 65536;
     guide_.stepsPerFrame = 4;
+    guide_.warmupFrames = 32;
+    guide_.warmupStepsPerFrame = 16;
     guide_.minRoughness = 0.2f;
     guide_.seed = 1;
     guide_.positionScale = 1.0f;
@@ -932,8 +934,9 @@ void Engine::allocGuide() {
     // one positive-weight record on average (most paths end early or carry
     // no radiance), so sparser recording would leave the Adam batches short of
     // batchSize.  The collection pass samples the pool down to
-    // batchSize x stepsPerFrame records.
-    const uint64_t poolCap = uint64_t(guide_.batchSize) * guide_.stepsPerFrame;
+    // batchSize x (the frame's Adam steps) records; the pool holds the
+    // warm-up's or the steady count, whichever is larger.
+    const uint64_t poolCap = uint64_t(guide_.batchSize) * std::max(guide_.stepsPerFrame, guide_.warmupStepsPerFrame);
     const uint64_t recBytes = uint64_t(n) * levels * 64;
     guideTrainStride_ = uint32_t(std::max<uint64_t>(1, (recBytes + kPathRecordBytes - 1) / kPathRecordBytes));
     const uint32_t trainPaths = (n + guideTrainStride_ - 1) / guideTrainStride_;
@@ -1087,7 +1090,8 @@ void Engine::fillConsts(FrameConsts& g) const {
     g.guideBoxScale = make_float4(ps, ps, ps, 0.0f);
 }
 
-void Engine::enqueueFrame(hipStream_t s, bool timed, const FrameConsts* g, const dev::BatchEntry* entry) {
+void Engine::enqueueFrame(hipStream_t s, bool timed, const FrameConsts* g, const dev::BatchEntry* entry,
+                          uint32_t offset) {
     const uint32_t pixels = frameW_ * frameH_;
     const uint32_t maxB = std::min(settings_.maxBounces, 16u);
     const bool guide = guideFrame();
@@ -1135,7 +1139,7 @@ void Engine::enqueueFrame(hipStream_t s, bool timed, const FrameConsts* g, const
         stats_.stageMs[3] += elapsed(ev_[4], ev_[5]);
     }
     if (timed) MRT_HIP(hipEventRecord(ev_[10], s));
-    enqueueGuideTraining(s, g);
+    enqueueGuideTraining(s, g, offset);
     if (timed) MRT_HIP(hipEventRecord(ev_[6], s));
     MRT_HIP(dev::launchFinishBatch(bv, s));  // accumulate + resolve
     if (timed) {
@@ -1182,12 +1186,18 @@ dev::TraceShape Engine::traceShape(bool guidePrep) const {
     return t;
 }
 
-// Training records of this frame -> pool -> stepsPerFrame Adam steps, batch t
+uint32_t Engine::trainSteps(uint32_t offset) const {
+    if (!pathView().pathRec) return 0;
+    return uint64_t(trainedFrames_) + offset < guide_.warmupFrames ? guide_.warmupStepsPerFrame : guide_.stepsPerFrame;
+}
+
+// Training records of this frame -> pool -> trainSteps Adam steps, batch t
 // taking pool rows t, t + steps, t + 2 steps, ...
-void Engine::enqueueGuideTraining(hipStream_t s, const FrameConsts* g) {
-    if (!pathView().pathRec) return;
-    const uint32_t steps = guide_.stepsPerFrame;
+void Engine::enqueueGuideTraining(hipStream_t s, const FrameConsts* g, uint32_t offset) {
+    const uint32_t steps = trainSteps(offset);
+    if (steps == 0) return;
     const uint32_t poolCap = guide_.batchSize * steps;
+    lastPoolCap_ = poolCap;
     dev::GuideCollectArgs ca;
     ca.pathRec = dPathRec_.as<const float4>();
     ca.recMask = dRecMask_.as<const uint32_t>();
@@ -1245,7 +1255,9 @@ void Engine::renderFrame(mrtFrameInfo* out) {
             }
             stats_.status |= c[dev::kStatusWord];
         } else if (useGraph_) {
+            if (graphExec_ && graphTrainSteps_ != trainSteps(0)) invalidateGraph();  // warm-up over
             if (!graphExec_) {
+                graphTrainSteps_ = trainSteps(0);
                 MRT_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
                 try {
                     enqueueFrame(stream_, false, dConsts_.as<const FrameConsts>(), entry);
@@ -1264,6 +1276,7 @@ void Engine::renderFrame(mrtFrameInfo* out) {
             enqueueFrame(stream_, false, dConsts_.as<const FrameConsts>(), entry);
             MRT_HIP(hipStreamSynchronize(stream_));
         }
+        if (pathView().pathRec) ++trainedFrames_;
         ++frameIndex_;
     }
     stats_.frameMs = msSince(t0);
@@ -1337,7 +1350,7 @@ uint32_t Engine::captureFrames(uint32_t frames, hipGraph_t* out) {
     const dev::BatchEntry* dEntries = dBatchEntries_.as<const dev::BatchEntry>();
     MRT_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     try {
-        for (uint32_t i = 0; i < frames; ++i) enqueueFrame(stream_, false, consts + i, dEntries + i);
+        for (uint32_t i = 0; i < frames; ++i) enqueueFrame(stream_, false, consts + i, dEntries + i, i);
     } catch (...) {
         hipGraph_t g2 = nullptr;
         (void)hipStreamEndCapture(stream_, &g2);
@@ -1352,6 +1365,7 @@ uint32_t Engine::captureFrames(uint32_t frames, hipGraph_t* out) {
 void Engine::framesDone(uint32_t frames) {
     for (uint32_t i = 0; i < frames; ++i)
         if (++guideStamp_ == 0) guideStamp_ = 1;
+    if (pathView().pathRec) trainedFrames_ += frames;
     frameIndex_ += frames;
 }
 
@@ -1442,6 +1456,8 @@ void Engine::setGuiding(const mrtxGuidingOptions& in) {
     if (o.learningRate == 0.0f) o.learningRate = 1e-3f;
     if (o.batchSize == 0) o.batchSize = 65536;
     if (o.stepsPerFrame == 0) o.stepsPerFrame = 4;
+    if (o.warmupFrames == 0) o.warmupFrames = 32;
+    if (o.warmupStepsPerFrame == 0) o.warmupStepsPerFrame = 16;
     if (o.positionScale == 0.0f) o.positionScale = 1.0f;
     if (!(o.positionScale > 0.0f) || !std::isfinite(o.positionScale))
         throw Error(MRT_ERROR_INVALID_ARGUMENT, "guiding positionScale must be > 0");
@@ -1452,10 +1468,13 @@ void Engine::setGuiding(const mrtxGuidingOptions& in) {
     if (o.batchSize < 256 || o.batchSize > (1u << 20))
         throw Error(MRT_ERROR_INVALID_ARGUMENT, "guiding batchSize must be in [256, 1048576]");
     if (o.stepsPerFrame > 64) throw Error(MRT_ERROR_INVALID_ARGUMENT, "guiding stepsPerFrame must be <= 64");
+    if (o.warmupStepsPerFrame > 64)
+        throw Error(MRT_ERROR_INVALID_ARGUMENT, "guiding warmupStepsPerFrame must be <= 64");
     if (!(o.minRoughness >= 0.0f)) throw Error(MRT_ERROR_INVALID_ARGUMENT, "guiding minRoughness must be >= 0");
     DeviceScope scope(device_);
     MRT_HIP(hipStreamSynchronize(stream_));
     const bool realloc = o.batchSize != guide_.batchSize || o.stepsPerFrame != guide_.stepsPerFrame ||
+                         o.warmupStepsPerFrame != guide_.warmupStepsPerFrame ||
                          guidePaths_ != frameW_ * frameH_ || guideLevels_ != guideLevels();
     guide_ = o;
     if (guideActive()) {
@@ -1476,7 +1495,7 @@ void Engine::guidingStats(mrtxGuidingStats& o) {
     o.steps = w[dev::kGwStep];
     o.lastBatch = w[dev::kGwLastBatch];
     std::memcpy(&o.lastLoss, &w[dev::kGwLastLoss], sizeof(float));
-    o.poolRecords = std::min(w[dev::kGwPool], guide_.batchSize * guide_.stepsPerFrame);
+    o.poolRecords = std::min(w[dev::kGwPool], lastPoolCap_);
     o.candidates = w[dev::kGwCand0 + (guideStamp_ & 1u)];
     o.lastFrameRows = stats_.guideRows;
     o.envBrightShare = hasEnv_ ? envSplitA_ : 0.0f;
@@ -1510,6 +1529,7 @@ void Engine::setGuideParams(const float* src) {
     MRT_HIP(hipStreamSynchronize(stream_));
     ensureGuideNet();
     guideNet_->setParams(src);
+    trainedFrames_ = 0;
 }
 
 void Engine::exportScene(mrtxSceneExport& o) {

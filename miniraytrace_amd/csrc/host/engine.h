@@ -103,7 +103,9 @@ public:
     bool guideFrame() const;
     void enqueueGuideFrameBegin(hipStream_t s, const FrameConsts* g);
     void enqueueGuideInfer(hipStream_t s, uint32_t bounce);
-    void enqueueGuideTraining(hipStream_t s, const FrameConsts* g);
+    // offset: the frame's position after the next frame to render (captured
+    // batches enqueue several frames before any of them runs).
+    void enqueueGuideTraining(hipStream_t s, const FrameConsts* g, uint32_t offset = 0);
     // BatchEntry::tailAt of this engine's requests.
     uint32_t tailAt() const;
     // Trace-launch shape (guidePrep: the variant that writes query records).
@@ -161,7 +163,12 @@ private:
     void uploadTextures();
     void allocFrame();
     void fillConsts(FrameConsts& g) const;
-    void enqueueFrame(hipStream_t s, bool timed, const FrameConsts* g, const dev::BatchEntry* entry);
+    void enqueueFrame(hipStream_t s, bool timed, const FrameConsts* g, const dev::BatchEntry* entry,
+                      uint32_t offset = 0);
+    // Adam steps of the frame `offset` frames after the next one (0 when
+    // the engine does not train): the warm-up's count for the network's
+    // first warmupFrames trained frames, then stepsPerFrame.
+    uint32_t trainSteps(uint32_t offset) const;
     void fillEntries(dev::BatchEntry* out, const FrameConsts* g0, uint32_t frames) const;
     void invalidateGraph();
     void chooseShapes();
@@ -240,6 +247,9 @@ private:
     uint32_t guidePaths_ = 0, guideLevels_ = 0;
     uint32_t guideTrainStride_ = 1;  // every guideTrainStride_-th path writes training records
     uint32_t guideStamp_ = 0;
+    uint32_t trainedFrames_ = 0;   // frames trained since the network was created or its parameters set
+    uint32_t graphTrainSteps_ = 0; // Adam steps per frame in the captured frame graph
+    uint32_t lastPoolCap_ = 0;     // training pool size of the last frame enqueued
     float sceneBoxMin_[3] = { 0.0f, 0.0f, 0.0f };
     float sceneBoxScale_ = 1.0f;
 

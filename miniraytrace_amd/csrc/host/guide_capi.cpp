@@ -62,12 +62,19 @@ extern "C" {
 
 // ---- engine guiding ----------------------------------------------------------
 
+// Callers built before the warm-up fields pass the shorter struct: the
+// fields it lacks take their defaults (set) or are not written (get).
+constexpr size_t kGuidingOptionsV1 = offsetof(mrtxGuidingOptions, warmupFrames);
+
 MRT_API mrtResult mrtxSetGuiding(mrtEngine* e, const mrtxGuidingOptions* o) {
     mrt::Engine* eng = unwrap(e);
     if (!eng) return MRT_ERROR_INVALID_HANDLE;
-    if (!validStruct(o)) return MRT_ERROR_INVALID_ARGUMENT;
+    if (!o || o->structSize < kGuidingOptionsV1) return MRT_ERROR_INVALID_ARGUMENT;
     return mrt::guarded([&] {
-        eng->setGuiding(*o);
+        mrtxGuidingOptions full{};
+        std::memcpy(&full, o, std::min(o->structSize, sizeof(full)));
+        full.structSize = sizeof(full);
+        eng->setGuiding(full);
         return MRT_SUCCESS;
     });
 }
@@ -75,8 +82,12 @@ MRT_API mrtResult mrtxSetGuiding(mrtEngine* e, const mrtxGuidingOptions* o) {
 MRT_API mrtResult mrtxGetGuiding(mrtEngine* e, mrtxGuidingOptions* o) {
     mrt::Engine* eng = unwrap(e);
     if (!eng) return MRT_ERROR_INVALID_HANDLE;
-    if (!validStruct(o)) return MRT_ERROR_INVALID_ARGUMENT;
-    eng->getGuiding(*o);
+    if (!o || o->structSize < kGuidingOptionsV1) return MRT_ERROR_INVALID_ARGUMENT;
+    mrtxGuidingOptions full;
+    eng->getGuiding(full);
+    const size_t n = std::min(o->structSize, sizeof(full));
+    full.structSize = o->structSize;
+    std::memcpy(o, &full, n);
     return MRT_SUCCESS;
 }
 

@@ -224,11 +224,15 @@ class Engine:
     # -- path guiding (mini_raytrace_ext.h section 5) --
     def set_guiding(self, sampling: bool = True, training: bool = True, select_prob: float = 0.5,
                     learning_rate: float = 1e-3, batch_size: int = 65536, steps_per_frame: int = 4,
-                    min_roughness: float = 0.2, seed: int = 1, position_scale: float = 1.0) -> None:
+                    min_roughness: float = 0.2, seed: int = 1, position_scale: float = 1.0,
+                    warmup_frames: int = 32, warmup_steps_per_frame: int = 16) -> None:
+        """warmup_steps_per_frame = steps_per_frame: no warm-up (the network's
+        first warmup_frames trained frames otherwise take that many steps)."""
         o = L.mrtxGuidingOptions(enableSampling=int(sampling), enableTraining=int(training),
                                  selectProb=select_prob, learningRate=learning_rate, batchSize=batch_size,
                                  stepsPerFrame=steps_per_frame, minRoughness=min_roughness, seed=seed,
-                                 positionScale=position_scale)
+                                 positionScale=position_scale, warmupFrames=warmup_frames,
+                                 warmupStepsPerFrame=warmup_steps_per_frame)
         L.check(L.lib().mrtxSetGuiding(self._h, C.byref(o)), "mrtxSetGuiding")
 
     def guiding(self) -> dict:

@@ -420,6 +420,47 @@ def test_training_batches_are_full_at_720p(gpu):
     assert st["lastBatch"] >= 0.95 * 65536
 
 
+def test_training_warmup_schedule(gpu):
+    """The first warmupFrames trained frames of a network take
+    warmupStepsPerFrame Adam steps, later frames stepsPerFrame: through the
+    single-frame graph (re-captured when the warm-up ends) and through a
+    merged batch (one graph holding frames of both phases).  A caller built
+    before the warm-up fields passes the shorter struct and gets the
+    defaults."""
+    import ctypes as C
+    from miniraytrace_amd import _lib as L
+    from miniraytrace_amd.batch import render_batch
+    _print("case: training warm-up schedule")
+    W, H = 64, 36
+    e = _room_engine(W, H, 10**6, depth=4)
+    e.set_guiding(sampling=True, training=True, batch_size=1024, steps_per_frame=2, warmup_frames=3,
+                  warmup_steps_per_frame=5)
+    counts = []
+    for _ in range(5):
+        e.render_frame()
+        counts.append(e.guiding_stats()["steps"])
+    _print("steps after each frame:", counts)
+    assert counts == [5, 10, 15, 17, 19]
+    b = _room_engine(W, H, 10**6, depth=4)
+    b.set_guiding(sampling=True, training=True, batch_size=1024, steps_per_frame=2, warmup_frames=3,
+                  warmup_steps_per_frame=5)
+    st = render_batch([b], [5])
+    assert st["framesDone"] == [5] and b.guiding_stats()["steps"] == 19
+    render_batch([b], [2])
+    assert b.guiding_stats()["steps"] == 23
+    # the struct without the warm-up fields: accepted, defaults (32 frames x 16 steps)
+    o = L.mrtxGuidingOptions(enableSampling=1, enableTraining=1, batchSize=1024, stepsPerFrame=2)
+    o.structSize = L.mrtxGuidingOptions.warmupFrames.offset
+    c = _room_engine(W, H, 10**6, depth=4)
+    L.check(L.lib().mrtxSetGuiding(c.handle, C.byref(o)), "mrtxSetGuiding")
+    opts = c.guiding()
+    assert opts["warmupFrames"] == 32 and opts["warmupStepsPerFrame"] == 16 and opts["stepsPerFrame"] == 2
+    c.render_frame()
+    assert c.guiding_stats()["steps"] == 16
+    for x in (e, b, c):
+        x.close()
+
+
 def test_autotune_does_not_train_or_change_settings(gpu):
     """mrtxAutotunePath's trial frames leave a training engine's network,
     Adam step count and spp limit as they were (a size class no other test
