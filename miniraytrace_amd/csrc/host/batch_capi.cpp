@@ -7,6 +7,7 @@
 #include "mini_raytrace_ext.h"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <map>
 #include <set>
@@ -38,14 +39,34 @@ struct BatchGraphs {
 // that also writes guided requests' query records.
 mrt::dev::TraceShape voteShape(const std::vector<mrt::Engine*>& engines, const std::vector<uint32_t>& frames,
                                bool shadow, bool guidePrep) {
-    std::map<std::pair<uint32_t, uint32_t>, double> weight;
-    for (size_t i = 0; i < engines.size(); ++i) {
-        const mrt::dev::TraceShape& s = shadow ? engines[i]->shadowShape() : engines[i]->extendShape();
-        weight[{ s.block, s.stack }] += double(engines[i]->pixels()) * frames[i];
+    constexpr int K = mrt::Engine::kShapeCandidates;
+    // When every request was autotuned: the candidate with the least total
+    // measured time, each request's per-frame time times its frames (a sum
+    // over many requests evens out their single-frame measurements).
+    std::array<double, K> total{};
+    bool timed = true;
+    for (size_t i = 0; i < engines.size() && timed; ++i) {
+        float ms[K];
+        timed = engines[i]->shapeTimes(ms);
+        for (int k = 0; timed && k < K; ++k) total[k] += double(ms[k]) * frames[i];
     }
-    auto best = std::max_element(weight.begin(), weight.end(),
-                                 [](const auto& a, const auto& b) { return a.second < b.second; });
-    mrt::dev::TraceShape t{ best->first.first, best->first.second, 0 };
+    mrt::dev::TraceShape t{ 0, 0, 0 };
+    if (timed) {
+        const int k = int(std::min_element(total.begin(), total.end()) - total.begin());
+        t.block = mrt::Engine::kShapeCand[k][0];
+        t.stack = mrt::Engine::kShapeCand[k][1];
+    } else {
+        // Otherwise the shape most of the work (pixels x frames) runs with.
+        std::map<std::pair<uint32_t, uint32_t>, double> weight;
+        for (size_t i = 0; i < engines.size(); ++i) {
+            const mrt::dev::TraceShape& s = shadow ? engines[i]->shadowShape() : engines[i]->extendShape();
+            weight[{ s.block, s.stack }] += double(engines[i]->pixels()) * frames[i];
+        }
+        auto best = std::max_element(weight.begin(), weight.end(),
+                                     [](const auto& a, const auto& b) { return a.second < b.second; });
+        t.block = best->first.first;
+        t.stack = best->first.second;
+    }
     int perCU = 0;
     MRT_HIP(mrt::dev::batchTraceBlocksPerCU(t.block, t.stack, guidePrep, &perCU));
     t.grid = uint32_t(std::max(1, perCU) * engines[0]->cuCount());

@@ -4,6 +4,7 @@
 #include "bvh_gpu_build.h"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -1375,6 +1376,15 @@ void Engine::framesDone(uint32_t frames) {
 // the image; the trial frames are discarded (accumulation restarts) and do
 // not train a guiding network.  Results are cached per (triangle count, pixel
 // count, depth, compute units) class.
+const uint32_t Engine::kShapeCand[Engine::kShapeCandidates][2] = { { 64, 16 }, { 128, 8 }, { 128, 16 },
+                                                                    { 128, 32 }, { 256, 8 }, { 256, 16 } };
+
+bool Engine::shapeTimes(float out[kShapeCandidates]) const {
+    if (!shapeTimed_) return false;
+    std::copy(shapeMs_, shapeMs_ + kShapeCandidates, out);
+    return true;
+}
+
 void Engine::autotuneShapes(mrtxPathOptions& chosen) {
     struct Key {
         int tris, pixels, depth, cus;
@@ -1383,7 +1393,11 @@ void Engine::autotuneShapes(mrtxPathOptions& chosen) {
         }
     };
     static std::mutex cacheMutex;
-    static std::map<Key, std::pair<dev::TraceShape, dev::TraceShape>> cache;
+    struct Entry {
+        dev::TraceShape extend, shadow;
+        std::array<float, kShapeCandidates> ms;
+    };
+    static std::map<Key, Entry> cache;
     commit(nullptr);
     // The class includes the device's compute-unit count: the best shape
     // depends on how many waves the chip holds.
@@ -1395,15 +1409,15 @@ void Engine::autotuneShapes(mrtxPathOptions& chosen) {
         std::lock_guard<std::mutex> lock(cacheMutex);
         auto it = cache.find(key);
         if (it != cache.end()) {
-            extendShape_.block = it->second.first.block;
-            extendShape_.stack = it->second.first.stack;
-            shadowShape_.block = it->second.second.block;
-            shadowShape_.stack = it->second.second.stack;
+            extendShape_.block = it->second.extend.block;
+            extendShape_.stack = it->second.extend.stack;
+            shadowShape_.block = it->second.shadow.block;
+            shadowShape_.stack = it->second.shadow.stack;
+            std::copy(it->second.ms.begin(), it->second.ms.end(), shapeMs_);
             hit = true;
         }
     }
     if (!hit) {
-        static const uint32_t kCand[][2] = { { 64, 16 }, { 128, 8 }, { 128, 16 }, { 128, 32 }, { 256, 8 }, { 256, 16 } };
         // Trial frames run profiled, without an spp limit and without
         // training (so the network, its Adam state and the training-record
         // stamp are as before); the guard restores all of it on any exit.
@@ -1425,13 +1439,14 @@ void Engine::autotuneShapes(mrtxPathOptions& chosen) {
         // with the least time in those kernels over a whole frame.
         float best = 1e30f;
         dev::TraceShape be = extendShape_, bs = shadowShape_;
-        for (const auto& c : kCand) {
-            extendShape_.block = shadowShape_.block = c[0];
-            extendShape_.stack = shadowShape_.stack = c[1];
+        for (int k = 0; k < kShapeCandidates; ++k) {
+            extendShape_.block = shadowShape_.block = kShapeCand[k][0];
+            extendShape_.stack = shadowShape_.stack = kShapeCand[k][1];
             chooseShapes();
             resetAccumulation();
             renderFrame(nullptr);
             const float t = stats_.stageMs[1] + stats_.stageMs[3] + stats_.stageMs[5];
+            shapeMs_[k] = t;
             if (t < best) {
                 best = t;
                 be = extendShape_;
@@ -1440,9 +1455,12 @@ void Engine::autotuneShapes(mrtxPathOptions& chosen) {
         }
         extendShape_ = be;
         shadowShape_ = bs;
+        Entry entry{ be, bs, {} };
+        std::copy(shapeMs_, shapeMs_ + kShapeCandidates, entry.ms.begin());
         std::lock_guard<std::mutex> lock(cacheMutex);
-        cache[key] = { be, bs };
+        cache[key] = entry;
     }
+    shapeTimed_ = true;
     chooseShapes();
     resetAccumulation();
     getPathOptions(chosen);

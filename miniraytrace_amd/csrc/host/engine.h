@@ -117,6 +117,12 @@ public:
     int cuCount() const { return cuCount_; }
     void framesDone(uint32_t frames);
     void autotuneShapes(mrtxPathOptions& chosen);
+    // Launch shapes the autotuner tries (block, LDS stack depth), and this
+    // engine's measured trace + tail time per frame for each (false until
+    // the engine has been autotuned).
+    static constexpr int kShapeCandidates = 6;
+    static const uint32_t kShapeCand[kShapeCandidates][2];
+    bool shapeTimes(float out[kShapeCandidates]) const;
     int device() const { return device_; }
     hipStream_t stream() const { return stream_; }
 
@@ -248,6 +254,8 @@ private:
     uint32_t guideTrainStride_ = 1;  // every guideTrainStride_-th path writes training records
     uint32_t guideStamp_ = 0;
     uint32_t trainedFrames_ = 0;   // frames trained since the network was created or its parameters set
+    float shapeMs_[kShapeCandidates] = {};
+    bool shapeTimed_ = false;
     uint32_t graphTrainSteps_ = 0; // Adam steps per frame in the captured frame graph
     uint32_t lastPoolCap_ = 0;     // training pool size of the last frame enqueued
     float sceneBoxMin_[3] = { 0.0f, 0.0f, 0.0f };
