@@ -1178,7 +1178,8 @@ void Engine::enqueueGuideInfer(hipStream_t s, uint32_t bounce) {
 // training frames (per-sample variance guided / plain, ms per frame): 1/4
 // 0.383 at 6.99 ms, 1/16 0.220 at 7.30, 1/64 0.219 at 7.38, never 0.217 at
 // 7.38.  The deep levels pay since roulette reads the BSDF-only throughput
-// (PathView::rrScale).
+// (PathView::rrScale).  With the training warm-up (equal-RMSE time against
+// plain): 1/8 0.304, 1/16 0.204, 1/32 0.206.
 uint32_t Engine::tailAt() const { return guideFrame() ? frameW_ * frameH_ / 16 : ~0u; }
 
 dev::TraceShape Engine::traceShape(bool guidePrep) const {
