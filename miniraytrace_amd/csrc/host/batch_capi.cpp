@@ -107,7 +107,7 @@ void renderMerged(const std::vector<mrt::Engine*>& engines, std::vector<uint32_t
     mrt::DeviceBuffer dTable, dCursors;
     dTable.allocate(table.size() * sizeof(mrt::dev::BatchEntry));
     dTable.upload(table.data(), table.size() * sizeof(mrt::dev::BatchEntry));
-    dCursors.allocate(mrt::dev::kMaxBounceLevels * 4 * sizeof(uint32_t));
+    dCursors.allocate(mrt::dev::kCursorWords * sizeof(uint32_t));
     const mrt::dev::TraceShape ext = voteShape(engines, frames, false, guided);
     const mrt::dev::TraceShape sha = voteShape(engines, frames, true, false);
     const uint32_t shadeGrid = engines[0]->shadeGrid(guided);

@@ -892,21 +892,22 @@ void Engine::allocFrame() {
     const size_t n = size_t(settings_.width) * settings_.height;
     frameW_ = settings_.width;
     frameH_ = settings_.height;
+    const size_t q = queueSlots();  // ray and shadow queues: kQueueSegs segments
     dConsts_.ensure(sizeof(FrameConsts));
     dBeta_.allocate(n * 16);
     dRad_.allocate(n * 16);
     dRng_.allocate(n * 4);
     for (int i = 0; i < 2; ++i) {
-        dRayO_[i].allocate(n * 16);
-        dRayD_[i].allocate(n * 16);
+        dRayO_[i].allocate(q * 16);
+        dRayD_[i].allocate(q * 16);
     }
-    dHit_.allocate(n * 16);
-    dHitInst_.allocate(n * 4);
-    dShO_.allocate(n * 16);
-    dShD_.allocate(n * 16);
-    dShC_.allocate(n * 16);
+    dHit_.allocate(q * 16);
+    dHitInst_.allocate(q * 4);
+    dShO_.allocate(q * 16);
+    dShD_.allocate(q * 16);
+    dShC_.allocate(q * 16);
     dCounters_.ensure(dev::kCounterWords * 4);
-    dCursors_.ensure(dev::kMaxBounceLevels * 4 * 4);
+    dCursors_.ensure(dev::kCursorWords * 4);
     dEntry_.ensure(sizeof(dev::BatchEntry));
     hEntry_.ensure(sizeof(dev::BatchEntry));
     dAccum_.allocate(n * 16);
@@ -926,7 +927,7 @@ void Engine::allocFrame() {
 void Engine::allocGuide() {
     const uint32_t n = frameW_ * frameH_;
     const uint32_t levels = guideLevels();
-    dGuideIdx_.allocate(size_t(n) * 4);
+    dGuideIdx_.allocate(queueSlots() * 4);  // per ray-queue slot
     dGuideRec_.allocate(size_t(n) * 3 * 16);
     dGuideMix_.allocate(size_t(n) * dev::kGuideMixFloat4 * 16);
     // Training paths: every path of the frame keeps its per-level records
@@ -946,7 +947,7 @@ void Engine::allocGuide() {
     dRecMask_.allocate(size_t(trainPaths) * 4);  // reset per frame by ray generation
     dGuidePool_.allocate(size_t(poolCap) * 4 * 16);
     dRrScale_.allocate(size_t(n) * 4);
-    dShX_.allocate(size_t(n) * 8);
+    dShX_.allocate(queueSlots() * 8);  // per shadow-queue slot
     dNeeStats_.allocate(size_t(kNeeCells) * 16);
     dNeeStats_.zero(stream_);
     MRT_HIP(hipStreamSynchronize(stream_));
@@ -991,6 +992,7 @@ dev::SceneView Engine::sceneView() const {
 dev::PathView Engine::pathView() const {
     dev::PathView p;
     p.capacity = frameW_ * frameH_;
+    p.segCap = dev::queueSegCap(p.capacity);
     p.beta = dBeta_.as<float4>();
     p.radiance = dRad_.as<float4>();
     p.rngCounter = dRng_.as<uint32_t>();
@@ -1168,7 +1170,7 @@ void Engine::enqueueGuideInfer(hipStream_t s, uint32_t bounce) {
     dev::RecView rv;
     rv.base = pv.guideRec;
     rv.stride = 3;
-    MRT_HIP(dev::launchGuideInfer(rv, pv.counters + bounce * dev::kCntPerBounce + dev::kCntGuideRecords,
+    MRT_HIP(dev::launchGuideInfer(rv, pv.counters + dev::cntWord(bounce, dev::kCntGuideRecords, 0),
                                   frameW_ * frameH_, guideNet_->deviceImage(), pv.guideMix, true, s));
 }
 
@@ -1249,11 +1251,14 @@ void Engine::renderFrame(mrtFrameInfo* out) {
             const uint32_t* c = static_cast<const uint32_t*>(hCounters_.get());
             stats_.guideRows = 0;
             for (uint32_t b = 0; b < dev::kMaxBounceLevels; ++b) {
-                stats_.guideRows += c[b * dev::kCntPerBounce + dev::kCntGuideRecords];
-                stats_.rays[b] = c[b * dev::kCntPerBounce + dev::kCntRays] +
-                                 c[b * dev::kCntPerBounce + dev::kCntTailRays];
-                stats_.shadowRays[b] = c[b * dev::kCntPerBounce + dev::kCntShadows] +
-                                       c[b * dev::kCntPerBounce + dev::kCntTailShadows];
+                stats_.guideRows += c[dev::cntWord(b, dev::kCntGuideRecords, 0)];
+                stats_.rays[b] = 0;
+                stats_.shadowRays[b] = 0;
+                for (uint32_t q = 0; q < dev::kQueueSegs; ++q) {
+                    stats_.rays[b] += c[dev::cntWord(b, dev::kCntRays, q)] + c[dev::cntWord(b, dev::kCntTailRays, q)];
+                    stats_.shadowRays[b] +=
+                        c[dev::cntWord(b, dev::kCntShadows, q)] + c[dev::cntWord(b, dev::kCntTailShadows, q)];
+                }
             }
             stats_.status |= c[dev::kStatusWord];
         } else if (useGraph_) {

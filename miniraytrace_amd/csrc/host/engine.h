@@ -179,6 +179,8 @@ private:
     void invalidateGraph();
     void chooseShapes();
     uint32_t guideLevels() const { return std::min(settings_.maxBounces, 16u) + 1; }
+    // Slots of each ray and shadow-ray queue (kQueueSegs segments of the frame's paths).
+    size_t queueSlots() const { return size_t(dev::kQueueSegs) * dev::queueSegCap(frameW_ * frameH_); }
     void ensureGuideNet();
     void allocGuide();
 

@@ -153,35 +153,66 @@ __device__ __forceinline__ uint32_t batchRequestOfSlot(const BatchView& bv, uint
     return m ? 63u - uint32_t(__builtin_clzll(m)) : 0u;
 }
 
-// Work split of one queue stage over all requests: lane j holds request j's
-// queue length and the inclusive prefix of 64-item chunks.
+// Work split of one queue segment of a stage over all requests: lane j holds
+// request j's item count in the segment and the inclusive prefix of 64-item
+// chunks.
 struct BatchChunks {
     uint32_t n;      // lane j: items of request j in this split
     uint32_t incl;   // lane j: chunks of requests 0..j
     uint32_t total;
-    bool tail;       // lane j: the tail kernel serves request j at this level
 };
 
-// Which part of a level's queues a kernel works on.  The tail kernel serves
-// request j at a level when all requests together hold at most tailPaths
-// rays there and request j holds at most its own limit (BatchEntry::tailAt;
-// guided requests hand over only their last few paths); the per-level
-// kernels serve the rest.
+// Lane j: request j's counter array (null when request j is absent or does
+// not render this frame).
+__device__ __forceinline__ const uint32_t* laneCounters(const BatchView& bv) {
+    const uint32_t j = laneId();
+    return j < bv.count && bv.entries[j].active ? bv.entries[j].pv.counters : nullptr;
+}
+
+// Lane j: request j's items of (level, slot) in each segment.
+__device__ __forceinline__ void laneSegmentCounts(const uint32_t* cnt, uint32_t level, uint32_t slot,
+                                                  uint32_t (&n)[kQueueSegs]) {
+#pragma unroll
+    for (uint32_t s = 0; s < kQueueSegs; ++s) n[s] = cnt ? cnt[cntWord(level, slot, s)] : 0u;
+}
+
+// Lane s (< kQueueSegs): the 64-item chunks of segment s over the requests
+// whose lane has `keep` set (lane j: request j's counts n).
+__device__ __forceinline__ uint32_t segmentTotals(const uint32_t (&n)[kQueueSegs], bool keep) {
+    uint32_t out = 0;
+#pragma unroll
+    for (uint32_t s = 0; s < kQueueSegs; ++s) {
+        uint32_t c = keep ? (n[s] + 63u) / 64u : 0u;
+        for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off, 64);
+        if (laneId() == s) out = c;
+    }
+    return out;
+}
+
+// Lane j: whether the tail kernel serves request j at a level where it holds
+// n rays: all requests together hold at most tailPaths rays there and
+// request j holds at most its own limit (BatchEntry::tailAt; guided requests
+// hand over only their last few paths).  The per-level kernels serve the rest.
+__device__ __forceinline__ bool batchTailLane(const BatchView& bv, const uint32_t (&n)[kQueueSegs],
+                                              uint32_t tailPaths) {
+    uint32_t mine = 0;
+#pragma unroll
+    for (uint32_t s = 0; s < kQueueSegs; ++s) mine += n[s];
+    const uint32_t tailAt = mine > 0 ? bv.entries[laneId()].tailAt : 0u;
+    uint32_t all = mine;
+    for (int off = 32; off >= 1; off >>= 1) all += __shfl_xor(all, off, 64);
+    return mine > 0 && all <= tailPaths && mine <= tailAt;
+}
+
+// Which part of a level's queues a kernel works on (tail: batchTailLane).
 enum BatchPart : int { kPartAll = 0, kPartLevel = 1, kPartTail = 2 };
 
-__device__ __forceinline__ BatchChunks batchChunks(const BatchView& bv, uint32_t bounce, uint32_t slot,
-                                                   uint32_t tailPaths = 0, int part = kPartAll) {
+__device__ __forceinline__ BatchChunks batchChunks(const uint32_t* cnt, uint32_t bounce, uint32_t slot, uint32_t seg,
+                                                   bool tail = false, int part = kPartAll) {
     const uint32_t j = laneId();
     BatchChunks bc;
-    uint32_t n = 0, tailAt = 0;
-    if (j < bv.count && bv.entries[j].active) {
-        n = bv.entries[j].pv.counters[bounce * kCntPerBounce + slot];
-        tailAt = bv.entries[j].tailAt;
-    }
-    uint32_t all = n;
-    for (int off = 32; off >= 1; off >>= 1) all += __shfl_xor(all, off, 64);
-    bc.tail = n > 0 && all <= tailPaths && n <= tailAt;
-    if ((part == kPartLevel && bc.tail) || (part == kPartTail && !bc.tail)) n = 0;
+    uint32_t n = cnt ? cnt[cntWord(bounce, slot, seg)] : 0u;
+    if ((part == kPartLevel && tail) || (part == kPartTail && !tail)) n = 0;
     bc.n = n;
     uint32_t c = (n + 63u) / 64u;
     for (int off = 1; off < 64; off <<= 1) {
@@ -211,56 +242,79 @@ __device__ __forceinline__ uint32_t batchFetch(uint32_t* cursor) {
     return __builtin_amdgcn_readfirstlane(chunk);
 }
 
-// Next chunk of the stage for this wave: request r (wave-uniform), the
-// request's queue length and the chunk's first item.  False when done.
-// An empty stage returns before touching the cursor: every wave of the
-// persistent grid would otherwise queue one atomic on the same word.
-__device__ __forceinline__ bool batchNext(const BatchChunks& bc, uint32_t* cursor, uint32_t& r, uint32_t& count,
-                                          uint32_t& base) {
-    if (bc.total == 0) return false;
-    const uint32_t chunk = batchFetch(cursor);
-    if (chunk >= bc.total) return false;
-    batchLocate(bc, chunk, r, count, base);
-    return true;
-}
-
-// Per frame: every request's counters and the batch's work cursors to zero.
-__global__ __launch_bounds__(256) void kClearBatch(BatchView bv) {
-    const uint32_t words = bv.count * kCounterWords + kMaxBounceLevels * 4;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x) {
-        if (i < bv.count * kCounterWords) bv.entries[i / kCounterWords].pv.counters[i % kCounterWords] = 0u;
-        else bv.cursors[i - bv.count * kCounterWords] = 0u;
+// Work loop of a stage over its queue segments (cursors: the stage's segment
+// cursors, tot: lane s holds the chunks of segment s).  A wave takes chunks
+// of its XCD's segment, then of the others in turn, one atomic per chunk;
+// a segment without chunks costs nothing and a spent one a single atomic.
+// prepare(seg) loads the work split of segment seg once the wave holds a
+// chunk of it; work(seg, chunk) runs the chunk.
+template <class Prepare, class Work>
+__device__ __forceinline__ void segmentLoop(uint32_t* cursors, uint32_t tot, Prepare&& prepare, Work&& work) {
+    const uint32_t home = xccId();
+#pragma unroll 1
+    for (uint32_t t = 0; t < kQueueSegs; ++t) {
+        const uint32_t seg = (home + t) % kQueueSegs;
+        const uint32_t total = __builtin_amdgcn_readlane(tot, seg);
+        if (total == 0) continue;
+        bool ready = false;
+        while (true) {
+            const uint32_t chunk = batchFetch(cursors + seg * kLineWords);
+            if (chunk >= total) break;
+            if (!ready) {
+                prepare(seg);
+                ready = true;
+            }
+            work(seg, chunk);
+        }
     }
 }
 
+// Per frame: every request's counters and the batch's work cursors to zero
+// (the first word of each counter and cursor line; the rest is padding).
+__global__ __launch_bounds__(256) void kClearBatch(BatchView bv) {
+    const uint32_t lines = bv.count * kCounterLines + kCursorLines;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < lines; i += gridDim.x * blockDim.x) {
+        if (i < bv.count * kCounterLines)
+            bv.entries[i / kCounterLines].pv.counters[(i % kCounterLines) * kLineWords] = 0u;
+        else
+            bv.cursors[(i - bv.count * kCounterLines) * kLineWords] = 0u;
+    }
+}
+
+// Camera ray of pixel slot p goes to queue slot p: segment p / segCap.
 __global__ __launch_bounds__(256) void kGenerateBatch(BatchView bv) {
     for (uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; w < bv.pixelSlots; w += gridDim.x * blockDim.x) {
         const uint32_t r = batchRequestOfSlot(bv, w);
         const BatchEntry& e = bv.entries[r];
         if (!e.active) continue;
         const uint32_t p = w - e.pixelBase + laneId();
-        if (p == 0) e.pv.counters[kCntRays] = e.pixels;
+        if (p < kQueueSegs) {
+            const uint32_t lo = min(e.pixels, p * e.pv.segCap);
+            e.pv.counters[cntWord(0, kCntRays, p)] = min(e.pv.segCap, e.pixels - lo);
+        }
         if (p < e.pixels) generatePixel(e.pv, *e.g, p);
     }
 }
 
 // ------------------------------------------------------------------ extend --
-// Closest hits for queue slots [base, base + 64) of one request's ray queue.
-// PREP && prep (a guided request at a level the tail kernel does not take):
-// the same wave then forms the guiding network's query record of every hit
-// on a surface that may be guided (opaque, roughness >= guideMinRoughness)
-// -- (pos01, n, wo) at a compacted row -- and the row number in guideIdx
-// (~0u elsewhere), while the hit is still in registers.
+// Closest hits for items [base, base + 64) of segment `seg` of one request's
+// ray queue (count items in the segment).  PREP && prep (a guided request at
+// a level the tail kernel does not take): the same wave then forms the
+// guiding network's query record of every hit on a surface that may be
+// guided (opaque, roughness >= guideMinRoughness) -- (pos01, n, wo) at a
+// compacted row -- and the row number in guideIdx (~0u elsewhere), while the
+// hit is still in registers.
 template <int BLOCK, int S, bool PREP>
 __device__ __forceinline__ void extendChunk(const SceneView& sc, const PathView& pv, const FrameConsts* __restrict__ gp,
-                                            uint32_t bounce, uint32_t count, uint32_t base, uint2* lds,
+                                            uint32_t bounce, uint32_t seg, uint32_t count, uint32_t base, uint2* lds,
                                             bool prep, uint32_t* waveStack) {
-    const uint32_t k = base + laneId();
+    const uint32_t i = base + laneId();
+    const uint32_t k = seg * pv.segCap + i;  // queue slot
     bool overflow = false;
     bool hit = false;
     SceneHit h;
     V3 dir = v3(0.0f, 0.0f, 0.0f);
-    if (k < count) {
+    if (i < count) {
         const float4 o = pv.rayO[bounce & 1][k], d = pv.rayD[bounce & 1][k];
         dir = v3(d);
         // Camera rays (level 0) come in 8 x 8 pixel tiles per wave: the wave
@@ -285,8 +339,8 @@ __device__ __forceinline__ void extendChunk(const SceneView& sc, const PathView&
             pos01 = (s.pos - v3(gp->guideBoxMin)) * v3(gp->guideBoxScale);
             n = s.nShade;
         }
-        const uint32_t row = waveAppend(pv.counters + bounce * kCntPerBounce + kCntGuideRecords, guide);
-        if (k < count) pv.guideIdx[k] = guide ? row : kNoHit;
+        const uint32_t row = waveAppend(pv.counters + cntWord(bounce, kCntGuideRecords, 0), guide);
+        if (i < count) pv.guideIdx[k] = guide ? row : kNoHit;
         if (guide) {
             float4* r = pv.guideRec + size_t(row) * 3;
             r[0] = f4(pos01, 0.0f);
@@ -501,30 +555,32 @@ __device__ __forceinline__ ShadeOut shadePath(const SceneView& sc, const PathVie
     return o;
 }
 
-// Shading of queue slots [base, base + 64) of ray queue `bounce`: new rays
-// to queue bounce + 1, shadow rays to the shadow queue.
+// Shading of items [base, base + 64) of segment `seg` of ray queue
+// `bounce`: new rays to the same segment of queue bounce + 1, shadow rays to
+// the same segment of the shadow queue (each item adds at most one of each,
+// so a segment never outgrows its segCap slots).
 template <bool GUIDE>
 __device__ __forceinline__ void shadeChunk(const SceneView& sc, const PathView& pv, const FrameConsts& g,
-                                           uint32_t bounce, uint32_t count, uint32_t base) {
-    uint32_t* cnt = pv.counters + bounce * kCntPerBounce;
-    uint32_t* next = pv.counters + (bounce + 1) * kCntPerBounce;
-    const uint32_t k = base + laneId();
+                                           uint32_t bounce, uint32_t seg, uint32_t count, uint32_t base) {
+    const uint32_t segBase = seg * pv.segCap;
+    const uint32_t i = base + laneId();
+    const uint32_t k = segBase + i;  // queue slot
     ShadeOut o;
     o.emitRay = false;
     o.emitShadow = false;
     uint32_t p = 0;
-    if (k < count) {
+    if (i < count) {
         const float4 ro = pv.rayO[bounce & 1][k], rd = pv.rayD[bounce & 1][k];
         p = __float_as_uint(ro.w);
         o = shadePath<GUIDE>(sc, pv, g, bounce, p, v3(rd), pv.hit[k], pv.hitInst[k],
                              GUIDE && pv.guideIdx ? pv.guideIdx[k] : kNoHit);
     }
-    const uint32_t rs = waveAppend(next + kCntRays, o.emitRay);
+    const uint32_t rs = segBase + waveAppend(pv.counters + cntWord(bounce + 1, kCntRays, seg), o.emitRay);
     if (o.emitRay) {
         pv.rayO[(bounce + 1) & 1][rs] = f4(o.newO, __uint_as_float(p));
         pv.rayD[(bounce + 1) & 1][rs] = f4(o.newD, 0.0f);
     }
-    const uint32_t ss = waveAppend(cnt + kCntShadows, o.emitShadow);
+    const uint32_t ss = segBase + waveAppend(pv.counters + cntWord(bounce, kCntShadows, seg), o.emitShadow);
     if (o.emitShadow) {
         pv.shadowO[ss] = f4(o.shO, o.shMax);
         pv.shadowD[ss] = f4(o.shD, __uint_as_float(p));
@@ -542,13 +598,22 @@ __device__ __forceinline__ void shadeChunk(const SceneView& sc, const PathView& 
 // MIS with selection probability 0, still unbiased).
 template <bool GUIDE>
 __global__ __launch_bounds__(256) void kShadeBatch(BatchView bv, uint32_t bounce, uint32_t tailPaths) {
-    const BatchChunks bc = batchChunks(bv, bounce, kCntRays, tailPaths, kPartLevel);
-    uint32_t r, count, base;
-    while (batchNext(bc, bv.cursors + bounce * 4 + 1, r, count, base)) {
-        const BatchEntry& e = bv.entries[r];
-        const FrameConsts g = *e.g;
-        shadeChunk<GUIDE>(e.sc, e.pv, g, bounce, count, base);
-    }
+    uint32_t n[kQueueSegs];
+    laneSegmentCounts(laneCounters(bv), bounce, kCntRays, n);
+    const bool tail = batchTailLane(bv, n, tailPaths);
+    const uint32_t tot = segmentTotals(n, !tail);
+    if (__ballot(tot > 0) == 0) return;  // nothing for this kernel: no atomics
+    BatchChunks bc;
+    segmentLoop(
+        bv.cursors + cursorWord(bounce, 1, 0), tot,
+        [&](uint32_t seg) { bc = batchChunks(laneCounters(bv), bounce, kCntRays, seg, tail, kPartLevel); },
+        [&](uint32_t seg, uint32_t chunk) {
+            uint32_t r, count, base;
+            batchLocate(bc, chunk, r, count, base);
+            const BatchEntry& e = bv.entries[r];
+            const FrameConsts g = *e.g;
+            shadeChunk<GUIDE>(e.sc, e.pv, g, bounce, seg, count, base);
+        });
 }
 
 // ------------------------------------------------------------------ shadow --
@@ -570,14 +635,15 @@ __device__ __forceinline__ void applyShadow(const PathView& pv, const FrameConst
     }
 }
 
-// Any hit for shadow slots [base, base + 64) of one request; unoccluded
-// slots add their contribution to the path's radiance.
+// Any hit for items [base, base + 64) of segment `seg` of one request's
+// shadow queue; unoccluded ones add their contribution to the path's radiance.
 template <int BLOCK, int S>
 __device__ __forceinline__ void shadowChunk(const SceneView& sc, const PathView& pv, const FrameConsts* __restrict__ gp,
-                                            uint32_t bounce, uint32_t count, uint32_t base, uint2* lds) {
-    const uint32_t k = base + laneId();
+                                            uint32_t bounce, uint32_t seg, uint32_t count, uint32_t base, uint2* lds) {
+    const uint32_t i = base + laneId();
+    const uint32_t k = seg * pv.segCap + i;  // shadow queue slot
     bool overflow = false;
-    if (k < count) {
+    if (i < count) {
         const float4 o = pv.shadowO[k], d = pv.shadowD[k];
         SceneHit h;
         const bool occluded = traceSceneWide<true, S, BLOCK>(sc, gp->tlasWideRoot, gp->instanceCount,
@@ -624,30 +690,36 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PREP ? (t
                                                      uint32_t tailPaths) {
     __shared__ uint2 ldsStack[S * BLOCK];
     __shared__ uint32_t packetStack[BLOCK / 64][kPacketStack];  // per wave, for camera rays
-    const BatchChunks be = extendLevel != kNoLevel ? batchChunks(bv, extendLevel, kCntRays, tailPaths)
-                                                   : BatchChunks{ 0, 0, 0, false };
-    const BatchChunks bs = shadowLevel != kNoLevel ? batchChunks(bv, shadowLevel, kCntShadows)
-                                                   : BatchChunks{ 0, 0, 0, false };
-    if (be.total == 0 && bs.total == 0) return;  // levels the tail kernel finished: no atomics
-    uint32_t* cursor = bv.cursors + (extendLevel != kNoLevel ? extendLevel * 4 + 0 : shadowLevel * 4 + 2);
-    while (true) {
-        const uint32_t chunk = batchFetch(cursor);
-        uint32_t r, count, base;
-        if (chunk < be.total) {
-            batchLocate(be, chunk, r, count, base);
-            const BatchEntry& e = bv.entries[r];
-            // query records for guided requests whose level the tail kernel does not take
-            const bool prep = PREP && e.pv.guideIdx != nullptr && !__shfl(int(be.tail), int(r), 64);
-            extendChunk<BLOCK, S, PREP>(e.sc, e.pv, e.g, extendLevel, count, base, ldsStack + threadIdx.x, prep,
-                                        packetStack[threadIdx.x >> 6]);
-        } else if (chunk - be.total < bs.total) {
-            batchLocate(bs, chunk - be.total, r, count, base);
-            const BatchEntry& e = bv.entries[r];
-            shadowChunk<BLOCK, S>(e.sc, e.pv, e.g, shadowLevel, count, base, ldsStack + threadIdx.x);
-        } else {
-            break;
-        }
-    }
+    uint32_t n[kQueueSegs];
+    laneSegmentCounts(extendLevel != kNoLevel ? laneCounters(bv) : nullptr, extendLevel, kCntRays, n);
+    // query records for guided requests whose level the tail kernel does not take
+    const bool tail = PREP && batchTailLane(bv, n, tailPaths);
+    uint32_t tot = segmentTotals(n, true);
+    laneSegmentCounts(shadowLevel != kNoLevel ? laneCounters(bv) : nullptr, shadowLevel, kCntShadows, n);
+    tot += segmentTotals(n, true);
+    if (__ballot(tot > 0) == 0) return;  // levels the tail kernel finished: no atomics
+    BatchChunks be, bs;
+    segmentLoop(
+        bv.cursors + (extendLevel != kNoLevel ? cursorWord(extendLevel, 0, 0) : cursorWord(shadowLevel, 2, 0)), tot,
+        [&](uint32_t seg) {
+            const uint32_t* cnt = laneCounters(bv);
+            be = extendLevel != kNoLevel ? batchChunks(cnt, extendLevel, kCntRays, seg) : BatchChunks{ 0, 0, 0 };
+            bs = shadowLevel != kNoLevel ? batchChunks(cnt, shadowLevel, kCntShadows, seg) : BatchChunks{ 0, 0, 0 };
+        },
+        [&](uint32_t seg, uint32_t chunk) {
+            uint32_t r, count, base;
+            if (chunk < be.total) {
+                batchLocate(be, chunk, r, count, base);
+                const BatchEntry& e = bv.entries[r];
+                const bool prep = PREP && e.pv.guideIdx != nullptr && !__shfl(int(tail), int(r), 64);
+                extendChunk<BLOCK, S, PREP>(e.sc, e.pv, e.g, extendLevel, seg, count, base, ldsStack + threadIdx.x,
+                                            prep, packetStack[threadIdx.x >> 6]);
+            } else {
+                batchLocate(bs, chunk - be.total, r, count, base);
+                const BatchEntry& e = bv.entries[r];
+                shadowChunk<BLOCK, S>(e.sc, e.pv, e.g, shadowLevel, seg, count, base, ldsStack + threadIdx.x);
+            }
+        });
 }
 
 // ------------------------------------------------------------------- tail --
@@ -663,17 +735,18 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PREP ? (t
 // it covers go to the kCntTail* words (statistics; the queues stay empty).
 template <int BLOCK, int S>
 __device__ __forceinline__ void tailChunk(const SceneView& sc, const PathView& pv, const FrameConsts& g,
-                                          uint32_t bounce, uint32_t count, uint32_t base, uint2* lds) {
-    const uint32_t k = base + laneId();
+                                          uint32_t bounce, uint32_t seg, uint32_t count, uint32_t base, uint2* lds) {
+    const uint32_t i = base + laneId();
+    const uint32_t k = seg * pv.segCap + i;  // queue slot
     bool overflow = false;
-    if (k < count) {
+    if (i < count) {
         const uint32_t p = __float_as_uint(pv.rayO[bounce & 1][k].w);
         V3 dir = v3(pv.rayD[bounce & 1][k]);
         float4 h = pv.hit[k];
         uint32_t inst = pv.hitInst[k];
         for (uint32_t b = bounce;; ++b) {
             const ShadeOut o = shadePath<false>(sc, pv, g, b, p, dir, h, inst, kNoHit);
-            waveAppend(pv.counters + b * kCntPerBounce + kCntTailShadows, o.emitShadow);
+            waveAppend(pv.counters + cntWord(b, kCntTailShadows, seg), o.emitShadow);
             if (o.emitShadow) {
                 SceneHit sh;
                 const bool occluded = traceSceneWide<true, S, BLOCK, false>(sc, g.tlasWideRoot, g.instanceCount,
@@ -683,7 +756,7 @@ __device__ __forceinline__ void tailChunk(const SceneView& sc, const PathView& p
                 if (!occluded) applyShadow(pv, g, b, p, o.shC);
             }
             if (!o.emitRay) break;
-            waveAppend(pv.counters + (b + 1) * kCntPerBounce + kCntTailRays, true);
+            waveAppend(pv.counters + cntWord(b + 1, kCntTailRays, seg), true);
             SceneHit hh;
             const bool hit = traceSceneWide<false, S, BLOCK, false>(sc, g.tlasWideRoot, g.instanceCount,
                                                                 make_float3(o.newO.x, o.newO.y, o.newO.z),
@@ -706,14 +779,22 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
                                                                                             uint32_t bounce,
                                                                                             uint32_t tailPaths) {
     __shared__ uint2 ldsStack[S * BLOCK];
-    const BatchChunks bc = batchChunks(bv, bounce, kCntRays, tailPaths, kPartTail);
-    if (bc.total == 0) return;
-    uint32_t r, count, base;
-    while (batchNext(bc, bv.cursors + bounce * 4 + 3, r, count, base)) {
-        const BatchEntry& e = bv.entries[r];
-        const FrameConsts g = *e.g;
-        tailChunk<BLOCK, S>(e.sc, e.pv, g, bounce, count, base, ldsStack + threadIdx.x);
-    }
+    uint32_t n[kQueueSegs];
+    laneSegmentCounts(laneCounters(bv), bounce, kCntRays, n);
+    const bool tail = batchTailLane(bv, n, tailPaths);
+    const uint32_t tot = segmentTotals(n, tail);
+    if (__ballot(tot > 0) == 0) return;
+    BatchChunks bc;
+    segmentLoop(
+        bv.cursors + cursorWord(bounce, 3, 0), tot,
+        [&](uint32_t seg) { bc = batchChunks(laneCounters(bv), bounce, kCntRays, seg, tail, kPartTail); },
+        [&](uint32_t seg, uint32_t chunk) {
+            uint32_t r, count, base;
+            batchLocate(bc, chunk, r, count, base);
+            const BatchEntry& e = bv.entries[r];
+            const FrameConsts g = *e.g;
+            tailChunk<BLOCK, S>(e.sc, e.pv, g, bounce, seg, count, base, ldsStack + threadIdx.x);
+        });
 }
 
 // -------------------------------------------------------------- accumulate --
@@ -815,8 +896,8 @@ static hipError_t traceBatchT(const BatchView& bv, uint32_t extendLevel, uint32_
 }
 
 hipError_t launchClearBatch(const BatchView& bv, hipStream_t s) {
-    const uint32_t words = bv.count * kCounterWords + kMaxBounceLevels * 4;
-    hipLaunchKernelGGL(kClearBatch, dim3(gridFor(words, 256, 1024)), dim3(256), 0, s, bv);
+    const uint32_t lines = bv.count * kCounterLines + kCursorLines;
+    hipLaunchKernelGGL(kClearBatch, dim3(gridFor(lines, 256, 1024)), dim3(256), 0, s, bv);
     return hipGetLastError();
 }
 

@@ -547,6 +547,11 @@ __device__ __forceinline__ float balanceHeuristic(float a, float b) { return a /
 __device__ __forceinline__ uint32_t laneId() {
     return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }
+// The XCD this wave runs on, modulo 8 (hardware register HW_REG_XCC_ID = 20,
+// bits [3:0]); wave-uniform.  Work placement only: any value is correct.
+__device__ __forceinline__ uint32_t xccId() {
+    return __builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 7u;
+}
 // Wave-aggregated append: one atomic per wave; returns this lane's slot
 // (only meaningful where pred is true).  All lanes of the wave that are
 // active must call it.

@@ -12,8 +12,16 @@ namespace dev {
 // Bounce levels 0..16 (maxBounces is clamped to 16) plus one for the queue
 // a level-16 shade would fill.
 constexpr uint32_t kMaxBounceLevels = 18;
-constexpr uint32_t kStatusWord = kMaxBounceLevels * kCntPerBounce;  // bit 0: stack overflow
-constexpr uint32_t kCounterWords = kStatusWord + 8;
+constexpr uint32_t kStatusWord = cntWord(kMaxBounceLevels, 0, 0);  // bit 0: stack overflow
+constexpr uint32_t kCounterWords = kStatusWord + kLineWords;
+constexpr uint32_t kCounterLines = kCounterWords / kLineWords;  // words in use: the first of each line
+// Work cursors of a frame, one line each: stage 0 trace (closest hits, then
+// the shadow rays sharing the launch), 1 shade, 2 shadow-only trace, 3 tail.
+__host__ __device__ constexpr uint32_t cursorWord(uint32_t level, uint32_t stage, uint32_t seg) {
+    return ((level * 4 + stage) * kQueueSegs + seg) * kLineWords;
+}
+constexpr uint32_t kCursorLines = kMaxBounceLevels * 4 * kQueueSegs;
+constexpr uint32_t kCursorWords = kCursorLines * kLineWords;
 constexpr uint32_t kNoLevel = 0xFFFFFFFFu;  // "no queue" for launchTraceBatch
 
 struct TraceShape {

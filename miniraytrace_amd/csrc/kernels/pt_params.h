@@ -34,19 +34,41 @@ struct SceneView {
     const float4* tex32;             // RGBA32F texel pool
 };
 
-// Per-bounce counters, 8 words per bounce level.
+// Queue segments.  Every ray and shadow-ray queue of a request is split into
+// kQueueSegs segments of PathView::segCap slots (slot = segment x segCap +
+// item), each with its own counters and work cursors.  Device-scope atomics
+// on one address serialise (about 11 ns each on the MI355X, whatever the
+// number of waves), so a queue with one counter and one work cursor is
+// capped near 90M appends or chunk fetches per second; eight segments, each
+// word on a 128-byte line of its own, spread them over eight addresses.  A
+// wave starts on the segment of its XCD and moves on to the others when that
+// one is spent.  Level-0 segments are contiguous ranges of camera-ray slots
+// (image bands), and a chunk's new rays and shadow rays go to its own
+// segment, so a segment never holds more than its level-0 rays.
+constexpr uint32_t kQueueSegs = 8;
+constexpr uint32_t kLineWords = 32;  // one 128-byte line
+
+// Per-bounce counters: word cntWord(level, slot, segment).
 enum CounterSlot : uint32_t {
-    kCntRays = 0,        // rays queued for this bounce
-    kCntShadows = 1,     // shadow rays queued at this bounce
-    kCntShadeFetch = 3,  // work-fetch cursor of guided shading
-    kCntGuideRecords = 5,
-    kCntTailRays = 6,    // rays traced at this bounce by the tail kernel (statistics)
-    kCntTailShadows = 7, // shadow rays traced at this bounce by the tail kernel (statistics)
-    kCntPerBounce = 8,
+    kCntRays = 0,          // rays queued for this bounce, per segment
+    kCntShadows = 1,       // shadow rays queued at this bounce, per segment
+    kCntGuideRecords = 2,  // guiding query records of this bounce (segment 0)
+    kCntTailRays = 3,      // rays traced at this bounce by the tail kernel (statistics, per segment)
+    kCntTailShadows = 4,   // shadow rays traced at this bounce by the tail kernel (statistics, per segment)
+    kCntSlots = 5,
 };
+__host__ __device__ constexpr uint32_t cntWord(uint32_t level, uint32_t slot, uint32_t seg) {
+    return ((level * kCntSlots + slot) * kQueueSegs + seg) * kLineWords;
+}
+// Slots per queue segment for `paths` paths: a multiple of 64, so a wave's
+// 64 slots never straddle two segments.
+__host__ __device__ constexpr uint32_t queueSegCap(uint32_t paths) {
+    return ((paths + kQueueSegs - 1) / kQueueSegs + 63u) & ~63u;
+}
 
 struct PathView {
     uint32_t capacity;               // paths (= pixels) per frame
+    uint32_t segCap;                 // queueSegCap(capacity): queues hold kQueueSegs x segCap slots
     float4* beta;                    // rgb throughput + previous BSDF pdf
     float4* radiance;                // rgb radiance + BSDF pdf of the last continuation (guided shading)
     uint32_t* rngCounter;            // draws consumed so far
@@ -57,7 +79,7 @@ struct PathView {
     float4* shadowO;                 // origin xyz + tmax
     float4* shadowD;                 // direction xyz + path index bits
     float4* shadowC;                 // unoccluded contribution rgb
-    uint32_t* counters;              // kCntPerBounce words per bounce
+    uint32_t* counters;              // kCounterWords: cntWord(level, slot, segment), then the status word
     float4* accum;                   // rgb sum + sample count, per pixel
     uint32_t* outRgba8;              // resolved image
     float4* outRgba32;               // accum / spp, alpha 1
@@ -96,7 +118,7 @@ struct BatchView {
     const BatchEntry* entries;       // count entries, pixelBase ascending
     uint32_t count;                  // <= kMaxBatchRequests
     uint32_t pixelSlots;             // padded pixel space of all requests
-    uint32_t* cursors;               // [bounce level][4] work cursors, cleared per frame
+    uint32_t* cursors;               // kCursorWords: cursorWord(level, stage, segment), cleared per frame
 };
 
 } // namespace dev
