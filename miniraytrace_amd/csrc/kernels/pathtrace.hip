@@ -733,52 +733,117 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PREP ? (t
 // per-level launch then mostly waits for its slowest ray, while here each
 // wave only waits for its own paths.  Ray and shadow-ray counts of the levels
 // it covers go to the kCntTail* words (statistics; the queues stay empty).
+//
+// A vertex's shadow ray and its continuation are independent, so they are
+// traced together: the shadow ray of a lane that also continues moves to a
+// lane of the wave that has no ray of its own this round (its path ended, or
+// ends here without a shadow ray), and one traversal serves closest-hit and
+// any-hit lanes alike.  Deep levels hold few live paths per wave, so a
+// level's wait becomes the longer of the two walks instead of their sum.
+// Shadow rays for which no lane is free are traced by their own lane after.
+// pairs: 128 words of LDS for this wave.
 template <int BLOCK, int S>
 __device__ __forceinline__ void tailChunk(const SceneView& sc, const PathView& pv, const FrameConsts& g,
-                                          uint32_t bounce, uint32_t seg, uint32_t count, uint32_t base, uint2* lds) {
-    const uint32_t i = base + laneId();
+                                          uint32_t bounce, uint32_t seg, uint32_t count, uint32_t base, uint2* lds,
+                                          uint32_t* pairs) {
+    const uint32_t lane = laneId();
+    const uint32_t i = base + lane;
     const uint32_t k = seg * pv.segCap + i;  // queue slot
     bool overflow = false;
-    if (i < count) {
-        const uint32_t p = __float_as_uint(pv.rayO[bounce & 1][k].w);
-        V3 dir = v3(pv.rayD[bounce & 1][k]);
-        float4 h = pv.hit[k];
-        uint32_t inst = pv.hitInst[k];
-        for (uint32_t b = bounce;; ++b) {
-            const ShadeOut o = shadePath<false>(sc, pv, g, b, p, dir, h, inst, kNoHit);
-            waveAppend(pv.counters + cntWord(b, kCntTailShadows, seg), o.emitShadow);
-            if (o.emitShadow) {
-                SceneHit sh;
-                const bool occluded = traceSceneWide<true, S, BLOCK, false>(sc, g.tlasWideRoot, g.instanceCount,
-                                                                        make_float3(o.shO.x, o.shO.y, o.shO.z),
-                                                                        make_float3(o.shD.x, o.shD.y, o.shD.z), kEpsRay,
-                                                                        o.shMax, lds, sh, overflow);
-                if (!occluded) applyShadow(pv, g, b, p, o.shC);
-            }
-            if (!o.emitRay) break;
-            waveAppend(pv.counters + cntWord(b + 1, kCntTailRays, seg), true);
-            SceneHit hh;
-            const bool hit = traceSceneWide<false, S, BLOCK, false>(sc, g.tlasWideRoot, g.instanceCount,
-                                                                make_float3(o.newO.x, o.newO.y, o.newO.z),
-                                                                make_float3(o.newD.x, o.newD.y, o.newD.z), kEpsRay, 1e30f,
-                                                                lds, hh, overflow);
-            h = hit ? make_float4(hh.t, hh.b1, hh.b2, __uint_as_float(hh.tri))
-                    : make_float4(1e30f, 0.0f, 0.0f, __uint_as_float(kNoHit));
-            inst = hit ? hh.inst : kNoHit;
+    bool alive = i < count;
+    uint32_t p = 0, inst = kNoHit;
+    V3 dir = v3(0.0f, 0.0f, 0.0f);
+    float4 h = make_float4(1e30f, 0.0f, 0.0f, __uint_as_float(kNoHit));
+    if (alive) {
+        p = __float_as_uint(pv.rayO[bounce & 1][k].w);
+        dir = v3(pv.rayD[bounce & 1][k]);
+        h = pv.hit[k];
+        inst = pv.hitInst[k];
+    }
+    for (uint32_t b = bounce; __ballot(alive) != 0; ++b) {
+        ShadeOut o;
+        o.emitRay = false;
+        o.emitShadow = false;
+        if (alive) o = shadePath<false>(sc, pv, g, b, p, dir, h, inst, kNoHit);
+        waveAppend(pv.counters + cntWord(b, kCntTailShadows, seg), o.emitShadow);
+        waveAppend(pv.counters + cntWord(b + 1, kCntTailRays, seg), o.emitRay);
+        // Pair the r-th lane that needs a helper with the r-th free lane.
+        const bool cont = o.emitRay;
+        const bool need = o.emitShadow && cont;
+        const bool freeLane = !cont && !o.emitShadow;
+        const uint64_t needMask = __ballot(need), freeMask = __ballot(freeLane);
+        const uint32_t needRank = __builtin_amdgcn_mbcnt_hi(uint32_t(needMask >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo(uint32_t(needMask), 0u));
+        const uint32_t freeRank = __builtin_amdgcn_mbcnt_hi(uint32_t(freeMask >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo(uint32_t(freeMask), 0u));
+        const uint32_t nNeed = uint32_t(__builtin_popcountll(needMask));
+        const uint32_t nFree = uint32_t(__builtin_popcountll(freeMask));
+        if (need) pairs[needRank] = lane;
+        if (freeLane) pairs[64 + freeRank] = lane;
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        const bool helper = freeLane && freeRank < nNeed;
+        const bool helped = need && needRank < nFree;
+        const uint32_t partner = helper ? pairs[freeRank] : (helped ? pairs[64 + needRank] : lane);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        const float pox = __shfl(o.shO.x, int(partner), 64), poy = __shfl(o.shO.y, int(partner), 64),
+                    poz = __shfl(o.shO.z, int(partner), 64);
+        const float pdx = __shfl(o.shD.x, int(partner), 64), pdy = __shfl(o.shD.y, int(partner), 64),
+                    pdz = __shfl(o.shD.z, int(partner), 64);
+        const float pmax = __shfl(o.shMax, int(partner), 64);
+        // One walk for every lane with a ray: continuations (closest hit),
+        // shadow rays of lanes that end here, and the shadow rays handed over.
+        const bool ownShadow = o.emitShadow && !cont;
+        const bool anyRay = ownShadow || helper;
+        bool traced = false;
+        SceneHit hh;
+        if (cont || anyRay) {
+            const float3 ro = cont ? make_float3(o.newO.x, o.newO.y, o.newO.z)
+                                   : (ownShadow ? make_float3(o.shO.x, o.shO.y, o.shO.z) : make_float3(pox, poy, poz));
+            const float3 rd = cont ? make_float3(o.newD.x, o.newD.y, o.newD.z)
+                                   : (ownShadow ? make_float3(o.shD.x, o.shD.y, o.shD.z) : make_float3(pdx, pdy, pdz));
+            const float tmax = cont ? 1e30f : (ownShadow ? o.shMax : pmax);
+            traced = traceSceneWide<false, S, BLOCK, false>(sc, g.tlasWideRoot, g.instanceCount, ro, rd, kEpsRay, tmax,
+                                                            lds, hh, overflow, anyRay);
+        }
+        const uint64_t occHelped = __ballot(helper && traced);
+        bool occluded = ownShadow && traced;
+        if (helped) occluded = ((occHelped >> partner) & 1u) != 0;
+        if (need && !helped) {
+            SceneHit sh;
+            occluded = traceSceneWide<true, S, BLOCK, false>(sc, g.tlasWideRoot, g.instanceCount,
+                                                             make_float3(o.shO.x, o.shO.y, o.shO.z),
+                                                             make_float3(o.shD.x, o.shD.y, o.shD.z), kEpsRay, o.shMax,
+                                                             lds, sh, overflow);
+        }
+        if (o.emitShadow && !occluded) applyShadow(pv, g, b, p, o.shC);
+        if (cont) {
+            h = traced ? make_float4(hh.t, hh.b1, hh.b2, __uint_as_float(hh.tri))
+                       : make_float4(1e30f, 0.0f, 0.0f, __uint_as_float(kNoHit));
+            inst = traced ? hh.inst : kNoHit;
             dir = o.newD;
         }
+        alive = cont;
     }
     if (overflow) atomicOr(pv.counters + kStatusWord, 1u);
 }
 
 // Serves the requests batchChunks assigns to the tail at level `bounce`
 // (kShadeBatch serves the others).  Held to at most 168 VGPRs (3 waves per SIMD,
-// a few spills): measured faster than the compiler's 183 at 2 waves.
+// a few spills): measured faster than 2 waves without spills (1080p frame
+// 4.85 against 5.02 ms).  Shapes whose LDS (S x 8 bytes per lane and 512
+// bytes per wave of hand-over slots, 160 KB per CU) holds fewer waves get
+// the registers of the waves they can hold.
+constexpr int tailWavesPerSimd(int block, int stack) {
+    const int blocksPerCU = (160 * 1024) / (stack * 8 * block + (block / 64) * 512);
+    const int waves = blocksPerCU * (block / 64) / 4;
+    return waves < 1 ? 1 : (waves > 3 ? 3 : waves);
+}
 template <int BLOCK, int S>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void kTailBatch(BatchView bv,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(tailWavesPerSimd(BLOCK, S)))) void kTailBatch(BatchView bv,
                                                                                             uint32_t bounce,
                                                                                             uint32_t tailPaths) {
     __shared__ uint2 ldsStack[S * BLOCK];
+    __shared__ uint32_t pairLds[BLOCK / 64][128];  // per wave: shadow-ray hand-over
     uint32_t n[kQueueSegs];
     laneSegmentCounts(laneCounters(bv), bounce, kCntRays, n);
     const bool tail = batchTailLane(bv, n, tailPaths);
@@ -793,7 +858,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
             batchLocate(bc, chunk, r, count, base);
             const BatchEntry& e = bv.entries[r];
             const FrameConsts g = *e.g;
-            tailChunk<BLOCK, S>(e.sc, e.pv, g, bounce, seg, count, base, ldsStack + threadIdx.x);
+            tailChunk<BLOCK, S>(e.sc, e.pv, g, bounce, seg, count, base, ldsStack + threadIdx.x,
+                                pairLds[threadIdx.x >> 6]);
         });
 }
 

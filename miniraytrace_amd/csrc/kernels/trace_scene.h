@@ -40,10 +40,11 @@ __device__ __forceinline__ void toObject(const TraceInstance& ti, float ox, floa
 template <bool ANY>
 __device__ __forceinline__ bool testTriangle(const RayK& r, const float4& v0, const float4& v1, const float4& v2,
                                              uint32_t k, uint32_t curInst, float tmax, float& best, bool& found,
-                                             uint32_t& bestTri, uint32_t& bestInst, float& bestB1, float& bestB2) {
+                                             uint32_t& bestTri, uint32_t& bestInst, float& bestB1, float& bestB2,
+                                             bool anyLane = false) {
     float t, b1, b2;
     if (!isectTri(r, v0, v1, v2, best, t, b1, b2)) return false;
-    if (ANY) return t < tmax;
+    if (ANY || anyLane) return t < tmax;
     const bool better = t < best ||
                         (found && t == best && (curInst < bestInst || (curInst == bestInst && k < bestTri)));
     if (better) {
@@ -60,10 +61,10 @@ __device__ __forceinline__ bool testTriangle(const RayK& r, const float4& v0, co
 template <bool ANY>
 __device__ __forceinline__ bool blasTriangle(const SceneView& sc, const RayK& r, uint32_t k, uint32_t curInst,
                                              float tmax, float& best, bool& found, uint32_t& bestTri,
-                                             uint32_t& bestInst, float& bestB1, float& bestB2) {
+                                             uint32_t& bestInst, float& bestB1, float& bestB2, bool anyLane = false) {
     const float4* tv = sc.triVerts + 3 * size_t(k);
     return testTriangle<ANY>(r, ldGlobal4(tv), ldGlobal4(tv + 1), ldGlobal4(tv + 2), k, curInst, tmax, best, found,
-                             bestTri, bestInst, bestB1, bestB2);
+                             bestTri, bestInst, bestB1, bestB2, anyLane);
 }
 
 // ANY = false: closest hit in (tmin, tmax) -> returns true and fills `out`.
@@ -200,11 +201,13 @@ __device__ bool traceScene(const SceneView& sc, uint32_t tlasRoot, uint32_t inst
 // children farthest first; a leaf step is as in traceScene.  The hits are
 // those of traceScene: the box tests are conservative and the tie rule is
 // the same, so neither tree shape nor visiting order changes a result.
-// Camera-ray packets keep the binary walk (traceScenePacket).
+// Camera-ray packets keep the binary walk (traceScenePacket).  anyLane:
+// this lane's ray is an any-hit query whatever ANY says (lanes of one call
+// may mix closest-hit and any-hit rays).
 template <bool ANY, int S, int BLOCK, bool TRI_STEP = true>
 __device__ bool traceSceneWide(const SceneView& sc, uint32_t tlasWideRoot, uint32_t instanceCount,
                                float3 o, float3 d, float tmin, float tmax, uint2* ldsLane,
-                               SceneHit& out, bool& overflow) {
+                               SceneHit& out, bool& overflow, bool anyLane = false) {
     if (instanceCount == 0) return false;
     RayK r = setupRayBox(o.x, o.y, o.z, d.x, d.y, d.z, tmin);
     WideRows rows = wideRows(r);
@@ -258,7 +261,7 @@ __device__ bool traceSceneWide(const SceneView& sc, uint32_t tlasWideRoot, uint3
                 }
             } else if (inBlas) {
                 if constexpr (TRI_STEP) {
-                    if (blasTriangle<ANY>(sc, r, lp, curInst, tmax, best, found, bestTri, bestInst, bestB1, bestB2))
+                    if (blasTriangle<ANY>(sc, r, lp, curInst, tmax, best, found, bestTri, bestInst, bestB1, bestB2, anyLane))
                         return true;
                     if (cnt > 1) {
                         cur = Cursor{ lp + 1, cnt - 1 };
@@ -266,7 +269,7 @@ __device__ bool traceSceneWide(const SceneView& sc, uint32_t tlasWideRoot, uint3
                     }
                 } else {
                     for (uint32_t k = lp; k < lp + cnt; ++k)
-                        if (blasTriangle<ANY>(sc, r, k, curInst, tmax, best, found, bestTri, bestInst, bestB1, bestB2))
+                        if (blasTriangle<ANY>(sc, r, k, curInst, tmax, best, found, bestTri, bestInst, bestB1, bestB2, anyLane))
                             return true;
                 }
             } else {
@@ -300,7 +303,7 @@ __device__ bool traceSceneWide(const SceneView& sc, uint32_t tlasWideRoot, uint3
         if (!have && !pendingInst) break;
     }
     if (stack.overflow) overflow = true;
-    if (ANY || !found) return false;
+    if (ANY || anyLane || !found) return false;
     out.t = best;
     out.b1 = bestB1;
     out.b2 = bestB2;
