@@ -178,6 +178,9 @@ typedef struct mrtxFrameStats {
     uint32_t geometryRepacks;   /* commits that repacked every mesh, since creation */
     uint32_t meshesWritten;     /* meshes rewritten alone (in place, or appended
                                    behind the others), since creation */
+    uint32_t graphTailLevel;    /* level at which the last frame's graph handed every
+                                   path left to the tail kernel and ended (0xFFFFFFFF:
+                                   per-level launches to the last bounce, or no graph) */
 } mrtxFrameStats;
 
 /* Profiling mode: each frame runs stage by stage with HIP events between the

@@ -139,7 +139,7 @@ class mrtxFrameStats(_Sized):
                 ("rays", C.c_uint32 * MRTX_MAX_BOUNCE_LEVELS),
                 ("shadowRays", C.c_uint32 * MRTX_MAX_BOUNCE_LEVELS), ("status", C.c_uint32),
                 ("commitMs", C.c_float), ("commitUploadBytes", C.c_uint64), ("geometryRepacks", C.c_uint32),
-                ("meshesWritten", C.c_uint32)]
+                ("meshesWritten", C.c_uint32), ("graphTailLevel", C.c_uint32)]
 
 
 class mrtxBatchRequest(C.Structure):

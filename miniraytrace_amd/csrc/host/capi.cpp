@@ -251,11 +251,13 @@ MRT_API mrtResult mrtxGetFrameStats(mrtEngine* e, mrtxFrameStats* out) {
         out->shadowRays[b] = s.shadowRays[b];
     }
     out->status = s.status;
-    if (out->structSize < sizeof(mrtxFrameStats)) return MRT_SUCCESS;
+    if (out->structSize < offsetof(mrtxFrameStats, graphTailLevel)) return MRT_SUCCESS;
     out->commitMs = s.commitMs;
     out->commitUploadBytes = s.commitUploadBytes;
     out->geometryRepacks = s.geometryRepacks;
     out->meshesWritten = s.meshesWritten;
+    if (out->structSize < offsetof(mrtxFrameStats, graphTailLevel) + sizeof(uint32_t)) return MRT_SUCCESS;
+    out->graphTailLevel = s.graphTailLevel;
     return MRT_SUCCESS;
 }
 

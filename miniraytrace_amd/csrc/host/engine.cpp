@@ -919,6 +919,8 @@ void Engine::allocFrame() {
     MRT_HIP(hipStreamSynchronize(stream_));
     hConsts_.ensure(sizeof(FrameConsts));
     hCounters_.ensure(dev::kCounterWords * 4);
+    hLevelRays_.ensure(dev::kMaxBounceLevels * 4);
+    levelRaysFresh_ = false;
     if (guideActive()) allocGuide();
     invalidateGraph();
 }
@@ -1019,6 +1021,7 @@ dev::PathView Engine::pathView() const {
     p.shadowX = nee ? dShX_.as<uint2>() : nullptr;
     p.neeStats = nee ? dNeeStats_.as<float4>() : nullptr;
     p.rrScale = guide && guide_.enableSampling ? dRrScale_.as<float>() : nullptr;
+    p.levelRays = static_cast<uint32_t*>(hLevelRays_.get());
     return p;
 }
 
@@ -1093,8 +1096,40 @@ void Engine::fillConsts(FrameConsts& g) const {
     g.guideBoxScale = make_float4(ps, ps, ps, 0.0f);
 }
 
+// Depth of a captured plain frame.  Once a level's queue holds at most
+// tailPaths_ rays, the tail kernel finishes every path of it, so the later
+// per-level launches find empty queues (about 7 us each, 16 of them at 1080p
+// with depth 8).  The graph therefore ends with a tail launch that takes
+// every path left at the first level the last frame held at most tailPaths_
+// rays in.  The tail shades exactly as the per-level kernels do, so the image
+// does not depend on the level chosen; a level that holds more rays than the
+// threshold in a later frame only makes the tail slower, and the graph is
+// captured again (tailLevelFits).  Guided frames keep every launch: their
+// tail vertices are shaded unguided.
+uint32_t Engine::fittedTailLevel() const {
+    if (!levelRaysFresh_ || tailPaths_ == 0 || guideFrame()) return ~0u;
+    const uint32_t* n = static_cast<const uint32_t*>(hLevelRays_.get());
+    const uint32_t maxB = std::min(settings_.maxBounces, 16u);
+    for (uint32_t b = 0; b <= maxB; ++b)
+        if (n[b] <= tailPaths_) return b;
+    return ~0u;
+}
+
+// Whether a graph ending at `level` still suits the last frame: its last
+// level within 1/8 above the threshold, the level before it more than 1/8
+// below (the margins keep a queue size near the threshold from capturing
+// the graph again every frame).
+bool Engine::tailLevelFits(uint32_t level) const {
+    if (!levelRaysFresh_) return true;
+    if (level == ~0u) return fittedTailLevel() == ~0u;
+    if (tailPaths_ == 0 || guideFrame()) return false;
+    const uint32_t* n = static_cast<const uint32_t*>(hLevelRays_.get());
+    const uint64_t tp = tailPaths_;
+    return n[level] <= tp + tp / 8 && (level == 0 || n[level - 1] > tp - tp / 8);
+}
+
 void Engine::enqueueFrame(hipStream_t s, bool timed, const FrameConsts* g, const dev::BatchEntry* entry,
-                          uint32_t offset) {
+                          uint32_t offset, uint32_t tailLevel) {
     const uint32_t pixels = frameW_ * frameH_;
     const uint32_t maxB = std::min(settings_.maxBounces, 16u);
     const bool guide = guideFrame();
@@ -1117,14 +1152,18 @@ void Engine::enqueueFrame(hipStream_t s, bool timed, const FrameConsts* g, const
     // rays of level b - 1), the network inference, shade and tail; the
     // deepest level's shadow rays get a trace launch of their own.
     for (uint32_t b = 0; b <= maxB; ++b) {
+        // At tailLevel the tail kernel takes every path (threshold ~0u; the
+        // requests of a plain frame have no limit of their own).
+        const bool last = b == tailLevel;
+        const uint32_t tp = last ? ~0u : tailPaths_;
         if (timed) MRT_HIP(hipEventRecord(ev_[2], s));
-        MRT_HIP(dev::launchTraceBatch(bv, b, b == 0 ? dev::kNoLevel : b - 1, traceShape(guide), tailPaths_, guide, s));
+        MRT_HIP(dev::launchTraceBatch(bv, b, b == 0 ? dev::kNoLevel : b - 1, traceShape(guide), tp, guide, s));
         if (timed) MRT_HIP(hipEventRecord(ev_[3], s));
         enqueueGuideInfer(s, b);
         if (timed) MRT_HIP(hipEventRecord(ev_[9], s));
-        MRT_HIP(dev::launchShadeBatch(bv, b, tailPaths_, guide, shadeGrid(guide), s));
+        if (!last) MRT_HIP(dev::launchShadeBatch(bv, b, tp, guide, shadeGrid(guide), s));
         if (timed) MRT_HIP(hipEventRecord(ev_[4], s));
-        MRT_HIP(dev::launchTailBatch(bv, b, tailPaths_, tailShape(), s));
+        MRT_HIP(dev::launchTailBatch(bv, b, tp, tailShape(), s));
         if (timed) {
             MRT_HIP(hipEventRecord(ev_[8], s));
             MRT_HIP(hipEventSynchronize(ev_[8]));
@@ -1133,9 +1172,10 @@ void Engine::enqueueFrame(hipStream_t s, bool timed, const FrameConsts* g, const
             stats_.stageMs[2] += elapsed(ev_[9], ev_[4]);
             stats_.stageMs[5] += elapsed(ev_[4], ev_[8]);
         }
+        if (last) break;
     }
     if (timed) MRT_HIP(hipEventRecord(ev_[4], s));
-    MRT_HIP(dev::launchTraceBatch(bv, dev::kNoLevel, maxB, shadowShape_, tailPaths_, false, s));
+    if (tailLevel > maxB) MRT_HIP(dev::launchTraceBatch(bv, dev::kNoLevel, maxB, shadowShape_, tailPaths_, false, s));
     if (timed) {
         MRT_HIP(hipEventRecord(ev_[5], s));
         MRT_HIP(hipEventSynchronize(ev_[5]));
@@ -1242,6 +1282,7 @@ void Engine::renderFrame(mrtFrameInfo* out) {
         fillEntries(he, dConsts_.as<const FrameConsts>(), 1);
         MRT_HIP(hipMemcpyAsync(dEntry_.get(), he, sizeof(dev::BatchEntry), hipMemcpyHostToDevice, stream_));
         const dev::BatchEntry* entry = dEntry_.as<const dev::BatchEntry>();
+        stats_.graphTailLevel = ~0u;
         if (profiling_) {
             std::memset(stats_.stageMs, 0, sizeof(stats_.stageMs));
             enqueueFrame(stream_, true, dConsts_.as<const FrameConsts>(), entry);
@@ -1263,11 +1304,13 @@ void Engine::renderFrame(mrtFrameInfo* out) {
             stats_.status |= c[dev::kStatusWord];
         } else if (useGraph_) {
             if (graphExec_ && graphTrainSteps_ != trainSteps(0)) invalidateGraph();  // warm-up over
+            if (graphExec_ && !tailLevelFits(graphTailLevel_)) invalidateGraph();
             if (!graphExec_) {
                 graphTrainSteps_ = trainSteps(0);
+                graphTailLevel_ = fittedTailLevel();
                 MRT_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
                 try {
-                    enqueueFrame(stream_, false, dConsts_.as<const FrameConsts>(), entry);
+                    enqueueFrame(stream_, false, dConsts_.as<const FrameConsts>(), entry, 0, graphTailLevel_);
                 } catch (...) {
                     hipGraph_t g2 = nullptr;
                     (void)hipStreamEndCapture(stream_, &g2);
@@ -1279,10 +1322,12 @@ void Engine::renderFrame(mrtFrameInfo* out) {
             }
             MRT_HIP(hipGraphLaunch(graphExec_, stream_));
             MRT_HIP(hipStreamSynchronize(stream_));
+            stats_.graphTailLevel = graphTailLevel_;
         } else {
             enqueueFrame(stream_, false, dConsts_.as<const FrameConsts>(), entry);
             MRT_HIP(hipStreamSynchronize(stream_));
         }
+        levelRaysFresh_ = true;  // kFinishBatch wrote this frame's queue profile
         if (pathView().pathRec) ++trainedFrames_;
         ++frameIndex_;
     }

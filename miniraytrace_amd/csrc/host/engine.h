@@ -60,6 +60,7 @@ struct FrameStats {
     uint32_t shadowRays[dev::kMaxBounceLevels] = {};
     uint32_t status = 0;
     uint32_t guideRows = 0;  // guiding-network query records of the frame (all levels)
+    uint32_t graphTailLevel = ~0u;  // tailLevel of the graph the last frame replayed (~0u: none)
 };
 
 class Engine {
@@ -169,8 +170,12 @@ private:
     void uploadTextures();
     void allocFrame();
     void fillConsts(FrameConsts& g) const;
+    // tailLevel: the tail kernel takes every path left at that level and the
+    // frame ends there (~0u: per-level launches to the last bounce).
     void enqueueFrame(hipStream_t s, bool timed, const FrameConsts* g, const dev::BatchEntry* entry,
-                      uint32_t offset = 0);
+                      uint32_t offset = 0, uint32_t tailLevel = ~0u);
+    uint32_t fittedTailLevel() const;
+    bool tailLevelFits(uint32_t level) const;
     // Adam steps of the frame `offset` frames after the next one (0 when
     // the engine does not train): the warm-up's count for the network's
     // first warmupFrames trained frames, then stepsPerFrame.
@@ -259,6 +264,9 @@ private:
     float shapeMs_[kShapeCandidates] = {};
     bool shapeTimed_ = false;
     uint32_t graphTrainSteps_ = 0; // Adam steps per frame in the captured frame graph
+    uint32_t graphTailLevel_ = ~0u;  // tailLevel of the captured frame graph
+    PinnedBuffer hLevelRays_;        // rays queued per level in the last frame (written by the GPU)
+    bool levelRaysFresh_ = false;    // hLevelRays_ holds a frame of the current frame setup
     uint32_t lastPoolCap_ = 0;     // training pool size of the last frame enqueued
     float sceneBoxMin_[3] = { 0.0f, 0.0f, 0.0f };
     float sceneBoxScale_ = 1.0f;

@@ -905,6 +905,17 @@ __device__ __forceinline__ void resolvePixel(const PathView& pv, const FrameCons
 
 // Accumulate and resolve of every active request of a batched frame.
 __global__ __launch_bounds__(256) void kFinishBatch(BatchView bv) {
+    if (blockIdx.x == 0 && threadIdx.x < bv.count) {
+        // The frame's queue profile, for the host's choice of graph depth.
+        const PathView& pv = bv.entries[threadIdx.x].pv;
+        if (bv.entries[threadIdx.x].active && pv.levelRays) {
+            for (uint32_t b = 0; b < kMaxBounceLevels; ++b) {
+                uint32_t n = 0;
+                for (uint32_t q = 0; q < kQueueSegs; ++q) n += pv.counters[cntWord(b, kCntRays, q)];
+                pv.levelRays[b] = n;
+            }
+        }
+    }
     for (uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; w < bv.pixelSlots; w += gridDim.x * blockDim.x) {
         const uint32_t r = batchRequestOfSlot(bv, w);
         const BatchEntry& e = bv.entries[r];

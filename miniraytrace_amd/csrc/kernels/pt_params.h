@@ -97,6 +97,9 @@ struct PathView {
     // beta * rrScale is the throughput BSDF sampling alone would carry.
     // Russian roulette reads that throughput.
     float* rrScale;
+    // Host-mapped pinned words (null when not recorded): kFinishBatch writes
+    // the rays queued at each level of the frame (kMaxBounceLevels words).
+    uint32_t* levelRays;
 };
 
 // One request of a batched frame (requests keep their own queues, counters

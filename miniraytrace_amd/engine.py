@@ -270,7 +270,8 @@ class Engine:
         return {"frameMs": s.frameMs, "gpuMs": dict(zip(names, list(s.gpuMs))),
                 "rays": list(s.rays), "shadowRays": list(s.shadowRays), "status": s.status,
                 "commitMs": s.commitMs, "commitUploadBytes": s.commitUploadBytes,
-                "geometryRepacks": s.geometryRepacks, "meshesWritten": s.meshesWritten}
+                "geometryRepacks": s.geometryRepacks, "meshesWritten": s.meshesWritten,
+                "graphTailLevel": s.graphTailLevel}
 
     def export_scene(self) -> dict:
         """Packed device scene as numpy arrays (two calls: counts, then data)."""
