@@ -60,3 +60,40 @@ def test_fitted_graph_depth_renders_the_same_image(torch_gpu):
     # the first frame has no queue profile yet; the graph captured after it ends at level 2
     assert graph[1][0] == NONE and all(lv == 2 for lv in graph[1][1:])
     assert np.array_equal(graph[0].view(np.uint32), eager[0].view(np.uint32))
+
+
+def test_graph_depth_follows_the_queue_profile(torch_gpu):
+    """The camera turns to the sky: level 1 empties, the graph ending at level
+    2 no longer fits (the level before its last holds far fewer rays than the
+    threshold), the next frame captures one ending at level 1; images still
+    equal the launch-by-launch ones frame for frame."""
+    from miniraytrace_amd import scenes
+    W, H = 320, 180
+    e = _engine(W, H, 1, 0, False)
+    e.set_profiling(True)
+    e.render_frame()
+    tail = int(e.frame_stats()["rays"][2])
+    e.close()
+    cam = dict(scenes.CONFIG2["camera"])
+    sky_cam = dict(cam, target=(cam["position"][0], cam["position"][1] + 10.0, cam["position"][2] - 0.5))
+    _print(f"case: graph depth after a camera turn, {W}x{H}, tail threshold {tail}")
+    runs = {}
+    for use_graph in (False, True):
+        e = _engine(W, H, 1 << 20, tail, use_graph)
+        levels, images = [], []
+        for f in range(7):
+            if f == 3:
+                e.set_camera(**sky_cam)
+            e.render_frame()
+            st = e.frame_stats()
+            levels.append(st["graphTailLevel"])
+            images.append(e.read_rgba32f(W, H).copy())
+        runs[use_graph] = (levels, images)
+        e.close()
+    levels = runs[True][0]
+    _print(f"graph tail levels per frame: {levels}")
+    assert runs[False][0] == [NONE] * 7
+    assert levels[0] == NONE and levels[1] == levels[2] == levels[3] == 2
+    assert levels[5] == levels[6] and levels[5] in (0, 1)
+    for f in range(7):
+        assert np.array_equal(runs[True][1][f].view(np.uint32), runs[False][1][f].view(np.uint32)), f"frame {f}"
