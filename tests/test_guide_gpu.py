@@ -339,28 +339,33 @@ def test_guided_mean_matches_plain_at_1080p(gpu):
 def test_guided_lowers_error(gpu):
     """Error against a 4096-spp plain render (the unbiased estimator the
     oracle pins, so a bias of the guided estimator is not shared by the
-    reference): at 64 spp the guided image's MSE is below plain's (measured
-    ratio about 0.83; the reference's own error adds the same amount to
+    reference): at 64 spp the guided image's MSE is below plain's.  The
+    room's radiance is heavy-tailed (sun paths through the window), so one
+    64-spp image per estimator is a noisy measure of its error: the MSEs are
+    averaged over 4 independent seeds each (measured: 0.60 averaged,
+    0.52-0.69 per seed; the reference's own error adds the same amount to
     both)."""
-    _print("case: guided vs plain error at equal samples, plain reference")
-    W, H, N = 96, 54, 64
+    _print("case: guided vs plain error at equal samples, plain reference, 4 seeds")
+    W, H, N, SEEDS = 96, 54, 64, 4
     ref_e = _room_engine(W, H, 10**6)
     ref_e.set_path_options(seed=777)
     ref = _render(ref_e, 4096, W, H)
     plain = _room_engine(W, H, 10**6)
-    plain.set_path_options(seed=5)
-    ip = _render(plain, N, W, H)
     guided = _room_engine(W, H, 10**6)
     guided.set_guiding(sampling=True, training=True, batch_size=4096, steps_per_frame=2)
     _render(guided, 64, W, H)  # warm-up training
-    guided.set_path_options(seed=6)
-    guided.reset_accumulation()
-    ig = _render(guided, N, W, H)
-    mse_p = ((ip - ref) ** 2).mean()
-    mse_g = ((ig - ref) ** 2).mean()
-    _print(f"MSE at {N} spp: plain {mse_p:.4e}, guided {mse_g:.4e}, ratio {mse_g / mse_p:.3f}; "
-           f"stats {guided.guiding_stats()}")
-    assert mse_g < 0.95 * mse_p
+    mse_p, mse_g = [], []
+    for i in range(SEEDS):
+        plain.set_path_options(seed=5 + 2 * i)
+        plain.reset_accumulation()
+        mse_p.append(((_render(plain, N, W, H) - ref) ** 2).mean())
+        guided.set_path_options(seed=6 + 2 * i)
+        guided.reset_accumulation()
+        mse_g.append(((_render(guided, N, W, H) - ref) ** 2).mean())
+    mp, mg = float(np.mean(mse_p)), float(np.mean(mse_g))
+    _print(f"MSE at {N} spp: plain {mp:.4e}, guided {mg:.4e}, ratio {mg / mp:.3f} "
+           f"(per seed {[round(float(g / p), 3) for g, p in zip(mse_g, mse_p)]}); stats {guided.guiding_stats()}")
+    assert mg < 0.95 * mp
 
 
 def test_environment_strata_are_learned(gpu):
